@@ -1,0 +1,188 @@
+"""MI355X-native end-to-end GPU dynamic-programming planner (Python mirror of the C ABI).
+
+The reference pipeline is two executables (src/build_model.cu, src/spvi_2.cu) joined by
+.npy files. Here the same two stages are library calls on one GPU:
+
+    m = build_model(params, fields)          # build_sparse_transition_model_at_T_at_a,
+                                             # all t and a (src/build_model.cu:696-892)
+    S1, S2, val, R = m.export_coo()          # save_master_Coos_to_file (:1246-1314)
+    policy, value, stats = m.solve(thresh)   # solver_spvi_solve (src/spvi_2.cu:650-726)
+
+Fields are numpy arrays (host) or torch tensors already on the GPU (device pointers are
+passed through; nothing is copied back). The HIP library must be built; there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import E2EError, check, lib
+from .configs import Params as PyParams
+
+__all__ = ["Model", "build_model", "import_coo", "c_params", "device_count", "E2EError",
+           "pack_params", "actions"]
+
+_PFIELDS = [f[0] for f in _abi.Params._fields_]
+
+
+def c_params(p) -> _abi.Params:
+    """Accepts a configs.Params, a dict, or an _abi.Params."""
+    if isinstance(p, _abi.Params):
+        return p
+    d = p if isinstance(p, dict) else p.__dict__
+    cp = _abi.Params()
+    for k in _PFIELDS:
+        if k in d:
+            setattr(cp, k, d[k])
+    return cp
+
+
+def device_count() -> int:
+    return int(lib().e2e_device_count())
+
+
+def pack_params(p) -> np.ndarray:
+    """The reference's 32-float prob_params vector (src/build_model.cu:1032-1055)."""
+    out = np.zeros(32, np.float32)
+    check(lib().e2e_params_pack(C.byref(c_params(p)), out.ctypes.data))
+    return out
+
+
+def actions(p):
+    cp = c_params(p)
+    na = cp.num_ac_speeds * cp.num_ac_angles
+    sp = np.zeros(na, np.float32)
+    an = np.zeros(na, np.float32)
+    check(lib().e2e_actions(C.byref(cp), sp.ctypes.data, an.ctypes.data))
+    return sp, an
+
+
+def _ptr(x, keep, dtype):
+    if x is None:
+        return None, False
+    if hasattr(x, "data_ptr") and getattr(x, "is_cuda", False):
+        assert x.is_contiguous(), "device field tensors must be contiguous"
+        keep.append(x)
+        return x.data_ptr(), True
+    a = np.ascontiguousarray(x, dtype=dtype)
+    keep.append(a)
+    return a.ctypes.data, False
+
+
+class Model:
+    """A device-resident transition model (state-major CSR + R) with its solve buffers."""
+
+    def __init__(self, handle: int, params=None):
+        self._h = C.c_void_p(handle)
+        self.params = params
+
+    # -- lifetime
+    def close(self):
+        if self._h:
+            lib().e2e_model_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self) -> int:
+        return self._h.value
+
+    # -- info
+    @property
+    def info(self) -> dict:
+        i = _abi.ModelInfo()
+        check(lib().e2e_model_info_get(self._h, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in i._fields_}
+
+    # -- export
+    def export_coo(self, index_dtype=np.int64):
+        """Reference on-disk order (rows a*Ns+s sorted, S2 ascending), R of Na*Ns."""
+        inf = self.info
+        nnz, rows = inf["nnz"], inf["ns"] * inf["na"]
+        it = np.dtype(index_dtype)
+        S1 = np.empty(nnz, it)
+        S2 = np.empty(nnz, it)
+        val = np.empty(nnz, np.float32)
+        R = np.empty(rows, np.float32)
+        fn = lib().e2e_model_export_coo if it == np.int64 else lib().e2e_model_export_coo_i32
+        check(fn(self._h, S1.ctypes.data, S2.ctypes.data, val.ctypes.data, R.ctypes.data))
+        return S1, S2, val, R
+
+    def export_csr(self):
+        inf = self.info
+        nnz, rows = inf["nnz"], inf["ns"] * inf["na"]
+        rp = np.empty(rows + 1, np.uint32)
+        col = np.empty(nnz, np.int32)
+        val = np.empty(nnz, np.float32)
+        R = np.empty(rows, np.float32)
+        check(lib().e2e_model_export_csr(self._h, rp.ctypes.data, col.ctypes.data, val.ctypes.data,
+                                         R.ctypes.data))
+        return rp, col, val, R
+
+    # -- solve
+    def solve(self, thresh: float = 0.1, discount: float = 1.0, max_sweeps: int = 0,
+              sweeps_per_check: int = 0, use_graph: bool = False, lanes_per_row: int = 0,
+              timeout_ms: float = 0.0, raise_on_nonconv: bool = True):
+        """Sparse VI. Returns (policy uint32[Ns], value float32[Ns], stats dict)."""
+        ns = self.info["ns"]
+        o = _abi.SolveOpts(thresh, discount, max_sweeps, sweeps_per_check, int(use_graph),
+                           lanes_per_row, timeout_ms)
+        st = _abi.SolveStats()
+        pol = np.empty(ns, np.uint32)
+        val = np.empty(ns, np.float32)
+        code = lib().e2e_vi_solve(self._h, C.byref(o), pol.ctypes.data, val.ctypes.data, C.byref(st))
+        stats = {k: getattr(st, k) for k, _ in st._fields_}
+        stats["status"] = code
+        if code != 0 and (raise_on_nonconv or code not in (4, 5)):
+            check(code)
+        return pol, val, stats
+
+
+def build_model(params, fields: dict, keep_rsum: bool = False, slot_k: int = 0,
+                radix_bits: int = 0, slices_per_launch: int = 0, stream: int = 0) -> Model:
+    """Monte Carlo build on the GPU. `fields` holds u, v, ui, vi, yi and optionally s,
+    mask (numpy arrays, or contiguous torch CUDA tensors)."""
+    keep = []
+    ptrs = {}
+    on_dev = None
+    for k, dt in (("u", np.float32), ("v", np.float32), ("ui", np.float32), ("vi", np.float32),
+                  ("yi", np.float32), ("s", np.float32), ("mask", np.int32)):
+        p, dev = _ptr(fields.get(k), keep, dt)
+        ptrs[k] = p
+        if p is not None:
+            if on_dev is None:
+                on_dev = dev
+            elif on_dev != dev:
+                raise ValueError("fields must be all on the host or all on the device")
+    f = _abi.Fields(**ptrs)
+    o = _abi.BuildOpts(int(bool(on_dev)), int(keep_rsum), slot_k, radix_bits, slices_per_launch, 0,
+                       stream or None)
+    h = C.c_void_p()
+    check(lib().e2e_model_build(C.byref(c_params(params)), C.byref(f), C.byref(o), C.byref(h)))
+    return Model(h.value, params)
+
+
+def import_coo(ns: int, na: int, S1, S2, val, R, stream: int = 0) -> Model:
+    """Model from the reference COO layout (what src/spvi_2.cu:393-504 loads)."""
+    S1 = np.ascontiguousarray(S1, np.int32).ravel()
+    S2 = np.ascontiguousarray(S2, np.int32).ravel()
+    val = np.ascontiguousarray(val, np.float32).ravel()
+    R = np.ascontiguousarray(R, np.float32).ravel()
+    h = C.c_void_p()
+    check(lib().e2e_model_import_coo(ns, na, len(S1), S1.ctypes.data, S2.ctypes.data, val.ctypes.data,
+                                     R.ctypes.data, stream or None, C.byref(h)))
+    return Model(h.value)

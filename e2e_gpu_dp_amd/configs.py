@@ -1,0 +1,111 @@
+"""Named workloads: problem parameters plus the seeded field recipe.
+
+Parameter names follow src/input_to_build_model.h:3-48. Every workload uses dx = dy =
+dt = 1, x0 = y0 = 0.5, one speed F = 1, reward method 0 ("time", -dt),
+r_outbound = -100, r_terminal = +100, window m = 21, and one terminal cell
+(term_subgrid_size 1) at a seeded interior location.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from .fields import make_fields
+
+
+@dataclasses.dataclass
+class Params:
+    gsize: int
+    nt: int
+    nrzns: int
+    nmodes: int
+    num_ac_speeds: int = 1
+    num_ac_angles: int = 8
+    i_term: int = 0
+    j_term: int = 0
+    term_subgrid_size: int = 1
+    reward_type: int = 0
+    neighb_gsize: int = 21
+    is_stationary: int = 0
+    F: float = 1.0
+    dt: float = 1.0
+    dx: float = 1.0
+    dy: float = 1.0
+    x0: float = 0.5
+    y0: float = 0.5
+    r_outbound: float = -100.0
+    r_terminal: float = 100.0
+    alpha: float = 0.0
+    z: float = -9999.0
+
+    @property
+    def na(self) -> int:
+        return self.num_ac_speeds * self.num_ac_angles
+
+    @property
+    def nc(self) -> int:
+        return self.gsize * self.gsize
+
+    @property
+    def ns(self) -> int:
+        return self.nc * self.nt
+
+    def replace(self, **kw) -> "Params":
+        return dataclasses.replace(self, **kw)
+
+
+def _goal(g: int, seed: int):
+    rng = np.random.default_rng(seed + 77)
+    return int(rng.integers(g // 4, 3 * g // 4)), int(rng.integers(g // 4, 3 * g // 4))
+
+
+@dataclasses.dataclass
+class Workload:
+    name: str
+    params: Params
+    seed: int
+    sigmas: list
+
+    def fields(self):
+        p = self.params
+        return make_fields(p.gsize, p.nt, p.nmodes, p.nrzns, self.seed, self.sigmas, F=p.F)
+
+    @property
+    def lanes(self) -> int:
+        p = self.params
+        return (p.nt - 1) * p.nc * p.na * p.nrzns
+
+
+def workload(name: str) -> Workload:
+    """C1 (small, CPU-runnable), C2 (100^2 x 120, the single-GPU bench case),
+    C4 (200^2 x 200, large); C3 and C5 reuse these builders."""
+    if name == "C1":
+        g, nt, na, nr, nm, seed = 25, 20, 8, 1000, 4, 1234
+        sig = [1.0, 0.5, 0.25, 0.125]
+    elif name == "C2":
+        g, nt, na, nr, nm, seed = 100, 120, 16, 2500, 5, 2345
+        sig = [2.0 ** -k for k in range(nm)]
+    elif name == "C4":
+        g, nt, na, nr, nm, seed = 200, 200, 16, 5000, 6, 4567
+        sig = [2.0 ** -k for k in range(nm)]
+    else:
+        raise KeyError(name)
+    it, jt = _goal(g, seed)
+    p = Params(gsize=g, nt=nt, nrzns=nr, nmodes=nm, num_ac_angles=na, i_term=it, j_term=jt)
+    return Workload(name, p, seed, sig)
+
+
+def small(g=7, nt=4, nr=40, nm=2, na=8, seed=7, **kw) -> Workload:
+    """Tiny workloads for brute-force parity (the reference's own debug size is
+    5x5, nt=3, nr=5, 1 mode: prel_tests/build_model.cu:944-974)."""
+    it, jt = _goal(g, seed)
+    base = dict(gsize=g, nt=nt, nrzns=nr, nmodes=nm, num_ac_angles=na, i_term=it, j_term=jt)
+    base.update(kw)
+    return Workload(f"small{g}x{nt}", Params(**base), seed, [2.0 ** -k for k in range(nm)])
+
+
+def goal_cells(g: int, k: int = 64, seed: int = 3456):
+    """k distinct goal cells drawn uniformly from the grid (cell index i*g + j)."""
+    rng = np.random.default_rng(seed)
+    return rng.choice(g * g, size=k, replace=False).astype(np.int32)

@@ -1,0 +1,1139 @@
+// build.hip — Monte Carlo transition-model build on the GPU: (a) move, (b) LDS segmented
+// radix sort + count-by-key, (c) fused CSR build; plus the reference-order COO
+// export and the COO import used by the solver driver.
+//
+// Reference semantics: src/build_model.cu (transition_calc :483-559, move :268-356,
+// extract_velocity :361-404, rewards :231-265, obstacles :419-477, compute_mean
+// :563-570, COO emission :597-644, host loop :1142-1314), with the counting done as
+// in the sort-based builder prel_tests/build_model.cu:410-487,705-706 (sort the S2 keys
+// of each (s,a) segment, one entry per distinct key, count = run length), so every S2
+// is counted (no 21x21 window drop; out-of-window lanes are reported).
+//
+// Layout in HBM
+//   fields (read-only): u,v,s [nt][g][g], ui,vi [nt][nm][g][g], yi [nt][nr][nm], mask
+//   staging  : uint2 [rows][K]  (sp2, count) per distinct S2 of a row; rows = ns*na,
+//              row = (t*nc + cell)*na + a  (state-major, so the solver reads all
+//              actions of a state from one contiguous nnz range)
+//   arena    : uint2 overflow entries of rows with more than K distinct S2
+//   row_nnz  : uint32 [rows]; R, Rsum : float [rows]
+//   CSR      : row_ptr uint32 [rows+1], nz int2 [nnz] = (col, bits(val))
+#pragma clang fp contract(off)
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "internal.h"
+
+namespace e2e {
+
+namespace {
+
+constexpr int kWG = 256;     // threads per build workgroup (4 wave64)
+constexpr int kWaves = kWG / 64;
+
+struct BuildArgs {
+    const float* u;
+    const float* v;
+    const float* ui;
+    const float* vi;
+    const float* yi;
+    const float* s;
+    const int32_t* mask;
+    const int32_t* slice_obs;  // [nt] any obstacle in slice (NULL: no mask)
+    const float* xs;
+    const float* ys;
+    float xs0, xsn, ys0, ysn, Dj, Di;
+    double hDj, hDi;           // 0.5*Dj, 0.5*Di in double (move :318-330)
+    double late_pen;           // 0.05*r_outbound in double (move :353)
+    float cell_diag;           // fabsf(xs[1]-xs[0])*1.414213 rounded to float (:427)
+    int g, nc, nt, nm, nr, na;
+    int i_term, j_term, tsg, reward_type, m;
+    float dt, r_out, r_term, alpha;
+    int t0, K, radix_bits, R, tc;
+    int stage_y;
+    uint2* staging;
+    uint32_t* row_nnz;
+    float* Rout;
+    float* Rsum;
+    uint2* arena;
+    unsigned long long* arena_top;
+    unsigned long long arena_cap;
+    unsigned long long* stats;  // 0 oow lanes, 1 overflow rows, 2 max nuq, 3 nonfinite,
+                                // 4 arena exhausted, 5 multipass rows
+    float act_speed[kMaxActions];
+    float act_fcos[kMaxActions];
+    float act_fsin[kMaxActions];
+};
+
+__device__ __forceinline__ bool is_terminal(const BuildArgs& A, int i, int j) {
+    return (i >= A.i_term && i < A.i_term + A.tsg) && (j >= A.j_term && j < A.j_term + A.tsg);
+}
+
+// calculate_one_step_reward :231-265
+__device__ __forceinline__ float step_reward(const BuildArgs& A, float ac_speed, float rad1,
+                                             float rad2) {
+    const int method = A.reward_type;
+    const float Cr = 1.0f, Cf = 1.0f, Ct = (float)0.01;
+    const float dt = A.dt;
+    if (method == 0) return -dt;
+    if (method == 1) return -(Cf * ac_speed * ac_speed + Ct) * dt;
+    if (method == 2) return ((Cr * (rad2 + rad1) / 2) - (Cf * ac_speed * ac_speed) - Ct) * dt;
+    if (method == 3) return ((Cr * (rad2 + rad1) / 2) - Ct) * dt;
+    if (method == 4) {
+        float energy_reward = -(ac_speed * ac_speed) * dt;
+        float time_reward = -dt;
+        return A.alpha * energy_reward + (1 - A.alpha) * time_reward;
+    }
+    return 0.0f;
+}
+
+// is_within_band :419-441
+__device__ bool within_band(const BuildArgs& A, int i, int j, int i1, int j1, int i2, int j2) {
+    if (i1 == i2 || j1 == j2) return true;
+    const int g = A.g;
+    float x = A.xs[j], y = A.ys[g - 1 - i];
+    float x1 = A.xs[j1], y1 = A.ys[g - 1 - i1];
+    float x2 = A.xs[j2], y2 = A.ys[g - 1 - i2];
+    float Aa = (y2 - y1) / (x2 - x1);
+    float B = -1.0f;
+    float C = y1 - (Aa * x1);
+    float dist = fabsf(Aa * x + B * y + C) / sqrtf((Aa * Aa) + (B * B));
+    return dist < A.cell_diag;
+}
+
+// goes_through_obstacle :444-477
+__device__ bool crosses_obstacle(const BuildArgs& A, int i1, int j1, int i2, int j2, int T) {
+    const int g = A.g;
+    const int imin = min(i1, i2), imax = max(i1, i2), jmin = min(j1, j2), jmax = max(j1, j2);
+    const int32_t* m0 = A.mask + (size_t)T * A.nc;
+    const int32_t* m1 = m0 + A.nc;
+    for (int i = imin; i <= imax; i++)
+        for (int j = jmin; j <= jmax; j++) {
+            if (!(i == i1 && j == j1)) {
+                int sp = j + g * i;
+                if (m0[sp] == 1 || m1[sp] == 1)
+                    if (within_band(A, i, j, i1, j1, i2, j2)) return true;
+            }
+        }
+    return false;
+}
+
+// move :268-356 for a lane that starts at the centre (xc, yc) of cell (pi, pj).
+__device__ __forceinline__ void lane_move(const BuildArgs& A, float xc, float yc, float fc,
+                                          float fs, float vx, float vy, int T, int& pi,
+                                          int& pj, float& r) {
+    const int n = A.g - 1;
+    float vnetx = fc + vx;
+    float vnety = fs + vy;
+    float xnew = xc + (vnetx * A.dt);
+    float ynew = yc + (vnety * A.dt);
+    r = 0.0f;
+    if (xnew > A.xsn) {
+        xnew = A.xsn;
+        r += A.r_out;
+    } else if (xnew < A.xs0) {
+        xnew = A.xs0;
+        r += A.r_out;
+    }
+    if (ynew > A.ysn) {
+        ynew = A.ysn;
+        r += A.r_out;
+    } else if (ynew < A.ys0) {
+        ynew = A.ys0;
+        r += A.r_out;
+    }
+    const float ax = xnew - A.xs0;
+    const float ay = -(ynew - A.ysn);
+    const float remx = remainderf(ax, A.Dj);
+    const float remy = remainderf(ay, A.Di);
+    int xind = (int)((ax - remx) / A.Dj);
+    int yind = (int)((ay - remy) / A.Di);
+    // the if/else-if ladder of :318-330 adds one to each index whose remainder is at
+    // least half a cell (compared in double)
+    xind += ((double)remx >= A.hDj) ? 1 : 0;
+    yind += ((double)remy >= A.hDi) ? 1 : 0;
+    pi = yind;
+    pj = xind;
+    if (pi == 0 || pi == n || pj == 0 || pj == n) r += A.r_out;
+    if (is_terminal(A, pi, pj))
+        r += A.r_term;
+    else if (T > A.nt - 20)
+        r = (float)((double)r + A.late_pen);
+}
+
+__device__ __forceinline__ int wave_min(int x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
+    return x;
+}
+__device__ __forceinline__ int wave_max(int x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
+    return x;
+}
+__device__ __forceinline__ int wave_sum(int x) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+// Exclusive scan over the 256 threads of the block. lds: >= 4 ints. Two barriers.
+__device__ __forceinline__ int block_excl_scan_int(int v, int* lds, int* total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds[wid] = x;
+    __syncthreads();
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+        int s = lds[w];
+        if (w < wid) pre += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + x - v;
+}
+
+// Peers of this lane among `valid` lanes whose low `nbits` bits of `digit` agree.
+__device__ __forceinline__ unsigned long long match_peers(uint32_t digit, int nbits,
+                                                          bool valid) {
+    unsigned long long peers = __ballot(valid);
+    for (int b = 0; b < nbits; b++) {
+        const bool bit = (digit >> b) & 1u;
+        const unsigned long long bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+    }
+    return peers;
+}
+
+// Writes entry u of the current row (to the K staging slots or the overflow arena).
+__device__ __forceinline__ void put_entry(const BuildArgs& A, size_t row, int u, int nuq,
+                                          unsigned long long arena_off, uint32_t sp2,
+                                          uint32_t count) {
+    if (nuq <= A.K) {
+        A.staging[row * A.K + u] = make_uint2(sp2, count);
+    } else if (arena_off + (unsigned long long)nuq <= A.arena_cap) {
+        A.arena[arena_off + u] = make_uint2(sp2, count);
+    }
+}
+
+// One workgroup = TC consecutive cells of one time slice t. For each cell it samples the
+// velocity of every realization once (registers), then for each action:
+//   move every lane (one (s, a, w) triple per lane-round), range-compress the S2 keys of
+//   the segment to nbits bits, LSD-radix-sort them in LDS with radix_bits-wide digits
+//   (ballot match ranking, stable scatter), count runs (count-by-key), and write the
+//   row's distinct S2 with their counts plus the row reward sum.
+// When nbits <= radix_bits a single counting pass orders the keys, and its digit
+// histogram is the count-by-key, so the scatter is skipped.
+template <int R_MAX>
+__global__ __launch_bounds__(kWG) void mc_move_count(const BuildArgs A) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int R = A.R;
+    const int nr = A.nr, nm = A.nm, g = A.g, nc = A.nc;
+    const int D = A.radix_bits, NBMAX = 1 << D;
+    double* rpart = reinterpret_cast<double*>(smem_raw);      // [256]
+    int* misc = reinterpret_cast<int*>(rpart + kWG);          // [64]
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(misc + 64);   // [4][NBMAX]
+    uint32_t* keysA = cnt + kWaves * NBMAX;                   // [R*256 + 4]
+    uint32_t* keysB = keysA + R * kWG + 4;                    // [R*256 + 4]
+    float* ylds = reinterpret_cast<float*>(keysB + R * kWG + 4);  // [nm][nr] when staged
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+
+    const int t = A.t0 + blockIdx.y;
+    const size_t tslice = (size_t)t * nc;
+    if (A.stage_y) {
+        const float* src = A.yi + (size_t)t * nr * nm;
+        for (int i = tid; i < nr * nm; i += kWG) {
+            int w = i / nm, k = i - w * nm;
+            ylds[k * nr + w] = src[i];
+        }
+        __syncthreads();
+    }
+    const bool obs_slice = A.slice_obs && (A.slice_obs[t] | A.slice_obs[t + 1]);
+    unsigned long long oow_acc = 0, multipass_acc = 0, overflow_acc = 0;
+    uint32_t maxnuq_acc = 0;
+    bool bad = false, bad_index = false;
+
+    for (int ci = 0; ci < A.tc; ci++) {
+        const int c = blockIdx.x * A.tc + ci;
+        if (c >= nc) break;
+        const int i1 = c / g, j1 = c - i1 * g;
+        const size_t mid = tslice + c;
+        // extract_velocity :361-404, hoisted out of the action loop
+        float mx[kMaxModes], my[kMaxModes];
+#pragma unroll
+        for (int k = 0; k < kMaxModes; k++) {
+            if (k < nm) {
+                const size_t o = ((size_t)t * nm + k) * nc + c;
+                mx[k] = A.ui[o];
+                my[k] = A.vi[o];
+            }
+        }
+        const float u0 = A.u[mid], v0 = A.v[mid];
+        float vx[R_MAX], vy[R_MAX];
+#pragma unroll
+        for (int r = 0; r < R_MAX; r++) {
+            const int w = r * kWG + tid;
+            vx[r] = 0.0f;
+            vy[r] = 0.0f;
+            if (r < R && w < nr) {
+                float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+                for (int k = 0; k < kMaxModes; k++) {
+                    if (k < nm) {
+                        const float y = A.stage_y ? ylds[k * nr + w]
+                                                  : A.yi[((size_t)t * nr + w) * nm + k];
+                        sx = sx + mx[k] * y;
+                        sy = sy + my[k] * y;
+                    }
+                }
+                vx[r] = u0 + sx;
+                vy[r] = v0 + sy;
+                if (!(isfinite(vx[r]) && isfinite(vy[r]))) bad = true;
+            }
+        }
+        const bool term1 = is_terminal(A, i1, j1);
+        const bool obs1 = A.mask && A.mask[mid] == 1;
+        const float rad1 = A.s ? A.s[mid] : 0.0f;
+        const float xc = A.xs[j1], yc = A.ys[g - 1 - i1];
+        const int del = (A.m - 1) / 2;
+
+        for (int a = 0; a < A.na; a++) {
+            const float speed = A.act_speed[a], fc = A.act_fcos[a], fs = A.act_fsin[a];
+            uint32_t key[R_MAX];
+            int dimin = 0x7fffffff, dimax = -0x7fffffff, djmin = 0x7fffffff, djmax = -0x7fffffff;
+            int oow = 0;
+            double part = 0.0;
+            // ---- (a) move: transition_calc :522-545
+#pragma unroll
+            for (int r = 0; r < R_MAX; r++) {
+                const int w = r * kWG + tid;
+                int di = 0, dj = 0;
+                if (r < R && w < nr) {
+                    int pi = i1, pj = j1;
+                    float rw = 0.0f;
+                    if (!term1) {
+                        if (!obs1) {
+                            lane_move(A, xc, yc, fc, fs, vx[r], vy[r], t, pi, pj, rw);
+                            if (pi < 0 || pi >= g || pj < 0 || pj >= g) {  // cannot happen for
+                                bad_index = true;                          // dx, dy > 0; guard
+                                pi = min(max(pi, 0), g - 1);               // the gathers below
+                                pj = min(max(pj, 0), g - 1);
+                            }
+                            const int sp2 = pj + g * pi;
+                            const float rad2 = A.s ? A.s[tslice + nc + sp2] : 0.0f;
+                            rw += step_reward(A, speed, rad1, rad2);
+                            if (obs_slice && crosses_obstacle(A, i1, j1, pi, pj, t)) rw = A.r_out;
+                        } else {
+                            rw = A.r_out;
+                        }
+                    }
+                    di = pi - i1;
+                    dj = pj - j1;
+                    part += (double)rw;
+                    dimin = min(dimin, di);
+                    dimax = max(dimax, di);
+                    djmin = min(djmin, dj);
+                    djmax = max(djmax, dj);
+                    if (di + del < 0 || di + del >= A.m || dj + del < 0 || dj + del >= A.m) oow++;
+                }
+                key[r] = ((uint32_t)(di + 32768) << 16) | (uint32_t)(dj + 32768);
+            }
+            dimin = wave_min(dimin);
+            dimax = wave_max(dimax);
+            djmin = wave_min(djmin);
+            djmax = wave_max(djmax);
+            oow = wave_sum(oow);
+            rpart[tid] = part;
+            if (lane == 0) {
+                misc[wv * 8 + 0] = dimin;
+                misc[wv * 8 + 1] = dimax;
+                misc[wv * 8 + 2] = djmin;
+                misc[wv * 8 + 3] = djmax;
+                misc[wv * 8 + 4] = oow;
+            }
+            for (int i = tid; i < kWaves * NBMAX; i += kWG) cnt[i] = 0u;
+            __syncthreads();
+            dimin = misc[0], dimax = misc[1], djmin = misc[2], djmax = misc[3];
+            int oow_blk = misc[4];
+#pragma unroll
+            for (int w = 1; w < kWaves; w++) {
+                dimin = min(dimin, misc[w * 8 + 0]);
+                dimax = max(dimax, misc[w * 8 + 1]);
+                djmin = min(djmin, misc[w * 8 + 2]);
+                djmax = max(djmax, misc[w * 8 + 3]);
+                oow_blk += misc[w * 8 + 4];
+            }
+            // reward sum in the canonical order: partial[tid] over ascending w, then the
+            // tree partial[i] += partial[i+o], o = 128..1 (compute_mean :563-570 divides)
+            double rsum = 0.0;
+            if (wv == 0) {
+                double p = (rpart[lane] + rpart[lane + 128]) + (rpart[lane + 64] + rpart[lane + 192]);
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) {
+                    double q = __shfl_down(p, o, 64);
+                    p = p + q;
+                }
+                rsum = p;
+            }
+            // ---- (b) range-compress: key = (di - dimin) * Wj + (dj - djmin); this keeps
+            // the (i2, j2) lexicographic order, i.e. ascending S2 = (t+1)*nc + j2 + g*i2
+            const uint32_t Wj = (uint32_t)(djmax - djmin + 1);
+            const uint64_t nkeys = (uint64_t)(dimax - dimin + 1) * Wj;
+            const int nbits = nkeys <= 1 ? 0 : 64 - __clzll((long long)(nkeys - 1));
+#pragma unroll
+            for (int r = 0; r < R_MAX; r++) {
+                const int di = (int)(key[r] >> 16) - 32768, dj = (int)(key[r] & 0xffffu) - 32768;
+                key[r] = (uint32_t)(di - dimin) * Wj + (uint32_t)(dj - djmin);
+            }
+            const size_t row = (mid)*A.na + a;
+            int nuq = 0;
+            unsigned long long arena_off = 0;
+            if (nbits <= D) {
+                // single counting pass: per-wave digit histogram by ballot matching
+#pragma unroll
+                for (int r = 0; r < R_MAX; r++) {
+                    if (r < R) {
+                        const bool valid = (r * kWG + tid) < nr;
+                        const unsigned long long peers = match_peers(key[r], nbits, valid);
+                        if (valid && lane == __ffsll((long long)peers) - 1)
+                            cnt[wv * NBMAX + key[r]] += (uint32_t)__popcll(peers);
+                    }
+                }
+                __syncthreads();
+                uint32_t tot = 0;
+                if (tid < NBMAX) {
+#pragma unroll
+                    for (int w = 0; w < kWaves; w++) tot += cnt[w * NBMAX + tid];
+                }
+                int total;
+                const int u = block_excl_scan_int(tot > 0 ? 1 : 0, misc + 48, &total);
+                nuq = total;
+                if (nuq > A.K) {
+                    if (tid == 0) {
+                        unsigned long long off = atomicAdd(A.arena_top, (unsigned long long)nuq);
+                        reinterpret_cast<unsigned long long*>(misc + 56)[0] = off;
+                    }
+                    __syncthreads();
+                    arena_off = reinterpret_cast<unsigned long long*>(misc + 56)[0];
+                }
+                if (tot > 0) {
+                    const int di = (int)(tid / Wj) + dimin, dj = (int)(tid % Wj) + djmin;
+                    put_entry(A, row, u, nuq, arena_off, (uint32_t)((j1 + dj) + g * (i1 + di)), tot);
+                }
+            } else {
+                // multi-pass LSD radix sort of the nr keys in LDS
+                const int passes = (nbits + D - 1) / D;
+                uint32_t* src = keysA;
+                uint32_t* dst = keysB;
+                uint32_t lrank[R_MAX];
+                for (int ps = 0; ps < passes; ps++) {
+                    const int shift = ps * D;
+                    const int dbits = min(D, nbits - shift);
+                    const int nbp = 1 << dbits;
+                    if (ps > 0) {
+#pragma unroll
+                        for (int r = 0; r < R_MAX; r++) {
+                            const int pos = wv * (R * 64) + r * 64 + lane;
+                            key[r] = (r < R && pos < nr) ? src[pos] : 0u;
+                        }
+                        for (int i = tid; i < kWaves * NBMAX; i += kWG) cnt[i] = 0u;
+                        __syncthreads();
+                    }
+                    // rank: order (wave, round, lane); pass 0 takes w = r*256 + tid
+#pragma unroll
+                    for (int r = 0; r < R_MAX; r++) {
+                        if (r < R) {
+                            const bool valid = ps == 0 ? (r * kWG + tid) < nr
+                                                       : (wv * (R * 64) + r * 64 + lane) < nr;
+                            const uint32_t d = (key[r] >> shift) & (uint32_t)(nbp - 1);
+                            const unsigned long long peers = match_peers(d, dbits, valid);
+                            const uint32_t base = valid ? cnt[wv * NBMAX + d] : 0u;
+                            lrank[r] = base + (uint32_t)__popcll(peers & lt_mask);
+                            if (valid && lane == __ffsll((long long)peers) - 1)
+                                cnt[wv * NBMAX + d] = base + (uint32_t)__popcll(peers);
+                        }
+                    }
+                    __syncthreads();
+                    // offsets: digit-major, then wave order (stable)
+                    uint32_t tot = 0, wc[kWaves];
+                    if (tid < nbp) {
+#pragma unroll
+                        for (int w = 0; w < kWaves; w++) {
+                            wc[w] = cnt[w * NBMAX + tid];
+                            tot += wc[w];
+                        }
+                    }
+                    int total;
+                    const int base = block_excl_scan_int((int)tot, misc + 48, &total);
+                    if (tid < nbp) {
+                        uint32_t run = (uint32_t)base;
+#pragma unroll
+                        for (int w = 0; w < kWaves; w++) {
+                            cnt[w * NBMAX + tid] = run;
+                            run += wc[w];
+                        }
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int r = 0; r < R_MAX; r++) {
+                        if (r < R) {
+                            const bool valid = ps == 0 ? (r * kWG + tid) < nr
+                                                       : (wv * (R * 64) + r * 64 + lane) < nr;
+                            if (valid) {
+                                const uint32_t d = (key[r] >> shift) & (uint32_t)(nbp - 1);
+                                dst[cnt[wv * NBMAX + d] + lrank[r]] = key[r];
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    uint32_t* tmp = src;
+                    src = dst;
+                    dst = tmp;
+                }
+                // count-by-key: run heads of the sorted keys (thread-contiguous blocks)
+                int heads = 0;
+                const int p0 = tid * R;
+                for (int k = 0; k < R; k++) {
+                    const int p = p0 + k;
+                    if (p < nr && (p == 0 || src[p] != src[p - 1])) heads++;
+                }
+                int total;
+                int u = block_excl_scan_int(heads, misc + 48, &total);
+                nuq = total;
+                for (int k = 0; k < R; k++) {
+                    const int p = p0 + k;
+                    if (p < nr && (p == 0 || src[p] != src[p - 1])) dst[u++] = (uint32_t)p;
+                }
+                if (tid == 0) {
+                    dst[nuq] = (uint32_t)nr;
+                    if (nuq > A.K) {
+                        unsigned long long off = atomicAdd(A.arena_top, (unsigned long long)nuq);
+                        reinterpret_cast<unsigned long long*>(misc + 56)[0] = off;
+                    }
+                }
+                __syncthreads();
+                if (nuq > A.K) arena_off = reinterpret_cast<unsigned long long*>(misc + 56)[0];
+                for (int q = tid; q < nuq; q += kWG) {
+                    const uint32_t p = dst[q];
+                    const uint32_t kk = src[p];
+                    const int di = (int)(kk / Wj) + dimin, dj = (int)(kk % Wj) + djmin;
+                    put_entry(A, row, q, nuq, arena_off, (uint32_t)((j1 + dj) + g * (i1 + di)),
+                              dst[q + 1] - p);
+                }
+                multipass_acc++;
+            }
+            if (tid == 0) {
+                A.row_nnz[row] = (uint32_t)nuq;
+                const float fs_ = (float)rsum;
+                A.Rout[row] = fs_ / nr;
+                if (A.Rsum) A.Rsum[row] = fs_;
+                oow_acc += (unsigned long long)oow_blk;
+                if ((uint32_t)nuq > maxnuq_acc) maxnuq_acc = (uint32_t)nuq;
+                if (nuq > A.K) {
+                    overflow_acc++;
+                    if (arena_off + (unsigned long long)nuq > A.arena_cap) atomicOr(A.stats + 4, 1ull);
+                    else A.staging[row * A.K] = make_uint2((uint32_t)arena_off, (uint32_t)(arena_off >> 32));
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (bad) atomicOr(A.stats + 3, 1ull);
+    if (bad_index) atomicOr(A.stats + 6, 1ull);
+    if (tid == 0) {
+        if (oow_acc) atomicAdd(A.stats + 0, oow_acc);
+        if (overflow_acc) atomicAdd(A.stats + 1, overflow_acc);
+        if (maxnuq_acc) atomicMax(A.stats + 2, (unsigned long long)maxnuq_acc);
+        if (multipass_acc) atomicAdd(A.stats + 5, multipass_acc);
+    }
+}
+
+// (c) CSR scatter: one thread per (row, slot). col = (t+1)*nc + sp2, val = count/nr
+// (reduce_kernel :635-637: S2 = state1D_from_spid(t+1, sp_id2), prob = count/nrzns).
+__global__ __launch_bounds__(256) void csr_scatter(const uint2* __restrict__ staging,
+                                                   const uint2* __restrict__ arena,
+                                                   const uint32_t* __restrict__ row_nnz,
+                                                   const uint32_t* __restrict__ row_ptr,
+                                                   int2* __restrict__ nz, int64_t rows, int K,
+                                                   int64_t rows_per_slice, int64_t nc, int nr) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = idx / K;
+    const int u0 = (int)(idx - row * K);
+    if (row >= rows) return;
+    const uint32_t n = row_nnz[row];
+    const int64_t t = row / rows_per_slice;
+    const int64_t colbase = (t + 1) * nc;
+    const uint32_t dst = row_ptr[row];
+    if ((int)n <= K) {
+        if (u0 < (int)n) {
+            uint2 e = staging[row * K + u0];
+            nz[dst + u0] = make_int2((int)(colbase + e.x), __float_as_int((float)e.y / nr));
+        }
+    } else {
+        const uint2 h = staging[row * K];
+        const unsigned long long off = (unsigned long long)h.x | ((unsigned long long)h.y << 32);
+        for (int u = u0; u < (int)n; u += K) {
+            uint2 e = arena[off + u];
+            nz[dst + u] = make_int2((int)(colbase + e.x), __float_as_int((float)e.y / nr));
+        }
+    }
+}
+
+__global__ void fill_tail(uint32_t* row_nnz, float* R, float* Rsum, int64_t from, int64_t to) {
+    int64_t i = from + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < to) {
+        row_nnz[i] = 0;
+        R[i] = 0.0f;
+        if (Rsum) Rsum[i] = 0.0f;
+    }
+}
+
+__global__ void slice_any(const int32_t* __restrict__ mask, int64_t nc, int32_t* flags) {
+    const int t = blockIdx.y;
+    int any = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nc;
+         i += (int64_t)gridDim.x * blockDim.x)
+        any |= (mask[t * nc + i] == 1);
+    if (__any(any) && (threadIdx.x & 63) == 0) atomicOr(flags + t, 1);
+}
+
+// ---- export to the reference COO order (rows a*ns + s)
+__global__ void counts_action_major(const uint32_t* __restrict__ row_ptr, uint32_t* __restrict__ out,
+                                    int64_t ns, int64_t na) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // action-major row
+    if (i >= ns * na) return;
+    int64_t a = i / ns, s = i - a * ns;
+    int64_t r = s * na + a;
+    out[i] = row_ptr[r + 1] - row_ptr[r];
+}
+
+template <typename IT>
+__global__ void export_rows(const uint32_t* __restrict__ row_ptr, const int2* __restrict__ nz,
+                           const uint32_t* __restrict__ off_am, const float* __restrict__ R,
+                           IT* __restrict__ S1, IT* __restrict__ S2, float* __restrict__ val,
+                           float* __restrict__ Ram, int64_t ns, int64_t na) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // state-major row
+    if (r >= ns * na) return;
+    int64_t s = r / na, a = r - s * na;
+    int64_t am = a * ns + s;
+    uint32_t b = row_ptr[r], e = row_ptr[r + 1], d = off_am[am];
+    for (uint32_t j = b; j < e; j++, d++) {
+        int2 x = nz[j];
+        if (S1) S1[d] = (IT)am;
+        if (S2) S2[d] = (IT)x.x;
+        if (val) val[d] = __int_as_float(x.y);
+    }
+    if (Ram) Ram[am] = R[r];
+}
+
+// ---- import from the reference COO order
+__global__ void counts_state_major(const uint32_t* __restrict__ rp_am, uint32_t* __restrict__ out,
+                                   int64_t ns, int64_t na) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // state-major row
+    if (r >= ns * na) return;
+    int64_t s = r / na, a = r - s * na;
+    int64_t am = a * ns + s;
+    out[r] = rp_am[am + 1] - rp_am[am];
+}
+
+__global__ void import_rows(const uint32_t* __restrict__ rp_am, const uint32_t* __restrict__ row_ptr,
+                            const int32_t* __restrict__ S2, const float* __restrict__ val,
+                            const float* __restrict__ Ram, int2* __restrict__ nz,
+                            float* __restrict__ R, int64_t ns, int64_t na) {
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= ns * na) return;
+    int64_t s = r / na, a = r - s * na;
+    int64_t am = a * ns + s;
+    uint32_t b = rp_am[am], e = rp_am[am + 1], d = row_ptr[r];
+    for (uint32_t j = b; j < e; j++, d++) nz[d] = make_int2(S2[j], __float_as_int(val[j]));
+    R[r] = Ram[am];
+}
+
+// ------------------------------------------------------------------ host helpers
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <typename T>
+    T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+e2e_status dmalloc(DevBuf& b, size_t bytes) {
+    hipError_t e = hipMalloc(&b.p, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        b.p = nullptr;
+        set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+        return E2E_ERR_NOMEM;
+    }
+    return E2E_OK;
+}
+
+template <int R_MAX>
+e2e_status launch_build(const BuildArgs& A, dim3 grid, size_t lds, hipStream_t s) {
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mc_move_count<R_MAX>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) {
+            set_error("hipFuncSetAttribute(%zu B LDS): %s", lds, hipGetErrorString(e));
+            return E2E_ERR_HIP;
+        }
+    }
+    hipLaunchKernelGGL(mc_move_count<R_MAX>, grid, dim3(kWG), lds, s, A);
+    return E2E_OK;
+}
+
+e2e_status dispatch_build(const BuildArgs& A, dim3 grid, size_t lds, hipStream_t s) {
+    const int R = A.R;
+    if (R <= 1) return launch_build<1>(A, grid, lds, s);
+    if (R <= 2) return launch_build<2>(A, grid, lds, s);
+    if (R <= 4) return launch_build<4>(A, grid, lds, s);
+    if (R <= 8) return launch_build<8>(A, grid, lds, s);
+    if (R <= 12) return launch_build<12>(A, grid, lds, s);
+    if (R <= 16) return launch_build<16>(A, grid, lds, s);
+    if (R <= 20) return launch_build<20>(A, grid, lds, s);
+    if (R <= 32) return launch_build<32>(A, grid, lds, s);
+    set_error("nrzns=%d exceeds the build kernel limit of %d", A.nr, 32 * kWG);
+    return E2E_ERR_CAPACITY;
+}
+
+void host_actions(const e2e_params* p, float* speed, float* angle, float* fcos, float* fsin) {
+    // populate_ac_angles :1403-1408 (double, stored float), populate_ac_speeds :1412-1434
+    const int nsp = p->num_ac_speeds, nan_ = p->num_ac_angles;
+    std::vector<float> sp(nsp), an(nan_);
+    for (int i = 0; i < nan_; i++) an[i] = (float)(i * (2 * M_PI) / nan_);
+    if (nsp == 1) {
+        sp[0] = p->F;
+    } else {
+        const float delF = p->F / (nsp);
+        for (int i = 0; i < nsp; i++) sp[i] = (i + 1) * delF;
+    }
+    for (int i = 0; i < nsp; i++)
+        for (int j = 0; j < nan_; j++) {
+            const int idx = j + nan_ * i;  // populate_actions :1449
+            if (speed) speed[idx] = sp[i];
+            if (angle) angle[idx] = an[j];
+            // F*cosf(ac_angle), F*sinf(ac_angle) of move :284-285, taken once per action
+            if (fcos) fcos[idx] = sp[i] * cosf(an[j]);
+            if (fsin) fsin[idx] = sp[i] * sinf(an[j]);
+        }
+}
+
+e2e_status validate_params(const e2e_params* p) {
+    E2E_CHECK(p, E2E_ERR_INVALID, "params is NULL");
+    E2E_CHECK(p->gsize >= 2 && p->gsize <= 32767, E2E_ERR_INVALID, "gsize=%d out of range [2, 32767]", p->gsize);
+    E2E_CHECK(p->nt >= 2, E2E_ERR_INVALID, "nt=%d must be >= 2", p->nt);
+    E2E_CHECK(p->nrzns >= 1, E2E_ERR_INVALID, "nrzns=%d must be >= 1", p->nrzns);
+    E2E_CHECK(p->nmodes >= 0 && p->nmodes <= kMaxModes, E2E_ERR_CAPACITY, "nmodes=%d must be in [0, %d]", p->nmodes, kMaxModes);
+    E2E_CHECK(p->num_ac_speeds >= 1 && p->num_ac_angles >= 1, E2E_ERR_INVALID, "need >= 1 speed and angle");
+    E2E_CHECK(p->num_ac_speeds * p->num_ac_angles <= kMaxActions, E2E_ERR_CAPACITY, "at most %d actions", kMaxActions);
+    E2E_CHECK(p->reward_type >= 0 && p->reward_type <= 4, E2E_ERR_INVALID, "reward_type=%d not in 0..4", p->reward_type);
+    E2E_CHECK(p->neighb_gsize >= 1 && (p->neighb_gsize % 2) == 1, E2E_ERR_INVALID, "neighb_gsize must be odd (build_model.cu:1028)");
+    E2E_CHECK(p->term_subgrid_size >= 0, E2E_ERR_INVALID, "term_subgrid_size < 0");
+    E2E_CHECK(p->dx > 0.0f && p->dy > 0.0f && std::isfinite(p->x0) && std::isfinite(p->y0) &&
+                  std::isfinite(p->dt) && std::isfinite(p->F),
+              E2E_ERR_INVALID, "dx, dy must be > 0 and x0, y0, dt, F finite");
+    return E2E_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ build
+
+e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build_opts* o,
+                       Model** out) {
+    const double t_start = now_ms();
+    E2E_TRY(validate_params(p));
+    E2E_CHECK(f && f->u && f->v && f->yi && (p->nmodes == 0 || (f->ui && f->vi)), E2E_ERR_INVALID,
+              "fields u, v, yi (and ui, vi when nmodes > 0) are required");
+    E2E_CHECK(!(p->reward_type == 2 || p->reward_type == 3) || f->s, E2E_ERR_INVALID,
+              "reward_type %d needs the scalar field s", p->reward_type);
+    e2e_build_opts opts{};
+    if (o) opts = *o;
+    const int K = opts.slot_k > 0 ? opts.slot_k : 16;
+    const int D = opts.radix_bits > 0 ? opts.radix_bits : 8;
+    E2E_CHECK(D >= 1 && D <= 8, E2E_ERR_INVALID, "radix_bits=%d not in 1..8", D);
+    E2E_CHECK(K >= 1 && K <= 4096, E2E_ERR_INVALID, "slot_k=%d not in 1..4096", K);
+
+    Model* m = new Model();
+    m->p = *p;
+    m->built = true;
+    const int64_t g = p->gsize, nt = p->nt, nc = g * g, na = (int64_t)p->num_ac_speeds * p->num_ac_angles;
+    m->nc = nc;
+    m->nt = nt;
+    m->ns = nc * nt;
+    m->na = na;
+    m->rows = m->ns * na;
+    e2e_status st = model_alloc_stream(m, opts.stream);
+    if (st != E2E_OK) {
+        delete m;
+        return st;
+    }
+    auto fail = [&](e2e_status s) {
+        free_model(m);
+        return s;
+    };
+    hipStream_t s = m->stream;
+    const int nm = p->nmodes, nr = p->nrzns;
+
+    // fields on the device
+    DevBuf dU, dV, dUi, dVi, dYi, dS, dM;
+    const float *u = f->u, *v = f->v, *ui = f->ui, *vi = f->vi, *yi = f->yi, *sf = f->s;
+    const int32_t* mask = f->mask;
+    if (!opts.fields_on_device) {
+        auto up = [&](DevBuf& b, const void* h, size_t bytes) -> e2e_status {
+            if (!h) return E2E_OK;
+            E2E_TRY(dmalloc(b, bytes));
+            E2E_HIP(hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, s));
+            return E2E_OK;
+        };
+        const size_t f1 = sizeof(float) * nt * nc, fm = f1 * (nm > 0 ? nm : 1);
+        if ((st = up(dU, f->u, f1)) || (st = up(dV, f->v, f1)) ||
+            (st = up(dUi, nm ? f->ui : nullptr, fm)) || (st = up(dVi, nm ? f->vi : nullptr, fm)) ||
+            (st = up(dYi, f->yi, sizeof(float) * nt * nr * (nm > 0 ? nm : 1))) ||
+            (st = up(dS, f->s, f1)) || (st = up(dM, f->mask, sizeof(int32_t) * nt * nc)))
+            return fail(st);
+        u = dU.as<float>();
+        v = dV.as<float>();
+        ui = nm ? dUi.as<float>() : nullptr;
+        vi = nm ? dVi.as<float>() : nullptr;
+        yi = dYi.as<float>();
+        sf = f->s ? dS.as<float>() : nullptr;
+        mask = f->mask ? dM.as<int32_t>() : nullptr;
+    }
+
+    // grid ticks (define_xs_or_ys :1395-1399) and actions
+    std::vector<float> xs(g), ys(g);
+    for (int i = 0; i < g; i++) {
+        xs[i] = p->x0 + i * p->dx;
+        ys[i] = p->y0 + i * p->dy;
+    }
+    BuildArgs A;
+    std::memset(&A, 0, sizeof(A));
+    host_actions(p, A.act_speed, nullptr, A.act_fcos, A.act_fsin);
+    DevBuf dXY, dFlags, dStats;
+    if ((st = dmalloc(dXY, sizeof(float) * 2 * g))) return fail(st);
+    E2E_HIP(hipMemcpyAsync(dXY.p, xs.data(), sizeof(float) * g, hipMemcpyHostToDevice, s));
+    E2E_HIP(hipMemcpyAsync(dXY.as<float>() + g, ys.data(), sizeof(float) * g, hipMemcpyHostToDevice, s));
+    if ((st = dmalloc(dStats, sizeof(unsigned long long) * 8))) return fail(st);
+    E2E_HIP(hipMemsetAsync(dStats.p, 0, sizeof(unsigned long long) * 8, s));
+    if (mask) {
+        if ((st = dmalloc(dFlags, sizeof(int32_t) * nt))) return fail(st);
+        E2E_HIP(hipMemsetAsync(dFlags.p, 0, sizeof(int32_t) * nt, s));
+        hipLaunchKernelGGL(slice_any, dim3((unsigned)std::min<int64_t>(ceil_div(nc, 256), 64), (unsigned)nt),
+                           dim3(256), 0, s, mask, nc, dFlags.as<int32_t>());
+    }
+
+    A.u = u; A.v = v; A.ui = ui; A.vi = vi; A.yi = yi; A.s = sf; A.mask = mask;
+    A.slice_obs = mask ? dFlags.as<int32_t>() : nullptr;
+    A.xs = dXY.as<float>();
+    A.ys = dXY.as<float>() + g;
+    const int n = (int)g - 1;
+    A.xs0 = xs[0]; A.xsn = xs[n]; A.ys0 = ys[0]; A.ysn = ys[n];
+    A.Dj = fabsf(xs[1] - xs[0]);
+    A.Di = fabsf(ys[1] - ys[0]);
+    A.hDj = 0.5 * A.Dj;
+    A.hDi = 0.5 * A.Di;
+    A.late_pen = 0.05 * (double)p->r_outbound;
+    A.cell_diag = (float)(fabsf(xs[1] - xs[0]) * 1.414213);
+    A.g = (int)g; A.nc = (int)nc; A.nt = (int)nt; A.nm = nm; A.nr = nr; A.na = (int)na;
+    A.i_term = p->i_term; A.j_term = p->j_term; A.tsg = p->term_subgrid_size;
+    A.reward_type = p->reward_type; A.m = p->neighb_gsize;
+    A.dt = p->dt; A.r_out = p->r_outbound; A.r_term = p->r_terminal; A.alpha = p->alpha;
+    A.K = K;
+    A.radix_bits = D;
+    A.R = (int)ceil_div(nr, kWG);
+    A.stats = dStats.as<unsigned long long>();
+
+    // row-indexed outputs
+    const int64_t rows = m->rows;
+    DevBuf dNnz, dStage, dArena, dTop;
+    if ((st = dmalloc(dNnz, sizeof(uint32_t) * rows))) return fail(st);
+    if ((st = dmalloc(dStage, sizeof(uint2) * rows * K))) return fail(st);
+    if (hipMalloc(&m->R, sizeof(float) * rows) != hipSuccess) {
+        set_error("hipMalloc R failed");
+        return fail(E2E_ERR_NOMEM);
+    }
+    if (opts.keep_rsum && hipMalloc(&m->Rsum, sizeof(float) * rows) != hipSuccess) {
+        set_error("hipMalloc Rsum failed");
+        return fail(E2E_ERR_NOMEM);
+    }
+    unsigned long long arena_cap = std::max<unsigned long long>(1ull << 20, (unsigned long long)rows / 4);
+    if ((st = dmalloc(dTop, sizeof(unsigned long long)))) return fail(st);
+
+    A.staging = dStage.as<uint2>();
+    A.row_nnz = dNnz.as<uint32_t>();
+    A.Rout = m->R;
+    A.Rsum = m->Rsum;
+
+    // last slice: no rows (build_model.cu:1142), R = 0 (:1173-1176)
+    {
+        const int64_t from = (nt - 1) * nc * na, to = rows;
+        hipLaunchKernelGGL(fill_tail, dim3((unsigned)ceil_div(to - from, 256)), dim3(256), 0, s,
+                           A.row_nnz, m->R, m->Rsum, from, to);
+    }
+
+    // LDS: rpart 2 KiB + misc 256 B + cnt + 2 key buffers (+ staged Yi)
+    const size_t lds_base = sizeof(double) * kWG + sizeof(int) * 64 +
+                            sizeof(uint32_t) * kWaves * (1 << D) + sizeof(uint32_t) * 2 * (A.R * kWG + 4);
+    const size_t ybytes = sizeof(float) * (size_t)nr * nm;
+    const bool stage = ybytes > 0 && ybytes <= 16 * 1024;
+    A.stage_y = stage ? 1 : 0;
+    A.tc = stage ? 4 : 1;
+    const size_t lds = lds_base + (stage ? ybytes : 0);
+    E2E_CHECK(lds <= 160 * 1024, E2E_ERR_CAPACITY, "build kernel needs %zu B of LDS", lds);
+
+    const int64_t nslices = nt - 1;
+    const int64_t spl = opts.slices_per_launch > 0 ? opts.slices_per_launch : nslices;
+    hipEvent_t e0, e1;
+    E2E_HIP(hipEventCreate(&e0));
+    E2E_HIP(hipEventCreate(&e1));
+    double move_ms = 0.0;
+    int launches = 0;
+    for (int attempt = 0; attempt < 4; attempt++) {
+        if ((st = dmalloc(dArena, sizeof(uint2) * arena_cap))) break;
+        E2E_HIP(hipMemsetAsync(dTop.p, 0, sizeof(unsigned long long), s));
+        E2E_HIP(hipMemsetAsync(dStats.p, 0, sizeof(unsigned long long) * 8, s));
+        A.arena = dArena.as<uint2>();
+        A.arena_top = dTop.as<unsigned long long>();
+        A.arena_cap = arena_cap;
+        move_ms = 0.0;
+        launches = 0;
+        for (int64_t t0 = 0; t0 < nslices; t0 += spl) {
+            const int64_t cnt = std::min(spl, nslices - t0);
+            A.t0 = (int)t0;
+            dim3 grid((unsigned)ceil_div(nc, A.tc), (unsigned)cnt);
+            E2E_HIP(hipEventRecord(e0, s));
+            if ((st = dispatch_build(A, grid, lds, s))) break;
+            E2E_HIP(hipGetLastError());
+            E2E_HIP(hipEventRecord(e1, s));
+            E2E_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            E2E_HIP(hipEventElapsedTime(&ms, e0, e1));
+            move_ms += ms;
+            launches++;
+        }
+        if (st) break;
+        unsigned long long hs[8];
+        E2E_HIP(hipMemcpyAsync(hs, dStats.p, sizeof(hs), hipMemcpyDeviceToHost, s));
+        E2E_HIP(hipStreamSynchronize(s));
+        if (hs[3]) {
+            set_error("non-finite velocity sample in the input fields");
+            st = E2E_ERR_FIELD;
+            break;
+        }
+        if (hs[6]) {
+            set_error("a lane left the grid (check dx, dy, x0, y0)");
+            st = E2E_ERR_INVALID;
+            break;
+        }
+        m->info.n_out_of_window = (int64_t)hs[0];
+        m->info.n_overflow_rows = (int64_t)hs[1];
+        m->info.max_row_nnz = (int64_t)hs[2];
+        m->info.n_multipass_rows = (int64_t)hs[5];
+        if (!hs[4]) break;
+        // arena exhausted: grow and rebuild
+        unsigned long long used = 0;
+        E2E_HIP(hipMemcpy(&used, dTop.p, sizeof(used), hipMemcpyDeviceToHost));
+        (void)hipFree(dArena.p);
+        dArena.p = nullptr;
+        arena_cap = std::max(arena_cap * 4, used + 1024);
+        if (attempt == 3) {
+            set_error("overflow arena exhausted after 4 attempts");
+            st = E2E_ERR_CAPACITY;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) return fail(st);
+    m->info.move_kernel_ms = move_ms;
+    m->info.move_launches = launches;
+
+    // (c) scan of row counts -> row_ptr, then scatter into the CSR
+    DevBuf dTmp, dTot;
+    if ((st = dmalloc(dTmp, sizeof(unsigned long long) * scan_tmp_words(rows)))) return fail(st);
+    if ((st = dmalloc(dTot, sizeof(unsigned long long)))) return fail(st);
+    if (hipMalloc(&m->row_ptr, sizeof(uint32_t) * (rows + 1)) != hipSuccess) {
+        set_error("hipMalloc row_ptr failed");
+        return fail(E2E_ERR_NOMEM);
+    }
+    hipEvent_t c0, c1, c2;
+    E2E_HIP(hipEventCreate(&c0));
+    E2E_HIP(hipEventCreate(&c1));
+    E2E_HIP(hipEventCreate(&c2));
+    E2E_HIP(hipEventRecord(c0, s));
+    if ((st = scan_u32(A.row_nnz, m->row_ptr, rows, dTmp.as<unsigned long long>(),
+                       dTot.as<unsigned long long>(), s)))
+        return fail(st);
+    E2E_HIP(hipEventRecord(c1, s));
+    unsigned long long total = 0;
+    E2E_HIP(hipMemcpyAsync(&total, dTot.p, sizeof(total), hipMemcpyDeviceToHost, s));
+    E2E_HIP(hipStreamSynchronize(s));
+    E2E_CHECK(total < (1ull << 32) && total < (1ull << 31), E2E_ERR_CAPACITY,
+              "nnz=%llu exceeds the 2^31 column-index range", total);
+    m->nnz = (int64_t)total;
+    if (hipMalloc(&m->nz, sizeof(int2) * (total ? total : 1)) != hipSuccess) {
+        set_error("hipMalloc nz (%llu entries) failed", total);
+        return fail(E2E_ERR_NOMEM);
+    }
+    hipLaunchKernelGGL(csr_scatter, dim3((unsigned)ceil_div(rows * K, 256)), dim3(256), 0, s,
+                       dStage.as<uint2>(), dArena.as<uint2>(), A.row_nnz, m->row_ptr, m->nz, rows, K,
+                       nc * na, nc, nr);
+    E2E_HIP(hipGetLastError());
+    E2E_HIP(hipEventRecord(c2, s));
+    E2E_HIP(hipEventSynchronize(c2));
+    float ms1 = 0, ms2 = 0;
+    E2E_HIP(hipEventElapsedTime(&ms1, c0, c1));
+    E2E_HIP(hipEventElapsedTime(&ms2, c1, c2));
+    (void)hipEventDestroy(c0);
+    (void)hipEventDestroy(c1);
+    (void)hipEventDestroy(c2);
+    m->info.scan_ms = ms1;
+    m->info.csr_ms = ms2;
+    m->info.lanes = (nt - 1) * nc * na * (int64_t)nr;
+    m->info.build_ms = now_ms() - t_start;
+    *out = m;
+    return E2E_OK;
+}
+
+e2e_status model_import(int64_t ns, int64_t na, int64_t nnz, const int32_t* S1, const int32_t* S2,
+                        const float* val, const float* R, void* stream, Model** out) {
+    E2E_CHECK(ns > 0 && na > 0 && nnz >= 0 && S1 && S2 && val && R, E2E_ERR_INVALID,
+              "import: bad arguments");
+    E2E_CHECK(na <= kMaxActions, E2E_ERR_CAPACITY, "import: at most %d actions", kMaxActions);
+    E2E_CHECK(nnz < (1ll << 31) && ns * na < (1ll << 31), E2E_ERR_CAPACITY, "import: model too large");
+    const int64_t rows = ns * na;
+    // action-major row pointers; S1 must be sorted by row (the reference writes it so)
+    std::vector<uint32_t> rp(rows + 1, 0);
+    for (int64_t j = 0; j < nnz; j++) {
+        E2E_CHECK(S1[j] >= 0 && S1[j] < rows, E2E_ERR_INVALID, "import: S1[%lld]=%d out of range",
+                  (long long)j, S1[j]);
+        E2E_CHECK(j == 0 || S1[j] >= S1[j - 1], E2E_ERR_INVALID, "import: S1 not sorted at %lld",
+                  (long long)j);
+        E2E_CHECK(S2[j] >= 0 && S2[j] < ns, E2E_ERR_INVALID, "import: S2[%lld]=%d out of range",
+                  (long long)j, S2[j]);
+        rp[S1[j] + 1]++;
+    }
+    for (int64_t r = 0; r < rows; r++) rp[r + 1] += rp[r];
+    Model* m = new Model();
+    m->built = false;
+    m->ns = ns;
+    m->na = na;
+    m->rows = rows;
+    m->nnz = nnz;
+    e2e_status st = model_alloc_stream(m, stream);
+    if (st) {
+        delete m;
+        return st;
+    }
+    auto fail = [&](e2e_status s) {
+        free_model(m);
+        return s;
+    };
+    hipStream_t s = m->stream;
+    DevBuf dRp, dS2, dVal, dR, dCnt, dTmp, dTot;
+    if ((st = dmalloc(dRp, sizeof(uint32_t) * (rows + 1))) || (st = dmalloc(dS2, sizeof(int32_t) * nnz)) ||
+        (st = dmalloc(dVal, sizeof(float) * nnz)) || (st = dmalloc(dR, sizeof(float) * rows)) ||
+        (st = dmalloc(dCnt, sizeof(uint32_t) * rows)) ||
+        (st = dmalloc(dTmp, sizeof(unsigned long long) * scan_tmp_words(rows))) ||
+        (st = dmalloc(dTot, sizeof(unsigned long long))))
+        return fail(st);
+    if (hipMalloc(&m->row_ptr, sizeof(uint32_t) * (rows + 1)) != hipSuccess ||
+        hipMalloc(&m->nz, sizeof(int2) * (nnz ? nnz : 1)) != hipSuccess ||
+        hipMalloc(&m->R, sizeof(float) * rows) != hipSuccess) {
+        set_error("import: device allocation failed");
+        return fail(E2E_ERR_NOMEM);
+    }
+    E2E_HIP(hipMemcpyAsync(dRp.p, rp.data(), sizeof(uint32_t) * (rows + 1), hipMemcpyHostToDevice, s));
+    E2E_HIP(hipMemcpyAsync(dS2.p, S2, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
+    E2E_HIP(hipMemcpyAsync(dVal.p, val, sizeof(float) * nnz, hipMemcpyHostToDevice, s));
+    E2E_HIP(hipMemcpyAsync(dR.p, R, sizeof(float) * rows, hipMemcpyHostToDevice, s));
+    const unsigned nb = (unsigned)ceil_div(rows, 256);
+    hipLaunchKernelGGL(counts_state_major, dim3(nb), dim3(256), 0, s, dRp.as<uint32_t>(),
+                       dCnt.as<uint32_t>(), ns, na);
+    if ((st = scan_u32(dCnt.as<uint32_t>(), m->row_ptr, rows, dTmp.as<unsigned long long>(),
+                       dTot.as<unsigned long long>(), s)))
+        return fail(st);
+    hipLaunchKernelGGL(import_rows, dim3(nb), dim3(256), 0, s, dRp.as<uint32_t>(), m->row_ptr,
+                       dS2.as<int32_t>(), dVal.as<float>(), dR.as<float>(), m->nz, m->R, ns, na);
+    E2E_HIP(hipGetLastError());
+    E2E_HIP(hipStreamSynchronize(s));
+    *out = m;
+    return E2E_OK;
+}
+
+template <typename IT>
+static e2e_status export_coo_t(const Model* m, IT* S1, IT* S2, float* val, float* R) {
+    hipStream_t s = m->stream;
+    const int64_t rows = m->rows, nnz = m->nnz;
+    DevBuf dCnt, dOff, dTmp, dTot, dS1, dS2, dVal, dR;
+    e2e_status st;
+    if ((st = dmalloc(dCnt, sizeof(uint32_t) * rows)) || (st = dmalloc(dOff, sizeof(uint32_t) * (rows + 1))) ||
+        (st = dmalloc(dTmp, sizeof(unsigned long long) * scan_tmp_words(rows))) ||
+        (st = dmalloc(dTot, sizeof(unsigned long long))))
+        return st;
+    if (S1 && (st = dmalloc(dS1, sizeof(IT) * nnz))) return st;
+    if (S2 && (st = dmalloc(dS2, sizeof(IT) * nnz))) return st;
+    if (val && (st = dmalloc(dVal, sizeof(float) * nnz))) return st;
+    if (R && (st = dmalloc(dR, sizeof(float) * rows))) return st;
+    const unsigned nb = (unsigned)ceil_div(rows, 256);
+    hipLaunchKernelGGL(counts_action_major, dim3(nb), dim3(256), 0, s, m->row_ptr, dCnt.as<uint32_t>(),
+                       m->ns, m->na);
+    E2E_TRY(scan_u32(dCnt.as<uint32_t>(), dOff.as<uint32_t>(), rows, dTmp.as<unsigned long long>(),
+                     dTot.as<unsigned long long>(), s));
+    hipLaunchKernelGGL(export_rows<IT>, dim3(nb), dim3(256), 0, s, m->row_ptr, m->nz, dOff.as<uint32_t>(),
+                       m->R, dS1.as<IT>(), dS2.as<IT>(), dVal.as<float>(), dR.as<float>(), m->ns, m->na);
+    E2E_HIP(hipGetLastError());
+    if (S1) E2E_HIP(hipMemcpyAsync(S1, dS1.p, sizeof(IT) * nnz, hipMemcpyDeviceToHost, s));
+    if (S2) E2E_HIP(hipMemcpyAsync(S2, dS2.p, sizeof(IT) * nnz, hipMemcpyDeviceToHost, s));
+    if (val) E2E_HIP(hipMemcpyAsync(val, dVal.p, sizeof(float) * nnz, hipMemcpyDeviceToHost, s));
+    if (R) E2E_HIP(hipMemcpyAsync(R, dR.p, sizeof(float) * rows, hipMemcpyDeviceToHost, s));
+    E2E_HIP(hipStreamSynchronize(s));
+    return E2E_OK;
+}
+
+e2e_status model_export_coo64(const Model* m, int64_t* S1, int64_t* S2, float* val, float* R) {
+    return export_coo_t<int64_t>(m, S1, S2, val, R);
+}
+
+e2e_status model_export_coo32(const Model* m, int32_t* S1, int32_t* S2, float* val, float* R) {
+    E2E_CHECK(m->rows < (1ll << 31), E2E_ERR_CAPACITY, "row index a*Ns+s exceeds int32");
+    return export_coo_t<int32_t>(m, S1, S2, val, R);
+}
+
+e2e_status model_export_csr(const Model* m, uint32_t* row_ptr, int32_t* col, float* val, float* R) {
+    hipStream_t s = m->stream;
+    if (row_ptr) E2E_HIP(hipMemcpyAsync(row_ptr, m->row_ptr, sizeof(uint32_t) * (m->rows + 1), hipMemcpyDeviceToHost, s));
+    if (R) E2E_HIP(hipMemcpyAsync(R, m->R, sizeof(float) * m->rows, hipMemcpyDeviceToHost, s));
+    if (col || val) {
+        std::vector<int2> h(m->nnz);
+        E2E_HIP(hipMemcpyAsync(h.data(), m->nz, sizeof(int2) * m->nnz, hipMemcpyDeviceToHost, s));
+        E2E_HIP(hipStreamSynchronize(s));
+        for (int64_t j = 0; j < m->nnz; j++) {
+            if (col) col[j] = h[j].x;
+            if (val) {
+                int bits = h[j].y;
+                std::memcpy(&val[j], &bits, sizeof(float));
+            }
+        }
+    }
+    E2E_HIP(hipStreamSynchronize(s));
+    return E2E_OK;
+}
+
+void actions_table(const e2e_params* p, float* speed, float* angle) {
+    host_actions(p, speed, angle, nullptr, nullptr);
+}
+
+}  // namespace e2e

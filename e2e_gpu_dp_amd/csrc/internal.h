@@ -1,0 +1,100 @@
+// internal.h — shared declarations of libe2edp (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/e2edp.h"
+
+namespace e2e {
+
+void set_error(const char* fmt, ...);
+
+#define E2E_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ::e2e::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #call,               \
+                             hipGetErrorString(e_));                                    \
+            return E2E_ERR_HIP;                                                         \
+        }                                                                               \
+    } while (0)
+
+#define E2E_CHECK(cond, code, ...)                                                      \
+    do {                                                                                \
+        if (!(cond)) {                                                                  \
+            ::e2e::set_error(__VA_ARGS__);                                              \
+            return (code);                                                              \
+        }                                                                               \
+    } while (0)
+
+#define E2E_TRY(expr)                                                                   \
+    do {                                                                                \
+        e2e_status s_ = (expr);                                                         \
+        if (s_ != E2E_OK) return s_;                                                    \
+    } while (0)
+
+constexpr int kMaxActions = 64;
+constexpr int kMaxModes = 16;
+
+// Device-side solver state of one solve (one per model).
+struct VIState {
+    uint32_t k;          // 1-based index of the next sweep
+    int32_t conv;        // sweep at which the residual first fell below thresh, -1 if none
+    uint32_t res[2];     // residual bits of sweeps k (parity slots); max via atomicMax
+    uint32_t done;       // blocks of the current sweep that finished
+    float last_res;      // residual of the last completed sweep
+    uint32_t pad[2];
+};
+
+// Per-column state of a batched (multi-goal) solve.
+struct BatchState {
+    uint32_t k;          // 1-based index of the next sweep
+    uint32_t done;       // blocks finished in the current sweep
+    int32_t all_conv;    // 1 once every column converged
+    uint32_t pad;
+};
+
+struct Model {
+    e2e_params p{};
+    bool built = false;       // false: imported from COO
+    int64_t ns = 0, na = 0, nc = 0, nt = 0, nnz = 0, rows = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+
+    // state-major CSR, row = s*na + a
+    uint32_t* row_ptr = nullptr;  // [rows+1]
+    int2* nz = nullptr;           // [nnz] (col, val bits)
+    float* R = nullptr;           // [rows]
+    float* Rsum = nullptr;        // [rows] (optional)
+
+    // plain solve buffers
+    float* V[2] = {nullptr, nullptr};
+    int32_t* pi = nullptr;
+    VIState* st = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_chunk = 0;
+    int graph_lpr = 0;
+    float graph_thresh = 0.f, graph_discount = 0.f;
+
+    e2e_model_info info{};
+};
+
+e2e_status model_alloc_stream(Model* m, void* stream);
+e2e_status free_model(Model* m);
+
+// scan.hip: exclusive scan of n uint32 counts into out[0..n] (out[n] = total).
+// tmp must hold scan_tmp_words(n) uint64 words. Returns total via *total_dev (uint64 on device).
+size_t scan_tmp_words(int64_t n);
+e2e_status scan_u32(const uint32_t* in, uint32_t* out, int64_t n, unsigned long long* tmp,
+                    unsigned long long* total_dev, hipStream_t s);
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+double now_ms();
+
+}  // namespace e2e

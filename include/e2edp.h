@@ -1,0 +1,203 @@
+/*
+ * e2edp.h — C ABI of the MI355X-native end-to-end dynamic-programming planner.
+ *
+ * The library (libe2edp.so) replaces the two in-process host functions and the
+ * two executables of the reference pipeline:
+ *
+ *   reference                                            replaced by
+ *   ---------------------------------------------------  ---------------------------------
+ *   build_sparse_transition_model_at_T_at_a(...)          e2e_model_build (all t, all a in
+ *     src/build_model.cu:696-892 (+ the t x a host loop     one device-resident build)
+ *     src/build_model.cu:1142-1165)
+ *   save_master_Coos_to_file(...) src/build_model.cu:1246 e2e_model_export_coo[_i32]
+ *   get_mdp_model(path)           src/spvi_2.cu:393-504   e2e_model_import_coo
+ *   solver_spvi_solve(...)        src/spvi_2.cu:650-726   e2e_vi_solve
+ *
+ * Conventions
+ *   - Every entry point returns an e2e_status; on failure e2e_last_error() gives a
+ *     thread-local message. No C++ exception crosses this boundary.
+ *   - Host buffers are owned by the caller. Device memory is owned by the library
+ *     through opaque handles.
+ *   - A handle is bound to one HIP stream (given at creation or created by the
+ *     library). One handle must not be used by two host threads at once;
+ *     distinct handles may.
+ *   - All results are deterministic: no floating-point atomics touch results.
+ */
+#ifndef E2EDP_H
+#define E2EDP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum e2e_status {
+    E2E_OK = 0,
+    E2E_ERR_INVALID = 1,      /* bad argument / inconsistent shapes */
+    E2E_ERR_HIP = 2,          /* a HIP runtime call failed */
+    E2E_ERR_NOMEM = 3,        /* device or host allocation failed */
+    E2E_ERR_NOT_CONVERGED = 4,/* max_sweeps reached before the residual fell below thresh */
+    E2E_ERR_TIMEOUT = 5,      /* wall-clock limit reached (spvi_2.cu:652 max_solver_time_s) */
+    E2E_ERR_IO = 6,           /* file read/write or format error */
+    E2E_ERR_CAPACITY = 7,     /* a size limit of this implementation was exceeded */
+    E2E_ERR_FIELD = 8         /* non-finite velocity sample found in the input fields */
+} e2e_status;
+
+/* Problem parameters. Field-for-field the variables of src/input_to_build_model.h:3-48
+ * plus alpha (src/build_model.cu:934-943) and the reward method code
+ * (get_reward_type, src/build_model.cu:898-918). The reference packs them as a
+ * 32-float vector (src/build_model.cu:1032-1055); e2e_params_pack() writes that
+ * vector. */
+typedef struct e2e_params {
+    int32_t gsize;             /* grid cells per side (square grid)            h:11 */
+    int32_t nt;                /* time steps                                    h:9  */
+    int32_t nrzns;             /* Monte Carlo realizations                      h:23 */
+    int32_t nmodes;            /* DO modes                                      h:41 */
+    int32_t num_ac_speeds;     /*                                               h:19 */
+    int32_t num_ac_angles;     /*                                               h:20 */
+    int32_t i_term;            /* top-left row of the terminal subgrid          h:33 */
+    int32_t j_term;            /* top-left column of the terminal subgrid       h:34 */
+    int32_t term_subgrid_size; /* side of the terminal subgrid (0: none)        h:40 */
+    int32_t reward_type;       /* 0 time,1 energy1,2 energy2,3 energy3,4 custom1     */
+    int32_t neighb_gsize;      /* odd window side m (out-of-window is reported) h:44 */
+    int32_t is_stationary;     /* carried into prob_params only                 h:46 */
+    float F;                   /* max vehicle speed                             h:18 */
+    float dt;                  /*                                               h:10 */
+    float dx, dy;              /*                                               h:13 */
+    float x0, y0;              /* centre of cell column 0 / row gsize-1         h:15 */
+    float r_outbound;          /*                                               h:26 */
+    float r_terminal;          /*                                               h:27 */
+    float alpha;               /* energy/time mix of reward_type 4              */
+    float z;                   /* filler of prob_params slots 20..31            h:48 */
+} e2e_params;
+
+/* Input fields, C-order, layouts of src/build_model.cu:380-398 and :978-985:
+ *   u, v, s      : float [nt][gsize][gsize]
+ *   ui, vi       : float [nt][nmodes][gsize][gsize]
+ *   yi           : float [nt][nrzns][nmodes]
+ *   mask         : int32 [nt][gsize][gsize]  (1 = obstacle)
+ * s may be NULL when reward_type is 0, 1 or 4; mask may be NULL (no obstacles). */
+typedef struct e2e_fields {
+    const float* u;
+    const float* v;
+    const float* ui;
+    const float* vi;
+    const float* yi;
+    const float* s;
+    const int32_t* mask;
+} e2e_fields;
+
+typedef struct e2e_build_opts {
+    int32_t fields_on_device;  /* 1: e2e_fields pointers are device pointers */
+    int32_t keep_rsum;         /* 1: also keep per-row reward sums (needed by e2e_vi_solve_goals) */
+    int32_t slot_k;            /* staging entries per row before the overflow arena (0: 16) */
+    int32_t radix_bits;        /* digit width of the LDS segmented radix sort, 1..8 (0: 8) */
+    int32_t slices_per_launch; /* time slices per build-kernel launch (0: all) */
+    int32_t reserved;
+    void* stream;              /* hipStream_t to run on; NULL: the library creates one */
+} e2e_build_opts;
+
+typedef struct e2e_model_info {
+    int64_t ns;                /* states = gsize^2 * nt                 (DP_relv_params[0]) */
+    int64_t na;                /* actions = num_ac_speeds*num_ac_angles (DP_relv_params[1]) */
+    int64_t nnz;               /* stored transitions                                        */
+    int64_t nc;                /* cells per slice (0 for an imported model)                 */
+    int64_t nt;                /* slices (0 for an imported model)                          */
+    int64_t lanes;             /* Monte Carlo lanes evaluated = (nt-1)*nc*na*nrzns          */
+    int64_t n_out_of_window;   /* lanes whose S2 fell outside the neighb_gsize window       */
+    int64_t max_row_nnz;
+    int64_t n_overflow_rows;   /* rows with more unique S2 than slot_k                     */
+    int64_t n_multipass_rows;  /* rows whose radix sort needed more than one digit pass     */
+    int64_t device_bytes;      /* device memory held by the model                          */
+    double build_ms;           /* wall time of e2e_model_build                             */
+    double move_kernel_ms;     /* summed HIP-event time of the move/sort/count kernel      */
+    double scan_ms;            /* HIP-event time of the row-count scan                     */
+    double csr_ms;             /* HIP-event time of the CSR scatter                        */
+    int32_t has_rsum;
+    int32_t move_launches;
+} e2e_model_info;
+
+typedef struct e2e_solve_opts {
+    float thresh;              /* stop at the first sweep whose sup-norm residual < thresh (spvi_2.cu:685) */
+    float discount;            /* Q = R + discount * P V (spvi_2.cu:61,189) */
+    int32_t max_sweeps;        /* 0: nt + 2 for a built model, 100000 for an imported one */
+    int32_t sweeps_per_check;  /* sweeps launched between host checks of the converged flag (0: 16) */
+    int32_t use_graph;         /* 1: replay the sweep chain as a captured hipGraph */
+    int32_t lanes_per_row;     /* 0: chosen from nnz/row; else 1,2,4,8,16 */
+    double timeout_ms;         /* 0: none */
+} e2e_solve_opts;
+
+typedef struct e2e_solve_stats {
+    int32_t sweeps;            /* sweeps performed (the reference's num_iterations) */
+    int32_t converged;
+    float final_residual;      /* residual of the last sweep */
+    int32_t lanes_per_row;
+    double solve_ms;           /* wall time of the solve */
+    double sweep_kernel_ms;    /* HIP-event time of all sweep launches (incl. early-exit ones) */
+    double sweep_kernel_ms_per_sweep; /* sweep_kernel_ms / launches that did work */
+    double bytes_per_sweep;    /* algorithmic bytes of one sweep */
+    int32_t launches;
+    int32_t reserved;
+} e2e_solve_stats;
+
+const char* e2e_last_error(void);
+const char* e2e_version(void);
+
+/* Number of HIP devices visible (0 without a GPU; never fails). */
+int32_t e2e_device_count(void);
+
+/* sizeof of the ABI structs, for binding checks: 0 e2e_params, 1 e2e_fields,
+ * 2 e2e_build_opts, 3 e2e_model_info, 4 e2e_solve_opts, 5 e2e_solve_stats;
+ * -1 for an unknown code. */
+int32_t e2e_struct_size(int32_t which);
+
+/* Writes the reference's 32-float prob_params vector (src/build_model.cu:1032-1055). */
+e2e_status e2e_params_pack(const e2e_params* p, float out32[32]);
+
+/* Action table of src/build_model.cu:1403-1457: action = speed_idx*num_ac_angles + angle_idx.
+ * speed/angle may be NULL. */
+e2e_status e2e_actions(const e2e_params* p, float* speed, float* angle);
+
+/* Monte Carlo build of the sparse transition model on the GPU.
+ * Replaces the per-(t,a) build_sparse_transition_model_at_T_at_a
+ * (src/build_model.cu:696-892) and the t x a loop of main (:1142-1165). */
+e2e_status e2e_model_build(const e2e_params* p, const e2e_fields* f,
+                           const e2e_build_opts* opts, void** model_out);
+
+/* Loads a model given in the reference COO layout (rows a*Ns+s, sorted by row,
+ * S2 ascending within a row) — what get_mdp_model reads (src/spvi_2.cu:393-504).
+ * Host arrays. */
+e2e_status e2e_model_import_coo(int64_t ns, int64_t na, int64_t nnz,
+                                const int32_t* S1, const int32_t* S2,
+                                const float* val, const float* R,
+                                void* stream, void** model_out);
+
+e2e_status e2e_model_info_get(const void* model, e2e_model_info* out);
+e2e_status e2e_model_destroy(void* model);
+
+/* Writes the model in the reference's on-disk order (src/build_model.cu:1246-1314):
+ * rows a*Ns + t*g^2 + sp sorted by (a,t,sp), S2 ascending; R of length Na*Ns with
+ * the last slice 0. Host buffers of nnz (S1,S2,val) and na*ns (R) elements.
+ * Any pointer may be NULL to skip it. */
+e2e_status e2e_model_export_coo(const void* model, int64_t* S1, int64_t* S2,
+                                float* val, float* R);
+/* Same, int32 indices (src/convert_model_datatype.py:19-23). */
+e2e_status e2e_model_export_coo_i32(const void* model, int32_t* S1, int32_t* S2,
+                                    float* val, float* R);
+
+/* Internal state-major CSR (row = s*Na + a): row_ptr[na*ns+1], col[nnz], val[nnz],
+ * R[na*ns]. Host buffers; any may be NULL. */
+e2e_status e2e_model_export_csr(const void* model, uint32_t* row_ptr, int32_t* col,
+                                float* val, float* R);
+
+/* Sparse value iteration: replaces solver_spvi_solve (src/spvi_2.cu:650-726).
+ * policy/value are host arrays of ns elements (either may be NULL). */
+e2e_status e2e_vi_solve(void* model, const e2e_solve_opts* opts,
+                        uint32_t* policy, float* value, e2e_solve_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* E2EDP_H */

@@ -1,0 +1,53 @@
+"""Shared helpers for the parity tests (oracle side = checker)."""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle
+
+
+def oracle_model(params, fields, nthreads=0):
+    return oracle.build(params, fields, nthreads=nthreads)
+
+
+def assert_same_model(gpu, ref, what=""):
+    """Bit-exact comparison of GPU export (S1, S2, val, R) with the oracle's."""
+    S1, S2, val, R = gpu
+    assert len(S1) == len(ref["S1"]), f"{what}: nnz {len(S1)} != {len(ref['S1'])}"
+    bad = np.flatnonzero((S1 != ref["S1"]) | (S2 != ref["S2"]))
+    assert bad.size == 0, (f"{what}: {bad.size} index mismatches, first at {bad[0]}: "
+                           f"gpu ({S1[bad[0]]},{S2[bad[0]]}) ref ({ref['S1'][bad[0]]},{ref['S2'][bad[0]]})")
+    vb = np.flatnonzero(val.view(np.uint32) != ref["val"].view(np.uint32))
+    assert vb.size == 0, f"{what}: {vb.size} prob mismatches, first at {vb[0]}: {val[vb[0]]} vs {ref['val'][vb[0]]}"
+    rb = np.flatnonzero(R.view(np.uint32) != ref["R"].view(np.uint32))
+    assert rb.size == 0, f"{what}: {rb.size} R mismatches, first at {rb[0]}: {R[rb[0]]} vs {ref['R'][rb[0]]}"
+
+
+def rel_err(v, v64):
+    """|v - v64| / max(|v64|, 1): relative error with a floor of one step cost."""
+    return np.abs(v.astype(np.float64) - v64) / np.maximum(np.abs(v64), 1.0)
+
+
+def q_values_f64(ns, na, S1, S2, val, R, V, discount=1.0):
+    """Q64[a, s] = R + discount * P V (V float64), rows a*ns + s."""
+    acc = np.zeros(na * ns)
+    np.add.at(acc, S1, val.astype(np.float64) * V[S2])
+    return (R.astype(np.float64) + discount * acc).reshape(na, ns)
+
+
+def policy_ties_ok(pol_gpu, pi64, Q64, tol):
+    """Policies must agree except where the FP64 Q of both choices are within tol."""
+    pol = pol_gpu.astype(np.int64)
+    pol[pol == 0xFFFFFFFF] = -1
+    diff = np.flatnonzero(pol != pi64)
+    if diff.size == 0:
+        return 0
+    s = diff
+    a_g = pol[s]
+    a_r = pi64[s]
+    assert (a_g >= 0).all() and (a_r >= 0).all(), "policy -1 mismatch"
+    qg = Q64[a_g, s]
+    qr = Q64[a_r, s]
+    gap = np.abs(qg - qr) / np.maximum(np.abs(qr), 1.0)
+    assert (gap <= tol).all(), f"{(gap > tol).sum()} policy differences beyond ties (max gap {gap.max()})"
+    return diff.size

@@ -1,0 +1,89 @@
+"""GPU parity of the fused SpMV value iteration (d) against the FP64 oracle."""
+import numpy as np
+import pytest
+
+import e2e_gpu_dp_amd as e2e
+import oracle
+from e2e_gpu_dp_amd.configs import small, workload
+from helpers import oracle_model, policy_ties_ok, q_values_f64, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5  # |v - v64| <= 1e-5 * max(|v64|, 1)
+
+
+def _check_solve(ref, pol, val, stats, tol=TOL):
+    ns, na, nc = ref["ns"], ref["na"], ref["nc"]
+    V64, pi64 = oracle.vi_backward_f64(ns, na, nc, ref["S1"], ref["S2"], ref["val"], ref["R"])
+    err = rel_err(val, V64)
+    assert err.max() <= tol, f"max rel err {err.max()} at {err.argmax()}"
+    Q64 = q_values_f64(ns, na, ref["S1"], ref["S2"], ref["val"], ref["R"], V64)
+    policy_ties_ok(pol, pi64, Q64, 4 * tol)
+    # final residual of the GPU values under an FP64 backup
+    r64 = oracle.bellman_residual_f64(ns, na, ref["S1"], ref["S2"], ref["val"], ref["R"], val)
+    assert r64 <= tol * max(1.0, np.abs(V64).max())
+    return err.max()
+
+
+@pytest.mark.parametrize("lpr", [0, 1, 2, 8])
+def test_vi_small(gpu, lpr):
+    w = small(g=9, nt=12, nr=300, nm=2, seed=41)
+    f = w.fields()
+    ref = oracle_model(w.params, f)
+    with e2e.build_model(w.params, f) as m:
+        pol, val, st = m.solve(thresh=1e-6, lanes_per_row=lpr)
+        assert st["converged"] and st["sweeps"] <= w.params.nt + 1
+        _check_solve(ref, pol, val, st)
+
+
+def test_vi_c1_and_sweep_count(gpu):
+    w = workload("C1")
+    f = w.fields()
+    ref = oracle_model(w.params, f)
+    _, _, it64, _ = oracle.vi_sweeps_f64(ref["ns"], ref["na"], ref["S1"], ref["S2"], ref["val"],
+                                         ref["R"], 1e-6)
+    with e2e.build_model(w.params, f) as m:
+        pol, val, st = m.solve(thresh=1e-6)
+        assert st["converged"]
+        assert st["sweeps"] == it64
+        assert st["final_residual"] == 0.0
+        _check_solve(ref, pol, val, st)
+        # eager and graph-replayed sweeps give identical bits
+        pol2, val2, st2 = m.solve(thresh=1e-6, use_graph=True, sweeps_per_check=4)
+        assert st2["sweeps"] == st["sweeps"]
+        assert np.array_equal(val.view(np.uint32), val2.view(np.uint32))
+        assert np.array_equal(pol, pol2)
+
+
+def test_vi_reference_threshold_stops_early(gpu):
+    """The drop-in default thresh 0.1 (spvi_2.cu:757) stops at the first sweep with
+    residual < 0.1, as the reference loop does (spvi_2.cu:669-698)."""
+    w = small(g=9, nt=15, nr=300, nm=2, seed=43)
+    f = w.fields()
+    ref = oracle_model(w.params, f)
+    V64, pi64, it64, res64 = oracle.vi_sweeps_f64(ref["ns"], ref["na"], ref["S1"], ref["S2"],
+                                                  ref["val"], ref["R"], 0.1)
+    with e2e.build_model(w.params, f) as m:
+        pol, val, st = m.solve(thresh=0.1, sweeps_per_check=3)
+        assert st["sweeps"] == it64
+        assert rel_err(val, V64).max() <= TOL
+
+
+def test_vi_imported_coo_matches_built(gpu):
+    w = small(g=10, nt=8, nr=250, nm=2, seed=47)
+    f = w.fields()
+    with e2e.build_model(w.params, f) as m:
+        S1, S2, val, R = m.export_coo(np.int32)
+        pol, v, st = m.solve(thresh=1e-6)
+    with e2e.import_coo(w.params.ns, w.params.na, S1, S2, val, R) as mi:
+        pol2, v2, st2 = mi.solve(thresh=1e-6)
+    assert st2["sweeps"] == st["sweeps"]
+    assert np.array_equal(v.view(np.uint32), v2.view(np.uint32))
+    assert np.array_equal(pol, pol2)
+
+
+def test_vi_not_converged_status(gpu):
+    w = small(g=8, nt=10, nr=100, nm=1, seed=49)
+    with e2e.build_model(w.params, w.fields()) as m:
+        pol, val, st = m.solve(thresh=1e-6, max_sweeps=3, sweeps_per_check=1, raise_on_nonconv=False)
+        assert st["status"] == 4 and st["sweeps"] == 3 and not st["converged"]
