@@ -33,6 +33,7 @@ COMMON = [
 ]
 
 LIB_SOURCES = ["abi.cpp", "build.hip", "scan.hip", "vi.hip"]
+DRIVERS = {"build_model": "build_model_main.cpp", "spvi": "spvi_main.cpp"}
 
 
 def _compile(src: Path) -> Path:
@@ -60,9 +61,23 @@ def build(jobs: int = 0, verbose: bool = True) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_drivers()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {', '.join(DRIVERS)} in {BIN}")
     return LIB
+
+
+def build_drivers() -> None:
+    """The drop-in executables: plain C++ over the C ABI (no HIP types), linked to
+    libe2edp.so next to them."""
+    BIN.mkdir(exist_ok=True)
+    for exe, src in DRIVERS.items():
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", str(PKG.parent / "include"),
+               "-o", str(BIN / exe), str(CSRC / src), "-L", str(PKG), "-le2edp",
+               "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
 
 
 def main(argv=None):
