@@ -22,7 +22,7 @@ from . import _abi
 from ._abi import E2EError, check, lib
 from .configs import Params as PyParams
 
-__all__ = ["Model", "build_model", "import_coo", "c_params", "device_count", "E2EError",
+__all__ = ["Model", "Server", "build_model", "import_coo", "c_params", "device_count", "E2EError",
            "pack_params", "actions"]
 
 _PFIELDS = [f[0] for f in _abi.Params._fields_]
@@ -152,6 +152,29 @@ class Model:
         return pol, val, stats
 
 
+    def solve_goals(self, goal_cells, thresh: float = 0.1, discount: float = 1.0,
+                    max_sweeps: int = 0, sweeps_per_check: int = 0, variant: int = -1,
+                    use_graph: bool = False, timeout_ms: float = 0.0, raise_on_nonconv: bool = True):
+        """Batched multi-goal VI (CSR x dense-block SpMM). Returns (policy uint32[k, Ns],
+        value float32[k, Ns], per-column sweeps int32[k], stats dict)."""
+        goals = np.ascontiguousarray(goal_cells, np.int32).ravel()
+        k = len(goals)
+        ns = self.info["ns"]
+        o = _abi.SolveOpts(thresh, discount, max_sweeps, sweeps_per_check, 0, 0, timeout_ms)
+        bo = _abi.BatchOpts(variant, int(use_graph))
+        st = _abi.BatchStats()
+        pol = np.empty((k, ns), np.uint32)
+        val = np.empty((k, ns), np.float32)
+        sw = np.empty(k, np.int32)
+        code = lib().e2e_vi_solve_goals(self._h, goals.ctypes.data, k, C.byref(o), C.byref(bo),
+                                        pol.ctypes.data, val.ctypes.data, sw.ctypes.data, C.byref(st))
+        stats = {kk: getattr(st, kk) for kk, _ in st._fields_}
+        stats["status"] = code
+        if code != 0 and (raise_on_nonconv or code != 4):
+            check(code)
+        return pol, val, sw, stats
+
+
 def build_model(params, fields: dict, keep_rsum: bool = False, slot_k: int = 0,
                 radix_bits: int = 0, slices_per_launch: int = 0, stream: int = 0) -> Model:
     """Monte Carlo build on the GPU. `fields` holds u, v, ui, vi, yi and optionally s,
@@ -186,3 +209,81 @@ def import_coo(ns: int, na: int, S1, S2, val, R, stream: int = 0) -> Model:
     check(lib().e2e_model_import_coo(ns, na, len(S1), S1.ctypes.data, S2.ctypes.data, val.ctypes.data,
                                      R.ctypes.data, stream or None, C.byref(h)))
     return Model(h.value)
+
+
+REQUEST_STATUS = {0: "queued", 1: "running", 2: "done", 3: "rejected", 4: "failed"}
+
+
+class Server:
+    """Continuous-batching planning server (C++ scheduler + builder threads in libe2edp)."""
+
+    def __init__(self, hbm_budget_bytes: int = 0, max_cols: int = 64, sweeps_per_step: int = 4,
+                 keep_full_results: bool = False, serial: bool = False, thresh: float = 1e-6):
+        o = _abi.ServerOpts(hbm_budget_bytes, max_cols, sweeps_per_step, int(keep_full_results),
+                            int(serial), thresh, 0)
+        h = C.c_void_p()
+        check(lib().e2e_server_create(C.byref(o), C.byref(h)))
+        self._h = h
+        self._ns = {}
+
+    def add_env(self, env_id: int, params, fields: dict):
+        keep = []
+        ptrs = {k: _ptr(fields.get(k), keep, np.int32 if k == "mask" else np.float32)[0]
+                for k in ("u", "v", "ui", "vi", "yi", "s", "mask")}
+        check(lib().e2e_server_add_env(self._h, env_id, C.byref(c_params(params)),
+                                       C.byref(_abi.Fields(**ptrs))))
+        cp = c_params(params)
+        self._ns[env_id] = cp.gsize * cp.gsize * cp.nt
+
+    def submit(self, env_id: int, goal_cell: int, start_cell: int) -> int:
+        rid = C.c_uint64()
+        check(lib().e2e_server_submit(self._h, env_id, int(goal_cell), int(start_cell), C.byref(rid)))
+        return rid.value
+
+    @staticmethod
+    def _res(r):
+        d = {k: getattr(r, k) for k, _ in r._fields_}
+        d["status_name"] = REQUEST_STATUS.get(d["status"], "?")
+        return d
+
+    def poll(self, req_id: int) -> dict:
+        r = _abi.RequestResult()
+        check(lib().e2e_server_poll(self._h, req_id, C.byref(r)))
+        return self._res(r)
+
+    def wait(self, req_id: int, timeout_ms: float = 0.0) -> dict:
+        r = _abi.RequestResult()
+        check(lib().e2e_server_wait(self._h, req_id, timeout_ms, C.byref(r)))
+        return self._res(r)
+
+    def columns(self, req_id: int, env_id: int):
+        ns = self._ns[env_id]
+        pol = np.empty(ns, np.uint32)
+        val = np.empty(ns, np.float32)
+        check(lib().e2e_server_result_columns(self._h, req_id, pol.ctypes.data, val.ctypes.data))
+        return pol, val
+
+    def drain(self, timeout_ms: float = 0.0):
+        check(lib().e2e_server_drain(self._h, timeout_ms))
+
+    def stats(self) -> dict:
+        st = _abi.ServerStats()
+        check(lib().e2e_server_stats_get(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if self._h:
+            lib().e2e_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -74,7 +74,38 @@ class SolveStats(C.Structure):
     ]
 
 
-STRUCTS = [Params, Fields, BuildOpts, ModelInfo, SolveOpts, SolveStats]
+class BatchOpts(C.Structure):
+    _fields_ = [("variant", C.c_int32), ("use_graph", C.c_int32), ("reserved", C.c_int32 * 2)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("sweeps", "converged", "variant", "cols_per_lane",
+                                          "states_per_wave", "col_block")] + \
+        [(n, C.c_double) for n in ("solve_ms", "sweep_kernel_ms", "tune_ms", "bytes_per_sweep")] + \
+        [("launches", C.c_int32), ("reserved", C.c_int32)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_cols", C.c_int32),
+                ("sweeps_per_step", C.c_int32), ("keep_full_results", C.c_int32),
+                ("serial", C.c_int32), ("thresh", C.c_float), ("reserved", C.c_int32)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("sweeps", C.c_int32), ("value_at_start", C.c_float),
+                ("action_at_start", C.c_int32), ("latency_ms", C.c_double), ("queue_ms", C.c_double),
+                ("cache_hit", C.c_int32), ("reserved", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("submitted", "completed", "rejected", "models_built",
+                                          "model_evictions", "cache_hits", "peak_reserved_bytes",
+                                          "sweeps_launched")] + \
+        [(n, C.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "wall_ms", "solves_per_s")]
+
+
+STRUCTS = [Params, Fields, BuildOpts, ModelInfo, SolveOpts, SolveStats, BatchOpts, BatchStats,
+           ServerOpts, RequestResult, ServerStats]
 
 VP = C.c_void_p
 P = C.POINTER
@@ -95,6 +126,17 @@ SIGNATURES = {
     "e2e_model_export_coo_i32": (C.c_int, [VP, VP, VP, VP, VP]),
     "e2e_model_export_csr": (C.c_int, [VP, VP, VP, VP, VP]),
     "e2e_vi_solve": (C.c_int, [VP, P(SolveOpts), VP, VP, P(SolveStats)]),
+    "e2e_vi_solve_goals": (C.c_int, [VP, VP, C.c_int32, P(SolveOpts), P(BatchOpts), VP, VP, VP,
+                                     P(BatchStats)]),
+    "e2e_server_create": (C.c_int, [P(ServerOpts), P(VP)]),
+    "e2e_server_add_env": (C.c_int, [VP, C.c_int64, P(Params), P(Fields)]),
+    "e2e_server_submit": (C.c_int, [VP, C.c_int64, C.c_int32, C.c_int32, P(C.c_uint64)]),
+    "e2e_server_poll": (C.c_int, [VP, C.c_uint64, P(RequestResult)]),
+    "e2e_server_wait": (C.c_int, [VP, C.c_uint64, C.c_double, P(RequestResult)]),
+    "e2e_server_result_columns": (C.c_int, [VP, C.c_uint64, VP, VP]),
+    "e2e_server_drain": (C.c_int, [VP, C.c_double]),
+    "e2e_server_stats_get": (C.c_int, [VP, P(ServerStats)]),
+    "e2e_server_destroy": (C.c_int, [VP]),
 }
 
 _lib = None

@@ -43,6 +43,7 @@ e2e_status free_model(Model* m) {
     if (!m) return E2E_OK;
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     if (m->graph) (void)hipGraphExecDestroy(m->graph);
+    batch_cache_free(m->batch);
     void* ptrs[] = {m->row_ptr, m->nz, m->R, m->Rsum, m->V[0], m->V[1], m->pi, m->st};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -61,6 +62,9 @@ e2e_status model_export_csr(const Model* m, uint32_t* row_ptr, int32_t* col, flo
 e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* value,
                     e2e_solve_stats* stats);
 void actions_table(const e2e_params* p, float* speed, float* angle);
+e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve_opts* o,
+                          const e2e_batch_opts* bo, uint32_t* policy, float* value, int32_t* col_sweeps,
+                          e2e_batch_stats* stats);
 
 template <typename F>
 static e2e_status guarded(F&& f) {
@@ -112,6 +116,11 @@ int32_t e2e_struct_size(int32_t which) {
         case 3: return (int32_t)sizeof(e2e_model_info);
         case 4: return (int32_t)sizeof(e2e_solve_opts);
         case 5: return (int32_t)sizeof(e2e_solve_stats);
+        case 6: return (int32_t)sizeof(e2e_batch_opts);
+        case 7: return (int32_t)sizeof(e2e_batch_stats);
+        case 8: return (int32_t)sizeof(e2e_server_opts);
+        case 9: return (int32_t)sizeof(e2e_request_result);
+        case 10: return (int32_t)sizeof(e2e_server_stats);
         default: return -1;
     }
 }
@@ -246,6 +255,19 @@ e2e_status e2e_vi_solve(void* model, const e2e_solve_opts* opts, uint32_t* polic
         return E2E_ERR_INVALID;
     }
     return guarded([&] { return vi_solve(static_cast<Model*>(model), opts, policy, value, stats); });
+}
+
+e2e_status e2e_vi_solve_goals(void* model, const int32_t* goal_cells, int32_t k,
+                              const e2e_solve_opts* opts, const e2e_batch_opts* bopts, uint32_t* policy,
+                              float* value, int32_t* col_sweeps, e2e_batch_stats* stats) {
+    if (!model) {
+        set_error("e2e_vi_solve_goals: model is NULL");
+        return E2E_ERR_INVALID;
+    }
+    return guarded([&] {
+        return vi_solve_goals(static_cast<Model*>(model), goal_cells, k, opts, bopts, policy, value, col_sweeps,
+                              stats);
+    });
 }
 
 }  // extern "C"

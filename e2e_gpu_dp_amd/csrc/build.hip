@@ -957,6 +957,11 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st) return fail(st);
+    if (mask) {
+        std::vector<int32_t> flags(nt);
+        E2E_HIP(hipMemcpy(flags.data(), dFlags.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost));
+        for (int32_t fl : flags) m->has_obstacles = m->has_obstacles || fl != 0;
+    }
     m->info.move_kernel_ms = move_ms;
     m->info.move_launches = launches;
 

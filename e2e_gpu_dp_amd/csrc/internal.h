@@ -59,6 +59,9 @@ struct BatchState {
     uint32_t pad;
 };
 
+struct BatchCache;
+void batch_cache_free(BatchCache* c);
+
 struct Model {
     e2e_params p{};
     bool built = false;       // false: imported from COO
@@ -80,6 +83,9 @@ struct Model {
     int graph_chunk = 0;
     int graph_lpr = 0;
     float graph_thresh = 0.f, graph_discount = 0.f;
+
+    bool has_obstacles = false;
+    BatchCache* batch = nullptr;  // buffers + tuning of the batched multi-goal solve
 
     e2e_model_info info{};
 };

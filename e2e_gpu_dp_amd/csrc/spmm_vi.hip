@@ -1,0 +1,724 @@
+// spmm_vi.hip — batched multi-goal value iteration: k goal columns that share one
+// transition model, solved together as a CSR x dense-block SpMM per sweep.
+//
+// Column j is the model the reference would build with its terminal cell at goal g_j
+// (term_subgrid_size 1), solved by the reference loop (src/spvi_2.cu:650-726). From one
+// shared build with no terminal cell (term_subgrid_size 0, per-row reward sums kept):
+//   * rows of a non-goal cell are identical in both models (the terminal test of
+//     transition_calc, src/build_model.cu:523, only concerns S1), except for the reward of
+//     lanes that land on the goal: move() adds r_terminal instead of the late-step
+//     penalty (src/build_model.cu:343-354). With c the number of such lanes and
+//     delta_t = r_terminal - (t > nt-20 ? 0.05*r_outbound : 0),
+//         R_j(s,a) = (Rsum(s,a) + c * delta_t) / nr,
+//     exact for integer-valued rewards (c = count is recovered exactly from
+//     prob = count/nr);
+//   * rows of the goal cell are self-loops with R = 0 in the per-goal model, so
+//     V_j = 0 and pi_j = 0 there at every t; the kernel writes that directly.
+//
+// Layout: V[2][ns][KP] (KP = padded column count, columns contiguous per state), pi
+// [ns][KP]. Variant (CPL, SPW): a wave takes SPW states, 64/SPW lanes per state, CPL
+// adjacent columns per lane (float/float2/float4 gathers of V), so a wave covers
+// KB = 64/SPW*CPL columns of each state; grid.y walks the column blocks.
+//
+// Each column freezes at the first sweep whose own residual is below thresh, exactly
+// where its plain solve would stop; every column's arithmetic is independent of which
+// other columns share the batch.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "internal.h"
+
+namespace e2e {
+
+struct BatchArgs {
+    const uint32_t* rp;
+    const int2* nz;
+    const float* R;
+    const float* Rsum;
+    float* V0;
+    float* V1;
+    int32_t* pi;
+    const int32_t* goal;   // [KP] goal cell per column (-1: none)
+    int32_t* conv;         // [KP] sweep at which the column converged, -1 running
+    uint32_t* res;         // [2][KP] residual bits per sweep parity
+    float* last_res;       // [KP]
+    BatchState* st;
+    int64_t ns;
+    int nc, na, nt, kp, nr;
+    float thresh, r_term, discount;
+    double late_pen;       // 0.05 * r_outbound
+};
+
+namespace {
+
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+    s = a + b;
+    const float bp = s - a;
+    e = (a - (s - bp)) + (b - bp);
+}
+
+template <int CPL>
+struct VecLoad;
+template <>
+struct VecLoad<1> {
+    __device__ static void ld(const float* p, float* x) { x[0] = *p; }
+};
+template <>
+struct VecLoad<2> {
+    __device__ static void ld(const float* p, float* x) {
+        float2 v = *reinterpret_cast<const float2*>(p);
+        x[0] = v.x;
+        x[1] = v.y;
+    }
+};
+template <>
+struct VecLoad<4> {
+    __device__ static void ld(const float* p, float* x) {
+        float4 v = *reinterpret_cast<const float4*>(p);
+        x[0] = v.x;
+        x[1] = v.y;
+        x[2] = v.z;
+        x[3] = v.w;
+    }
+};
+
+template <int CPL, int SPW>
+__global__ __launch_bounds__(256) void spmm_vi_sweep(const BatchArgs A) {
+    constexpr int LPS = 64 / SPW;
+    constexpr int KB = LPS * CPL;
+    __shared__ float red[4][KB];
+    __shared__ int is_last;
+    BatchState* st = A.st;
+    if (st->all_conv) return;
+    const uint32_t k = st->k;
+    const float* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
+    float* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int grp = lane / LPS, l = lane % LPS;
+    const int j0 = blockIdx.y * KB + l * CPL;
+    const int KP = A.kp;
+    int goal[CPL];
+    bool live[CPL];
+    float rmax[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; c++) {
+        goal[c] = A.goal[j0 + c];
+        live[c] = A.conv[j0 + c] < 0;
+        rmax[c] = 0.0f;
+    }
+    const int64_t stride = (int64_t)gridDim.x * 4 * SPW;
+    for (int64_t s = ((int64_t)blockIdx.x * 4 + wv) * SPW + grp; s < A.ns; s += stride) {
+        const int t = (int)(s / A.nc);
+        const int cell = (int)(s - (int64_t)t * A.nc);
+        const int colbase = (t + 1) * A.nc;
+        const float delta = (float)((double)A.r_term - (t > A.nt - 20 ? A.late_pen : 0.0));
+        float best[CPL];
+        int bi[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; c++) {
+            best[c] = -INFINITY;
+            bi[c] = -1;
+        }
+        for (int a = 0; a < A.na; a++) {
+            const int64_t row = s * A.na + a;
+            const uint32_t beg = A.rp[row], end = A.rp[row + 1];
+            float sh[CPL], sl[CPL];
+            int hit[CPL];
+#pragma unroll
+            for (int c = 0; c < CPL; c++) {
+                sh[c] = 0.0f;
+                sl[c] = 0.0f;
+                hit[c] = -1;
+            }
+            for (uint32_t j = beg; j < end; j++) {
+                const int2 e = A.nz[j];
+                const float v = __int_as_float(e.y);
+                float x[CPL];
+                VecLoad<CPL>::ld(Vin + (int64_t)e.x * KP + j0, x);
+                const int cc = e.x - colbase;
+#pragma unroll
+                for (int c = 0; c < CPL; c++) {
+                    const float p = v * x[c];
+                    const float pe = fmaf(v, x[c], -p);
+                    float tt, err;
+                    two_sum(sh[c], p, tt, err);
+                    sh[c] = tt;
+                    sl[c] = sl[c] + (err + pe);
+                    if (cc == goal[c]) hit[c] = (int)rintf(v * (float)A.nr);
+                }
+            }
+            const float rr = A.R[row];
+#pragma unroll
+            for (int c = 0; c < CPL; c++) {
+                const float rj = hit[c] >= 0 ? (A.Rsum[row] + (float)hit[c] * delta) / A.nr : rr;
+                float h = sh[c], lo = sl[c];
+                if (A.discount != 1.0f) {
+                    h *= A.discount;
+                    lo *= A.discount;
+                }
+                float tt, err;
+                two_sum(rj, h, tt, err);
+                const float q = tt + (err + lo);
+                if (q > best[c]) {  // strict '>' in action order (spvi_2.cu:112)
+                    best[c] = q;
+                    bi[c] = a;
+                }
+            }
+        }
+        float vo[CPL];
+        int po[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; c++) {
+            if (cell == goal[c]) {
+                vo[c] = 0.0f;
+                po[c] = 0;
+            } else if (best[c] > -1e6f) {
+                vo[c] = best[c];
+                po[c] = bi[c];
+            } else {
+                vo[c] = -1e6f;
+                po[c] = -1;
+            }
+        }
+        const int64_t o = s * KP + j0;
+#pragma unroll
+        for (int c = 0; c < CPL; c++) {
+            if (live[c]) {
+                Vout[o + c] = vo[c];
+                A.pi[o + c] = po[c];
+                rmax[c] = fmaxf(rmax[c], fabsf(vo[c] - Vin[o + c]));
+            }
+        }
+    }
+    // per-column residual: lanes of the same column across the SPW groups, then waves
+#pragma unroll
+    for (int c = 0; c < CPL; c++)
+        for (int off = LPS; off < 64; off <<= 1) rmax[c] = fmaxf(rmax[c], __shfl_xor(rmax[c], off, 64));
+    if (grp == 0) {
+#pragma unroll
+        for (int c = 0; c < CPL; c++) red[wv][l * CPL + c] = rmax[c];
+    }
+    __syncthreads();
+    if (tid < KB) {
+        const float m = fmaxf(fmaxf(red[0][tid], red[1][tid]), fmaxf(red[2][tid], red[3][tid]));
+        atomicMax(&A.res[(k & 1u) * KP + blockIdx.y * KB + tid], __float_as_uint(m));
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t ticket = atomicAdd(&st->done, 1u);
+        is_last = ticket == gridDim.x * gridDim.y - 1;
+    }
+    __syncthreads();
+    if (is_last) {
+        __threadfence();
+        int running = 0;
+        for (int j = tid; j < KP; j += blockDim.x) {
+            const float r = __uint_as_float(atomicAdd(&A.res[(k & 1u) * KP + j], 0u));
+            if (A.conv[j] < 0) {
+                A.last_res[j] = r;
+                if (r < A.thresh) A.conv[j] = (int32_t)k;
+                else running++;
+            }
+            A.res[((k + 1u) & 1u) * KP + j] = 0u;
+        }
+        const int n_running = __syncthreads_count(running > 0);
+        if (tid == 0) {
+            st->all_conv = n_running == 0;
+            st->done = 0u;
+            st->k = k + 1u;
+        }
+    }
+}
+
+__global__ void batch_init(float* V0, int64_t n, const int32_t* goal, int32_t* conv, int32_t* start,
+                           uint32_t* res, float* last_res, int kp, int k, BatchState* st) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) V0[i] = 0.0f;
+    if (i < kp) {
+        conv[i] = i < k ? -1 : 0;  // padding columns never run
+        start[i] = 1;
+        res[i] = 0u;
+        res[kp + i] = 0u;
+        last_res[i] = 0.0f;
+    }
+    if (i == 0) {
+        st->k = 1u;
+        st->done = 0u;
+        st->all_conv = 0;
+    }
+}
+
+// out[j][s] = V[conv_j parity][s][j]; pol[j][s] likewise from pi (written only while live)
+__global__ void batch_gather(const float* V0, const float* V1, const int32_t* pi, const int32_t* conv,
+                             int64_t ns, int kp, int k, int last_k, float* vout, int32_t* pout) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns * k) return;
+    const int j = (int)(i / ns);
+    const int64_t s = i - (int64_t)j * ns;
+    const int cj = conv[j] >= 0 ? conv[j] : last_k;
+    const float* V = (cj & 1) ? V1 : V0;
+    vout[i] = V[s * kp + j];
+    pout[i] = pi[s * kp + j];
+}
+
+using BatchFn = void (*)(const BatchArgs);
+
+struct Variant {
+    int cpl, spw;
+    BatchFn fn;
+};
+
+const Variant kVariants[] = {
+    {1, 1, spmm_vi_sweep<1, 1>}, {2, 2, spmm_vi_sweep<2, 2>}, {4, 4, spmm_vi_sweep<4, 4>},
+    {1, 2, spmm_vi_sweep<1, 2>}, {2, 4, spmm_vi_sweep<2, 4>}, {4, 8, spmm_vi_sweep<4, 8>},
+    {1, 4, spmm_vi_sweep<1, 4>}, {2, 8, spmm_vi_sweep<2, 8>}, {4, 16, spmm_vi_sweep<4, 16>},
+};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+int kb_of(const Variant& v) { return (64 / v.spw) * v.cpl; }
+
+struct BatchBufs {
+    int kp = 0;
+    int64_t ns = 0;
+    float* V[2] = {nullptr, nullptr};
+    int32_t* pi = nullptr;
+    int32_t* goal = nullptr;
+    int32_t* conv = nullptr;
+    int32_t* start = nullptr;  // sweep index at which each column was admitted
+    uint32_t* res = nullptr;
+    float* last_res = nullptr;
+    BatchState* st = nullptr;
+    float* vout = nullptr;
+    int32_t* pout = nullptr;
+    void release() {
+        void* ps[] = {V[0], V[1], pi, goal, conv, start, res, last_res, st, vout, pout};
+        for (void* p : ps)
+            if (p) (void)hipFree(p);
+        *this = BatchBufs();
+    }
+};
+
+}  // namespace
+
+struct BatchCache {
+    BatchBufs b;
+    int tuned_kp = -1;
+    int tuned_variant = -1;
+    hipGraphExec_t graph = nullptr;
+    int graph_variant = -1, graph_chunk = 0, graph_grid = 0;
+    float graph_thresh = 0.f, graph_discount = 0.f;
+};
+
+void batch_cache_free(BatchCache* c) {
+    if (!c) return;
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    c->b.release();
+    delete c;
+}
+
+static e2e_status alloc_bufs(const Model* m, BatchBufs& b, int kp) {
+    const size_t n = (size_t)m->ns * kp;
+    if (hipMalloc(&b.V[0], n * 4) != hipSuccess || hipMalloc(&b.V[1], n * 4) != hipSuccess ||
+        hipMalloc(&b.pi, n * 4) != hipSuccess || hipMalloc(&b.goal, kp * 4) != hipSuccess ||
+        hipMalloc(&b.conv, kp * 4) != hipSuccess || hipMalloc(&b.start, kp * 4) != hipSuccess ||
+        hipMalloc(&b.res, 2 * kp * 4) != hipSuccess ||
+        hipMalloc(&b.last_res, kp * 4) != hipSuccess || hipMalloc(&b.st, sizeof(BatchState)) != hipSuccess ||
+        hipMalloc(&b.vout, n * 4) != hipSuccess || hipMalloc(&b.pout, n * 4) != hipSuccess) {
+        (void)hipGetLastError();
+        b.release();
+        set_error("batched solve: device allocation failed");
+        return E2E_ERR_NOMEM;
+    }
+    b.kp = kp;
+    b.ns = m->ns;
+    return E2E_OK;
+}
+
+static e2e_status ensure_bufs(Model* m, BatchCache* c, int kp) {
+    BatchBufs& b = c->b;
+    if (b.kp == kp && b.ns == m->ns) return E2E_OK;
+    b.release();
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    return alloc_bufs(m, b, kp);
+}
+
+static BatchArgs make_args(const Model* m, const BatchBufs& b, const e2e_solve_opts& o) {
+    BatchArgs A{};
+    A.rp = m->row_ptr;
+    A.nz = m->nz;
+    A.R = m->R;
+    A.Rsum = m->Rsum;
+    A.V0 = b.V[0];
+    A.V1 = b.V[1];
+    A.pi = b.pi;
+    A.goal = b.goal;
+    A.conv = b.conv;
+    A.res = b.res;
+    A.last_res = b.last_res;
+    A.st = b.st;
+    A.ns = m->ns;
+    A.nc = (int)m->nc;
+    A.na = (int)m->na;
+    A.nt = (int)m->nt;
+    A.kp = b.kp;
+    A.nr = m->p.nrzns;
+    A.thresh = o.thresh;
+    A.r_term = m->p.r_terminal;
+    A.discount = o.discount;
+    A.late_pen = 0.05 * (double)m->p.r_outbound;
+    return A;
+}
+
+static dim3 grid_of(const Model* m, const Variant& v, int kp) {
+    const int64_t states_per_block = 4 * v.spw;
+    int64_t gx = ceil_div(m->ns, states_per_block);
+    gx = std::min<int64_t>(gx, 4096);  // grid-stride beyond this
+    return dim3((unsigned)gx, (unsigned)(kp / kb_of(v)));
+}
+
+e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve_opts* o,
+                          const e2e_batch_opts* bo, uint32_t* policy, float* value, int32_t* col_sweeps,
+                          e2e_batch_stats* stats) {
+    const double t_start = now_ms();
+    E2E_CHECK(m->built, E2E_ERR_INVALID, "batched goals need a model built by e2e_model_build");
+    E2E_CHECK(m->Rsum, E2E_ERR_INVALID, "batched goals need keep_rsum=1 at build time");
+    E2E_CHECK(m->p.term_subgrid_size == 0, E2E_ERR_INVALID,
+              "batched goals need the shared model built without a terminal cell (term_subgrid_size 0)");
+    E2E_CHECK(!m->has_obstacles, E2E_ERR_INVALID, "batched goals are not defined for models with obstacles");
+    E2E_CHECK(k >= 1 && k <= 64 && goals, E2E_ERR_INVALID, "k=%d goals must be in 1..64", k);
+    for (int j = 0; j < k; j++)
+        E2E_CHECK(goals[j] >= 0 && goals[j] < m->nc, E2E_ERR_INVALID, "goal %d out of range", goals[j]);
+    e2e_solve_opts opts{};
+    opts.thresh = 0.1f;
+    opts.discount = 1.0f;
+    if (o) opts = *o;
+    e2e_batch_opts bopt{};
+    bopt.variant = -1;
+    if (bo) bopt = *bo;
+    const int kp = std::max(16, (int)ceil_div(k, 16) * 16);
+    if (!m->batch) m->batch = new BatchCache();
+    BatchCache* c = m->batch;
+    E2E_TRY(ensure_bufs(m, c, kp));
+    BatchBufs& b = c->b;
+    hipStream_t s = m->stream;
+    std::vector<int32_t> hg(kp, -1);
+    for (int j = 0; j < k; j++) hg[j] = goals[j];
+    E2E_HIP(hipMemcpyAsync(b.goal, hg.data(), kp * 4, hipMemcpyHostToDevice, s));
+    BatchArgs A = make_args(m, b, opts);
+    const int64_t nV = m->ns * kp;
+    auto init = [&]() {
+        hipLaunchKernelGGL(batch_init, dim3((unsigned)ceil_div(nV, 256)), dim3(256), 0, s, b.V[0], nV, b.goal,
+                           b.conv, b.start, b.res, b.last_res, kp, k, b.st);
+    };
+    const int max_sweeps = opts.max_sweeps > 0 ? opts.max_sweeps : (int)m->nt + 2;
+    const int spc = opts.sweeps_per_check > 0 ? opts.sweeps_per_check : 8;
+
+    // ---- autotune: time a few sweeps of every valid variant on this model
+    double tune_ms = 0.0;
+    int variant = bopt.variant;
+    hipEvent_t e0, e1;
+    E2E_HIP(hipEventCreate(&e0));
+    E2E_HIP(hipEventCreate(&e1));
+    if (variant < 0 || variant >= kNumVariants || kp % kb_of(kVariants[variant])) {
+        if (c->tuned_kp == kp && c->tuned_variant >= 0) {
+            variant = c->tuned_variant;
+        } else {
+            const double t0 = now_ms();
+            float best = 1e30f;
+            const int trial = std::min(3, max_sweeps);
+            for (int vi = 0; vi < kNumVariants; vi++) {
+                const Variant& v = kVariants[vi];
+                if (kp % kb_of(v)) continue;
+                init();
+                E2E_HIP(hipEventRecord(e0, s));
+                for (int i = 0; i < trial; i++) hipLaunchKernelGGL(v.fn, grid_of(m, v, kp), dim3(256), 0, s, A);
+                E2E_HIP(hipEventRecord(e1, s));
+                E2E_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                E2E_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    variant = vi;
+                }
+            }
+            c->tuned_kp = kp;
+            c->tuned_variant = variant;
+            tune_ms = now_ms() - t0;
+        }
+    }
+    const Variant& V = kVariants[variant];
+    const dim3 grid = grid_of(m, V, kp);
+
+    if (bopt.use_graph &&
+        (!c->graph || c->graph_variant != variant || c->graph_chunk != spc || c->graph_grid != (int)grid.x ||
+         c->graph_thresh != opts.thresh || c->graph_discount != opts.discount)) {
+        if (c->graph) {
+            (void)hipGraphExecDestroy(c->graph);
+            c->graph = nullptr;
+        }
+        hipGraph_t gr;
+        E2E_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < spc; i++) hipLaunchKernelGGL(V.fn, grid, dim3(256), 0, s, A);
+        E2E_HIP(hipStreamEndCapture(s, &gr));
+        hipError_t e = hipGraphInstantiate(&c->graph, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e != hipSuccess) {
+            c->graph = nullptr;
+            set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+            return E2E_ERR_HIP;
+        }
+        c->graph_variant = variant;
+        c->graph_chunk = spc;
+        c->graph_grid = (int)grid.x;
+        c->graph_thresh = opts.thresh;
+        c->graph_discount = opts.discount;
+    }
+
+    init();
+    E2E_HIP(hipGetLastError());
+    double kms = 0.0;
+    int launched = 0;
+    BatchState hst{};
+    while (launched < max_sweeps) {
+        const int n = std::min(spc, max_sweeps - launched);
+        E2E_HIP(hipEventRecord(e0, s));
+        if (bopt.use_graph && n == spc) {
+            E2E_HIP(hipGraphLaunch(c->graph, s));
+        } else {
+            for (int i = 0; i < n; i++) hipLaunchKernelGGL(V.fn, grid, dim3(256), 0, s, A);
+            E2E_HIP(hipGetLastError());
+        }
+        E2E_HIP(hipEventRecord(e1, s));
+        E2E_HIP(hipMemcpyAsync(&hst, b.st, sizeof(hst), hipMemcpyDeviceToHost, s));
+        E2E_HIP(hipStreamSynchronize(s));
+        float ms = 0.f;
+        E2E_HIP(hipEventElapsedTime(&ms, e0, e1));
+        kms += ms;
+        launched += n;
+        if (hst.all_conv) break;
+        if (opts.timeout_ms > 0 && now_ms() - t_start > opts.timeout_ms) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const int last_k = (int)hst.k - 1;
+    hipLaunchKernelGGL(batch_gather, dim3((unsigned)ceil_div(m->ns * k, 256)), dim3(256), 0, s, b.V[0], b.V[1],
+                       b.pi, b.conv, m->ns, kp, k, last_k, b.vout, b.pout);
+    E2E_HIP(hipGetLastError());
+    std::vector<int32_t> hconv(kp);
+    std::vector<float> hres(kp);
+    E2E_HIP(hipMemcpyAsync(hconv.data(), b.conv, kp * 4, hipMemcpyDeviceToHost, s));
+    E2E_HIP(hipMemcpyAsync(hres.data(), b.last_res, kp * 4, hipMemcpyDeviceToHost, s));
+    if (value) E2E_HIP(hipMemcpyAsync(value, b.vout, sizeof(float) * m->ns * k, hipMemcpyDeviceToHost, s));
+    if (policy) E2E_HIP(hipMemcpyAsync(policy, b.pout, sizeof(int32_t) * m->ns * k, hipMemcpyDeviceToHost, s));
+    E2E_HIP(hipStreamSynchronize(s));
+    int nconv = 0;
+    for (int j = 0; j < k; j++) {
+        if (hconv[j] >= 0) nconv++;
+        if (col_sweeps) col_sweeps[j] = hconv[j] >= 0 ? hconv[j] : -last_k;
+    }
+    if (stats) {
+        stats->sweeps = last_k;
+        stats->converged = nconv;
+        stats->variant = variant;
+        stats->cols_per_lane = V.cpl;
+        stats->states_per_wave = V.spw;
+        stats->col_block = kb_of(V);
+        stats->solve_ms = now_ms() - t_start;
+        stats->sweep_kernel_ms = kms;
+        stats->tune_ms = tune_ms;
+        stats->bytes_per_sweep = 8.0 * m->nnz + 8.0 * m->rows + 4.0 + 12.0 * (double)k * m->ns;
+        stats->launches = launched;
+    }
+    if (nconv < k) {
+        set_error("%d of %d goal columns did not converge in %d sweeps", k - nconv, k, last_k);
+        return E2E_ERR_NOT_CONVERGED;
+    }
+    return E2E_OK;
+}
+
+// ------------------------------------------------------------------ live batches
+// A long-lived batch of goal columns on one model, for continuous batching: columns
+// are admitted at sweep boundaries into free slots, every launch advances all live
+// columns by one sweep, and a column leaves when its own residual falls below thresh.
+// A column admitted before sweep k starts from V = 0 in the buffer sweep k reads, so
+// after n of its sweeps it holds exactly the values of a solo n-sweep solve.
+
+namespace {
+
+// admit n columns: goal[col] = g, conv = -1, start = k, zero the column in the buffer
+// the next sweep reads; clears all_conv so launches do work again.
+__global__ void live_admit_kernel(float* V0, float* V1, int64_t ns, int kp, const int32_t* cols,
+                                  const int32_t* goals, int n, int32_t* goal, int32_t* conv, int32_t* start,
+                                  float* last_res, BatchState* st) {
+    const uint32_t k = st->k;
+    float* Vin = (k & 1u) ? V0 : V1;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ns * n) {
+        const int c = (int)(i / ns);
+        const int64_t s = i - (int64_t)c * ns;
+        Vin[s * kp + cols[c]] = 0.0f;
+    }
+    if (i < n) {
+        goal[cols[i]] = goals[i];
+        conv[cols[i]] = -1;
+        start[cols[i]] = (int32_t)k;
+        last_res[cols[i]] = 0.0f;
+    }
+    if (i == 0) st->all_conv = 0;
+}
+
+__global__ void live_column(const float* V0, const float* V1, const int32_t* pi, const int32_t* conv,
+                            int64_t ns, int kp, int col, float* vout, int32_t* pout) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const int cj = conv[col];
+    const float* V = (cj & 1) ? V1 : V0;
+    vout[s] = V[s * kp + col];
+    pout[s] = pi[s * kp + col];
+}
+
+__global__ void live_init(float* V0, float* V1, int64_t n, int32_t* goal, int32_t* conv, int32_t* start,
+                          uint32_t* res, float* last_res, int kp, BatchState* st) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        V0[i] = 0.0f;
+        V1[i] = 0.0f;
+    }
+    if (i < kp) {
+        goal[i] = -1;
+        conv[i] = 0;  // free slot
+        start[i] = 1;
+        res[i] = 0u;
+        res[kp + i] = 0u;
+        last_res[i] = 0.0f;
+    }
+    if (i == 0) {
+        st->k = 1u;
+        st->done = 0u;
+        st->all_conv = 1;
+    }
+}
+
+}  // namespace
+
+struct LiveBatch {
+    Model* m = nullptr;
+    BatchBufs b;
+    int variant = 0;
+    dim3 grid;
+    hipStream_t s = nullptr;
+    BatchArgs A{};
+    int32_t* d_list = nullptr;  // admission list scratch: [2][kp]
+    int64_t bytes = 0;
+};
+
+int64_t live_batch_bytes(const Model* m, int kp) {
+    return (int64_t)m->ns * kp * 4 * 5 + (int64_t)kp * 40 + 4096;
+}
+
+e2e_status live_create(Model* m, int kp, float thresh, hipStream_t s, LiveBatch** out) {
+    E2E_CHECK(m->built && m->Rsum && m->p.term_subgrid_size == 0 && !m->has_obstacles, E2E_ERR_INVALID,
+              "live batch needs a shared goal model (keep_rsum, no terminal, no obstacles)");
+    E2E_CHECK(kp >= 16 && kp <= 64 && kp % 16 == 0, E2E_ERR_INVALID, "kp=%d", kp);
+    LiveBatch* L = new LiveBatch();
+    L->m = m;
+    L->s = s;
+    e2e_status st = alloc_bufs(m, L->b, kp);
+    if (st != E2E_OK || hipMalloc(&L->d_list, 2 * kp * sizeof(int32_t)) != hipSuccess) {
+        L->b.release();
+        delete L;
+        if (st == E2E_OK) set_error("live batch: allocation failed");
+        return st != E2E_OK ? st : E2E_ERR_NOMEM;
+    }
+    e2e_solve_opts o{};
+    o.thresh = thresh;
+    o.discount = 1.0f;
+    L->A = make_args(m, L->b, o);
+    // variant: the model's tuned choice for this width if one exists, else columns across
+    // the whole wave (one state per wave)
+    L->variant = 0;
+    if (m->batch && m->batch->tuned_kp == kp && m->batch->tuned_variant >= 0) L->variant = m->batch->tuned_variant;
+    if (kp % kb_of(kVariants[L->variant])) L->variant = kp % 32 == 0 ? 3 : 6;
+    L->grid = grid_of(m, kVariants[L->variant], kp);
+    const int64_t n = m->ns * kp;
+    hipLaunchKernelGGL(live_init, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, L->b.V[0], L->b.V[1], n,
+                       L->b.goal, L->b.conv, L->b.start, L->b.res, L->b.last_res, kp, L->b.st);
+    E2E_HIP(hipGetLastError());
+    L->bytes = live_batch_bytes(m, kp);
+    *out = L;
+    return E2E_OK;
+}
+
+void live_free(LiveBatch* L) {
+    if (!L) return;
+    if (L->s) (void)hipStreamSynchronize(L->s);
+    if (L->d_list) (void)hipFree(L->d_list);
+    L->b.release();
+    delete L;
+}
+
+e2e_status live_admit(LiveBatch* L, const int32_t* cols, const int32_t* goals, int n) {
+    if (n <= 0) return E2E_OK;
+    const int kp = L->b.kp;
+    E2E_CHECK(n <= kp, E2E_ERR_INVALID, "too many admissions");
+    std::vector<int32_t> h(2 * kp);
+    for (int i = 0; i < n; i++) {
+        E2E_CHECK(cols[i] >= 0 && cols[i] < kp && goals[i] >= 0 && goals[i] < L->m->nc, E2E_ERR_INVALID,
+                  "bad admission");
+        h[i] = cols[i];
+        h[kp + i] = goals[i];
+    }
+    E2E_HIP(hipMemcpyAsync(L->d_list, h.data(), 2 * kp * sizeof(int32_t), hipMemcpyHostToDevice, L->s));
+    const int64_t tot = L->m->ns * n;
+    hipLaunchKernelGGL(live_admit_kernel, dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, L->s, L->b.V[0],
+                       L->b.V[1], L->m->ns, kp, L->d_list, L->d_list + kp, n, L->b.goal, L->b.conv, L->b.start,
+                       L->b.last_res, L->b.st);
+    E2E_HIP(hipGetLastError());
+    // the host copy must outlive the async memcpy
+    E2E_HIP(hipStreamSynchronize(L->s));
+    return E2E_OK;
+}
+
+e2e_status live_step(LiveBatch* L, int nsweeps) {
+    for (int i = 0; i < nsweeps; i++)
+        hipLaunchKernelGGL(kVariants[L->variant].fn, L->grid, dim3(256), 0, L->s, L->A);
+    E2E_HIP(hipGetLastError());
+    return E2E_OK;
+}
+
+// Waits for the stream, then reports each slot's convergence sweep (conv) and
+// admission sweep (start).
+e2e_status live_state(LiveBatch* L, BatchState* st, int32_t* conv, int32_t* start) {
+    const int kp = L->b.kp;
+    E2E_HIP(hipMemcpyAsync(st, L->b.st, sizeof(BatchState), hipMemcpyDeviceToHost, L->s));
+    E2E_HIP(hipMemcpyAsync(conv, L->b.conv, kp * 4, hipMemcpyDeviceToHost, L->s));
+    E2E_HIP(hipMemcpyAsync(start, L->b.start, kp * 4, hipMemcpyDeviceToHost, L->s));
+    E2E_HIP(hipStreamSynchronize(L->s));
+    return E2E_OK;
+}
+
+// Copies one converged column: V(start_state), pi(start_state), and optionally the full
+// value/policy columns (host arrays of ns).
+e2e_status live_harvest(LiveBatch* L, int col, int64_t start_state, float* v_start, int32_t* a_start,
+                        float* vcol, uint32_t* pcol) {
+    const int64_t ns = L->m->ns;
+    hipLaunchKernelGGL(live_column, dim3((unsigned)ceil_div(ns, 256)), dim3(256), 0, L->s, L->b.V[0], L->b.V[1],
+                       L->b.pi, L->b.conv, ns, L->b.kp, col, L->b.vout, L->b.pout);
+    E2E_HIP(hipGetLastError());
+    E2E_HIP(hipMemcpyAsync(v_start, L->b.vout + start_state, 4, hipMemcpyDeviceToHost, L->s));
+    E2E_HIP(hipMemcpyAsync(a_start, L->b.pout + start_state, 4, hipMemcpyDeviceToHost, L->s));
+    if (vcol) E2E_HIP(hipMemcpyAsync(vcol, L->b.vout, sizeof(float) * ns, hipMemcpyDeviceToHost, L->s));
+    if (pcol) E2E_HIP(hipMemcpyAsync(pcol, L->b.pout, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, L->s));
+    E2E_HIP(hipStreamSynchronize(L->s));
+    return E2E_OK;
+}
+
+int live_kp(const LiveBatch* L) { return L->b.kp; }
+int64_t live_bytes(const LiveBatch* L) { return L->bytes; }
+
+}  // namespace e2e

@@ -141,6 +141,56 @@ typedef struct e2e_solve_stats {
     int32_t reserved;
 } e2e_solve_stats;
 
+typedef struct e2e_batch_opts {
+    int32_t variant;           /* SpMM kernel variant; -1: autotune on this model */
+    int32_t use_graph;         /* 1: replay the sweep chain as a captured hipGraph */
+    int32_t reserved[2];
+} e2e_batch_opts;
+
+typedef struct e2e_batch_stats {
+    int32_t sweeps;            /* sweeps run until the last column converged */
+    int32_t converged;         /* columns that converged */
+    int32_t variant;           /* kernel variant used */
+    int32_t cols_per_lane;     /* its column width per lane */
+    int32_t states_per_wave;   /* its states (= na rows each) per wavefront */
+    int32_t col_block;         /* columns covered by one wavefront */
+    double solve_ms;
+    double sweep_kernel_ms;
+    double tune_ms;            /* autotuning time included in solve_ms (0 when cached) */
+    double bytes_per_sweep;    /* algorithmic bytes of one batched sweep */
+    int32_t launches;
+    int32_t reserved;
+} e2e_batch_stats;
+
+typedef struct e2e_server_opts {
+    int64_t hbm_budget_bytes;  /* device bytes for models + batches; 0: 80% of free memory */
+    int32_t max_cols;          /* goal columns per model batch, 1..64 (0: 64) */
+    int32_t sweeps_per_step;   /* sweeps launched per model between scheduler checks (0: 4) */
+    int32_t keep_full_results; /* 1: keep each request's full value/policy columns on the host */
+    int32_t serial;            /* 1: one request on the GPU at a time (queue-only baseline) */
+    float thresh;              /* VI stopping threshold (0: 1e-6) */
+    int32_t reserved;
+} e2e_server_opts;
+
+typedef struct e2e_request_result {
+    int32_t status;            /* 0 queued, 1 running, 2 done, 3 rejected (too big), 4 failed */
+    int32_t sweeps;            /* sweeps of this request's solve */
+    float value_at_start;      /* V(start cell, t = 0) */
+    int32_t action_at_start;   /* pi(start cell, t = 0) */
+    double latency_ms;         /* arrival -> result */
+    double queue_ms;           /* arrival -> admitted into a batch slot */
+    int32_t cache_hit;         /* its model was resident when it arrived */
+    int32_t reserved;
+} e2e_request_result;
+
+typedef struct e2e_server_stats {
+    int64_t submitted, completed, rejected, models_built, model_evictions, cache_hits;
+    int64_t peak_reserved_bytes, sweeps_launched;
+    double p50_ms, p99_ms, mean_ms, max_ms;
+    double wall_ms;            /* first arrival -> last completion */
+    double solves_per_s;
+} e2e_server_stats;
+
 const char* e2e_last_error(void);
 const char* e2e_version(void);
 
@@ -148,7 +198,9 @@ const char* e2e_version(void);
 int32_t e2e_device_count(void);
 
 /* sizeof of the ABI structs, for binding checks: 0 e2e_params, 1 e2e_fields,
- * 2 e2e_build_opts, 3 e2e_model_info, 4 e2e_solve_opts, 5 e2e_solve_stats;
+ * 2 e2e_build_opts, 3 e2e_model_info, 4 e2e_solve_opts, 5 e2e_solve_stats,
+ * 6 e2e_batch_opts, 7 e2e_batch_stats, 8 e2e_server_opts, 9 e2e_request_result,
+ * 10 e2e_server_stats;
  * -1 for an unknown code. */
 int32_t e2e_struct_size(int32_t which);
 
@@ -195,6 +247,37 @@ e2e_status e2e_model_export_csr(const void* model, uint32_t* row_ptr, int32_t* c
  * policy/value are host arrays of ns elements (either may be NULL). */
 e2e_status e2e_vi_solve(void* model, const e2e_solve_opts* opts,
                         uint32_t* policy, float* value, e2e_solve_stats* stats);
+
+/* Batched multi-goal solve (no counterpart call in the reference, which rebuilds and
+ * re-solves the model per goal: src/input_to_build_model.h:33-40 fix one terminal per
+ * build). Column j equals the solve of the model built with its terminal cell at
+ * goal_cells[j] (cell index i*gsize + j, term_subgrid_size 1), by solver_spvi_solve
+ * (src/spvi_2.cu:650-726). The model must come from e2e_model_build with keep_rsum=1 and
+ * term_subgrid_size 0, without obstacles. k <= 64. policy/value are host arrays [k][ns]
+ * (either may be NULL); col_sweeps [k] gets each column's sweep count (or NULL). */
+e2e_status e2e_vi_solve_goals(void* model, const int32_t* goal_cells, int32_t k,
+                              const e2e_solve_opts* opts, const e2e_batch_opts* bopts,
+                              uint32_t* policy, float* value, int32_t* col_sweeps,
+                              e2e_batch_stats* stats);
+
+/* Continuous-batching planning server (no counterpart in the reference, whose pipeline
+ * is one build + one solve per goal: src/Makefile:7-19). Environments (fields + params,
+ * keyed by env_id, e.g. the field seed) are registered once; requests (env, goal cell,
+ * start cell at t = 0) are answered from device-resident models. */
+e2e_status e2e_server_create(const e2e_server_opts* opts, void** server_out);
+e2e_status e2e_server_add_env(void* server, int64_t env_id, const e2e_params* p,
+                              const e2e_fields* host_fields);
+e2e_status e2e_server_submit(void* server, int64_t env_id, int32_t goal_cell,
+                             int32_t start_cell, uint64_t* req_id);
+e2e_status e2e_server_poll(void* server, uint64_t req_id, e2e_request_result* out);
+e2e_status e2e_server_wait(void* server, uint64_t req_id, double timeout_ms,
+                           e2e_request_result* out);
+/* Full value/policy columns (ns entries) of a finished request (keep_full_results=1). */
+e2e_status e2e_server_result_columns(void* server, uint64_t req_id, uint32_t* policy,
+                                     float* value);
+e2e_status e2e_server_drain(void* server, double timeout_ms);
+e2e_status e2e_server_stats_get(void* server, e2e_server_stats* out);
+e2e_status e2e_server_destroy(void* server);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,80 @@
+"""Continuous-batching server: every request's result is the same alone and in the mix;
+HBM-budget admission evicts idle models or rejects what cannot fit."""
+import numpy as np
+import pytest
+
+import e2e_gpu_dp_amd as e2e
+from e2e_gpu_dp_amd.configs import small
+
+pytestmark = pytest.mark.gpu
+
+
+def _envs():
+    a = small(g=16, nt=12, nr=300, nm=2, seed=81, na=8)
+    b = small(g=20, nt=15, nr=500, nm=3, seed=82, na=16)
+    return {1: (a.params, a.fields()), 2: (b.params, b.fields())}
+
+
+def _solo(params, fields, goal):
+    with e2e.build_model(params.replace(term_subgrid_size=0), fields, keep_rsum=True) as m:
+        pol, val, sw, _ = m.solve_goals([goal], thresh=1e-6)
+    return pol[0], val[0], sw[0]
+
+
+def test_server_mix_equals_solo(gpu):
+    envs = _envs()
+    rng = np.random.default_rng(7)
+    with e2e.Server(keep_full_results=True, thresh=1e-6, sweeps_per_step=3) as srv:
+        for eid, (p, f) in envs.items():
+            srv.add_env(eid, p, f)
+        reqs = []
+        for i in range(40):
+            eid = 1 + (i % 2)
+            nc = envs[eid][0].nc
+            g, s0 = (int(x) for x in rng.integers(0, nc, 2))
+            reqs.append((srv.submit(eid, g, s0), eid, g, s0))
+        srv.drain(timeout_ms=120000)
+        st = srv.stats()
+        assert st["completed"] == 40 and st["models_built"] == 2
+        assert st["p99_ms"] >= st["p50_ms"] > 0
+        for rid, eid, g, s0 in reqs[:6] + reqs[-4:]:
+            r = srv.wait(rid)
+            assert r["status_name"] == "done"
+            pol, val = srv.columns(rid, eid)
+            spol, sval, ssw = _solo(*envs[eid], g)
+            assert r["sweeps"] == ssw
+            assert np.array_equal(val.view(np.uint32), sval.view(np.uint32))
+            assert np.array_equal(pol, spol)
+            assert r["value_at_start"] == val[s0] and r["action_at_start"] == int(np.int32(pol[s0]))
+
+
+def test_server_budget_evicts_and_rejects(gpu):
+    envs = _envs()
+    with e2e.Server(hbm_budget_bytes=120 << 20, keep_full_results=False) as srv:
+        for eid, (p, f) in envs.items():
+            srv.add_env(eid, p, f)
+        ids = [srv.submit(1 + (i % 2), 5 + i, 3) for i in range(8)]
+        srv.drain(timeout_ms=120000)
+        st = srv.stats()
+        assert st["completed"] == 8
+        assert st["peak_reserved_bytes"] <= 120 << 20
+        assert all(srv.poll(i)["status_name"] == "done" for i in ids)
+    with e2e.Server(hbm_budget_bytes=1 << 20) as srv:
+        p, f = envs[2]
+        srv.add_env(2, p, f)
+        rid = srv.submit(2, 10, 3)
+        r = srv.wait(rid, timeout_ms=30000)
+        assert r["status_name"] == "rejected"
+
+
+def test_server_serial_queue_same_answers(gpu):
+    envs = _envs()
+    p, f = envs[1]
+    with e2e.Server(serial=True, max_cols=1, keep_full_results=True) as srv:
+        srv.add_env(1, p, f)
+        ids = [srv.submit(1, g, 0) for g in (7, 100, 200)]
+        srv.drain(timeout_ms=60000)
+        for rid, g in zip(ids, (7, 100, 200)):
+            pol, val = srv.columns(rid, 1)
+            spol, sval, _ = _solo(p, f, g)
+            assert np.array_equal(val.view(np.uint32), sval.view(np.uint32))
