@@ -44,7 +44,7 @@ class Fields(C.Structure):
 class BuildOpts(C.Structure):
     _fields_ = [
         ("fields_on_device", C.c_int32), ("keep_rsum", C.c_int32), ("slot_k", C.c_int32),
-        ("radix_bits", C.c_int32), ("slices_per_launch", C.c_int32), ("reserved", C.c_int32),
+        ("radix_bits", C.c_int32), ("slices_per_launch", C.c_int32), ("block_threads", C.c_int32),
         ("stream", C.c_void_p),
     ]
 

@@ -29,8 +29,6 @@ namespace e2e {
 
 namespace {
 
-constexpr int kWG = 256;     // threads per build workgroup (4 wave64)
-constexpr int kWaves = kWG / 64;
 
 struct BuildArgs {
     const float* u;
@@ -178,7 +176,8 @@ __device__ __forceinline__ int wave_sum(int x) {
     return x;
 }
 
-// Exclusive scan over the 256 threads of the block. lds: >= 4 ints. Two barriers.
+// Exclusive scan over the NW waves of a block. lds: >= NW ints. Two barriers.
+template <int NW>
 __device__ __forceinline__ int block_excl_scan_int(int v, int* lds, int* total) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     int x = v;
@@ -191,7 +190,7 @@ __device__ __forceinline__ int block_excl_scan_int(int v, int* lds, int* total) 
     __syncthreads();
     int pre = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < kWaves; w++) {
+    for (int w = 0; w < NW; w++) {
         int s = lds[w];
         if (w < wid) pre += s;
         tot += s;
@@ -224,34 +223,53 @@ __device__ __forceinline__ void put_entry(const BuildArgs& A, size_t row, int u,
     }
 }
 
-// One workgroup = TC consecutive cells of one time slice t. For each cell it samples the
-// velocity of every realization once (registers), then for each action:
-//   move every lane (one (s, a, w) triple per lane-round), range-compress the S2 keys of
-//   the segment to nbits bits, LSD-radix-sort them in LDS with radix_bits-wide digits
-//   (ballot match ranking, stable scatter), count runs (count-by-key), and write the
-//   row's distinct S2 with their counts plus the row reward sum.
-// When nbits <= radix_bits a single counting pass orders the keys, and its digit
-// histogram is the count-by-key, so the scatter is skipped.
-template <int R_MAX>
-__global__ __launch_bounds__(kWG) void mc_move_count(const BuildArgs A) {
+constexpr int kMiscInts = 256;  // wave range data [0,128), scan scratch [128,144), u64 at [160]
+
+// LDS bytes of mc_move_count<WG> for R = ceil(nr / WG) rounds.
+__host__ __device__ inline size_t build_lds_bytes(int WG, int R, int D, int nm, int nr, bool stage_y) {
+    const int NW = WG / 64;
+    const size_t NRP = (size_t)R * WG;
+    return sizeof(double) * R * NW + sizeof(int) * kMiscInts + sizeof(uint32_t) * NW * (1 << D) +
+           sizeof(float2) * NRP + 2 * sizeof(uint32_t) * (NRP + 4) +
+           (stage_y ? sizeof(float) * (size_t)nr * nm : 0);
+}
+
+// One workgroup = TC consecutive cells of one time slice t; its threads own the
+// realizations w = r*WG + tid (rounds r < R). For each cell the velocity sample of every
+// realization is formed once (extract_velocity) and kept in LDS; then for each action:
+//   (a) move every (s, a, w) lane, keep its relative S2 offset in LDS and its reward in
+//       the canonical chunk tree;
+//   (b) range-compress the segment's S2 keys to nbits bits and LSD-radix-sort them in
+//       LDS with radix_bits-wide digits (ballot match ranking per wave, stable scatter
+//       through per-(wave, digit) running offsets), then count runs (count-by-key);
+//       when nbits <= radix_bits one counting pass orders the keys and its digit
+//       histogram is the count-by-key, so the scatter is skipped;
+//   write the row's distinct S2 with their counts and the row reward sum.
+template <int WG, bool OBS>
+__global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
+    constexpr int NW = WG / 64;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int R = A.R;
     const int nr = A.nr, nm = A.nm, g = A.g, nc = A.nc;
     const int D = A.radix_bits, NBMAX = 1 << D;
-    double* rpart = reinterpret_cast<double*>(smem_raw);      // [256]
-    int* misc = reinterpret_cast<int*>(rpart + kWG);          // [64]
-    uint32_t* cnt = reinterpret_cast<uint32_t*>(misc + 64);   // [4][NBMAX]
-    uint32_t* keysA = cnt + kWaves * NBMAX;                   // [R*256 + 4]
-    uint32_t* keysB = keysA + R * kWG + 4;                    // [R*256 + 4]
-    float* ylds = reinterpret_cast<float*>(keysB + R * kWG + 4);  // [nm][nr] when staged
+    const int NRP = R * WG;
+    double* csum = reinterpret_cast<double*>(smem_raw);          // [R*NW] chunk sums
+    int* misc = reinterpret_cast<int*>(csum + R * NW);           // [kMiscInts]
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(misc + kMiscInts);  // [NW][NBMAX]
+    float2* vxy = reinterpret_cast<float2*>(cnt + NW * NBMAX);   // [NRP]
+    uint32_t* keysA = reinterpret_cast<uint32_t*>(vxy + NRP);    // [NRP + 4]
+    uint32_t* keysB = keysA + NRP + 4;                           // [NRP + 4]
+    float* ylds = reinterpret_cast<float*>(keysB + NRP + 4);     // [nm][nr] when staged
+    unsigned long long* arena_slot = reinterpret_cast<unsigned long long*>(misc + 160);
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    const int nchunks = (nr + 63) / 64;
 
     const int t = A.t0 + blockIdx.y;
     const size_t tslice = (size_t)t * nc;
     if (A.stage_y) {
         const float* src = A.yi + (size_t)t * nr * nm;
-        for (int i = tid; i < nr * nm; i += kWG) {
+        for (int i = tid; i < nr * nm; i += WG) {
             int w = i / nm, k = i - w * nm;
             ylds[k * nr + w] = src[i];
         }
@@ -261,98 +279,85 @@ __global__ __launch_bounds__(kWG) void mc_move_count(const BuildArgs A) {
     unsigned long long oow_acc = 0, multipass_acc = 0, overflow_acc = 0;
     uint32_t maxnuq_acc = 0;
     bool bad = false, bad_index = false;
+    const int del = (A.m - 1) / 2;
 
     for (int ci = 0; ci < A.tc; ci++) {
         const int c = blockIdx.x * A.tc + ci;
         if (c >= nc) break;
         const int i1 = c / g, j1 = c - i1 * g;
         const size_t mid = tslice + c;
-        // extract_velocity :361-404, hoisted out of the action loop
-        float mx[kMaxModes], my[kMaxModes];
-#pragma unroll
-        for (int k = 0; k < kMaxModes; k++) {
-            if (k < nm) {
-                const size_t o = ((size_t)t * nm + k) * nc + c;
-                mx[k] = A.ui[o];
-                my[k] = A.vi[o];
-            }
-        }
+        // extract_velocity :361-404: mode sum from 0 in k order, then added to the mean
         const float u0 = A.u[mid], v0 = A.v[mid];
-        float vx[R_MAX], vy[R_MAX];
-#pragma unroll
-        for (int r = 0; r < R_MAX; r++) {
-            const int w = r * kWG + tid;
-            vx[r] = 0.0f;
-            vy[r] = 0.0f;
-            if (r < R && w < nr) {
+        for (int r = 0; r < R; r++) {
+            const int w = r * WG + tid;
+            if (w < nr) {
                 float sx = 0.0f, sy = 0.0f;
-#pragma unroll
-                for (int k = 0; k < kMaxModes; k++) {
-                    if (k < nm) {
-                        const float y = A.stage_y ? ylds[k * nr + w]
-                                                  : A.yi[((size_t)t * nr + w) * nm + k];
-                        sx = sx + mx[k] * y;
-                        sy = sy + my[k] * y;
-                    }
+                for (int k = 0; k < nm; k++) {
+                    const size_t o = ((size_t)t * nm + k) * nc + c;
+                    const float y = A.stage_y ? ylds[k * nr + w] : A.yi[((size_t)t * nr + w) * nm + k];
+                    sx = sx + A.ui[o] * y;
+                    sy = sy + A.vi[o] * y;
                 }
-                vx[r] = u0 + sx;
-                vy[r] = v0 + sy;
-                if (!(isfinite(vx[r]) && isfinite(vy[r]))) bad = true;
+                const float vx = u0 + sx, vy = v0 + sy;
+                if (!(isfinite(vx) && isfinite(vy))) bad = true;
+                vxy[w] = make_float2(vx, vy);
             }
         }
         const bool term1 = is_terminal(A, i1, j1);
         const bool obs1 = A.mask && A.mask[mid] == 1;
         const float rad1 = A.s ? A.s[mid] : 0.0f;
         const float xc = A.xs[j1], yc = A.ys[g - 1 - i1];
-        const int del = (A.m - 1) / 2;
 
         for (int a = 0; a < A.na; a++) {
             const float speed = A.act_speed[a], fc = A.act_fcos[a], fs = A.act_fsin[a];
-            uint32_t key[R_MAX];
             int dimin = 0x7fffffff, dimax = -0x7fffffff, djmin = 0x7fffffff, djmax = -0x7fffffff;
             int oow = 0;
-            double part = 0.0;
             // ---- (a) move: transition_calc :522-545
-#pragma unroll
-            for (int r = 0; r < R_MAX; r++) {
-                const int w = r * kWG + tid;
-                int di = 0, dj = 0;
-                if (r < R && w < nr) {
+            for (int r = 0; r < R; r++) {
+                const int w = r * WG + tid;
+                const bool valid = w < nr;
+                float rw = 0.0f;
+                if (valid) {
                     int pi = i1, pj = j1;
-                    float rw = 0.0f;
                     if (!term1) {
                         if (!obs1) {
-                            lane_move(A, xc, yc, fc, fs, vx[r], vy[r], t, pi, pj, rw);
-                            if (pi < 0 || pi >= g || pj < 0 || pj >= g) {  // cannot happen for
-                                bad_index = true;                          // dx, dy > 0; guard
-                                pi = min(max(pi, 0), g - 1);               // the gathers below
+                            const float2 v = vxy[w];
+                            lane_move(A, xc, yc, fc, fs, v.x, v.y, t, pi, pj, rw);
+                            if (pi < 0 || pi >= g || pj < 0 || pj >= g) {  // impossible for dx, dy > 0;
+                                bad_index = true;                          // guards the gathers below
+                                pi = min(max(pi, 0), g - 1);
                                 pj = min(max(pj, 0), g - 1);
                             }
                             const int sp2 = pj + g * pi;
                             const float rad2 = A.s ? A.s[tslice + nc + sp2] : 0.0f;
                             rw += step_reward(A, speed, rad1, rad2);
-                            if (obs_slice && crosses_obstacle(A, i1, j1, pi, pj, t)) rw = A.r_out;
+                            if (OBS && obs_slice && crosses_obstacle(A, i1, j1, pi, pj, t)) rw = A.r_out;
                         } else {
                             rw = A.r_out;
                         }
                     }
-                    di = pi - i1;
-                    dj = pj - j1;
-                    part += (double)rw;
+                    const int di = pi - i1, dj = pj - j1;
                     dimin = min(dimin, di);
                     dimax = max(dimax, di);
                     djmin = min(djmin, dj);
                     djmax = max(djmax, dj);
                     if (di + del < 0 || di + del >= A.m || dj + del < 0 || dj + del >= A.m) oow++;
+                    keysA[w] = ((uint32_t)(di + 32768) << 16) | (uint32_t)(dj + 32768);
                 }
-                key[r] = ((uint32_t)(di + 32768) << 16) | (uint32_t)(dj + 32768);
+                // canonical reward sum, chunk (r*NW + wv) = w in [64c, 64c+63]: lane tree
+                double p = valid ? (double)rw : 0.0;
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) {
+                    const double q = __shfl_down(p, o, 64);
+                    p = p + q;
+                }
+                if (lane == 0) csum[r * NW + wv] = p;
             }
             dimin = wave_min(dimin);
             dimax = wave_max(dimax);
             djmin = wave_min(djmin);
             djmax = wave_max(djmax);
             oow = wave_sum(oow);
-            rpart[tid] = part;
             if (lane == 0) {
                 misc[wv * 8 + 0] = dimin;
                 misc[wv * 8 + 1] = dimax;
@@ -360,139 +365,118 @@ __global__ __launch_bounds__(kWG) void mc_move_count(const BuildArgs A) {
                 misc[wv * 8 + 3] = djmax;
                 misc[wv * 8 + 4] = oow;
             }
-            for (int i = tid; i < kWaves * NBMAX; i += kWG) cnt[i] = 0u;
+            for (int i = tid; i < NW * NBMAX; i += WG) cnt[i] = 0u;
             __syncthreads();
             dimin = misc[0], dimax = misc[1], djmin = misc[2], djmax = misc[3];
             int oow_blk = misc[4];
 #pragma unroll
-            for (int w = 1; w < kWaves; w++) {
+            for (int w = 1; w < NW; w++) {
                 dimin = min(dimin, misc[w * 8 + 0]);
                 dimax = max(dimax, misc[w * 8 + 1]);
                 djmin = min(djmin, misc[w * 8 + 2]);
                 djmax = max(djmax, misc[w * 8 + 3]);
                 oow_blk += misc[w * 8 + 4];
             }
-            // reward sum in the canonical order: partial[tid] over ascending w, then the
-            // tree partial[i] += partial[i+o], o = 128..1 (compute_mean :563-570 divides)
             double rsum = 0.0;
-            if (wv == 0) {
-                double p = (rpart[lane] + rpart[lane + 128]) + (rpart[lane + 64] + rpart[lane + 192]);
-#pragma unroll
-                for (int o = 32; o >= 1; o >>= 1) {
-                    double q = __shfl_down(p, o, 64);
-                    p = p + q;
-                }
-                rsum = p;
-            }
-            // ---- (b) range-compress: key = (di - dimin) * Wj + (dj - djmin); this keeps
-            // the (i2, j2) lexicographic order, i.e. ascending S2 = (t+1)*nc + j2 + g*i2
+            if (tid == 0)
+                for (int ch = 0; ch < nchunks; ch++) rsum = rsum + csum[ch];
+            // ---- (b) range-compress: key = (di - dimin) * Wj + (dj - djmin) keeps the
+            // (i2, j2) lexicographic order, i.e. ascending S2 = (t+1)*nc + j2 + g*i2
             const uint32_t Wj = (uint32_t)(djmax - djmin + 1);
             const uint64_t nkeys = (uint64_t)(dimax - dimin + 1) * Wj;
             const int nbits = nkeys <= 1 ? 0 : 64 - __clzll((long long)(nkeys - 1));
-#pragma unroll
-            for (int r = 0; r < R_MAX; r++) {
-                const int di = (int)(key[r] >> 16) - 32768, dj = (int)(key[r] & 0xffffu) - 32768;
-                key[r] = (uint32_t)(di - dimin) * Wj + (uint32_t)(dj - djmin);
+            for (int r = 0; r < R; r++) {
+                const int w = r * WG + tid;
+                if (w < nr) {
+                    const uint32_t k0 = keysA[w];
+                    const int di = (int)(k0 >> 16) - 32768, dj = (int)(k0 & 0xffffu) - 32768;
+                    keysA[w] = (uint32_t)(di - dimin) * Wj + (uint32_t)(dj - djmin);
+                }
             }
-            const size_t row = (mid)*A.na + a;
+            const size_t row = mid * A.na + a;
             int nuq = 0;
             unsigned long long arena_off = 0;
             if (nbits <= D) {
                 // single counting pass: per-wave digit histogram by ballot matching
-#pragma unroll
-                for (int r = 0; r < R_MAX; r++) {
-                    if (r < R) {
-                        const bool valid = (r * kWG + tid) < nr;
-                        const unsigned long long peers = match_peers(key[r], nbits, valid);
-                        if (valid && lane == __ffsll((long long)peers) - 1)
-                            cnt[wv * NBMAX + key[r]] += (uint32_t)__popcll(peers);
-                    }
+                for (int r = 0; r < R; r++) {
+                    const int w = r * WG + tid;
+                    const bool valid = w < nr;
+                    const uint32_t key = valid ? keysA[w] : 0u;
+                    const unsigned long long peers = match_peers(key, nbits, valid);
+                    if (valid && lane == __ffsll((long long)peers) - 1)
+                        cnt[wv * NBMAX + key] += (uint32_t)__popcll(peers);
                 }
                 __syncthreads();
                 uint32_t tot = 0;
-                if (tid < NBMAX) {
+                for (int d = tid; d < NBMAX; d += WG) {  // NBMAX <= 256 <= WG: one digit per thread
 #pragma unroll
-                    for (int w = 0; w < kWaves; w++) tot += cnt[w * NBMAX + tid];
+                    for (int w = 0; w < NW; w++) tot += cnt[w * NBMAX + d];
                 }
                 int total;
-                const int u = block_excl_scan_int(tot > 0 ? 1 : 0, misc + 48, &total);
+                const int u = block_excl_scan_int<NW>(tot > 0 ? 1 : 0, misc + 128, &total);
                 nuq = total;
                 if (nuq > A.K) {
-                    if (tid == 0) {
-                        unsigned long long off = atomicAdd(A.arena_top, (unsigned long long)nuq);
-                        reinterpret_cast<unsigned long long*>(misc + 56)[0] = off;
-                    }
+                    if (tid == 0) *arena_slot = atomicAdd(A.arena_top, (unsigned long long)nuq);
                     __syncthreads();
-                    arena_off = reinterpret_cast<unsigned long long*>(misc + 56)[0];
+                    arena_off = *arena_slot;
                 }
                 if (tot > 0) {
                     const int di = (int)(tid / Wj) + dimin, dj = (int)(tid % Wj) + djmin;
                     put_entry(A, row, u, nuq, arena_off, (uint32_t)((j1 + dj) + g * (i1 + di)), tot);
                 }
             } else {
-                // multi-pass LSD radix sort of the nr keys in LDS
+                // multi-pass LSD radix sort of the nr keys in LDS; element (wave, round,
+                // lane) of a pass is keysA[r*WG + tid] in pass 0 (any order is fine for the
+                // first pass) and src[wv*R*64 + r*64 + lane] afterwards (the stable order)
                 const int passes = (nbits + D - 1) / D;
                 uint32_t* src = keysA;
                 uint32_t* dst = keysB;
-                uint32_t lrank[R_MAX];
                 for (int ps = 0; ps < passes; ps++) {
                     const int shift = ps * D;
                     const int dbits = min(D, nbits - shift);
                     const int nbp = 1 << dbits;
                     if (ps > 0) {
-#pragma unroll
-                        for (int r = 0; r < R_MAX; r++) {
-                            const int pos = wv * (R * 64) + r * 64 + lane;
-                            key[r] = (r < R && pos < nr) ? src[pos] : 0u;
-                        }
-                        for (int i = tid; i < kWaves * NBMAX; i += kWG) cnt[i] = 0u;
+                        for (int i = tid; i < NW * NBMAX; i += WG) cnt[i] = 0u;
                         __syncthreads();
                     }
-                    // rank: order (wave, round, lane); pass 0 takes w = r*256 + tid
-#pragma unroll
-                    for (int r = 0; r < R_MAX; r++) {
-                        if (r < R) {
-                            const bool valid = ps == 0 ? (r * kWG + tid) < nr
-                                                       : (wv * (R * 64) + r * 64 + lane) < nr;
-                            const uint32_t d = (key[r] >> shift) & (uint32_t)(nbp - 1);
-                            const unsigned long long peers = match_peers(d, dbits, valid);
-                            const uint32_t base = valid ? cnt[wv * NBMAX + d] : 0u;
-                            lrank[r] = base + (uint32_t)__popcll(peers & lt_mask);
-                            if (valid && lane == __ffsll((long long)peers) - 1)
-                                cnt[wv * NBMAX + d] = base + (uint32_t)__popcll(peers);
-                        }
+                    for (int r = 0; r < R; r++) {  // count
+                        const int idx = ps == 0 ? r * WG + tid : wv * (R * 64) + r * 64 + lane;
+                        const bool valid = idx < nr;
+                        const uint32_t key = valid ? src[idx] : 0u;
+                        const uint32_t d = (key >> shift) & (uint32_t)(nbp - 1);
+                        const unsigned long long peers = match_peers(d, dbits, valid);
+                        if (valid && lane == __ffsll((long long)peers) - 1)
+                            cnt[wv * NBMAX + d] += (uint32_t)__popcll(peers);
                     }
                     __syncthreads();
                     // offsets: digit-major, then wave order (stable)
-                    uint32_t tot = 0, wc[kWaves];
+                    uint32_t tot = 0;
                     if (tid < nbp) {
 #pragma unroll
-                        for (int w = 0; w < kWaves; w++) {
-                            wc[w] = cnt[w * NBMAX + tid];
-                            tot += wc[w];
-                        }
+                        for (int w = 0; w < NW; w++) tot += cnt[w * NBMAX + tid];
                     }
                     int total;
-                    const int base = block_excl_scan_int((int)tot, misc + 48, &total);
+                    const int base = block_excl_scan_int<NW>((int)tot, misc + 128, &total);
                     if (tid < nbp) {
                         uint32_t run = (uint32_t)base;
 #pragma unroll
-                        for (int w = 0; w < kWaves; w++) {
+                        for (int w = 0; w < NW; w++) {
+                            const uint32_t cw = cnt[w * NBMAX + tid];
                             cnt[w * NBMAX + tid] = run;
-                            run += wc[w];
+                            run += cw;
                         }
                     }
                     __syncthreads();
-#pragma unroll
-                    for (int r = 0; r < R_MAX; r++) {
-                        if (r < R) {
-                            const bool valid = ps == 0 ? (r * kWG + tid) < nr
-                                                       : (wv * (R * 64) + r * 64 + lane) < nr;
-                            if (valid) {
-                                const uint32_t d = (key[r] >> shift) & (uint32_t)(nbp - 1);
-                                dst[cnt[wv * NBMAX + d] + lrank[r]] = key[r];
-                            }
-                        }
+                    for (int r = 0; r < R; r++) {  // scatter through running offsets
+                        const int idx = ps == 0 ? r * WG + tid : wv * (R * 64) + r * 64 + lane;
+                        const bool valid = idx < nr;
+                        const uint32_t key = valid ? src[idx] : 0u;
+                        const uint32_t d = (key >> shift) & (uint32_t)(nbp - 1);
+                        const unsigned long long peers = match_peers(d, dbits, valid);
+                        const uint32_t b0 = valid ? cnt[wv * NBMAX + d] : 0u;
+                        if (valid) dst[b0 + (uint32_t)__popcll(peers & lt_mask)] = key;
+                        if (valid && lane == __ffsll((long long)peers) - 1)
+                            cnt[wv * NBMAX + d] = b0 + (uint32_t)__popcll(peers);
                     }
                     __syncthreads();
                     uint32_t* tmp = src;
@@ -507,7 +491,7 @@ __global__ __launch_bounds__(kWG) void mc_move_count(const BuildArgs A) {
                     if (p < nr && (p == 0 || src[p] != src[p - 1])) heads++;
                 }
                 int total;
-                int u = block_excl_scan_int(heads, misc + 48, &total);
+                int u = block_excl_scan_int<NW>(heads, misc + 128, &total);
                 nuq = total;
                 for (int k = 0; k < R; k++) {
                     const int p = p0 + k;
@@ -515,14 +499,11 @@ __global__ __launch_bounds__(kWG) void mc_move_count(const BuildArgs A) {
                 }
                 if (tid == 0) {
                     dst[nuq] = (uint32_t)nr;
-                    if (nuq > A.K) {
-                        unsigned long long off = atomicAdd(A.arena_top, (unsigned long long)nuq);
-                        reinterpret_cast<unsigned long long*>(misc + 56)[0] = off;
-                    }
+                    if (nuq > A.K) *arena_slot = atomicAdd(A.arena_top, (unsigned long long)nuq);
                 }
                 __syncthreads();
-                if (nuq > A.K) arena_off = reinterpret_cast<unsigned long long*>(misc + 56)[0];
-                for (int q = tid; q < nuq; q += kWG) {
+                if (nuq > A.K) arena_off = *arena_slot;
+                for (int q = tid; q < nuq; q += WG) {
                     const uint32_t p = dst[q];
                     const uint32_t kk = src[p];
                     const int di = (int)(kk / Wj) + dimin, dj = (int)(kk % Wj) + djmin;
@@ -533,7 +514,7 @@ __global__ __launch_bounds__(kWG) void mc_move_count(const BuildArgs A) {
             }
             if (tid == 0) {
                 A.row_nnz[row] = (uint32_t)nuq;
-                const float fs_ = (float)rsum;
+                const float fs_ = (float)rsum;  // compute_mean :563-570
                 A.Rout[row] = fs_ / nr;
                 if (A.Rsum) A.Rsum[row] = fs_;
                 oow_acc += (unsigned long long)oow_blk;
@@ -546,6 +527,7 @@ __global__ __launch_bounds__(kWG) void mc_move_count(const BuildArgs A) {
             }
             __syncthreads();
         }
+        __syncthreads();  // vxy of this cell is done before the next cell overwrites it
     }
     if (bad) atomicOr(A.stats + 3, 1ull);
     if (bad_index) atomicOr(A.stats + 6, 1ull);
@@ -680,32 +662,49 @@ e2e_status dmalloc(DevBuf& b, size_t bytes) {
     return E2E_OK;
 }
 
-template <int R_MAX>
+template <int WG, bool OBS>
 e2e_status launch_build(const BuildArgs& A, dim3 grid, size_t lds, hipStream_t s) {
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mc_move_count<R_MAX>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mc_move_count<WG, OBS>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) {
             set_error("hipFuncSetAttribute(%zu B LDS): %s", lds, hipGetErrorString(e));
             return E2E_ERR_HIP;
         }
     }
-    hipLaunchKernelGGL(mc_move_count<R_MAX>, grid, dim3(kWG), lds, s, A);
+    hipLaunchKernelGGL((mc_move_count<WG, OBS>), grid, dim3(WG), lds, s, A);
     return E2E_OK;
 }
 
-e2e_status dispatch_build(const BuildArgs& A, dim3 grid, size_t lds, hipStream_t s) {
-    const int R = A.R;
-    if (R <= 1) return launch_build<1>(A, grid, lds, s);
-    if (R <= 2) return launch_build<2>(A, grid, lds, s);
-    if (R <= 4) return launch_build<4>(A, grid, lds, s);
-    if (R <= 8) return launch_build<8>(A, grid, lds, s);
-    if (R <= 12) return launch_build<12>(A, grid, lds, s);
-    if (R <= 16) return launch_build<16>(A, grid, lds, s);
-    if (R <= 20) return launch_build<20>(A, grid, lds, s);
-    if (R <= 32) return launch_build<32>(A, grid, lds, s);
-    set_error("nrzns=%d exceeds the build kernel limit of %d", A.nr, 32 * kWG);
-    return E2E_ERR_CAPACITY;
+// OBS: the goes_through_obstacle test (:444-477) is compiled in only for models whose
+// mask has obstacles; it costs registers the obstacle-free kernel does not need.
+e2e_status dispatch_build(const BuildArgs& A, int wg, bool obs, dim3 grid, size_t lds, hipStream_t s) {
+    if (obs) {
+        if (wg == 256) return launch_build<256, true>(A, grid, lds, s);
+        if (wg == 512) return launch_build<512, true>(A, grid, lds, s);
+        return launch_build<1024, true>(A, grid, lds, s);
+    }
+    if (wg == 256) return launch_build<256, false>(A, grid, lds, s);
+    if (wg == 512) return launch_build<512, false>(A, grid, lds, s);
+    return launch_build<1024, false>(A, grid, lds, s);
+}
+
+// Block size: the one that keeps the most waves resident per CU under the 160 KiB LDS
+// (capped at 32 waves), the smaller on ties.
+int choose_wg(int nr, int D, int nm, bool stage_y, int forced) {
+    if (forced == 256 || forced == 512 || forced == 1024) return forced;
+    int best = 256, best_w = -1;
+    for (int wg : {256, 512, 1024}) {
+        const int R = (nr + wg - 1) / wg;
+        const size_t lds = build_lds_bytes(wg, R, D, nm, nr, stage_y);
+        const int blocks = lds > 0 ? (int)((160 * 1024) / lds) : 0;
+        const int waves = std::min(32, blocks * wg / 64);
+        if (waves > best_w) {
+            best_w = waves;
+            best = wg;
+        }
+    }
+    return best;
 }
 
 void host_actions(const e2e_params* p, float* speed, float* angle, float* fcos, float* fsin) {
@@ -853,7 +852,6 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     A.dt = p->dt; A.r_out = p->r_outbound; A.r_term = p->r_terminal; A.alpha = p->alpha;
     A.K = K;
     A.radix_bits = D;
-    A.R = (int)ceil_div(nr, kWG);
     A.stats = dStats.as<unsigned long long>();
 
     // row-indexed outputs
@@ -884,16 +882,24 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
                            A.row_nnz, m->R, m->Rsum, from, to);
     }
 
-    // LDS: rpart 2 KiB + misc 256 B + cnt + 2 key buffers (+ staged Yi)
-    const size_t lds_base = sizeof(double) * kWG + sizeof(int) * 64 +
-                            sizeof(uint32_t) * kWaves * (1 << D) + sizeof(uint32_t) * 2 * (A.R * kWG + 4);
     const size_t ybytes = sizeof(float) * (size_t)nr * nm;
     const bool stage = ybytes > 0 && ybytes <= 16 * 1024;
     A.stage_y = stage ? 1 : 0;
     A.tc = stage ? 4 : 1;
-    const size_t lds = lds_base + (stage ? ybytes : 0);
-    E2E_CHECK(lds <= 160 * 1024, E2E_ERR_CAPACITY, "build kernel needs %zu B of LDS", lds);
+    const int wg = choose_wg(nr, D, nm, stage, opts.block_threads);
+    A.R = (int)ceil_div(nr, wg);
+    const size_t lds = build_lds_bytes(wg, A.R, D, nm, nr, stage);
+    E2E_CHECK(lds <= 160 * 1024, E2E_ERR_CAPACITY, "build kernel needs %zu B of LDS (nrzns too large)", lds);
+    m->info.n_multipass_rows = 0;
 
+    bool any_obs = false;
+    if (mask) {
+        std::vector<int32_t> flags(nt);
+        E2E_HIP(hipMemcpyAsync(flags.data(), dFlags.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, s));
+        E2E_HIP(hipStreamSynchronize(s));
+        for (int32_t fl : flags) any_obs = any_obs || fl != 0;
+    }
+    m->has_obstacles = any_obs;
     const int64_t nslices = nt - 1;
     const int64_t spl = opts.slices_per_launch > 0 ? opts.slices_per_launch : nslices;
     hipEvent_t e0, e1;
@@ -915,7 +921,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
             A.t0 = (int)t0;
             dim3 grid((unsigned)ceil_div(nc, A.tc), (unsigned)cnt);
             E2E_HIP(hipEventRecord(e0, s));
-            if ((st = dispatch_build(A, grid, lds, s))) break;
+            if ((st = dispatch_build(A, wg, any_obs, grid, lds, s))) break;
             E2E_HIP(hipGetLastError());
             E2E_HIP(hipEventRecord(e1, s));
             E2E_HIP(hipEventSynchronize(e1));
@@ -957,11 +963,6 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st) return fail(st);
-    if (mask) {
-        std::vector<int32_t> flags(nt);
-        E2E_HIP(hipMemcpy(flags.data(), dFlags.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost));
-        for (int32_t fl : flags) m->has_obstacles = m->has_obstacles || fl != 0;
-    }
     m->info.move_kernel_ms = move_ms;
     m->info.move_launches = launches;
 

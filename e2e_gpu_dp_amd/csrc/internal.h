@@ -45,10 +45,8 @@ constexpr int kMaxModes = 16;
 struct VIState {
     uint32_t k;          // 1-based index of the next sweep
     int32_t conv;        // sweep at which the residual first fell below thresh, -1 if none
-    uint32_t res[2];     // residual bits of sweeps k (parity slots); max via atomicMax
-    uint32_t done;       // blocks of the current sweep that finished
+    uint32_t done;       // blocks of the current sweep that finished (ticket)
     float last_res;      // residual of the last completed sweep
-    uint32_t pad[2];
 };
 
 // Per-column state of a batched (multi-goal) solve.
@@ -79,6 +77,7 @@ struct Model {
     float* V[2] = {nullptr, nullptr};
     int32_t* pi = nullptr;
     VIState* st = nullptr;
+    float* vi_part = nullptr;     // per-block residual partials of the running sweep
     hipGraphExec_t graph = nullptr;
     int graph_chunk = 0;
     int graph_lpr = 0;

@@ -41,7 +41,7 @@ struct BatchArgs {
     int32_t* pi;
     const int32_t* goal;   // [KP] goal cell per column (-1: none)
     int32_t* conv;         // [KP] sweep at which the column converged, -1 running
-    uint32_t* res;         // [2][KP] residual bits per sweep parity
+    float* part;           // [gridDim.x][KP] per-block column residual maxima
     float* last_res;       // [KP]
     BatchState* st;
     int64_t ns;
@@ -83,11 +83,16 @@ struct VecLoad<4> {
     }
 };
 
+constexpr int kBThreads = 1024;  // 16 waves per block
+constexpr int kBWaves = kBThreads / 64;
+constexpr int kBMaxBlocks = 512;
+
 template <int CPL, int SPW>
-__global__ __launch_bounds__(256) void spmm_vi_sweep(const BatchArgs A) {
+__global__ __launch_bounds__(kBThreads) void spmm_vi_sweep(const BatchArgs A) {
     constexpr int LPS = 64 / SPW;
     constexpr int KB = LPS * CPL;
-    __shared__ float red[4][KB];
+    __shared__ float red[kBWaves][KB];
+    __shared__ float fin[kBThreads];
     __shared__ int is_last;
     BatchState* st = A.st;
     if (st->all_conv) return;
@@ -107,8 +112,8 @@ __global__ __launch_bounds__(256) void spmm_vi_sweep(const BatchArgs A) {
         live[c] = A.conv[j0 + c] < 0;
         rmax[c] = 0.0f;
     }
-    const int64_t stride = (int64_t)gridDim.x * 4 * SPW;
-    for (int64_t s = ((int64_t)blockIdx.x * 4 + wv) * SPW + grp; s < A.ns; s += stride) {
+    const int64_t stride = (int64_t)gridDim.x * kBWaves * SPW;
+    for (int64_t s = ((int64_t)blockIdx.x * kBWaves + wv) * SPW + grp; s < A.ns; s += stride) {
         const int t = (int)(s / A.nc);
         const int cell = (int)(s - (int64_t)t * A.nc);
         const int colbase = (t + 1) * A.nc;
@@ -201,27 +206,39 @@ __global__ __launch_bounds__(256) void spmm_vi_sweep(const BatchArgs A) {
     }
     __syncthreads();
     if (tid < KB) {
-        const float m = fmaxf(fmaxf(red[0][tid], red[1][tid]), fmaxf(red[2][tid], red[3][tid]));
-        atomicMax(&A.res[(k & 1u) * KP + blockIdx.y * KB + tid], __float_as_uint(m));
+        float m = 0.0f;
+#pragma unroll
+        for (int w = 0; w < kBWaves; w++) m = fmaxf(m, red[w][tid]);
+        A.part[(int64_t)blockIdx.x * KP + blockIdx.y * KB + tid] = m;
     }
     __threadfence();
     __syncthreads();
-    if (tid == 0) {
-        const uint32_t ticket = atomicAdd(&st->done, 1u);
-        is_last = ticket == gridDim.x * gridDim.y - 1;
-    }
+    if (tid == 0) is_last = atomicAdd(&st->done, 1u) == gridDim.x * gridDim.y - 1;
     __syncthreads();
     if (is_last) {
         __threadfence();
+        // column j's residual = max over the x-blocks of part[bx][j]; thread t takes
+        // column t % KP and every (kBThreads/KP)-th x-block, then LDS combines
+        const int groups = kBThreads / KP;
+        {
+            const int j = tid % KP, g0 = tid / KP;
+            float r = 0.0f;
+#pragma unroll 8
+            for (unsigned bx = g0; bx < gridDim.x; bx += groups)
+                r = fmaxf(r, __hip_atomic_load(A.part + (int64_t)bx * KP + j, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT));
+            fin[tid] = r;
+        }
+        __syncthreads();
         int running = 0;
-        for (int j = tid; j < KP; j += blockDim.x) {
-            const float r = __uint_as_float(atomicAdd(&A.res[(k & 1u) * KP + j], 0u));
+        for (int j = tid; j < KP; j += kBThreads) {
+            float r = 0.0f;
+            for (int g2 = 0; g2 < groups; g2++) r = fmaxf(r, fin[g2 * KP + j]);
             if (A.conv[j] < 0) {
                 A.last_res[j] = r;
                 if (r < A.thresh) A.conv[j] = (int32_t)k;
                 else running++;
             }
-            A.res[((k + 1u) & 1u) * KP + j] = 0u;
         }
         const int n_running = __syncthreads_count(running > 0);
         if (tid == 0) {
@@ -233,14 +250,12 @@ __global__ __launch_bounds__(256) void spmm_vi_sweep(const BatchArgs A) {
 }
 
 __global__ void batch_init(float* V0, int64_t n, const int32_t* goal, int32_t* conv, int32_t* start,
-                           uint32_t* res, float* last_res, int kp, int k, BatchState* st) {
+                           float* last_res, int kp, int k, BatchState* st) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) V0[i] = 0.0f;
     if (i < kp) {
         conv[i] = i < k ? -1 : 0;  // padding columns never run
         start[i] = 1;
-        res[i] = 0u;
-        res[kp + i] = 0u;
         last_res[i] = 0.0f;
     }
     if (i == 0) {
@@ -287,13 +302,13 @@ struct BatchBufs {
     int32_t* goal = nullptr;
     int32_t* conv = nullptr;
     int32_t* start = nullptr;  // sweep index at which each column was admitted
-    uint32_t* res = nullptr;
+    float* part = nullptr;     // [kBMaxBlocks][kp]
     float* last_res = nullptr;
     BatchState* st = nullptr;
     float* vout = nullptr;
     int32_t* pout = nullptr;
     void release() {
-        void* ps[] = {V[0], V[1], pi, goal, conv, start, res, last_res, st, vout, pout};
+        void* ps[] = {V[0], V[1], pi, goal, conv, start, part, last_res, st, vout, pout};
         for (void* p : ps)
             if (p) (void)hipFree(p);
         *this = BatchBufs();
@@ -323,7 +338,7 @@ static e2e_status alloc_bufs(const Model* m, BatchBufs& b, int kp) {
     if (hipMalloc(&b.V[0], n * 4) != hipSuccess || hipMalloc(&b.V[1], n * 4) != hipSuccess ||
         hipMalloc(&b.pi, n * 4) != hipSuccess || hipMalloc(&b.goal, kp * 4) != hipSuccess ||
         hipMalloc(&b.conv, kp * 4) != hipSuccess || hipMalloc(&b.start, kp * 4) != hipSuccess ||
-        hipMalloc(&b.res, 2 * kp * 4) != hipSuccess ||
+        hipMalloc(&b.part, (size_t)kBMaxBlocks * kp * 4) != hipSuccess ||
         hipMalloc(&b.last_res, kp * 4) != hipSuccess || hipMalloc(&b.st, sizeof(BatchState)) != hipSuccess ||
         hipMalloc(&b.vout, n * 4) != hipSuccess || hipMalloc(&b.pout, n * 4) != hipSuccess) {
         (void)hipGetLastError();
@@ -358,7 +373,7 @@ static BatchArgs make_args(const Model* m, const BatchBufs& b, const e2e_solve_o
     A.pi = b.pi;
     A.goal = b.goal;
     A.conv = b.conv;
-    A.res = b.res;
+    A.part = b.part;
     A.last_res = b.last_res;
     A.st = b.st;
     A.ns = m->ns;
@@ -375,9 +390,9 @@ static BatchArgs make_args(const Model* m, const BatchBufs& b, const e2e_solve_o
 }
 
 static dim3 grid_of(const Model* m, const Variant& v, int kp) {
-    const int64_t states_per_block = 4 * v.spw;
+    const int64_t states_per_block = kBWaves * v.spw;
     int64_t gx = ceil_div(m->ns, states_per_block);
-    gx = std::min<int64_t>(gx, 4096);  // grid-stride beyond this
+    gx = std::min<int64_t>(gx, kBMaxBlocks);  // grid-stride beyond this
     return dim3((unsigned)gx, (unsigned)(kp / kb_of(v)));
 }
 
@@ -413,7 +428,7 @@ e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve
     const int64_t nV = m->ns * kp;
     auto init = [&]() {
         hipLaunchKernelGGL(batch_init, dim3((unsigned)ceil_div(nV, 256)), dim3(256), 0, s, b.V[0], nV, b.goal,
-                           b.conv, b.start, b.res, b.last_res, kp, k, b.st);
+                           b.conv, b.start, b.last_res, kp, k, b.st);
     };
     const int max_sweeps = opts.max_sweeps > 0 ? opts.max_sweeps : (int)m->nt + 2;
     const int spc = opts.sweeps_per_check > 0 ? opts.sweeps_per_check : 8;
@@ -436,7 +451,7 @@ e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve
                 if (kp % kb_of(v)) continue;
                 init();
                 E2E_HIP(hipEventRecord(e0, s));
-                for (int i = 0; i < trial; i++) hipLaunchKernelGGL(v.fn, grid_of(m, v, kp), dim3(256), 0, s, A);
+                for (int i = 0; i < trial; i++) hipLaunchKernelGGL(v.fn, grid_of(m, v, kp), dim3(kBThreads), 0, s, A);
                 E2E_HIP(hipEventRecord(e1, s));
                 E2E_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;
@@ -463,7 +478,7 @@ e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve
         }
         hipGraph_t gr;
         E2E_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < spc; i++) hipLaunchKernelGGL(V.fn, grid, dim3(256), 0, s, A);
+        for (int i = 0; i < spc; i++) hipLaunchKernelGGL(V.fn, grid, dim3(kBThreads), 0, s, A);
         E2E_HIP(hipStreamEndCapture(s, &gr));
         hipError_t e = hipGraphInstantiate(&c->graph, gr, nullptr, nullptr, 0);
         (void)hipGraphDestroy(gr);
@@ -490,7 +505,7 @@ e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve
         if (bopt.use_graph && n == spc) {
             E2E_HIP(hipGraphLaunch(c->graph, s));
         } else {
-            for (int i = 0; i < n; i++) hipLaunchKernelGGL(V.fn, grid, dim3(256), 0, s, A);
+            for (int i = 0; i < n; i++) hipLaunchKernelGGL(V.fn, grid, dim3(kBThreads), 0, s, A);
             E2E_HIP(hipGetLastError());
         }
         E2E_HIP(hipEventRecord(e1, s));
@@ -583,7 +598,7 @@ __global__ void live_column(const float* V0, const float* V1, const int32_t* pi,
 }
 
 __global__ void live_init(float* V0, float* V1, int64_t n, int32_t* goal, int32_t* conv, int32_t* start,
-                          uint32_t* res, float* last_res, int kp, BatchState* st) {
+                          float* last_res, int kp, BatchState* st) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         V0[i] = 0.0f;
@@ -593,8 +608,6 @@ __global__ void live_init(float* V0, float* V1, int64_t n, int32_t* goal, int32_
         goal[i] = -1;
         conv[i] = 0;  // free slot
         start[i] = 1;
-        res[i] = 0u;
-        res[kp + i] = 0u;
         last_res[i] = 0.0f;
     }
     if (i == 0) {
@@ -618,7 +631,7 @@ struct LiveBatch {
 };
 
 int64_t live_batch_bytes(const Model* m, int kp) {
-    return (int64_t)m->ns * kp * 4 * 5 + (int64_t)kp * 40 + 4096;
+    return (int64_t)m->ns * kp * 4 * 5 + (int64_t)kp * (40 + 4 * kBMaxBlocks) + 4096;
 }
 
 e2e_status live_create(Model* m, int kp, float thresh, hipStream_t s, LiveBatch** out) {
@@ -647,7 +660,7 @@ e2e_status live_create(Model* m, int kp, float thresh, hipStream_t s, LiveBatch*
     L->grid = grid_of(m, kVariants[L->variant], kp);
     const int64_t n = m->ns * kp;
     hipLaunchKernelGGL(live_init, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, L->b.V[0], L->b.V[1], n,
-                       L->b.goal, L->b.conv, L->b.start, L->b.res, L->b.last_res, kp, L->b.st);
+                       L->b.goal, L->b.conv, L->b.start, L->b.last_res, kp, L->b.st);
     E2E_HIP(hipGetLastError());
     L->bytes = live_batch_bytes(m, kp);
     *out = L;
@@ -686,7 +699,7 @@ e2e_status live_admit(LiveBatch* L, const int32_t* cols, const int32_t* goals, i
 
 e2e_status live_step(LiveBatch* L, int nsweeps) {
     for (int i = 0; i < nsweeps; i++)
-        hipLaunchKernelGGL(kVariants[L->variant].fn, L->grid, dim3(256), 0, L->s, L->A);
+        hipLaunchKernelGGL(kVariants[L->variant].fn, L->grid, dim3(kBThreads), 0, L->s, L->A);
     E2E_HIP(hipGetLastError());
     return E2E_OK;
 }

@@ -16,10 +16,12 @@
 // product", SIAM J. Sci. Comput. 26(6), 2005), so Q carries about one float rounding;
 // values stay within 1e-5 relative of a float64 solve over 200-step horizons.
 //
-// Convergence without host round trips: the device state holds the sweep index k,
-// the residual of the running sweep (atomicMax on the bits of a non-negative float),
-// and `conv`, the first sweep whose residual fell below thresh. The last block of a
-// sweep (ticket counter) publishes the residual, sets conv, and advances k. A launch
+// Convergence without host round trips: the device state holds the sweep index k and
+// `conv`, the first sweep whose residual fell below thresh. Each block stores its
+// residual maximum (plain store); the last block of a sweep (ticket counter) reduces
+// them, publishes the residual, sets conv, and advances k. Blocks are 1024 threads and
+// at most 512 per grid (grid-stride over states), so one sweep costs at most 512
+// same-address atomics. A launch
 // that finds conv >= 0 returns at once, so a fixed chain of launches (eager or a
 // captured hipGraph) stops doing work exactly where the reference loop stops
 // (spvi_2.cu:669-698).
@@ -40,11 +42,16 @@ struct VIArgs {
     float* V1;
     int32_t* pi;
     VIState* st;
+    float* part;   // [gridDim.x] per-block residual maxima of the running sweep
     int64_t ns;
     int na;
     float thresh;
     float discount;
 };
+
+constexpr int kVIThreads = 1024;  // 16 waves per block
+constexpr int kVIWaves = kVIThreads / 64;
+constexpr int kVIMaxBlocks = 512;  // 2 blocks per CU
 
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
     s = a + b;
@@ -53,113 +60,132 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
 }
 
 template <int LPR, int NAP>
-__global__ __launch_bounds__(256) void spmv_vi_sweep(const VIArgs A) {
+__global__ __launch_bounds__(kVIThreads) void spmv_vi_sweep(const VIArgs A) {
     constexpr int G = LPR * NAP;
     constexpr int SPW = 64 / G;
-    __shared__ float wmax[4];
+    __shared__ float wmax[kVIWaves];
+    __shared__ int is_last;
     VIState* st = A.st;
     if (st->conv >= 0) return;  // converged at an earlier sweep
     const uint32_t k = st->k;
     const float* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
     float* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int64_t s = ((int64_t)blockIdx.x * 4 + wv) * SPW + lane / G;
     const int a = (lane % G) / LPR;
     const int l = lane % LPR;
-    const bool valid = (s < A.ns) && (a < A.na);
-    float sh = 0.0f, sl = 0.0f, rr = 0.0f;
-    if (valid) {
-        const int64_t row = s * A.na + a;
-        const uint32_t beg = A.rp[row], end = A.rp[row + 1];
-        rr = A.R[row];
-        uint32_t j = beg + l;
-        // two independent loads in flight per lane
-        for (; j + LPR < end; j += 2 * LPR) {
-            const int2 e0 = A.nz[j];
-            const int2 e1 = A.nz[j + LPR];
-            const float v0 = __int_as_float(e0.y), v1 = __int_as_float(e1.y);
-            const float x0 = Vin[e0.x], x1 = Vin[e1.x];
-            float p, pe, t, err;
-            p = v0 * x0;
-            pe = fmaf(v0, x0, -p);
-            two_sum(sh, p, t, err);
-            sh = t;
-            sl = sl + (err + pe);
-            p = v1 * x1;
-            pe = fmaf(v1, x1, -p);
-            two_sum(sh, p, t, err);
-            sh = t;
-            sl = sl + (err + pe);
-        }
-        if (j < end) {
-            const int2 e0 = A.nz[j];
-            const float v0 = __int_as_float(e0.y);
-            const float x0 = Vin[e0.x];
-            float p = v0 * x0, pe = fmaf(v0, x0, -p), t, err;
-            two_sum(sh, p, t, err);
-            sh = t;
-            sl = sl + (err + pe);
-        }
-    }
-#pragma unroll
-    for (int o = 1; o < LPR; o <<= 1) {
-        const float h2 = __shfl_xor(sh, o, 64), l2 = __shfl_xor(sl, o, 64);
-        float t, err;
-        two_sum(sh, h2, t, err);
-        sh = t;
-        sl = sl + (l2 + err);
-    }
-    if (A.discount != 1.0f) {
-        sh *= A.discount;
-        sl *= A.discount;
-    }
-    float q;
-    {
-        float t, err;
-        two_sum(rr, sh, t, err);
-        q = t + (err + sl);
-    }
-    if (!valid || q != q) q = -INFINITY;
-    int ba = valid ? a : 0x7fffffff;
-#pragma unroll
-    for (int o = LPR; o < G; o <<= 1) {
-        const float q2 = __shfl_xor(q, o, 64);
-        const int a2 = __shfl_xor(ba, o, 64);
-        if (q2 > q || (q2 == q && a2 < ba)) {
-            q = q2;
-            ba = a2;
-        }
-    }
     float res = 0.0f;
-    if (s < A.ns && (lane % G) == 0) {
-        float vnew;
-        int32_t pol;
-        if (q > -1e6f) {
-            vnew = q;
-            pol = ba;
-        } else {
-            vnew = -1e6f;
-            pol = -1;
+    // grid-stride over state groups: a few thousand blocks, so the per-sweep residual
+    // and ticket atomics stay few
+    const int64_t stride = (int64_t)gridDim.x * kVIWaves * SPW;
+    for (int64_t s = ((int64_t)blockIdx.x * kVIWaves + wv) * SPW + lane / G; s - lane / G < A.ns; s += stride) {
+        const bool valid = (s < A.ns) && (a < A.na);
+        float sh = 0.0f, sl = 0.0f, rr = 0.0f;
+        if (valid) {
+            const int64_t row = s * A.na + a;
+            const uint32_t beg = A.rp[row], end = A.rp[row + 1];
+            rr = A.R[row];
+            uint32_t j = beg + l;
+            // two independent loads in flight per lane
+            for (; j + LPR < end; j += 2 * LPR) {
+                const int2 e0 = A.nz[j];
+                const int2 e1 = A.nz[j + LPR];
+                const float v0 = __int_as_float(e0.y), v1 = __int_as_float(e1.y);
+                const float x0 = Vin[e0.x], x1 = Vin[e1.x];
+                float p, pe, t, err;
+                p = v0 * x0;
+                pe = fmaf(v0, x0, -p);
+                two_sum(sh, p, t, err);
+                sh = t;
+                sl = sl + (err + pe);
+                p = v1 * x1;
+                pe = fmaf(v1, x1, -p);
+                two_sum(sh, p, t, err);
+                sh = t;
+                sl = sl + (err + pe);
+            }
+            if (j < end) {
+                const int2 e0 = A.nz[j];
+                const float v0 = __int_as_float(e0.y);
+                const float x0 = Vin[e0.x];
+                float p = v0 * x0, pe = fmaf(v0, x0, -p), t, err;
+                two_sum(sh, p, t, err);
+                sh = t;
+                sl = sl + (err + pe);
+            }
         }
-        Vout[s] = vnew;
-        A.pi[s] = pol;
-        res = fabsf(vnew - Vin[s]);
+#pragma unroll
+        for (int o = 1; o < LPR; o <<= 1) {
+            const float h2 = __shfl_xor(sh, o, 64), l2 = __shfl_xor(sl, o, 64);
+            float t, err;
+            two_sum(sh, h2, t, err);
+            sh = t;
+            sl = sl + (l2 + err);
+        }
+        if (A.discount != 1.0f) {
+            sh *= A.discount;
+            sl *= A.discount;
+        }
+        float q;
+        {
+            float t, err;
+            two_sum(rr, sh, t, err);
+            q = t + (err + sl);
+        }
+        if (!valid || q != q) q = -INFINITY;
+        int ba = valid ? a : 0x7fffffff;
+#pragma unroll
+        for (int o = LPR; o < G; o <<= 1) {
+            const float q2 = __shfl_xor(q, o, 64);
+            const int a2 = __shfl_xor(ba, o, 64);
+            if (q2 > q || (q2 == q && a2 < ba)) {
+                q = q2;
+                ba = a2;
+            }
+        }
+        if (s < A.ns && (lane % G) == 0) {
+            float vnew;
+            int32_t pol;
+            if (q > -1e6f) {
+                vnew = q;
+                pol = ba;
+            } else {
+                vnew = -1e6f;
+                pol = -1;
+            }
+            Vout[s] = vnew;
+            A.pi[s] = pol;
+            res = fmaxf(res, fabsf(vnew - Vin[s]));
+        }
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) res = fmaxf(res, __shfl_xor(res, o, 64));
     if (lane == 0) wmax[wv] = res;
     __syncthreads();
     if (tid == 0) {
-        const float bm = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
-        atomicMax(&st->res[k & 1u], __float_as_uint(bm));
+        float bm = 0.0f;
+#pragma unroll
+        for (int w = 0; w < kVIWaves; w++) bm = fmaxf(bm, wmax[w]);
+        A.part[blockIdx.x] = bm;
         __threadfence();
-        const uint32_t ticket = atomicAdd(&st->done, 1u);
-        if (ticket == gridDim.x - 1) {
-            __threadfence();
-            const float r = __uint_as_float(atomicAdd(&st->res[k & 1u], 0u));
-            st->last_res = r;
-            if (r < A.thresh) st->conv = (int32_t)k;
-            st->res[(k + 1u) & 1u] = 0u;
+        is_last = atomicAdd(&st->done, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (is_last) {  // the last block of the sweep: reduce the partials, publish
+        __threadfence();
+        float r = 0.0f;
+        for (unsigned b = tid; b < gridDim.x; b += kVIThreads)
+            r = fmaxf(r, __hip_atomic_load(A.part + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) r = fmaxf(r, __shfl_xor(r, o, 64));
+        __syncthreads();
+        if (lane == 0) wmax[wv] = r;
+        __syncthreads();
+        if (tid == 0) {
+            float m = 0.0f;
+#pragma unroll
+            for (int w = 0; w < kVIWaves; w++) m = fmaxf(m, wmax[w]);
+            st->last_res = m;
+            if (m < A.thresh) st->conv = (int32_t)k;
             st->done = 0u;
             st->k = k + 1u;
         }
@@ -172,8 +198,6 @@ __global__ void vi_init(float* V0, int64_t ns, VIState* st) {
     if (i == 0) {
         st->k = 1u;
         st->conv = -1;
-        st->res[0] = 0u;
-        st->res[1] = 0u;
         st->done = 0u;
         st->last_res = 0.0f;
     }
@@ -239,7 +263,8 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
         if (hipMalloc(&m->V[0], sizeof(float) * ns) != hipSuccess ||
             hipMalloc(&m->V[1], sizeof(float) * ns) != hipSuccess ||
             hipMalloc(&m->pi, sizeof(int32_t) * ns) != hipSuccess ||
-            hipMalloc(&m->st, sizeof(VIState)) != hipSuccess) {
+            hipMalloc(&m->st, sizeof(VIState)) != hipSuccess ||
+            hipMalloc(&m->vi_part, sizeof(float) * kVIMaxBlocks) != hipSuccess) {
             set_error("vi_solve: device allocation failed");
             return E2E_ERR_NOMEM;
         }
@@ -249,10 +274,9 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     int lpr = 1;
     SweepFn fn = pick_sweep((int)m->na, lpr_req, &lpr);
     const int G = next_pow2((int)m->na) * lpr;
-    const int64_t states_per_block = 4 * (64 / G);
-    const int64_t nblocks = ceil_div(ns, states_per_block);
-    E2E_CHECK(nblocks < (1ll << 31), E2E_ERR_CAPACITY, "too many states");
-    VIArgs A{m->row_ptr, m->nz, m->R, m->V[0], m->V[1], m->pi, m->st, ns, (int)m->na,
+    const int64_t states_per_block = kVIWaves * (64 / G);
+    const int64_t nblocks = std::min<int64_t>(ceil_div(ns, states_per_block), kVIMaxBlocks);
+    VIArgs A{m->row_ptr, m->nz, m->R, m->V[0], m->V[1], m->pi, m->st, m->vi_part, ns, (int)m->na,
              opts.thresh, opts.discount};
 
     hipLaunchKernelGGL(vi_init, dim3((unsigned)ceil_div(ns, 256)), dim3(256), 0, s, m->V[0], ns, m->st);
@@ -268,7 +292,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
             hipGraph_t gr;
             E2E_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             for (int i = 0; i < spc; i++)
-                hipLaunchKernelGGL(fn, dim3((unsigned)nblocks), dim3(256), 0, s, A);
+                hipLaunchKernelGGL(fn, dim3((unsigned)nblocks), dim3(kVIThreads), 0, s, A);
             E2E_HIP(hipStreamEndCapture(s, &gr));
             hipError_t e = hipGraphInstantiate(&m->graph, gr, nullptr, nullptr, 0);
             (void)hipGraphDestroy(gr);
@@ -297,7 +321,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
         if (opts.use_graph && n == spc) {
             E2E_HIP(hipGraphLaunch(m->graph, s));
         } else {
-            for (int i = 0; i < n; i++) hipLaunchKernelGGL(fn, dim3((unsigned)nblocks), dim3(256), 0, s, A);
+            for (int i = 0; i < n; i++) hipLaunchKernelGGL(fn, dim3((unsigned)nblocks), dim3(kVIThreads), 0, s, A);
             E2E_HIP(hipGetLastError());
         }
         E2E_HIP(hipEventRecord(e1, s));

@@ -345,17 +345,21 @@ static void blk_push(blk* b, int32_t k, int32_t n) {
 
 /* The reward sum of a segment. The reference adds the per-lane rewards with float
  * atomicAdd in an unspecified order (:553). The canonical order fixed here (and used by
- * the GPU kernel): 256 double partial sums, partial[w % 256] accumulating lanes w in
- * ascending order, then the tree partial[i] += partial[i + o] for o = 128, 64, ..., 1.
- * For integer-valued rewards every order gives the same exact sum. */
-#define RSUM_WIDTH 256
+ * the GPU kernel), in double: realizations are taken in chunks of 64 consecutive w;
+ * within a chunk, p[l] = r[64c + l] (0 past nr) is reduced by the tree
+ * p[l] += p[l + o] for o = 32, 16, ..., 1 (result p[0]); the chunk sums are then added
+ * in ascending chunk order starting from 0. For integer-valued rewards every order
+ * gives the same exact sum. */
 static double canonical_sum(const float* r, int nr) {
-    double part[RSUM_WIDTH];
-    for (int i = 0; i < RSUM_WIDTH; i++) part[i] = 0.0;
-    for (int w = 0; w < nr; w++) part[w % RSUM_WIDTH] += (double)r[w];
-    for (int o = RSUM_WIDTH / 2; o >= 1; o >>= 1)
-        for (int i = 0; i < o; i++) part[i] = part[i] + part[i + o];
-    return part[0];
+    double total = 0.0;
+    for (int c = 0; c * 64 < nr; c++) {
+        double p[64];
+        for (int l = 0; l < 64; l++) p[l] = (c * 64 + l < nr) ? (double)r[c * 64 + l] : 0.0;
+        for (int o = 32; o >= 1; o >>= 1)
+            for (int l = 0; l < o; l++) p[l] = p[l] + p[l + o];
+        total = total + p[0];
+    }
+    return total;
 }
 
 /* Count-by-key of one segment (prel_tests/build_model.cu:410-487 semantics: sort the

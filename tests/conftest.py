@@ -29,6 +29,9 @@ _ensure_built()
 
 @pytest.fixture(scope="session")
 def gpu():
+    # torch brings its own HIP runtime; it must initialise before libe2edp binds one
+    import torch
+    torch.cuda.is_available()
     import e2e_gpu_dp_amd as e2e
     if e2e.device_count() < 1:
         pytest.fail("no HIP device visible (gpu tests must run on the GPU box)")
