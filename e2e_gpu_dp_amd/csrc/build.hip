@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "internal.h"
@@ -644,8 +645,10 @@ __global__ void import_rows(const uint32_t* __restrict__ rp_am, const uint32_t* 
 
 struct DevBuf {
     void* p = nullptr;
-    ~DevBuf() {
+    ~DevBuf() { reset(); }
+    void reset() {
         if (p) (void)hipFree(p);
+        p = nullptr;
     }
     template <typename T>
     T* as() const { return reinterpret_cast<T*>(p); }
@@ -753,6 +756,14 @@ e2e_status validate_params(const e2e_params* p) {
 e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build_opts* o,
                        Model** out) {
     const double t_start = now_ms();
+    const bool trace = std::getenv("E2E_TRACE") != nullptr;
+    double t_mark = t_start;
+    auto phase = [&](const char* what) {
+        if (!trace) return;
+        const double t = now_ms();
+        std::fprintf(stderr, "[e2e build] %-28s %9.3f ms\n", what, t - t_mark);
+        t_mark = t;
+    };
     E2E_TRY(validate_params(p));
     E2E_CHECK(f && f->u && f->v && f->yi && (p->nmodes == 0 || (f->ui && f->vi)), E2E_ERR_INVALID,
               "fields u, v, yi (and ui, vi when nmodes > 0) are required");
@@ -834,6 +845,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
                            dim3(256), 0, s, mask, nc, dFlags.as<int32_t>());
     }
 
+    phase("fields + setup");
     A.u = u; A.v = v; A.ui = ui; A.vi = vi; A.yi = yi; A.s = sf; A.mask = mask;
     A.slice_obs = mask ? dFlags.as<int32_t>() : nullptr;
     A.xs = dXY.as<float>();
@@ -892,6 +904,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     E2E_CHECK(lds <= 160 * 1024, E2E_ERR_CAPACITY, "build kernel needs %zu B of LDS (nrzns too large)", lds);
     m->info.n_multipass_rows = 0;
 
+    phase("row buffers allocated");
     bool any_obs = false;
     if (mask) {
         std::vector<int32_t> flags(nt);
@@ -952,8 +965,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
         // arena exhausted: grow and rebuild
         unsigned long long used = 0;
         E2E_HIP(hipMemcpy(&used, dTop.p, sizeof(used), hipMemcpyDeviceToHost));
-        (void)hipFree(dArena.p);
-        dArena.p = nullptr;
+        dArena.reset();
         arena_cap = std::max(arena_cap * 4, used + 1024);
         if (attempt == 3) {
             set_error("overflow arena exhausted after 4 attempts");
@@ -963,6 +975,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st) return fail(st);
+    phase("move/sort/count kernel");
     m->info.move_kernel_ms = move_ms;
     m->info.move_launches = launches;
 
@@ -989,6 +1002,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     E2E_CHECK(total < (1ull << 32) && total < (1ull << 31), E2E_ERR_CAPACITY,
               "nnz=%llu exceeds the 2^31 column-index range", total);
     m->nnz = (int64_t)total;
+    phase("row-count scan");
     if (hipMalloc(&m->nz, sizeof(int2) * (total ? total : 1)) != hipSuccess) {
         set_error("hipMalloc nz (%llu entries) failed", total);
         return fail(E2E_ERR_NOMEM);
@@ -1005,9 +1019,15 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     (void)hipEventDestroy(c0);
     (void)hipEventDestroy(c1);
     (void)hipEventDestroy(c2);
+    phase("CSR allocated + scattered");
     m->info.scan_ms = ms1;
     m->info.csr_ms = ms2;
     m->info.lanes = (nt - 1) * nc * na * (int64_t)nr;
+    // release the staging buffers inside the timed build
+    dStage.reset();
+    dArena.reset();
+    dNnz.reset();
+    phase("staging freed");
     m->info.build_ms = now_ms() - t_start;
     *out = m;
     return E2E_OK;
