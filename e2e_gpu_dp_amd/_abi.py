@@ -54,7 +54,8 @@ class ModelInfo(C.Structure):
         "ns", "na", "nnz", "nc", "nt", "lanes", "n_out_of_window", "max_row_nnz",
         "n_overflow_rows", "n_multipass_rows", "device_bytes")] + \
         [(n, C.c_double) for n in ("build_ms", "move_kernel_ms", "scan_ms", "csr_ms")] + \
-        [("has_rsum", C.c_int32), ("move_launches", C.c_int32)]
+        [("has_rsum", C.c_int32), ("move_launches", C.c_int32), ("block_threads", C.c_int32),
+         ("unit_cells", C.c_int32)]
 
 
 class SolveOpts(C.Structure):

@@ -119,6 +119,12 @@ __device__ bool crosses_obstacle(const BuildArgs& A, int i1, int j1, int i2, int
 }
 
 // move :268-356 for a lane that starts at the centre (xc, yc) of cell (pi, pj).
+// UNIT: cells of width exactly 1 (Dj == Di == 1.0f, the reference's dx = dy = 1). Then
+// remainderf(x, 1) == x - rint(x) exactly (the IEEE remainder's quotient is
+// round-half-even(x / 1), and x minus its nearest integer is exact), (x - rem) / 1 ==
+// rint(x) exactly, and the double compare rem >= 0.5 * Dj equals rem >= 0.5f: the same
+// bits as the general path at a fraction of its cost.
+template <bool UNIT>
 __device__ __forceinline__ void lane_move(const BuildArgs& A, float xc, float yc, float fc,
                                           float fs, float vx, float vy, int T, int& pi,
                                           int& pj, float& r) {
@@ -144,14 +150,22 @@ __device__ __forceinline__ void lane_move(const BuildArgs& A, float xc, float yc
     }
     const float ax = xnew - A.xs0;
     const float ay = -(ynew - A.ysn);
-    const float remx = remainderf(ax, A.Dj);
-    const float remy = remainderf(ay, A.Di);
-    int xind = (int)((ax - remx) / A.Dj);
-    int yind = (int)((ay - remy) / A.Di);
-    // the if/else-if ladder of :318-330 adds one to each index whose remainder is at
-    // least half a cell (compared in double)
-    xind += ((double)remx >= A.hDj) ? 1 : 0;
-    yind += ((double)remy >= A.hDi) ? 1 : 0;
+    int xind, yind;
+    if (UNIT) {
+        const float nx = rintf(ax), ny = rintf(ay);
+        const float remx = ax - nx, remy = ay - ny;
+        xind = (int)nx + (remx >= 0.5f ? 1 : 0);
+        yind = (int)ny + (remy >= 0.5f ? 1 : 0);
+    } else {
+        const float remx = remainderf(ax, A.Dj);
+        const float remy = remainderf(ay, A.Di);
+        xind = (int)((ax - remx) / A.Dj);
+        yind = (int)((ay - remy) / A.Di);
+        // the if/else-if ladder of :318-330 adds one to each index whose remainder is at
+        // least half a cell (compared in double)
+        xind += ((double)remx >= A.hDj) ? 1 : 0;
+        yind += ((double)remy >= A.hDi) ? 1 : 0;
+    }
     pi = yind;
     pj = xind;
     if (pi == 0 || pi == n || pj == 0 || pj == n) r += A.r_out;
@@ -246,7 +260,7 @@ __host__ __device__ inline size_t build_lds_bytes(int WG, int R, int D, int nm, 
 //       when nbits <= radix_bits one counting pass orders the keys and its digit
 //       histogram is the count-by-key, so the scatter is skipped;
 //   write the row's distinct S2 with their counts and the row reward sum.
-template <int WG, bool OBS>
+template <int WG, bool OBS, bool UNIT>
 __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
     constexpr int NW = WG / 64;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -261,6 +275,7 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
     float2* vxy = reinterpret_cast<float2*>(cnt + NW * NBMAX);   // [NRP]
     uint32_t* keysA = reinterpret_cast<uint32_t*>(vxy + NRP);    // [NRP + 4]
     uint32_t* keysB = keysA + NRP + 4;                           // [NRP + 4]
+    float* rwl = reinterpret_cast<float*>(keysB);                // lane rewards (until the sum is formed)
     float* ylds = reinterpret_cast<float*>(keysB + NRP + 4);     // [nm][nr] when staged
     unsigned long long* arena_slot = reinterpret_cast<unsigned long long*>(misc + 160);
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
@@ -312,7 +327,8 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
         for (int a = 0; a < A.na; a++) {
             const float speed = A.act_speed[a], fc = A.act_fcos[a], fs = A.act_fsin[a];
             int dimin = 0x7fffffff, dimax = -0x7fffffff, djmin = 0x7fffffff, djmax = -0x7fffffff;
-            int oow = 0;
+            int oow = 0, ipart = 0;
+            bool allint = true;
             // ---- (a) move: transition_calc :522-545
             for (int r = 0; r < R; r++) {
                 const int w = r * WG + tid;
@@ -323,7 +339,7 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
                     if (!term1) {
                         if (!obs1) {
                             const float2 v = vxy[w];
-                            lane_move(A, xc, yc, fc, fs, v.x, v.y, t, pi, pj, rw);
+                            lane_move<UNIT>(A, xc, yc, fc, fs, v.x, v.y, t, pi, pj, rw);
                             if (pi < 0 || pi >= g || pj < 0 || pj >= g) {  // impossible for dx, dy > 0;
                                 bad_index = true;                          // guards the gathers below
                                 pi = min(max(pi, 0), g - 1);
@@ -345,42 +361,60 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
                     if (di + del < 0 || di + del >= A.m || dj + del < 0 || dj + del >= A.m) oow++;
                     keysA[w] = ((uint32_t)(di + 32768) << 16) | (uint32_t)(dj + 32768);
                 }
-                // canonical reward sum, chunk (r*NW + wv) = w in [64c, 64c+63]: lane tree
-                double p = valid ? (double)rw : 0.0;
-#pragma unroll
-                for (int o = 32; o >= 1; o >>= 1) {
-                    const double q = __shfl_down(p, o, 64);
-                    p = p + q;
+                if (valid) {
+                    rwl[w] = rw;
+                    // integer-valued rewards sum exactly in int in any order, which equals
+                    // the canonical double sum
+                    if (rw == rintf(rw) && fabsf(rw) < 1048576.0f) ipart += (int)rw;
+                    else allint = false;
                 }
-                if (lane == 0) csum[r * NW + wv] = p;
             }
             dimin = wave_min(dimin);
             dimax = wave_max(dimax);
             djmin = wave_min(djmin);
             djmax = wave_max(djmax);
             oow = wave_sum(oow);
+            ipart = wave_sum(ipart);
+            const bool wave_int = __all(allint);
             if (lane == 0) {
                 misc[wv * 8 + 0] = dimin;
                 misc[wv * 8 + 1] = dimax;
                 misc[wv * 8 + 2] = djmin;
                 misc[wv * 8 + 3] = djmax;
                 misc[wv * 8 + 4] = oow;
+                misc[wv * 8 + 5] = ipart;
+                misc[wv * 8 + 6] = wave_int ? 1 : 0;
             }
             for (int i = tid; i < NW * NBMAX; i += WG) cnt[i] = 0u;
             __syncthreads();
-            dimin = misc[0], dimax = misc[1], djmin = misc[2], djmax = misc[3];
-            int oow_blk = misc[4];
-#pragma unroll
-            for (int w = 1; w < NW; w++) {
-                dimin = min(dimin, misc[w * 8 + 0]);
-                dimax = max(dimax, misc[w * 8 + 1]);
-                djmin = min(djmin, misc[w * 8 + 2]);
-                djmax = max(djmax, misc[w * 8 + 3]);
-                oow_blk += misc[w * 8 + 4];
+            // block totals: lane l < NW reads wave l's entries, then wave reductions
+            {
+                const bool lw = lane < NW;
+                dimin = wave_min(lw ? misc[lane * 8 + 0] : 0x7fffffff);
+                dimax = wave_max(lw ? misc[lane * 8 + 1] : -0x7fffffff);
+                djmin = wave_min(lw ? misc[lane * 8 + 2] : 0x7fffffff);
+                djmax = wave_max(lw ? misc[lane * 8 + 3] : -0x7fffffff);
             }
-            double rsum = 0.0;
-            if (tid == 0)
-                for (int ch = 0; ch < nchunks; ch++) rsum = rsum + csum[ch];
+            const int oow_blk = wave_sum(lane < NW ? misc[lane * 8 + 4] : 0);
+            const long long isum = wave_sum(lane < NW ? misc[lane * 8 + 5] : 0);
+            const bool blk_int = __all(lane >= NW || misc[lane * 8 + 6] != 0);
+            double rsum = (double)isum;
+            if (!blk_int) {
+                for (int ch = wv; ch < nchunks; ch += NW) {
+                    const int w = ch * 64 + lane;
+                    double p = w < nr ? (double)rwl[w] : 0.0;
+#pragma unroll
+                    for (int o = 32; o >= 1; o >>= 1) {
+                        const double q = __shfl_down(p, o, 64);
+                        p = p + q;
+                    }
+                    if (lane == 0) csum[ch] = p;
+                }
+                __syncthreads();
+                rsum = 0.0;
+                if (tid == 0)
+                    for (int ch = 0; ch < nchunks; ch++) rsum = rsum + csum[ch];
+            }
             // ---- (b) range-compress: key = (di - dimin) * Wj + (dj - djmin) keeps the
             // (i2, j2) lexicographic order, i.e. ascending S2 = (t+1)*nc + j2 + g*i2
             const uint32_t Wj = (uint32_t)(djmax - djmin + 1);
@@ -665,43 +699,60 @@ e2e_status dmalloc(DevBuf& b, size_t bytes) {
     return E2E_OK;
 }
 
-template <int WG, bool OBS>
-e2e_status launch_build(const BuildArgs& A, dim3 grid, size_t lds, hipStream_t s) {
+// OBS: the goes_through_obstacle test (:444-477) is compiled in only for models whose
+// mask has obstacles; UNIT: the unit-cell move. Both cost registers the common kernel
+// does not need.
+using BuildFn = void (*)(const BuildArgs);
+
+template <int WG>
+BuildFn pick_build(bool obs, bool unit) {
+    if (obs) return unit ? mc_move_count<WG, true, true> : mc_move_count<WG, true, false>;
+    return unit ? mc_move_count<WG, false, true> : mc_move_count<WG, false, false>;
+}
+
+BuildFn pick_build(int wg, bool obs, bool unit) {
+    if (wg == 256) return pick_build<256>(obs, unit);
+    if (wg == 512) return pick_build<512>(obs, unit);
+    return pick_build<1024>(obs, unit);
+}
+
+e2e_status dispatch_build(const BuildArgs& A, int wg, bool obs, bool unit, dim3 grid, size_t lds,
+                          hipStream_t s) {
+    BuildFn fn = pick_build(wg, obs, unit);
     if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&mc_move_count<WG, OBS>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) {
             set_error("hipFuncSetAttribute(%zu B LDS): %s", lds, hipGetErrorString(e));
             return E2E_ERR_HIP;
         }
     }
-    hipLaunchKernelGGL((mc_move_count<WG, OBS>), grid, dim3(WG), lds, s, A);
+    hipLaunchKernelGGL(fn, grid, dim3(wg), lds, s, A);
     return E2E_OK;
 }
 
-// OBS: the goes_through_obstacle test (:444-477) is compiled in only for models whose
-// mask has obstacles; it costs registers the obstacle-free kernel does not need.
-e2e_status dispatch_build(const BuildArgs& A, int wg, bool obs, dim3 grid, size_t lds, hipStream_t s) {
-    if (obs) {
-        if (wg == 256) return launch_build<256, true>(A, grid, lds, s);
-        if (wg == 512) return launch_build<512, true>(A, grid, lds, s);
-        return launch_build<1024, true>(A, grid, lds, s);
-    }
-    if (wg == 256) return launch_build<256, false>(A, grid, lds, s);
-    if (wg == 512) return launch_build<512, false>(A, grid, lds, s);
-    return launch_build<1024, false>(A, grid, lds, s);
-}
-
-// Block size: the one that keeps the most waves resident per CU under the 160 KiB LDS
-// (capped at 32 waves), the smaller on ties.
-int choose_wg(int nr, int D, int nm, bool stage_y, int forced) {
+// Block size: the one that keeps the most waves resident per CU, given the 160 KiB of
+// LDS and the kernel's VGPR count (512 VGPRs per SIMD lane, 4 SIMDs per CU, at most 8
+// waves per SIMD); variants that spill to scratch are avoided; the smaller on ties.
+int choose_wg(int nr, int D, int nm, bool stage_y, bool obs, bool unit, int forced) {
     if (forced == 256 || forced == 512 || forced == 1024) return forced;
     int best = 256, best_w = -1;
     for (int wg : {256, 512, 1024}) {
         const int R = (nr + wg - 1) / wg;
         const size_t lds = build_lds_bytes(wg, R, D, nm, nr, stage_y);
-        const int blocks = lds > 0 ? (int)((160 * 1024) / lds) : 0;
-        const int waves = std::min(32, blocks * wg / 64);
+        int blocks = lds > 0 ? (int)((160 * 1024) / lds) : 0;
+        hipFuncAttributes fa{};
+        int per_simd = 8;
+        bool spills = false;
+        if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(pick_build(wg, obs, unit))) == hipSuccess) {
+            if (fa.numRegs > 0) per_simd = std::min(8, 512 / fa.numRegs);
+            spills = fa.localSizeBytes > 0;
+        } else {
+            (void)hipGetLastError();
+        }
+        blocks = std::min(blocks, (4 * per_simd) / (wg / 64));
+        int waves = blocks * wg / 64;
+        if (spills) waves /= 2;
         if (waves > best_w) {
             best_w = waves;
             best = wg;
@@ -879,7 +930,9 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
         set_error("hipMalloc Rsum failed");
         return fail(E2E_ERR_NOMEM);
     }
-    unsigned long long arena_cap = std::max<unsigned long long>(1ull << 20, (unsigned long long)rows / 4);
+    // overflow arena for rows with more than K distinct S2: 2 entries per row on average
+    // covers the named workloads (a retry with a larger arena re-runs the kernel)
+    unsigned long long arena_cap = std::max<unsigned long long>(1ull << 20, (unsigned long long)rows * 2);
     if ((st = dmalloc(dTop, sizeof(unsigned long long)))) return fail(st);
 
     A.staging = dStage.as<uint2>();
@@ -898,13 +951,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     const bool stage = ybytes > 0 && ybytes <= 16 * 1024;
     A.stage_y = stage ? 1 : 0;
     A.tc = stage ? 4 : 1;
-    const int wg = choose_wg(nr, D, nm, stage, opts.block_threads);
-    A.R = (int)ceil_div(nr, wg);
-    const size_t lds = build_lds_bytes(wg, A.R, D, nm, nr, stage);
-    E2E_CHECK(lds <= 160 * 1024, E2E_ERR_CAPACITY, "build kernel needs %zu B of LDS (nrzns too large)", lds);
-    m->info.n_multipass_rows = 0;
-
-    phase("row buffers allocated");
+    const bool unit = A.Dj == 1.0f && A.Di == 1.0f;
     bool any_obs = false;
     if (mask) {
         std::vector<int32_t> flags(nt);
@@ -913,6 +960,15 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
         for (int32_t fl : flags) any_obs = any_obs || fl != 0;
     }
     m->has_obstacles = any_obs;
+    const int wg = choose_wg(nr, D, nm, stage, any_obs, unit, opts.block_threads);
+    m->info.block_threads = wg;
+    m->info.unit_cells = unit ? 1 : 0;
+    A.R = (int)ceil_div(nr, wg);
+    const size_t lds = build_lds_bytes(wg, A.R, D, nm, nr, stage);
+    E2E_CHECK(lds <= 160 * 1024, E2E_ERR_CAPACITY, "build kernel needs %zu B of LDS (nrzns too large)", lds);
+    m->info.n_multipass_rows = 0;
+
+    phase("row buffers allocated");
     const int64_t nslices = nt - 1;
     const int64_t spl = opts.slices_per_launch > 0 ? opts.slices_per_launch : nslices;
     hipEvent_t e0, e1;
@@ -934,7 +990,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
             A.t0 = (int)t0;
             dim3 grid((unsigned)ceil_div(nc, A.tc), (unsigned)cnt);
             E2E_HIP(hipEventRecord(e0, s));
-            if ((st = dispatch_build(A, wg, any_obs, grid, lds, s))) break;
+            if ((st = dispatch_build(A, wg, any_obs, unit, grid, lds, s))) break;
             E2E_HIP(hipGetLastError());
             E2E_HIP(hipEventRecord(e1, s));
             E2E_HIP(hipEventSynchronize(e1));

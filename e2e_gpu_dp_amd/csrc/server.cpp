@@ -108,7 +108,7 @@ struct Server {
         const int64_t nc = (int64_t)p.gsize * p.gsize, rows = nc * p.nt * p.num_ac_speeds * p.num_ac_angles;
         const int64_t fields = 4 * nc * p.nt * (4 + 2 * (int64_t)p.nmodes) + 4 * (int64_t)p.nt * p.nrzns * p.nmodes;
         const int64_t model = rows * 12 + rows * 12 * 8;           // row_ptr, R, Rsum + nnz at ~12/row
-        const int64_t temps = rows * (16 * 8 + 4) + std::max<int64_t>(1 << 20, rows / 4) * 8;
+        const int64_t temps = rows * (16 * 8 + 4) + std::max<int64_t>(1 << 20, rows * 2) * 8;
         return fields + model + temps;
     }
 

@@ -116,6 +116,8 @@ typedef struct e2e_model_info {
     double csr_ms;             /* HIP-event time of the CSR scatter                        */
     int32_t has_rsum;
     int32_t move_launches;
+    int32_t block_threads;     /* workgroup size the build kernel ran with */
+    int32_t unit_cells;        /* 1: the unit-cell (dx = dy = 1) move was used */
 } e2e_model_info;
 
 typedef struct e2e_solve_opts {

@@ -50,6 +50,15 @@ def _case(name):
         f = make_fields(p.gsize, p.nt, p.nmodes, p.nrzns, 13, F=p.F, mean_speed_frac=9.0,
                         mode_rms_frac=6.0)
         return p, f, {}
+    if name == "cells_0p7":  # general remainderf path (cells of width 0.7)
+        w = small(g=11, nt=5, nr=300, nm=2, seed=19, dx=0.7, dy=0.7, x0=0.35, y0=0.35)
+        return w.params, w.fields(), {}
+    if name == "cells_2_wg512":  # width 2 cells, forced 512-thread blocks
+        w = small(g=9, nt=4, nr=600, nm=2, seed=23, dx=2.0, dy=2.0, x0=1.0, y0=1.0, F=2.0)
+        return w.params, w.fields(), {"block_threads": 512}
+    if name == "wg1024":
+        w = small(g=8, nt=4, nr=1500, nm=3, seed=29)
+        return w.params, w.fields(), {"block_threads": 1024}
     if name == "many_rounds":
         w = small(g=6, nt=3, nr=5000, nm=6, seed=17)
         return w.params, w.fields(), {}
@@ -57,7 +66,8 @@ def _case(name):
 
 
 CASES = ["tiny", "small", "obstacles_tsg2", "reward0", "reward1", "reward2", "reward3",
-         "reward4", "multipass", "radix2_overflow", "overflow_k1", "fast_flow_oow", "many_rounds"]
+         "reward4", "multipass", "radix2_overflow", "overflow_k1", "fast_flow_oow", "many_rounds",
+         "cells_0p7", "cells_2_wg512", "wg1024"]
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -76,6 +86,9 @@ def test_build_bit_exact(gpu, name):
             assert info["n_overflow_rows"] > 0
         if name == "fast_flow_oow":
             assert info["n_out_of_window"] > 0
+        assert info["unit_cells"] == (p.dx == 1.0 and p.dy == 1.0)
+        if "block_threads" in opts:
+            assert info["block_threads"] == opts["block_threads"]
         # the int32 export (src/convert_model_datatype.py) carries the same values
         S1, S2, val, R = m.export_coo(np.int32)
         assert (S1 == ref["S1"]).all() and (S2 == ref["S2"]).all()
