@@ -1,12 +1,22 @@
 #!/usr/bin/env python3
-"""Benchmark of the plain GPU path on one MI355X: Monte Carlo model build
-(move + LDS radix sort/count-by-key + CSR) followed by fused SpMV value iteration to
-residual 1e-6, on the 100x100 x nt=120, 16-heading, 5-mode, 2500-realization workload.
+"""Benchmark on one MI355X.
 
-One step = one end-to-end build + solve with the fields already resident in HBM.
-Prints one JSON line (rank 0). `value` = Monte Carlo transitions per second of the
-whole step (lanes / step time). Per-kernel times come from HIP events recorded inside
-the library on the stream the kernels run on.
+Headline (`value`): the plain GPU path — Monte Carlo model build (move + LDS radix
+sort/count-by-key + CSR) followed by fused SpMV value iteration to residual 1e-6 — on
+the 100x100 x nt=120, 16-heading, 5-mode, 2500-realization workload. One step = one
+end-to-end build + solve with the fields already resident in HBM; `value` = Monte Carlo
+transitions per second of the whole step.
+
+Also reported on the same line:
+  * "batched": 64 goal columns on the 25x25 x nt=20, 8-heading model, solved as one
+    SpMM-VI (autotuned, hipGraph sweeps) vs 64 sequential plain SpMV-VI solves of the
+    per-goal models, with the largest column difference (bar: 1e-5 relative);
+  * "serving": a Poisson(20/s) stream of planning requests over environments with
+    grids {50^2, 100^2, 150^2}, nt in U[60,150], 8 or 16 headings, 1000-5000
+    realizations: p50/p99 latency and solves/s of the continuous-batching server vs a
+    one-request-at-a-time queue.
+Per-kernel times come from HIP events recorded inside the library on the stream the
+kernels run on.
 """
 from __future__ import annotations
 
@@ -46,6 +56,96 @@ def _cpu_baseline(w, fields, budget_s: float):
     }
 
 
+def _batched(e2e, np, steps: int):
+    """C1 model x 64 goals: batched SpMM-VI vs 64 sequential plain SpMV-VI solves."""
+    from e2e_gpu_dp_amd.configs import goal_cells, workload
+    w = workload("C1")
+    f = w.fields()
+    p = w.params
+    goals = goal_cells(p.gsize, 64, 3456)
+    shared = e2e.build_model(p.replace(term_subgrid_size=0), f, keep_rsum=True)
+    shared.solve_goals(goals, thresh=1e-6, use_graph=True)  # autotune + graph capture
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pol_b, val_b, sw_b, st_b = shared.solve_goals(goals, thresh=1e-6, use_graph=True)
+    t_batch = (time.perf_counter() - t0) / steps
+    plains = [e2e.build_model(p.replace(i_term=int(g) // p.gsize, j_term=int(g) % p.gsize,
+                                        term_subgrid_size=1), f) for g in goals]
+    for m in plains[:2]:
+        m.solve(thresh=1e-6, use_graph=True)
+    t0 = time.perf_counter()
+    outs = [m.solve(thresh=1e-6, use_graph=True) for m in plains]
+    t_plain = time.perf_counter() - t0
+    err = 0.0
+    for j, (pp, vp, sp) in enumerate(outs):
+        v64 = vp.astype(np.float64)
+        err = max(err, float((np.abs(val_b[j] - v64) / np.maximum(np.abs(v64), 1.0)).max()))
+    for m in plains:
+        m.close()
+    shared.close()
+    return {
+        "workload": "25x25 x nt=20, 8 headings, 4 modes, 1000 realizations; 64 goal cells (seed 3456)",
+        "solves_per_s": 64 / t_batch, "plain_solves_per_s": 64 / t_plain,
+        "speedup_vs_plain": t_plain / t_batch, "batched_ms": t_batch * 1e3, "plain_64_ms": t_plain * 1e3,
+        "max_rel_diff_vs_plain": err, "variant": st_b["variant"], "cols_per_lane": st_b["cols_per_lane"],
+        "states_per_wave": st_b["states_per_wave"], "sweeps": st_b["sweeps"],
+        "sweep_kernel_ms": st_b["sweep_kernel_ms"] / max(st_b["launches"], 1),
+        "bytes_per_sweep": st_b["bytes_per_sweep"],
+    }
+
+
+def _serving(e2e, np, n_requests: int, seed: int = 5678):
+    """Poisson(20/s) planning requests over mixed environments; server vs serial queue."""
+    from e2e_gpu_dp_amd.configs import Params
+    from e2e_gpu_dp_amd.fields import make_fields
+    rng = np.random.default_rng(seed)
+    envs = {}
+    for eid in range(6):
+        g = int(rng.choice([50, 100, 150]))
+        nt = int(rng.integers(60, 151))
+        na = int(rng.choice([8, 16]))
+        nr = int(rng.integers(1000, 5001))
+        nm = 5
+        p = Params(gsize=g, nt=nt, nrzns=nr, nmodes=nm, num_ac_angles=na, term_subgrid_size=0)
+        envs[eid] = (p, make_fields(g, nt, nm, nr, seed + 1 + eid))
+    arrivals = np.cumsum(rng.exponential(1 / 20.0, n_requests))
+    reqs = []
+    for t in arrivals:
+        eid = int(rng.integers(0, len(envs)))
+        nc = envs[eid][0].nc
+        reqs.append((float(t), eid, int(rng.integers(0, nc)), int(rng.integers(0, nc))))
+
+    def run(**opts):
+        with e2e.Server(thresh=1e-6, **opts) as srv:
+            for eid, (p, f) in envs.items():
+                srv.add_env(eid, p, f)
+            t0 = time.perf_counter()
+            ids = []
+            for (t, eid, g, s0) in reqs:
+                dt = t - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                ids.append(srv.submit(eid, g, s0))
+            srv.drain(timeout_ms=600000)
+            st = srv.stats()
+            res = [srv.poll(i) for i in ids]
+        return st, res
+
+    st, res = run(max_cols=64)
+    st_q, res_q = run(max_cols=1, serial=True)
+    same = sum(1 for a, b in zip(res, res_q) if a["value_at_start"] == b["value_at_start"])
+    return {
+        "workload": f"{n_requests} requests, Poisson 20/s, 6 environments "
+                    f"(grids {{50,100,150}}^2, nt U[60,150], 8/16 headings, 1000-5000 realizations)",
+        "envs": [{"g": p.gsize, "nt": p.nt, "na": p.na, "nr": p.nrzns} for p, _ in envs.values()],
+        "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "solves_per_s": st["solves_per_s"],
+        "completed": st["completed"], "models_built": st["models_built"],
+        "queue_p50_ms": st_q["p50_ms"], "queue_p99_ms": st_q["p99_ms"],
+        "queue_solves_per_s": st_q["solves_per_s"],
+        "same_value_at_start_as_queue": f"{same}/{len(res)}",
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,6 +156,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--skip-vi", action="store_true", help="profiling aid: build only")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=120)
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -130,10 +233,16 @@ def main(argv=None):
         gbps = st["bytes_per_sweep"] / (per * 1e-3) / 1e9
         out["vi"] = {"sweeps": st["sweeps"], "solve_ms": float(np.mean([s["solve_ms"] for _, s in infos])),
                      "sweep_kernel_ms": per, "bytes_per_sweep": st["bytes_per_sweep"],
-                     "gbps": gbps, "lanes_per_row": st["lanes_per_row"]}
+                     "gbps": gbps, "loads_in_flight": st["loads_in_flight"]}
         out["roofline_vi"] = {"kernel": "spmv_vi_sweep", "bound": "hbm", "achieved": gbps,
                               "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": gbps / PEAK_HBM_GBPS,
                               "traffic": None}
+    if not args.no_batched:
+        print("[bench] batched goals", file=sys.stderr, flush=True)
+        out["batched"] = _batched(e2e, np, max(args.steps, 3))
+    if not args.no_serving:
+        print("[bench] serving", file=sys.stderr, flush=True)
+        out["serving"] = _serving(e2e, np, args.serving_requests)
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = _cpu_baseline(w, fields, args.cpu_budget)

@@ -61,7 +61,7 @@ class ModelInfo(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [
         ("thresh", C.c_float), ("discount", C.c_float), ("max_sweeps", C.c_int32),
-        ("sweeps_per_check", C.c_int32), ("use_graph", C.c_int32), ("lanes_per_row", C.c_int32),
+        ("sweeps_per_check", C.c_int32), ("use_graph", C.c_int32), ("loads_in_flight", C.c_int32),
         ("timeout_ms", C.c_double),
     ]
 
@@ -69,7 +69,7 @@ class SolveOpts(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [
         ("sweeps", C.c_int32), ("converged", C.c_int32), ("final_residual", C.c_float),
-        ("lanes_per_row", C.c_int32), ("solve_ms", C.c_double), ("sweep_kernel_ms", C.c_double),
+        ("loads_in_flight", C.c_int32), ("solve_ms", C.c_double), ("sweep_kernel_ms", C.c_double),
         ("sweep_kernel_ms_per_sweep", C.c_double), ("bytes_per_sweep", C.c_double),
         ("launches", C.c_int32), ("reserved", C.c_int32),
     ]

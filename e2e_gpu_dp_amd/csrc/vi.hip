@@ -5,11 +5,9 @@
 //   residual = max_s |V'(s) - V(s)|                            (spvi_2.cu:129-171,263-356)
 // fused, with Q never materialised and the convergence test done on the device.
 //
-// Mapping (CDNA wave64): a state's na rows are contiguous in the state-major CSR, so
-// a group of NAP*LPR lanes (NAP = next power of two >= na) takes one state: LPR lanes
-// per action row stride that row's nnz, their partial sums combine with xor shuffles,
-// then an xor-butterfly over the action lanes gives max/argmax. 64/(NAP*LPR) states
-// per wave, 4 waves per block.
+// Mapping (CDNA wave64): a state's na rows are contiguous in the state-major CSR, so a
+// wave takes 64/NAP consecutive states (NAP = next power of two >= na) and streams their
+// contiguous nnz range with independent coalesced loads (see spmv_vi_sweep).
 //
 // Accuracy: each row sum is accumulated with an error-free product (fma) and TwoSum
 // (the compensated dot product of Ogita, Rump and Oishi, "Accurate sum and dot
@@ -19,8 +17,8 @@
 // Convergence without host round trips: the device state holds the sweep index k and
 // `conv`, the first sweep whose residual fell below thresh. Each block stores its
 // residual maximum (plain store); the last block of a sweep (ticket counter) reduces
-// them, publishes the residual, sets conv, and advances k. Blocks are 1024 threads and
-// at most 512 per grid (grid-stride over states), so one sweep costs at most 512
+// them, publishes the residual, sets conv, and advances k. Blocks are 512 threads and
+// at most 1024 per grid (grid-stride over state chunks), so one sweep costs at most 1024
 // same-address atomics. A launch
 // that finds conv >= 0 returns at once, so a fixed chain of launches (eager or a
 // captured hipGraph) stops doing work exactly where the reference loop stops
@@ -49,9 +47,9 @@ struct VIArgs {
     float discount;
 };
 
-constexpr int kVIThreads = 1024;  // 16 waves per block
+constexpr int kVIThreads = 512;   // 8 waves per block
 constexpr int kVIWaves = kVIThreads / 64;
-constexpr int kVIMaxBlocks = 512;  // 2 blocks per CU
+constexpr int kVIMaxBlocks = 1024;  // 4 blocks per CU
 
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
     s = a + b;
@@ -59,82 +57,100 @@ __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
     e = (a - (s - bp)) + (b - bp);
 }
 
-template <int LPR, int NAP>
+// One wave takes a chunk of SPC = 64/NAP consecutive states (NAP = next power of two
+// >= na; lane = state_in_chunk * NAP + action). Per chunk:
+//   1. each lane loads its row's two row pointers;
+//   2. the chunk's nnz range is streamed in windows of 64*U entries: every lane issues U
+//      independent coalesced 8-byte loads and U value gathers, forms the error-free
+//      products (p = v*x, pe = fma(v, x, -p)) and parks them in its wave's LDS;
+//   3. each lane sums its own row from LDS in ascending order (TwoSum, compensation
+//      carried), adds R;
+//   4. an xor butterfly over the NAP lanes of a state gives max / first argmax.
+// All loads of a window are independent, so a wave keeps about 2*U*64 memory requests
+// in flight instead of one state's dependent chain.
+template <int NAP, int U>
 __global__ __launch_bounds__(kVIThreads) void spmv_vi_sweep(const VIArgs A) {
-    constexpr int G = LPR * NAP;
-    constexpr int SPW = 64 / G;
+    constexpr int SPC = 64 / NAP;
+    constexpr int WIN = 64 * U;
     __shared__ float wmax[kVIWaves];
     __shared__ int is_last;
+    __shared__ float2 stage[kVIWaves][WIN];
     VIState* st = A.st;
     if (st->conv >= 0) return;  // converged at an earlier sweep
     const uint32_t k = st->k;
     const float* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
     float* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int a = (lane % G) / LPR;
-    const int l = lane % LPR;
+    const int a = lane % NAP, sl = lane / NAP;
+    float2* wst = stage[wv];
     float res = 0.0f;
-    // grid-stride over state groups: a few thousand blocks, so the per-sweep residual
-    // and ticket atomics stay few
-    const int64_t stride = (int64_t)gridDim.x * kVIWaves * SPW;
-    for (int64_t s = ((int64_t)blockIdx.x * kVIWaves + wv) * SPW + lane / G; s - lane / G < A.ns; s += stride) {
+    const int64_t nchunks = (A.ns + SPC - 1) / SPC;
+    for (int64_t ch = (int64_t)blockIdx.x * kVIWaves + wv; ch < nchunks; ch += (int64_t)gridDim.x * kVIWaves) {
+        const int64_t s = ch * SPC + sl;
         const bool valid = (s < A.ns) && (a < A.na);
-        float sh = 0.0f, sl = 0.0f, rr = 0.0f;
+        const int64_t row = s * A.na + a;
+        uint32_t rb = 0, re = 0;
+        float rr = 0.0f;
         if (valid) {
-            const int64_t row = s * A.na + a;
-            const uint32_t beg = A.rp[row], end = A.rp[row + 1];
+            rb = A.rp[row];
+            re = A.rp[row + 1];
             rr = A.R[row];
-            uint32_t j = beg + l;
-            // two independent loads in flight per lane
-            for (; j + LPR < end; j += 2 * LPR) {
-                const int2 e0 = A.nz[j];
-                const int2 e1 = A.nz[j + LPR];
-                const float v0 = __int_as_float(e0.y), v1 = __int_as_float(e1.y);
-                const float x0 = Vin[e0.x], x1 = Vin[e1.x];
-                float p, pe, t, err;
-                p = v0 * x0;
-                pe = fmaf(v0, x0, -p);
-                two_sum(sh, p, t, err);
-                sh = t;
-                sl = sl + (err + pe);
-                p = v1 * x1;
-                pe = fmaf(v1, x1, -p);
-                two_sum(sh, p, t, err);
-                sh = t;
-                sl = sl + (err + pe);
-            }
-            if (j < end) {
-                const int2 e0 = A.nz[j];
-                const float v0 = __int_as_float(e0.y);
-                const float x0 = Vin[e0.x];
-                float p = v0 * x0, pe = fmaf(v0, x0, -p), t, err;
-                two_sum(sh, p, t, err);
-                sh = t;
-                sl = sl + (err + pe);
-            }
         }
+        // chunk range: first valid lane's begin, last valid lane's end
+        const uint64_t vm = __ballot(valid);
+        const uint32_t cb = __shfl(rb, __ffsll((long long)vm) - 1, 64);
+        const uint32_t ce = __shfl(re, 63 - __clzll((long long)vm), 64);
+        float sh = 0.0f, slo = 0.0f;
+        for (uint32_t w0 = cb; w0 < ce; w0 += WIN) {
+            const uint32_t wn = min((uint32_t)WIN, ce - w0);
+            int2 e[U];
 #pragma unroll
-        for (int o = 1; o < LPR; o <<= 1) {
-            const float h2 = __shfl_xor(sh, o, 64), l2 = __shfl_xor(sl, o, 64);
-            float t, err;
-            two_sum(sh, h2, t, err);
-            sh = t;
-            sl = sl + (l2 + err);
+            for (int u = 0; u < U; u++) {
+                const uint32_t i = (uint32_t)(u * 64 + lane);
+                e[u] = i < wn ? A.nz[w0 + i] : make_int2(0, 0);
+            }
+            float x[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint32_t i = (uint32_t)(u * 64 + lane);
+                x[u] = i < wn ? Vin[e[u].x] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint32_t i = (uint32_t)(u * 64 + lane);
+                if (i < wn) {
+                    const float v = __int_as_float(e[u].y);
+                    const float p = v * x[u];
+                    wst[i] = make_float2(p, fmaf(v, x[u], -p));
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (valid) {
+                const uint32_t jb = max(rb, w0), je = min(re, w0 + wn);
+                for (uint32_t j = jb; j < je; j++) {
+                    const float2 pp = wst[j - w0];
+                    float t, err;
+                    two_sum(sh, pp.x, t, err);
+                    sh = t;
+                    slo = slo + (err + pp.y);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
         }
         if (A.discount != 1.0f) {
             sh *= A.discount;
-            sl *= A.discount;
+            slo *= A.discount;
         }
         float q;
         {
             float t, err;
             two_sum(rr, sh, t, err);
-            q = t + (err + sl);
+            q = t + (err + slo);
         }
         if (!valid || q != q) q = -INFINITY;
         int ba = valid ? a : 0x7fffffff;
 #pragma unroll
-        for (int o = LPR; o < G; o <<= 1) {
+        for (int o = 1; o < NAP; o <<= 1) {
             const float q2 = __shfl_xor(q, o, 64);
             const int a2 = __shfl_xor(ba, o, 64);
             if (q2 > q || (q2 == q && a2 < ba)) {
@@ -142,7 +158,7 @@ __global__ __launch_bounds__(kVIThreads) void spmv_vi_sweep(const VIArgs A) {
                 ba = a2;
             }
         }
-        if (s < A.ns && (lane % G) == 0) {
+        if (s < A.ns && a == 0) {
             float vnew;
             int32_t pol;
             if (q > -1e6f) {
@@ -205,45 +221,38 @@ __global__ void vi_init(float* V0, int64_t ns, VIState* st) {
 
 using SweepFn = void (*)(const VIArgs);
 
-template <int NAP>
-SweepFn pick_lpr(int lpr) {
-    constexpr int maxl = 64 / NAP;
-    if (lpr >= 16 && maxl >= 16) return spmv_vi_sweep<(maxl >= 16 ? 16 : 1), NAP>;
-    if (lpr >= 8 && maxl >= 8) return spmv_vi_sweep<(maxl >= 8 ? 8 : 1), NAP>;
-    if (lpr >= 4 && maxl >= 4) return spmv_vi_sweep<(maxl >= 4 ? 4 : 1), NAP>;
-    if (lpr >= 2 && maxl >= 2) return spmv_vi_sweep<(maxl >= 2 ? 2 : 1), NAP>;
-    return spmv_vi_sweep<1, NAP>;
-}
-
 int next_pow2(int x) {
     int p = 1;
     while (p < x) p <<= 1;
     return p;
 }
 
-SweepFn pick_sweep(int na, int lpr, int* lpr_used) {
-    const int nap = next_pow2(na);
-    int maxl = 64 / nap;
-    int l = 1;
-    while (l * 2 <= lpr && l * 2 <= maxl && l < 16) l *= 2;
-    *lpr_used = l;
+// U: nnz loads in flight per lane per window (the window is 64*U entries).
+template <int U>
+SweepFn pick_nap(int nap) {
     switch (nap) {
-        case 1: return pick_lpr<1>(l);
-        case 2: return pick_lpr<2>(l);
-        case 4: return pick_lpr<4>(l);
-        case 8: return pick_lpr<8>(l);
-        case 16: return pick_lpr<16>(l);
-        case 32: return pick_lpr<32>(l);
-        default: return pick_lpr<64>(l);
+        case 1: return spmv_vi_sweep<1, U>;
+        case 2: return spmv_vi_sweep<2, U>;
+        case 4: return spmv_vi_sweep<4, U>;
+        case 8: return spmv_vi_sweep<8, U>;
+        case 16: return spmv_vi_sweep<16, U>;
+        case 32: return spmv_vi_sweep<32, U>;
+        default: return spmv_vi_sweep<64, U>;
     }
 }
 
-int auto_lpr(double nnz_per_row) {
-    if (nnz_per_row <= 1.5) return 1;
-    if (nnz_per_row <= 3.0) return 2;
-    if (nnz_per_row <= 12.0) return 4;
-    if (nnz_per_row <= 24.0) return 8;
-    return 16;
+SweepFn pick_sweep(int na, int u, int* u_used) {
+    const int nap = next_pow2(na);
+    if (u <= 2) {
+        *u_used = 2;
+        return pick_nap<2>(nap);
+    }
+    if (u <= 4) {
+        *u_used = 4;
+        return pick_nap<4>(nap);
+    }
+    *u_used = 8;
+    return pick_nap<8>(nap);
 }
 
 }  // namespace
@@ -269,12 +278,13 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
             return E2E_ERR_NOMEM;
         }
     }
-    int lpr_req = opts.lanes_per_row > 0 ? opts.lanes_per_row
-                                         : auto_lpr(m->rows ? (double)m->nnz / (double)m->rows : 1.0);
-    int lpr = 1;
-    SweepFn fn = pick_sweep((int)m->na, lpr_req, &lpr);
-    const int G = next_pow2((int)m->na) * lpr;
-    const int64_t states_per_block = kVIWaves * (64 / G);
+    // loads in flight per lane: a chunk holds about 64/NAP states of nnz/row * na entries
+    const int nap = next_pow2((int)m->na);
+    const double per_chunk = (m->rows ? (double)m->nnz / (double)m->rows : 1.0) * m->na * (64 / nap);
+    int u_req = opts.loads_in_flight > 0 ? opts.loads_in_flight : (per_chunk > 192 ? 8 : per_chunk > 96 ? 4 : 2);
+    int lpr = 0;
+    SweepFn fn = pick_sweep((int)m->na, u_req, &lpr);
+    const int64_t states_per_block = kVIWaves * (64 / nap);
     const int64_t nblocks = std::min<int64_t>(ceil_div(ns, states_per_block), kVIMaxBlocks);
     VIArgs A{m->row_ptr, m->nz, m->R, m->V[0], m->V[1], m->pi, m->st, m->vi_part, ns, (int)m->na,
              opts.thresh, opts.discount};
@@ -351,7 +361,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
         stats->sweeps = sweeps;
         stats->converged = hst.conv >= 0;
         stats->final_residual = hst.last_res;
-        stats->lanes_per_row = lpr;
+        stats->loads_in_flight = lpr;
         stats->solve_ms = now_ms() - t_start;
         stats->sweep_kernel_ms = kms;
         stats->sweep_kernel_ms_per_sweep = sweeps > 0 ? kms / sweeps : 0.0;

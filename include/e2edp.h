@@ -126,7 +126,7 @@ typedef struct e2e_solve_opts {
     int32_t max_sweeps;        /* 0: nt + 2 for a built model, 100000 for an imported one */
     int32_t sweeps_per_check;  /* sweeps launched between host checks of the converged flag (0: 16) */
     int32_t use_graph;         /* 1: replay the sweep chain as a captured hipGraph */
-    int32_t lanes_per_row;     /* 0: chosen from nnz/row; else 1,2,4,8,16 */
+    int32_t loads_in_flight;   /* nnz loads per lane per window: 2, 4 or 8 (0: chosen from nnz per state) */
     double timeout_ms;         /* 0: none */
 } e2e_solve_opts;
 
@@ -134,7 +134,7 @@ typedef struct e2e_solve_stats {
     int32_t sweeps;            /* sweeps performed (the reference's num_iterations) */
     int32_t converged;
     float final_residual;      /* residual of the last sweep */
-    int32_t lanes_per_row;
+    int32_t loads_in_flight;
     double solve_ms;           /* wall time of the solve */
     double sweep_kernel_ms;    /* HIP-event time of all sweep launches (incl. early-exit ones) */
     double sweep_kernel_ms_per_sweep; /* sweep_kernel_ms / launches that did work */

@@ -25,13 +25,13 @@ def _check_solve(ref, pol, val, stats, tol=TOL):
     return err.max()
 
 
-@pytest.mark.parametrize("lpr", [0, 1, 2, 8])
-def test_vi_small(gpu, lpr):
+@pytest.mark.parametrize("lif", [0, 2, 4, 8])
+def test_vi_small(gpu, lif):
     w = small(g=9, nt=12, nr=300, nm=2, seed=41)
     f = w.fields()
     ref = oracle_model(w.params, f)
     with e2e.build_model(w.params, f) as m:
-        pol, val, st = m.solve(thresh=1e-6, lanes_per_row=lpr)
+        pol, val, st = m.solve(thresh=1e-6, loads_in_flight=lif)
         assert st["converged"] and st["sweeps"] <= w.params.nt + 1
         _check_solve(ref, pol, val, st)
 
