@@ -159,6 +159,8 @@ def main(argv=None):
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=120)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run 3 device-to-device copies of BYTES (HBM counter calibration)")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -167,6 +169,16 @@ def main(argv=None):
 
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        n = args.copy_calibration // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 3}))
+        return 0
 
     import e2e_gpu_dp_amd as e2e
     from e2e_gpu_dp_amd.configs import workload

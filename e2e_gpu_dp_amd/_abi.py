@@ -62,7 +62,7 @@ class SolveOpts(C.Structure):
     _fields_ = [
         ("thresh", C.c_float), ("discount", C.c_float), ("max_sweeps", C.c_int32),
         ("sweeps_per_check", C.c_int32), ("use_graph", C.c_int32), ("loads_in_flight", C.c_int32),
-        ("timeout_ms", C.c_double),
+        ("timeout_ms", C.c_double), ("kernel_variant", C.c_int32), ("blocks_per_cu", C.c_int32),
     ]
 
 
@@ -71,7 +71,8 @@ class SolveStats(C.Structure):
         ("sweeps", C.c_int32), ("converged", C.c_int32), ("final_residual", C.c_float),
         ("loads_in_flight", C.c_int32), ("solve_ms", C.c_double), ("sweep_kernel_ms", C.c_double),
         ("sweep_kernel_ms_per_sweep", C.c_double), ("bytes_per_sweep", C.c_double),
-        ("launches", C.c_int32), ("reserved", C.c_int32),
+        ("launches", C.c_int32), ("kernel_variant", C.c_int32), ("blocks", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

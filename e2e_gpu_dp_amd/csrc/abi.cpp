@@ -44,7 +44,8 @@ e2e_status free_model(Model* m) {
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     if (m->graph) (void)hipGraphExecDestroy(m->graph);
     batch_cache_free(m->batch);
-    void* ptrs[] = {m->row_ptr, m->nz, m->R, m->Rsum, m->V[0], m->V[1], m->pi, m->st, m->vi_part};
+    void* ptrs[] = {m->row_ptr, m->nz,   m->R,  m->Rsum,    m->V[0],
+                    m->V[1],    m->pi,   m->st, m->vi_part, m->vi_wpart};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);

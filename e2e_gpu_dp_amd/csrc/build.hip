@@ -1059,10 +1059,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
               "nnz=%llu exceeds the 2^31 column-index range", total);
     m->nnz = (int64_t)total;
     phase("row-count scan");
-    if (hipMalloc(&m->nz, sizeof(int2) * (total ? total : 1)) != hipSuccess) {
-        set_error("hipMalloc nz (%llu entries) failed", total);
-        return fail(E2E_ERR_NOMEM);
-    }
+    if ((st = alloc_nz(m, (int64_t)total))) return fail(st);
     hipLaunchKernelGGL(csr_scatter, dim3((unsigned)ceil_div(rows * K, 256)), dim3(256), 0, s,
                        dStage.as<uint2>(), dArena.as<uint2>(), A.row_nnz, m->row_ptr, m->nz, rows, K,
                        nc * na, nc, nr);
@@ -1086,6 +1083,16 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     phase("staging freed");
     m->info.build_ms = now_ms() - t_start;
     *out = m;
+    return E2E_OK;
+}
+
+e2e_status alloc_nz(Model* m, int64_t nnz) {
+    if (hipMalloc(&m->nz, sizeof(int2) * (nnz + kNzPad)) != hipSuccess) {
+        m->nz = nullptr;
+        set_error("hipMalloc nz (%lld entries) failed", (long long)nnz);
+        return E2E_ERR_NOMEM;
+    }
+    E2E_HIP(hipMemsetAsync(m->nz + nnz, 0, sizeof(int2) * kNzPad, m->stream));
     return E2E_OK;
 }
 
@@ -1132,11 +1139,11 @@ e2e_status model_import(int64_t ns, int64_t na, int64_t nnz, const int32_t* S1, 
         (st = dmalloc(dTot, sizeof(unsigned long long))))
         return fail(st);
     if (hipMalloc(&m->row_ptr, sizeof(uint32_t) * (rows + 1)) != hipSuccess ||
-        hipMalloc(&m->nz, sizeof(int2) * (nnz ? nnz : 1)) != hipSuccess ||
         hipMalloc(&m->R, sizeof(float) * rows) != hipSuccess) {
         set_error("import: device allocation failed");
         return fail(E2E_ERR_NOMEM);
     }
+    if ((st = alloc_nz(m, nnz))) return fail(st);
     E2E_HIP(hipMemcpyAsync(dRp.p, rp.data(), sizeof(uint32_t) * (rows + 1), hipMemcpyHostToDevice, s));
     E2E_HIP(hipMemcpyAsync(dS2.p, S2, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
     E2E_HIP(hipMemcpyAsync(dVal.p, val, sizeof(float) * nnz, hipMemcpyHostToDevice, s));

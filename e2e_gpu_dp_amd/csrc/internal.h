@@ -40,6 +40,10 @@ void set_error(const char* fmt, ...);
 
 constexpr int kMaxActions = 64;
 constexpr int kMaxModes = 16;
+// The CSR entry array is allocated with this many zero entries (column 0, value 0) past
+// nnz, so that streaming kernels may load whole windows of up to kNzPad entries without
+// per-entry bounds checks.
+constexpr int64_t kNzPad = 4096;
 
 // Device-side solver state of one solve (one per model).
 struct VIState {
@@ -82,6 +86,11 @@ struct Model {
     int graph_chunk = 0;
     int graph_lpr = 0;
     float graph_thresh = 0.f, graph_discount = 0.f;
+    void* graph_fn = nullptr;
+    int64_t graph_blocks = 0;
+    uint32_t* vi_wpart = nullptr;  // [waves+1] first state of each wave's contiguous share
+    int64_t vi_wpart_waves = 0;
+    int vi_wpart_spc = 0;
 
     bool has_obstacles = false;
     BatchCache* batch = nullptr;  // buffers + tuning of the batched multi-goal solve
@@ -90,6 +99,8 @@ struct Model {
 };
 
 e2e_status model_alloc_stream(Model* m, void* stream);
+// allocates m->nz with nnz entries plus a zeroed tail of kNzPad entries (on m->stream)
+e2e_status alloc_nz(Model* m, int64_t nnz);
 e2e_status free_model(Model* m);
 
 // scan.hip: exclusive scan of n uint32 counts into out[0..n] (out[n] = total).

@@ -128,6 +128,9 @@ typedef struct e2e_solve_opts {
     int32_t use_graph;         /* 1: replay the sweep chain as a captured hipGraph */
     int32_t loads_in_flight;   /* nnz loads per lane per window: 2, 4 or 8 (0: chosen from nnz per state) */
     double timeout_ms;         /* 0: none */
+    int32_t kernel_variant;    /* 0: default; 1 staged + compensated, 2 staged + fma, 3 row per lane + fma,
+                                  4 staged + float64 products and sums */
+    int32_t blocks_per_cu;     /* resident sweep blocks per compute unit (0: the occupancy limit) */
 } e2e_solve_opts;
 
 typedef struct e2e_solve_stats {
@@ -140,6 +143,8 @@ typedef struct e2e_solve_stats {
     double sweep_kernel_ms_per_sweep; /* sweep_kernel_ms / launches that did work */
     double bytes_per_sweep;    /* algorithmic bytes of one sweep */
     int32_t launches;
+    int32_t kernel_variant;    /* variant used (1-based, as in e2e_solve_opts) */
+    int32_t blocks;            /* sweep grid size (blocks of 512 threads) */
     int32_t reserved;
 } e2e_solve_stats;
 

@@ -30,8 +30,7 @@ def rel_err(v, v64):
 
 def q_values_f64(ns, na, S1, S2, val, R, V, discount=1.0):
     """Q64[a, s] = R + discount * P V (V float64), rows a*ns + s."""
-    acc = np.zeros(na * ns)
-    np.add.at(acc, S1, val.astype(np.float64) * V[S2])
+    acc = np.bincount(S1, weights=val.astype(np.float64) * V[S2], minlength=na * ns)
     return (R.astype(np.float64) + discount * acc).reshape(na, ns)
 
 

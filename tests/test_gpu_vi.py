@@ -87,3 +87,81 @@ def test_vi_not_converged_status(gpu):
     with e2e.build_model(w.params, w.fields()) as m:
         pol, val, st = m.solve(thresh=1e-6, max_sweeps=3, sweeps_per_check=1, raise_on_nonconv=False)
         assert st["status"] == 4 and st["sweeps"] == 3 and not st["converged"]
+
+
+# every sweep kernel, with the loads-in-flight settings it supports; the fma variants
+# (2, 3) round each product, so they get a wider bound on these short horizons
+VARIANTS = [(1, 2), (1, 8), (2, 4), (3, 0), (4, 4), (5, 2), (5, 4), (5, 8), (6, 2), (6, 4), (6, 8)]
+
+
+@pytest.mark.parametrize("variant,lif", VARIANTS)
+def test_vi_kernel_variants(gpu, variant, lif):
+    w = small(g=11, nt=14, nr=400, nm=2, na=12, seed=53)  # 12 actions: 4 idle lanes per state
+    f = w.fields()
+    ref = oracle_model(w.params, f)
+    with e2e.build_model(w.params, f) as m:
+        pol, val, st = m.solve(thresh=1e-6, kernel_variant=variant, loads_in_flight=lif)
+        assert st["kernel_variant"] == variant and st["converged"]
+        _check_solve(ref, pol, val, st, tol=TOL if variant not in (2, 3) else 4 * TOL)
+
+
+def _random_coo(ns, na, seed, empty_frac=0.2):
+    """Random row-sorted COO with empty rows, and dead states whose rows are all empty
+    and whose rewards are all -2e6."""
+    rng = np.random.default_rng(seed)
+    S1, S2, val = [], [], []
+    dead = set(rng.choice(ns, size=max(1, ns // 10), replace=False).tolist())
+    for a in range(na):
+        for s in range(ns):
+            if s in dead or rng.random() < empty_frac:
+                continue
+            k = int(rng.integers(1, 40))
+            cols = np.unique(rng.integers(0, ns, size=k))
+            p = rng.random(len(cols)).astype(np.float32)
+            S1 += [a * ns + s] * len(cols)
+            S2 += cols.tolist()
+            val += (p / p.sum()).astype(np.float32).tolist()
+    R = (-rng.random(na * ns) * 3).astype(np.float32)
+    for s in dead:  # every Q of a dead state is below -1e6: value -1e6, policy -1
+        R[np.arange(na) * ns + s] = -2e6
+    return (np.array(S1, np.int32), np.array(S2, np.int32), np.array(val, np.float32), R)
+
+
+@pytest.mark.parametrize("na", [1, 3, 16, 33])
+def test_vi_streaming_matches_chunked_bits(gpu, na):
+    """The streaming kernels add the same products in the same order as the chunked
+    ones, so their values and policies are bit-identical; empty rows and dead states
+    (every Q below -1e6: value -1e6, policy -1, as in spvi_2.cu:93-122) included."""
+    ns = 3001
+    S1, S2, val, R = _random_coo(ns, na, seed=60 + na)
+    with e2e.import_coo(ns, na, S1, S2, val, R) as m:
+        for a_, b_ in ((1, 5), (4, 6)):
+            pa, va, sa = m.solve(thresh=1e-3, max_sweeps=6, kernel_variant=a_, raise_on_nonconv=False)
+            pb, vb, sb = m.solve(thresh=1e-3, max_sweeps=6, kernel_variant=b_, raise_on_nonconv=False)
+            assert sa["sweeps"] == sb["sweeps"]
+            assert np.array_equal(va.view(np.uint32), vb.view(np.uint32)), (a_, b_)
+            assert np.array_equal(pa, pb), (a_, b_)
+        V64, pi64, it64, _ = oracle.vi_sweeps_f64(ns, na, S1, S2, val, R, 1e-3, 6)
+        assert rel_err(vb, V64).max() <= TOL
+        assert (pi64 == -1).sum() >= ns // 10
+        assert (pb.astype(np.int64)[pi64 == -1] == 0xFFFFFFFF).all()
+        assert (vb[pi64 == -1] == -1e6).all()
+
+
+def test_vi_c2_accuracy(gpu):
+    """Full-size workload (100x100 grid, nt=120, 16 headings, 2500 realizations,
+    1.34e8 nnz) with the default kernel: values within 1e-5 relative of the float64
+    backward induction, policies equal except at ties, sweep count as in float64."""
+    w = workload("C2")
+    with e2e.build_model(w.params, w.fields()) as m:
+        pol, val, st = m.solve(thresh=1e-6, sweeps_per_check=8)
+        S1, S2, pv, R = m.export_coo(np.int32)
+    ns, na = w.params.ns, w.params.na
+    V64, pi64 = oracle.vi_backward_f64(ns, na, w.params.nc, S1, S2, pv, R)
+    err = rel_err(val, V64)
+    print(f"[c2] sweeps={st['sweeps']} max rel err {err.max():.3e} ms/sweep "
+          f"{st['sweep_kernel_ms_per_sweep']:.4f}", flush=True)
+    assert st["converged"] and st["sweeps"] == w.params.nt
+    assert err.max() <= TOL
+    Q64 = q_values_f64(ns, na, S1, S2, pv, R, V64)
+    policy_ties_ok(pol, pi64, Q64, 4 * TOL)
