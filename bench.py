@@ -34,6 +34,20 @@ PEAK_HBM_GBPS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 
 
+def _pmc_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` as measured with rocprofv3 PMC counters on this
+    workload (profiles/hbm_traffic.json, commands in profiles/c2_hbm_traffic.md); None
+    when no measurement for this workload and kernel is on file."""
+    f = ROOT / "profiles" / "hbm_traffic.json"
+    try:
+        d = json.loads(f.read_text())
+    except (OSError, ValueError):
+        return None
+    if d.get("workload") != workload or kernel not in d:
+        return None
+    return d[kernel]["traffic_bytes"]
+
+
 def _cpu_baseline(w, fields, budget_s: float):
     """The oracle's OpenMP build on the host cores over the first slices of the same
     workload (bounded sample)."""
@@ -236,7 +250,8 @@ def main(argv=None):
                   "csr_ms": infos[0][0]["csr_ms"]},
         "roofline": {"kernel": "mc_move_count", "bound": "valu", "achieved": build_tflops,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": build_tflops / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": build_tflops / PEAK_FP32_TFLOPS,
+                     "traffic": _pmc_traffic(args.workload, "mc_move_count"),
                      "flops_per_lane": flops_per_lane},
     }
     if not args.skip_vi:
@@ -246,9 +261,13 @@ def main(argv=None):
         out["vi"] = {"sweeps": st["sweeps"], "solve_ms": float(np.mean([s["solve_ms"] for _, s in infos])),
                      "sweep_kernel_ms": per, "bytes_per_sweep": st["bytes_per_sweep"],
                      "gbps": gbps, "loads_in_flight": st["loads_in_flight"]}
-        out["roofline_vi"] = {"kernel": "spmv_vi_sweep", "bound": "hbm", "achieved": gbps,
+        out["vi"]["kernel_variant"] = st["kernel_variant"]
+        out["vi"]["blocks"] = st["blocks"]
+        kname = {1: "spmv_vi_sweep", 2: "spmv_vi_sweep", 3: "spmv_vi_lean", 4: "spmv_vi_tile",
+                 5: "spmv_vi_jagged"}.get(st["kernel_variant"], "?")
+        out["roofline_vi"] = {"kernel": kname, "bound": "hbm", "achieved": gbps,
                               "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": gbps / PEAK_HBM_GBPS,
-                              "traffic": None}
+                              "traffic": _pmc_traffic(args.workload, kname)}
     if not args.no_batched:
         print("[bench] batched goals", file=sys.stderr, flush=True)
         out["batched"] = _batched(e2e, np, max(args.steps, 3))

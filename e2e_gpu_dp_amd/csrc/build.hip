@@ -677,28 +677,6 @@ __global__ void import_rows(const uint32_t* __restrict__ rp_am, const uint32_t* 
 
 // ------------------------------------------------------------------ host helpers
 
-struct DevBuf {
-    void* p = nullptr;
-    ~DevBuf() { reset(); }
-    void reset() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-    }
-    template <typename T>
-    T* as() const { return reinterpret_cast<T*>(p); }
-};
-
-e2e_status dmalloc(DevBuf& b, size_t bytes) {
-    hipError_t e = hipMalloc(&b.p, bytes ? bytes : 16);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        b.p = nullptr;
-        set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
-        return E2E_ERR_NOMEM;
-    }
-    return E2E_OK;
-}
-
 // OBS: the goes_through_obstacle test (:444-477) is compiled in only for models whose
 // mask has obstacles; UNIT: the unit-cell move. Both cost registers the common kernel
 // does not need.
@@ -1114,8 +1092,13 @@ e2e_status model_import(int64_t ns, int64_t na, int64_t nnz, const int32_t* S1, 
                   (long long)j, S2[j]);
         rp[S1[j] + 1]++;
     }
-    for (int64_t r = 0; r < rows; r++) rp[r + 1] += rp[r];
+    uint32_t max_len = 0;
+    for (int64_t r = 0; r < rows; r++) {
+        max_len = std::max(max_len, rp[r + 1]);
+        rp[r + 1] += rp[r];
+    }
     Model* m = new Model();
+    m->info.max_row_nnz = max_len;
     m->built = false;
     m->ns = ns;
     m->na = na;

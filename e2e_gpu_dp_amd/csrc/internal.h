@@ -88,15 +88,47 @@ struct Model {
     float graph_thresh = 0.f, graph_discount = 0.f;
     void* graph_fn = nullptr;
     int64_t graph_blocks = 0;
+    unsigned char graph_args[256] = {};  // kernel arguments the graph was captured with
     uint32_t* vi_wpart = nullptr;  // [waves+1] first state of each wave's contiguous share
     int64_t vi_wpart_waves = 0;
     int vi_wpart_spc = 0;
+    // jagged-diagonal copy of the CSR for the jagged sweep: per state, rows sorted by
+    // length (longest first); entry k of the rows longer than k stored contiguously
+    int2* vi_jnz = nullptr;       // [nnz + kNzPad]
+    uint32_t* vi_jmeta = nullptr; // [rows] length | (action << 16), sorted order
+    float* vi_jR = nullptr;       // [rows] R in sorted order
+    uint2* vi_tdesc = nullptr;    // tiled sweep: per tile (first column, column span)
+    uint32_t vi_tile_states = 0, vi_maxspan = 0;
+    int64_t vi_part_cap = 0;      // entries of vi_part
 
     bool has_obstacles = false;
     BatchCache* batch = nullptr;  // buffers + tuning of the batched multi-goal solve
 
     e2e_model_info info{};
 };
+
+// Scoped device allocation for temporaries.
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
+    template <typename T>
+    T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+inline e2e_status dmalloc(DevBuf& b, size_t bytes) {
+    hipError_t e = hipMalloc(&b.p, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        b.p = nullptr;
+        set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+        return E2E_ERR_NOMEM;
+    }
+    return E2E_OK;
+}
 
 e2e_status model_alloc_stream(Model* m, void* stream);
 // allocates m->nz with nnz entries plus a zeroed tail of kNzPad entries (on m->stream)

@@ -25,6 +25,7 @@
 // (spvi_2.cu:669-698).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "internal.h"
 
@@ -45,7 +46,13 @@ struct VIArgs {
     int na;
     float thresh;
     float discount;
-    const uint32_t* wpart;  // [waves+1] state partition of the streaming kernel
+    const uint32_t* wpart;  // [waves+1] state partition of the lean sweep
+    const uint2* tdesc;     // [tiles] (first column, column span) of the tiled kernel
+    uint32_t tile_states;   // states per tile (tiled kernel)
+    uint32_t vwin;          // LDS value-window capacity in floats (tiled kernel)
+    const int2* jnz;        // jagged sweep: entries in per-state jagged-diagonal order
+    const uint32_t* jmeta;  // jagged sweep: length | (action << 16) per sorted row
+    const float* jR;        // jagged sweep: R per sorted row
 };
 
 constexpr int kVIThreads = 512;   // 8 waves per block
@@ -87,7 +94,6 @@ __global__ __launch_bounds__(kVIThreads) void spmv_vi_sweep(const VIArgs A) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int a = lane % NAP, sl = lane / NAP;
     float2* wst = stage[wv];
-    double* wsd = reinterpret_cast<double*>(wst);
     float res = 0.0f;
     const int64_t nchunks = (A.ns + SPC - 1) / SPC;
     const int64_t cstride = (int64_t)gridDim.x * kVIWaves;
@@ -128,8 +134,7 @@ __global__ __launch_bounds__(kVIThreads) void spmv_vi_sweep(const VIArgs A) {
     // products of one window into LDS, then each lane adds its row's part of the window.
     // ACC 0: error-free products (p, pe) and TwoSum (compensated); ACC 1: (prob, value)
     // pairs and one fma per entry.
-    auto window = [&](const Meta& M, uint32_t w0, uint32_t wn, const int2 (&e)[U], float& sh, float& slo,
-                      double& sd) {
+    auto window = [&](const Meta& M, uint32_t w0, uint32_t wn, const int2 (&e)[U], float& sh, float& slo) {
         float x[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
@@ -144,8 +149,6 @@ __global__ __launch_bounds__(kVIThreads) void spmv_vi_sweep(const VIArgs A) {
                 if (ACC == 0) {
                     const float p = v * x[u];
                     wst[i] = make_float2(p, fmaf(v, x[u], -p));
-                } else if (ACC == 3) {
-                    wsd[i] = (double)v * (double)x[u];  // exact: 24x24-bit product
                 } else {
                     wst[i] = make_float2(v, x[u]);
                 }
@@ -155,17 +158,7 @@ __global__ __launch_bounds__(kVIThreads) void spmv_vi_sweep(const VIArgs A) {
         if (M.valid) {
             const uint32_t jb = max(M.rb, w0), je = min(M.re, w0 + wn);
             uint32_t j = jb;
-            if (ACC == 3) {
-                for (; j + 4 <= je; j += 4) {
-                    double d4[4];
-#pragma unroll
-                    for (int t = 0; t < 4; t++) d4[t] = wsd[j + t - w0];
-#pragma unroll
-                    for (int t = 0; t < 4; t++) sd += d4[t];
-                }
-                for (; j < je; j++) sd += wsd[j - w0];
-            }
-            for (; ACC != 3 && j + 4 <= je; j += 4) {
+            for (; j + 4 <= je; j += 4) {
                 float2 q4[4];
 #pragma unroll
                 for (int t = 0; t < 4; t++) q4[t] = wst[j + t - w0];
@@ -181,7 +174,7 @@ __global__ __launch_bounds__(kVIThreads) void spmv_vi_sweep(const VIArgs A) {
                     }
                 }
             }
-            for (; ACC != 3 && j < je; j++) {
+            for (; j < je; j++) {
                 const float2 pp = wst[j - w0];
                 if (ACC == 0) {
                     float tt, err;
@@ -207,9 +200,8 @@ __global__ __launch_bounds__(kVIThreads) void spmv_vi_sweep(const VIArgs A) {
         Meta nxt;
         load_meta(ch + cstride, nxt);  // in flight with this chunk's gathers
         float sh = 0.0f, slo = 0.0f;
-        double sd = 0.0;
         const uint32_t wn0 = min((uint32_t)WIN, ce - cb);
-        window(cur, cb, wn0, ecur, sh, slo, sd);
+        window(cur, cb, wn0, ecur, sh, slo);
         // the next chunk's first window goes out before this chunk's remaining work
         uint32_t nb, ne;
         range_of(nxt, nb, ne);
@@ -217,12 +209,10 @@ __global__ __launch_bounds__(kVIThreads) void spmv_vi_sweep(const VIArgs A) {
         issue_nz(nb, ne, enxt);
         for (uint32_t w0 = cb + wn0; w0 < ce; w0 += WIN) {  // chunks larger than a window
             issue_nz(w0, ce, ecur);  // ecur is consumed: reuse its registers
-            window(cur, w0, min((uint32_t)WIN, ce - w0), ecur, sh, slo, sd);
+            window(cur, w0, min((uint32_t)WIN, ce - w0), ecur, sh, slo);
         }
         float q;
-        if (ACC == 3) {
-            q = (float)((double)cur.rr + (double)A.discount * sd);
-        } else {
+        {
             if (A.discount != 1.0f) {
                 sh *= A.discount;
                 slo *= A.discount;
@@ -298,225 +288,163 @@ __global__ __launch_bounds__(kVIThreads) void spmv_vi_sweep(const VIArgs A) {
     }
 }
 
-// ACC 2: one lane per row, streaming that row's entries directly (no LDS): 4 loads
-// per lane in flight, one fma per entry. Same chunk mapping and pipelined row data.
-template <int NAP>
-__global__ __launch_bounds__(kVIThreads) void spmv_vi_rows(const VIArgs A) {
-    constexpr int SPC = 64 / NAP;
-    __shared__ float wmax[kVIWaves];
-    __shared__ int is_last;
-    VIState* st = A.st;
-    if (st->conv >= 0) return;
-    const uint32_t k = st->k;
-    const float* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
-    float* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int a = lane % NAP, sl = lane / NAP;
-    float res = 0.0f;
-    const int64_t nchunks = (A.ns + SPC - 1) / SPC;
-    const int64_t cstride = (int64_t)gridDim.x * kVIWaves;
-    for (int64_t ch = (int64_t)blockIdx.x * kVIWaves + wv; ch < nchunks; ch += cstride) {
-        const int64_t s = ch * SPC + sl;
-        const bool valid = (s < A.ns) && (a < A.na);
-        float acc = 0.0f, rr = 0.0f, vold = 0.0f;
-        if (valid) {
-            const int64_t row = s * A.na + a;
-            const uint32_t rb = A.rp[row], re = A.rp[row + 1];
-            rr = A.R[row];
-            if (a == 0) vold = Vin[s];
-            uint32_t j = rb;
-            for (; j + 4 <= re; j += 4) {
-                int2 e[4];
-#pragma unroll
-                for (int t = 0; t < 4; t++) e[t] = A.nz[j + t];
-                float x[4];
-#pragma unroll
-                for (int t = 0; t < 4; t++) x[t] = Vin[e[t].x];
-#pragma unroll
-                for (int t = 0; t < 4; t++) acc = fmaf(__int_as_float(e[t].y), x[t], acc);
-            }
-            for (; j < re; j++) {
-                const int2 e = A.nz[j];
-                acc = fmaf(__int_as_float(e.y), Vin[e.x], acc);
-            }
-        }
-        float q = valid ? rr + A.discount * acc : -INFINITY;
-        if (q != q) q = -INFINITY;
-        int ba = valid ? a : 0x7fffffff;
-#pragma unroll
-        for (int o = 1; o < NAP; o <<= 1) {
-            const float q2 = __shfl_xor(q, o, 64);
-            const int a2 = __shfl_xor(ba, o, 64);
-            if (q2 > q || (q2 == q && a2 < ba)) {
-                q = q2;
-                ba = a2;
-            }
-        }
-        if (s < A.ns && a == 0) {
-            float vnew;
-            int32_t pol;
-            if (q > -1e6f) {
-                vnew = q;
-                pol = ba;
-            } else {
-                vnew = -1e6f;
-                pol = -1;
-            }
-            Vout[s] = vnew;
-            A.pi[s] = pol;
-            res = fmaxf(res, fabsf(vnew - vold));
-        }
+template <typename T>
+using CPtr = const __attribute__((address_space(4))) T*;
+
+// p[i] with a 32-bit byte offset (i * sizeof(T) < 2^32)
+template <typename T>
+__device__ __forceinline__ T ld_off(const T* __restrict__ p, uint32_t i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + (i * (uint32_t)sizeof(T)));
+}
+
+// lane l takes lane l-n of its row of 16 (row_shr:n); lanes l < n of the row keep `id`
+__device__ __forceinline__ int dpp_row_shr(int v, int n, int id) {
+    switch (n) {
+        case 1: return __builtin_amdgcn_update_dpp(id, v, 0x111, 0xf, 0xf, false);
+        case 2: return __builtin_amdgcn_update_dpp(id, v, 0x112, 0xf, 0xf, false);
+        case 4: return __builtin_amdgcn_update_dpp(id, v, 0x114, 0xf, 0xf, false);
+        default: return __builtin_amdgcn_update_dpp(id, v, 0x118, 0xf, 0xf, false);
     }
+}
+__device__ __forceinline__ float dpp_row_shr(float v, int n, float id) {
+    return __int_as_float(dpp_row_shr(__float_as_int(v), n, __float_as_int(id)));
+}
+
+// A lane adds its row's products from LDS kRowReads at a time: all reads issued, one
+// wait, then the in-range ones added in ascending order (the same order, and so the
+// same bits, as one at a time). Reads past the row, up to kRowReads past the window,
+// land in the window's padding or on other rows and are not added.
+constexpr int kRowReads = 8;
+
+__device__ __forceinline__ void row_part(const double* q, uint32_t jb, uint32_t je, double& sd) {
+    while (jb < je) {
+        double d[kRowReads];
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) res = fmaxf(res, __shfl_xor(res, o, 64));
-    if (lane == 0) wmax[wv] = res;
-    __syncthreads();
-    if (tid == 0) {
-        float bm = 0.0f;
+        for (int i = 0; i < kRowReads; i++) d[i] = q[jb + i];
+        const uint32_t c = je - jb;
 #pragma unroll
-        for (int w = 0; w < kVIWaves; w++) bm = fmaxf(bm, wmax[w]);
-        A.part[blockIdx.x] = bm;
-        __threadfence();
-        is_last = atomicAdd(&st->done, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (is_last) {
-        __threadfence();
-        float r = 0.0f;
-        for (unsigned b = tid; b < gridDim.x; b += kVIThreads)
-            r = fmaxf(r, __hip_atomic_load(A.part + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) r = fmaxf(r, __shfl_xor(r, o, 64));
-        __syncthreads();
-        if (lane == 0) wmax[wv] = r;
-        __syncthreads();
-        if (tid == 0) {
-            float m = 0.0f;
-#pragma unroll
-            for (int w = 0; w < kVIWaves; w++) m = fmaxf(m, wmax[w]);
-            st->last_res = m;
-            if (m < A.thresh) st->conv = (int32_t)k;
-            st->done = 0u;
-            st->k = k + 1u;
-        }
+        for (int i = 0; i < kRowReads; i++) sd = ((uint32_t)i < c) ? sd + d[i] : sd;
+        jb += kRowReads;
     }
 }
 
-// ---------------------------------------------------------------------------------
-// Streaming sweep (the default). Each wavefront owns one contiguous range of states,
-// [wpart[w], wpart[w+1]), cut so that every wave has about the same nnz + 2 * rows.
-// Its rows are contiguous in the state-major CSR, so its entries form ONE contiguous
-// stream [rp[s_begin*na], rp[s_end*na]) that is read in fixed windows of 64*U entries,
-// with no dependence on the row pointers:
-//   window k+2's entries are requested, window k+1's value gathers are requested, and
-//   window k's products are formed and parked in LDS, all in the same step, so two
-//   windows of entries and one of gathers are in flight while a window is reduced.
-// Rows are consumed in groups of SPC = 64/NAP states (one row per lane, as above);
-// a group's row data (row pointers, R, old value) is requested two groups ahead. When a
-// window contains the end of a group, the group is finished: max / first argmax over
-// the NAP lanes of each state, value + policy stores, residual.
-// Windows may run past the wave's stream: the entry array carries kNzPad zero entries
-// past nnz, and entries outside a lane's row range are never summed.
-// XCD-aware: blocks are dealt to the 8 XCDs round robin, so block b takes partition
-// (b % 8) * (nb / 8) + b / 8 and each XCD (and its L2) works on one eighth of the states.
-// ACC 0: float products with their rounding errors (fma) and TwoSum; ACC 3: float64
-// products (exact) and float64 sums; both round Q to float once.
 constexpr int kStreamThreads = 256;
 constexpr int kStreamWaves = kStreamThreads / 64;
 
-template <int NAP, int U, int ACC>
-__global__ __launch_bounds__(kStreamThreads) void spmv_vi_stream(const VIArgs A) {
+// wpart[w] = first state of wave w: the smallest multiple of spc whose weight
+// rp[s*na] + 2*s*na reaches w/W of the total (monotone in w), wpart[W] = ns.
+__global__ void vi_partition(const uint32_t* __restrict__ rp, int64_t ns, int na, int64_t W, int spc,
+                             uint32_t* __restrict__ wpart) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w > W) return;
+    if (w == 0 || w == W) {
+        wpart[w] = w == 0 ? 0u : (uint32_t)ns;
+        return;
+    }
+    const double total = (double)rp[ns * na] + 2.0 * (double)(ns * na);
+    const double target = total * (double)w / (double)W;
+    int64_t lo = 0, hi = ns;  // smallest s with weight(s) >= target
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) / 2;
+        const double wt = (double)rp[mid * na] + 2.0 * (double)(mid * na);
+        if (wt >= target)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    int64_t s = (lo + spc / 2) / spc * spc;
+    wpart[w] = (uint32_t)(s < ns ? s : ns);
+}
+
+// ---------------------------------------------------------------------------------
+// Lean streaming sweep (the default). Same work split as spmv_vi_stream (one contiguous
+// state range per wave, balanced by entries + rows, entries read as one stream in
+// windows of 64*U entries), built for occupancy: at most 64 VGPRs, so 8 waves per SIMD
+// keep enough loads in flight without software pipelining (a gather-bound stream runs
+// about 1.5x faster at 8 waves per SIMD than at 4). Wave-uniform values live in SGPRs
+// (readfirstlane, scalar loads of the read-only partition and row pointers). The next
+// group's row data is requested when a group starts. Products and sums in float64.
+template <int NAP, int U>
+__global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
     constexpr int SPC = 64 / NAP;
     constexpr int WIN = 64 * U;
     __shared__ float wmax[kStreamWaves];
     __shared__ int is_last;
-    __shared__ double stage[kStreamWaves][WIN];
+    __shared__ double stage[kStreamWaves][WIN + kRowReads];
     VIState* st = A.st;
     if (st->conv >= 0) return;  // converged at an earlier sweep
     const uint32_t k = st->k;
     const float* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
     float* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int a = lane % NAP, sl = lane / NAP;
-    const int na = A.na;
+    const uint32_t na = (uint32_t)A.na;
     const unsigned nb = gridDim.x, b = blockIdx.x;
     const unsigned pb = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;
     const unsigned w = pb * kStreamWaves + wv;
-    const int64_t s_begin = A.wpart[w], s_end = A.wpart[w + 1];
+    const CPtr<uint32_t> wp = (CPtr<uint32_t>)A.wpart;
+    const CPtr<uint32_t> rps = (CPtr<uint32_t>)A.rp;  // uniform reads of the row pointers
+    const uint32_t s_begin = wp[w], s_end = wp[w + 1];
     double* wsd = stage[wv];
-    float2* wsf = reinterpret_cast<float2*>(stage[wv]);
     float res = 0.0f;
 
-    if (s_begin < s_end) {  // wave-uniform
-        const int64_t ngroups = (s_end - s_begin + SPC - 1) / SPC;
-        const uint32_t P0 = A.rp[s_begin * na];
-        const uint32_t Pend = A.rp[s_end * na];
+    if (s_begin < s_end) {
+        const uint32_t ngroups = (s_end - s_begin + SPC - 1) / SPC;
+        const uint32_t P0 = rps[s_begin * na], Pend = rps[s_end * na];
         const uint32_t nwin = (Pend - P0 + WIN - 1) / WIN;
-        const int2* __restrict__ nzp = A.nz + P0 + lane;
-
         struct Meta {
-            uint32_t rb, re, ce;
+            uint32_t rb, re;
             float rr, vold;
-            bool valid;
         };
-        auto load_meta = [&](int64_t g, Meta& M) {
-            const int64_t s0 = s_begin + g * SPC, s = s0 + sl;
-            M.valid = (g < ngroups) && (s < s_end) && (a < na);
-            M.rb = 0u;
-            M.re = 0u;
-            M.ce = 0u;
-            M.rr = 0.0f;
-            M.vold = 0.0f;
-            if (g < ngroups) M.ce = A.rp[min(s0 + SPC, s_end) * na];  // same address in all lanes
-            if (M.valid) {
-                const int64_t row = s * na + a;
-                M.rb = A.rp[row];
-                M.re = A.rp[row + 1];
-                M.rr = A.R[row];
-                if (a == 0) M.vold = Vin[s];
-            }
+        // row data of group g (clamped to the wave's last group, so always in range)
+        auto load_meta = [&](uint32_t g, Meta& M) {
+            const uint32_t s = min(s_begin + g * SPC + (uint32_t)sl, s_end - 1u);
+            const uint32_t row = s * na + min((uint32_t)a, na - 1u);
+            M.rb = ld_off(A.rp, row);
+            M.re = ld_off(A.rp, row + 1u);
+            M.rr = ld_off(A.R, row);
+            M.vold = ld_off(Vin, s);
         };
-        auto load_win = [&](uint32_t kw, int2(&e)[U]) {
-            const int2* p = nzp + (size_t)kw * WIN;
-#pragma unroll
-            for (int u = 0; u < U; u++) e[u] = p[u * 64];
-        };
-        auto gather = [&](const int2(&e)[U], float(&x)[U]) {
-#pragma unroll
-            for (int u = 0; u < U; u++) x[u] = Vin[e[u].x];
-        };
-
-        Meta m0, m1, m2;
-        int64_t g = 0;
+        Meta cur, nxt;
+        uint32_t g = 0;
+        load_meta(0, cur);
+        load_meta(min(1u, ngroups - 1u), nxt);
+        uint32_t ce = rps[min(s_begin + SPC, s_end) * na];
+        bool valid = (s_begin + (uint32_t)sl < s_end) && ((uint32_t)a < na);
         double sd = 0.0;
-        float sh = 0.0f, slo = 0.0f;
+
         auto finish_group = [&]() {
-            float q;
-            if (ACC == 3) {
-                q = (float)((double)m0.rr + (double)A.discount * sd);
-            } else {
-                float hs = sh, ls = slo;
-                if (A.discount != 1.0f) {
-                    hs *= A.discount;
-                    ls *= A.discount;
-                }
-                float tt, err;
-                two_sum(m0.rr, hs, tt, err);
-                q = tt + (err + ls);
-            }
-            if (!m0.valid || q != q) q = -INFINITY;
-            int ba = m0.valid ? a : 0x7fffffff;
+            float q = valid ? (float)((double)cur.rr + (double)A.discount * sd) : -INFINITY;
+            if (q != q) q = -INFINITY;
+            int ba = valid ? a : 0x7fffffff;
+            // max / first argmax over the NAP lanes of each state; the combine is
+            // associative and commutative, so any reduction tree gives the same result
+            if (NAP <= 16) {
+                // DPP row shifts: after shifts 1, 2, .., NAP/2 the state's last lane
+                // (a = NAP-1) has combined exactly its own NAP lanes
 #pragma unroll
-            for (int o = 1; o < NAP; o <<= 1) {
-                const float q2 = __shfl_xor(q, o, 64);
-                const int a2 = __shfl_xor(ba, o, 64);
-                if (q2 > q || (q2 == q && a2 < ba)) {
-                    q = q2;
-                    ba = a2;
+                for (int o = 1; o < NAP; o <<= 1) {
+                    const float q2 = dpp_row_shr(q, o, -INFINITY);
+                    const int a2 = dpp_row_shr(ba, o, 0x7fffffff);
+                    const bool take = q2 > q || (q2 == q && a2 < ba);
+                    q = take ? q2 : q;
+                    ba = take ? a2 : ba;
+                }
+            } else {
+#pragma unroll
+                for (int o = 1; o < NAP; o <<= 1) {
+                    const float q2 = __shfl_xor(q, o, 64);
+                    const int a2 = __shfl_xor(ba, o, 64);
+                    if (q2 > q || (q2 == q && a2 < ba)) {
+                        q = q2;
+                        ba = a2;
+                    }
                 }
             }
-            if (m0.valid && a == 0) {
-                const int64_t s = s_begin + g * SPC + sl;
+            const bool writer = (NAP <= 16) ? (a == NAP - 1) : (a == 0);
+            if (writer && g < ngroups && s_begin + g * SPC + (uint32_t)sl < s_end) {
+                const uint32_t s = s_begin + g * SPC + (uint32_t)sl;
                 float vnew;
                 int32_t pol;
                 if (q > -1e6f) {
@@ -528,72 +456,36 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_stream(const VIArgs A)
                 }
                 Vout[s] = vnew;
                 A.pi[s] = pol;
-                res = fmaxf(res, fabsf(vnew - m0.vold));
+                res = fmaxf(res, fabsf(vnew - cur.vold));
             }
             sd = 0.0;
-            sh = 0.0f;
-            slo = 0.0f;
             g++;
-            m0 = m1;
-            m1 = m2;
-            load_meta(g + 2, m2);
+            cur = nxt;
+            load_meta(min(g + 1u, ngroups - 1u), nxt);
+            const uint32_t s0 = s_begin + g * SPC;
+            ce = rps[min(s0 + SPC, s_end) * na];
+            valid = (g < ngroups) && (s0 + (uint32_t)sl < s_end) && ((uint32_t)a < na);
         };
 
-        int2 e0[U], e1[U], e2[U];
-        float x0[U], x1[U];
-        if (nwin > 0) load_win(0, e0);
-        if (nwin > 1) load_win(1, e1);
-        load_meta(0, m0);
-        load_meta(1, m1);
-        load_meta(2, m2);
-        if (nwin > 0) gather(e0, x0);
+        const int2* __restrict__ nzp = A.nz + P0 + lane;
         for (uint32_t kw = 0; kw < nwin; kw++) {
-            if (kw + 2 < nwin) load_win(kw + 2, e2);
-            if (kw + 1 < nwin) gather(e1, x1);
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const float v = __int_as_float(e0[u].y);
-                if (ACC == 3) {
-                    wsd[u * 64 + lane] = (double)v * (double)x0[u];  // exact: 24x24-bit product
-                } else {
-                    const float p = v * x0[u];
-                    wsf[u * 64 + lane] = make_float2(p, fmaf(v, x0[u], -p));
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
             const uint32_t w0 = P0 + kw * WIN, wend = w0 + WIN;
+            int2 e[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) e[u] = nzp[kw * WIN + u * 64];
+            float x[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) x[u] = ld_off(Vin, (uint32_t)e[u].x);
+#pragma unroll
+            for (int u = 0; u < U; u++) wsd[u * 64 + lane] = (double)__int_as_float(e[u].y) * (double)x[u];
+            __builtin_amdgcn_wave_barrier();
             for (;;) {
-                if (m0.valid) {  // this lane's row part inside the window, ascending
-                    const uint32_t jb = max(m0.rb, w0), je = min(m0.re, wend);
-                    if (ACC == 3) {
-                        uint32_t j = jb;
-                        for (; j + 2 <= je; j += 2) {
-                            const double d0 = wsd[j - w0], d1 = wsd[j + 1 - w0];
-                            sd += d0;
-                            sd += d1;
-                        }
-                        if (j < je) sd += wsd[j - w0];
-                    } else {
-                        for (uint32_t j = jb; j < je; j++) {
-                            const float2 pp = wsf[j - w0];
-                            float tt, err;
-                            two_sum(sh, pp.x, tt, err);
-                            sh = tt;
-                            slo = slo + (err + pp.y);
-                        }
-                    }
-                }
-                if (m0.ce > wend) break;  // the group continues in the next window
+                if (valid) row_part(wsd - w0, max(cur.rb, w0), min(cur.re, wend), sd);
+                if (ce > wend) break;  // the group continues in the next window
                 finish_group();
                 if (g >= ngroups) break;
             }
             __builtin_amdgcn_wave_barrier();
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                e0[u] = e1[u];
-                e1[u] = e2[u];
-                x0[u] = x1[u];
-            }
         }
         while (g < ngroups) finish_group();  // trailing groups whose rows are all empty
     }
@@ -633,29 +525,387 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_stream(const VIArgs A)
     }
 }
 
-// wpart[w] = first state of wave w: the smallest multiple of spc whose weight
-// rp[s*na] + 2*s*na reaches w/W of the total (monotone in w), wpart[W] = ns.
-__global__ void vi_partition(const uint32_t* __restrict__ rp, int64_t ns, int na, int64_t W, int spc,
-                             uint32_t* __restrict__ wpart) {
-    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (w > W) return;
-    if (w == 0 || w == W) {
-        wpart[w] = w == 0 ? 0u : (uint32_t)ns;
-        return;
+// ---------------------------------------------------------------------------------
+// Tiled sweep (the default). One block per tile of tile_states consecutive states;
+// each of its 4 waves streams the entries of a quarter of the tile exactly like
+// spmv_vi_lean. The difference is where the value gathers go: a 64-lane gather from
+// global memory costs the texture addresser about 5x the cycles of a coalesced load of
+// the same instruction count, and the gathers of a state go to a narrow band of next
+// states. So the block first copies the tile's band of V, V[cmin, cmin + span) (tdesc,
+// computed once per model), into LDS with coalesced loads, and the per-entry gathers
+// read LDS. A tile whose band is wider than the LDS window (vwin floats) gathers from
+// global memory instead. Entries that a window loads past its wave's rows are never
+// summed; their band index is clamped so that every LDS read stays in the window.
+template <int NAP, int U>
+__global__ __launch_bounds__(kStreamThreads) void spmv_vi_tile(const VIArgs A) {
+    constexpr int SPC = 64 / NAP;
+    constexpr int WIN = 64 * U;
+    extern __shared__ float vband[];  // [vwin]
+    __shared__ float wmax[kStreamWaves];
+    __shared__ int is_last;
+    __shared__ double stage[kStreamWaves][WIN + kRowReads];
+    VIState* st = A.st;
+    if (st->conv >= 0) return;  // converged at an earlier sweep
+    const uint32_t k = st->k;
+    const float* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
+    float* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int a = lane % NAP, sl = lane / NAP;
+    const uint32_t na = (uint32_t)A.na, ns = (uint32_t)A.ns;
+    const CPtr<uint32_t> rps = (CPtr<uint32_t>)A.rp;  // uniform reads of the row pointers
+    const CPtr<unsigned long long> tdq = (CPtr<unsigned long long>)A.tdesc;
+    const uint32_t tile = blockIdx.x;
+    const unsigned long long td = tdq[tile];
+    const uint32_t cmin = (uint32_t)td, span = (uint32_t)(td >> 32);
+    const bool in_lds = span > 0u && span <= A.vwin;  // block-uniform
+    if (in_lds) {
+        for (uint32_t i = tid; i < span; i += kStreamThreads) vband[i] = ld_off(Vin, cmin + i);
     }
-    const double total = (double)rp[ns * na] + 2.0 * (double)(ns * na);
-    const double target = total * (double)w / (double)W;
-    int64_t lo = 0, hi = ns;  // smallest s with weight(s) >= target
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) / 2;
-        const double wt = (double)rp[mid * na] + 2.0 * (double)(mid * na);
-        if (wt >= target)
-            hi = mid;
-        else
-            lo = mid + 1;
+    __syncthreads();
+    const uint32_t per_wave = A.tile_states / kStreamWaves;
+    const uint32_t s_begin = min(tile * A.tile_states + wv * per_wave, ns);
+    const uint32_t s_end = min(s_begin + per_wave, ns);
+    double* wsd = stage[wv];
+    float res = 0.0f;
+
+    if (s_begin < s_end) {
+        const uint32_t ngroups = (s_end - s_begin + SPC - 1) / SPC;
+        const uint32_t P0 = rps[s_begin * na], Pend = rps[s_end * na];
+        const uint32_t nwin = (Pend - P0 + WIN - 1) / WIN;
+        struct Meta {
+            uint32_t rb, re;
+            float rr, vold;
+        };
+        auto load_meta = [&](uint32_t g, Meta& M) {
+            const uint32_t s = min(s_begin + g * SPC + (uint32_t)sl, s_end - 1u);
+            const uint32_t row = s * na + min((uint32_t)a, na - 1u);
+            M.rb = ld_off(A.rp, row);
+            M.re = ld_off(A.rp, row + 1u);
+            M.rr = ld_off(A.R, row);
+            M.vold = ld_off(Vin, s);
+        };
+        Meta cur, nxt;
+        uint32_t g = 0;
+        load_meta(0, cur);
+        load_meta(min(1u, ngroups - 1u), nxt);
+        uint32_t ce = rps[min(s_begin + SPC, s_end) * na];
+        bool valid = (s_begin + (uint32_t)sl < s_end) && ((uint32_t)a < na);
+        double sd = 0.0;
+
+        auto finish_group = [&]() {
+            float q = valid ? (float)((double)cur.rr + (double)A.discount * sd) : -INFINITY;
+            if (q != q) q = -INFINITY;
+            int ba = valid ? a : 0x7fffffff;
+            if (NAP <= 16) {
+#pragma unroll
+                for (int o = 1; o < NAP; o <<= 1) {
+                    const float q2 = dpp_row_shr(q, o, -INFINITY);
+                    const int a2 = dpp_row_shr(ba, o, 0x7fffffff);
+                    const bool take = q2 > q || (q2 == q && a2 < ba);
+                    q = take ? q2 : q;
+                    ba = take ? a2 : ba;
+                }
+            } else {
+#pragma unroll
+                for (int o = 1; o < NAP; o <<= 1) {
+                    const float q2 = __shfl_xor(q, o, 64);
+                    const int a2 = __shfl_xor(ba, o, 64);
+                    if (q2 > q || (q2 == q && a2 < ba)) {
+                        q = q2;
+                        ba = a2;
+                    }
+                }
+            }
+            const bool writer = (NAP <= 16) ? (a == NAP - 1) : (a == 0);
+            if (writer && g < ngroups && s_begin + g * SPC + (uint32_t)sl < s_end) {
+                const uint32_t s = s_begin + g * SPC + (uint32_t)sl;
+                float vnew;
+                int32_t pol;
+                if (q > -1e6f) {
+                    vnew = q;
+                    pol = ba;
+                } else {
+                    vnew = -1e6f;
+                    pol = -1;
+                }
+                Vout[s] = vnew;
+                A.pi[s] = pol;
+                res = fmaxf(res, fabsf(vnew - cur.vold));
+            }
+            sd = 0.0;
+            g++;
+            cur = nxt;
+            load_meta(min(g + 1u, ngroups - 1u), nxt);
+            const uint32_t s0 = s_begin + g * SPC;
+            ce = rps[min(s0 + SPC, s_end) * na];
+            valid = (g < ngroups) && (s0 + (uint32_t)sl < s_end) && ((uint32_t)a < na);
+        };
+
+        const int2* __restrict__ nzp = A.nz + P0 + lane;
+        for (uint32_t kw = 0; kw < nwin; kw++) {
+            const uint32_t w0 = P0 + kw * WIN, wend = w0 + WIN;
+            int2 e[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) e[u] = nzp[kw * WIN + u * 64];
+            float x[U];
+            if (in_lds) {
+#pragma unroll
+                for (int u = 0; u < U; u++) x[u] = vband[min((uint32_t)e[u].x - cmin, span - 1u)];
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; u++) x[u] = ld_off(Vin, (uint32_t)e[u].x);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) wsd[u * 64 + lane] = (double)__int_as_float(e[u].y) * (double)x[u];
+            __builtin_amdgcn_wave_barrier();
+            for (;;) {
+                if (valid) row_part(wsd - w0, max(cur.rb, w0), min(cur.re, wend), sd);
+                if (ce > wend) break;  // the group continues in the next window
+                finish_group();
+                if (g >= ngroups) break;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        while (g < ngroups) finish_group();  // trailing groups whose rows are all empty
     }
-    int64_t s = (lo + spc / 2) / spc * spc;
-    wpart[w] = (uint32_t)(s < ns ? s : ns);
+
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) res = fmaxf(res, __shfl_xor(res, o, 64));
+    if (lane == 0) wmax[wv] = res;
+    __syncthreads();
+    if (tid == 0) {
+        float bm = 0.0f;
+#pragma unroll
+        for (int i = 0; i < kStreamWaves; i++) bm = fmaxf(bm, wmax[i]);
+        A.part[blockIdx.x] = bm;
+        __threadfence();
+        is_last = atomicAdd(&st->done, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (is_last) {  // the last block of the sweep: reduce the partials, publish
+        __threadfence();
+        float r = 0.0f;
+        for (unsigned i = tid; i < gridDim.x; i += kStreamThreads)
+            r = fmaxf(r, __hip_atomic_load(A.part + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) r = fmaxf(r, __shfl_xor(r, o, 64));
+        __syncthreads();
+        if (lane == 0) wmax[wv] = r;
+        __syncthreads();
+        if (tid == 0) {
+            float mx = 0.0f;
+#pragma unroll
+            for (int i = 0; i < kStreamWaves; i++) mx = fmaxf(mx, wmax[i]);
+            st->last_res = mx;
+            if (mx < A.thresh) st->conv = (int32_t)k;
+            st->done = 0u;
+            st->k = k + 1u;
+        }
+    }
+}
+
+// tdesc[t] = (smallest column, largest - smallest + 1) over the entries of tile t
+// (0 span for a tile without entries); *maxspan = the largest span.
+__global__ __launch_bounds__(256) void tile_band(const uint32_t* __restrict__ rp, const int2* __restrict__ nz,
+                                                 int64_t ns, int na, uint32_t ts, uint2* __restrict__ tdesc,
+                                                 unsigned* __restrict__ maxspan) {
+    __shared__ uint32_t smin[256], smax[256];
+    const int64_t t = blockIdx.x;
+    const int64_t s0 = t * ts, s1 = min(s0 + (int64_t)ts, ns);
+    const uint32_t b = rp[s0 * na], e = rp[s1 * na];
+    uint32_t mn = 0xffffffffu, mx = 0u;
+    for (uint32_t j = b + threadIdx.x; j < e; j += 256) {
+        const uint32_t c = (uint32_t)nz[j].x;
+        mn = min(mn, c);
+        mx = max(mx, c);
+    }
+    smin[threadIdx.x] = mn;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            smin[threadIdx.x] = min(smin[threadIdx.x], smin[threadIdx.x + o]);
+            smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t span = e > b ? smax[0] - smin[0] + 1u : 0u;
+        tdesc[t] = make_uint2(e > b ? smin[0] : 0u, span);
+        atomicMax(maxspan, span);
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Jagged sweep (the default). Layout (jds_convert, once per model): within each state
+// the na rows are ranked by length, longest first (ties by action), and the entries are
+// stored diagonal by diagonal: entry k of every row longer than k, in rank order, then
+// entry k+1, ... (a jagged-diagonal / sliced-ELL layout with one slice per state and no
+// padding). A state's entries keep their place [rp[s*na], rp[(s+1)*na]).
+// Mapping: one lane per row, NAP lanes per state, 64/NAP states per wave, grid-stride
+// over such chunks. At diagonal k the rows longer than k are ranks 0..c_k-1, so a lane
+// finds its entry at base + sum_{k'<k} c_k' + rank, with c_k counted by a ballot: no
+// row pointers per entry, no LDS, and each row's products are added in its own entry
+// order in a float64 register (the same bits as the other float64 sweeps). Loads of
+// rows shorter than the state's longest are masked, so the only waste is issue slots.
+template <int NAP, int UNR>
+__global__ __launch_bounds__(kStreamThreads) void spmv_vi_jagged(const VIArgs A) {
+    constexpr int SPC = 64 / NAP;
+    __shared__ float wmax[kStreamWaves];
+    __shared__ int is_last;
+    VIState* st = A.st;
+    if (st->conv >= 0) return;  // converged at an earlier sweep
+    const uint32_t k = st->k;
+    const float* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
+    float* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int rank = lane % NAP, sl = lane / NAP;
+    const uint32_t na = (uint32_t)A.na, ns = (uint32_t)A.ns;
+    const uint64_t gmask = (NAP == 64 ? ~0ull : ((1ull << NAP) - 1ull)) << (sl * NAP);
+    const uint32_t nchunks = (ns + SPC - 1) / SPC;
+    const uint32_t nwaves = gridDim.x * kStreamWaves;
+    float res = 0.0f;
+    for (uint32_t ch = blockIdx.x * kStreamWaves + wv; ch < nchunks; ch += nwaves) {
+        const uint32_t s = ch * SPC + (uint32_t)sl;
+        const bool vs = s < ns;
+        const bool vr = vs && (uint32_t)rank < na;
+        const uint32_t row = s * na + (uint32_t)rank;
+        const uint32_t meta = vr ? ld_off(A.jmeta, row) : 0u;
+        const uint32_t len = meta & 0xffffu, act = meta >> 16;
+        const float rr = vr ? ld_off(A.jR, row) : 0.0f;
+        const uint32_t base = vs ? ld_off(A.rp, s * na) : 0u;
+        const float vold = vs ? ld_off(Vin, s) : 0.0f;
+        double acc = 0.0;
+        uint32_t off = 0;
+        for (uint32_t kd = 0;; kd += UNR) {
+            bool on[UNR];
+            uint32_t pos[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; u++) {
+                on[u] = kd + u < len;
+                const uint64_t bm = __ballot(on[u]);
+                pos[u] = base + off + (uint32_t)rank;
+                off += (uint32_t)__popcll(bm & gmask);
+            }
+            if (__ballot(on[0]) == 0ull) break;  // wave-uniform: no row has entry kd
+            int2 e[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; u++) e[u] = on[u] ? ld_off(A.jnz, pos[u]) : make_int2(0, 0);
+            float x[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; u++) x[u] = on[u] ? ld_off(Vin, (uint32_t)e[u].x) : 0.0f;
+#pragma unroll
+            for (int u = 0; u < UNR; u++)
+                if (on[u]) acc += (double)__int_as_float(e[u].y) * (double)x[u];  // exact product
+        }
+        float q = vr ? (float)((double)rr + (double)A.discount * acc) : -INFINITY;
+        if (q != q) q = -INFINITY;
+        int ba = vr ? (int)act : 0x7fffffff;
+        if (NAP <= 16) {
+#pragma unroll
+            for (int o = 1; o < NAP; o <<= 1) {
+                const float q2 = dpp_row_shr(q, o, -INFINITY);
+                const int a2 = dpp_row_shr(ba, o, 0x7fffffff);
+                const bool take = q2 > q || (q2 == q && a2 < ba);
+                q = take ? q2 : q;
+                ba = take ? a2 : ba;
+            }
+        } else {
+#pragma unroll
+            for (int o = 1; o < NAP; o <<= 1) {
+                const float q2 = __shfl_xor(q, o, 64);
+                const int a2 = __shfl_xor(ba, o, 64);
+                if (q2 > q || (q2 == q && a2 < ba)) {
+                    q = q2;
+                    ba = a2;
+                }
+            }
+        }
+        const bool writer = (NAP <= 16) ? (rank == NAP - 1) : (rank == 0);
+        if (writer && vs) {
+            float vnew;
+            int32_t pol;
+            if (q > -1e6f) {
+                vnew = q;
+                pol = ba;
+            } else {
+                vnew = -1e6f;
+                pol = -1;
+            }
+            Vout[s] = vnew;
+            A.pi[s] = pol;
+            res = fmaxf(res, fabsf(vnew - vold));
+        }
+    }
+
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) res = fmaxf(res, __shfl_xor(res, o, 64));
+    if (lane == 0) wmax[wv] = res;
+    __syncthreads();
+    if (tid == 0) {
+        float bm = 0.0f;
+#pragma unroll
+        for (int i = 0; i < kStreamWaves; i++) bm = fmaxf(bm, wmax[i]);
+        A.part[blockIdx.x] = bm;
+        __threadfence();
+        is_last = atomicAdd(&st->done, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (is_last) {  // the last block of the sweep: reduce the partials, publish
+        __threadfence();
+        float r = 0.0f;
+        for (unsigned i = tid; i < gridDim.x; i += kStreamThreads)
+            r = fmaxf(r, __hip_atomic_load(A.part + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) r = fmaxf(r, __shfl_xor(r, o, 64));
+        __syncthreads();
+        if (lane == 0) wmax[wv] = r;
+        __syncthreads();
+        if (tid == 0) {
+            float mx = 0.0f;
+#pragma unroll
+            for (int i = 0; i < kStreamWaves; i++) mx = fmaxf(mx, wmax[i]);
+            st->last_res = mx;
+            if (mx < A.thresh) st->conv = (int32_t)k;
+            st->done = 0u;
+            st->k = k + 1u;
+        }
+    }
+}
+
+// CSR -> per-state jagged-diagonal layout; one wave per state, one lane per row
+// (na <= 64). Rank = place in (length descending, action ascending).
+__global__ __launch_bounds__(256) void jds_convert(const uint32_t* __restrict__ rp, const int2* __restrict__ nz,
+                                                   const float* __restrict__ R, int64_t ns, int na,
+                                                   int2* __restrict__ jnz, uint32_t* __restrict__ jmeta,
+                                                   float* __restrict__ jR) {
+    const int lane = threadIdx.x & 63;
+    const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (s >= ns) return;  // wave-uniform
+    const bool valid = lane < na;
+    const int64_t row = s * na + lane;
+    const uint32_t rb = valid ? rp[row] : 0u;
+    const uint32_t len = valid ? rp[row + 1] - rb : 0u;
+    uint32_t rank = 0, maxlen = len;
+    for (int b = 0; b < na; b++) {
+        const uint32_t lb = (uint32_t)__shfl((int)len, b, 64);
+        if (b != lane && (lb > len || (lb == len && b < lane))) rank++;
+        maxlen = max(maxlen, lb);
+    }
+    if (valid) {
+        jmeta[s * na + rank] = len | ((uint32_t)lane << 16);
+        jR[s * na + rank] = R[row];
+    }
+    const uint32_t base = rp[s * na];
+    uint32_t off = 0;
+    for (uint32_t kd = 0; kd < maxlen; kd++) {
+        const bool on = valid && kd < len;
+        const uint32_t c = (uint32_t)__popcll(__ballot(on));
+        if (on) jnz[base + off + rank] = nz[rb + kd];
+        off += c;
+    }
 }
 
 __global__ void vi_init(float* V0, int64_t ns, VIState* st) {
@@ -691,18 +941,6 @@ SweepFn pick_nap(int nap) {
     }
 }
 
-SweepFn pick_rows(int nap) {
-    switch (nap) {
-        case 1: return spmv_vi_rows<1>;
-        case 2: return spmv_vi_rows<2>;
-        case 4: return spmv_vi_rows<4>;
-        case 8: return spmv_vi_rows<8>;
-        case 16: return spmv_vi_rows<16>;
-        case 32: return spmv_vi_rows<32>;
-        default: return spmv_vi_rows<64>;
-    }
-}
-
 template <int ACC>
 SweepFn pick_u(int nap, int u, int* u_used) {
     if (u <= 2) {
@@ -717,47 +955,108 @@ SweepFn pick_u(int nap, int u, int* u_used) {
     return pick_nap<8, ACC>(nap);
 }
 
-template <int U, int ACC>
-SweepFn pick_stream_nap(int nap) {
+template <int U>
+SweepFn pick_lean_nap(int nap) {
     switch (nap) {
-        case 1: return spmv_vi_stream<1, U, ACC>;
-        case 2: return spmv_vi_stream<2, U, ACC>;
-        case 4: return spmv_vi_stream<4, U, ACC>;
-        case 8: return spmv_vi_stream<8, U, ACC>;
-        case 16: return spmv_vi_stream<16, U, ACC>;
-        case 32: return spmv_vi_stream<32, U, ACC>;
-        default: return spmv_vi_stream<64, U, ACC>;
+        case 1: return spmv_vi_lean<1, U>;
+        case 2: return spmv_vi_lean<2, U>;
+        case 4: return spmv_vi_lean<4, U>;
+        case 8: return spmv_vi_lean<8, U>;
+        case 16: return spmv_vi_lean<16, U>;
+        case 32: return spmv_vi_lean<32, U>;
+        default: return spmv_vi_lean<64, U>;
     }
 }
 
-template <int ACC>
-SweepFn pick_stream(int nap, int u, int* u_used) {
+SweepFn pick_lean(int nap, int u, int* u_used) {
     if (u <= 2) {
         *u_used = 2;
-        return pick_stream_nap<2, ACC>(nap);
+        return pick_lean_nap<2>(nap);
     }
     if (u <= 4) {
         *u_used = 4;
-        return pick_stream_nap<4, ACC>(nap);
+        return pick_lean_nap<4>(nap);
     }
     *u_used = 8;
-    return pick_stream_nap<8, ACC>(nap);
+    return pick_lean_nap<8>(nap);
 }
 
-// 0-based variants: 0 staged, compensated; 1 staged, fma; 2 row per lane, fma;
-// 3 staged, float64 products and sums; 4 streaming, compensated; 5 streaming, float64
-constexpr int kDefaultVariant = 5;
-bool is_stream_variant(int variant) { return variant == 4 || variant == 5; }
+template <int U>
+SweepFn pick_tile_nap(int nap) {
+    switch (nap) {
+        case 1: return spmv_vi_tile<1, U>;
+        case 2: return spmv_vi_tile<2, U>;
+        case 4: return spmv_vi_tile<4, U>;
+        case 8: return spmv_vi_tile<8, U>;
+        case 16: return spmv_vi_tile<16, U>;
+        case 32: return spmv_vi_tile<32, U>;
+        default: return spmv_vi_tile<64, U>;
+    }
+}
+
+SweepFn pick_tile(int nap, int u, int* u_used) {
+    if (u <= 2) {
+        *u_used = 2;
+        return pick_tile_nap<2>(nap);
+    }
+    if (u <= 4) {
+        *u_used = 4;
+        return pick_tile_nap<4>(nap);
+    }
+    *u_used = 8;
+    return pick_tile_nap<8>(nap);
+}
+
+template <int UNR>
+SweepFn pick_jagged_nap(int nap) {
+    switch (nap) {
+        case 1: return spmv_vi_jagged<1, UNR>;
+        case 2: return spmv_vi_jagged<2, UNR>;
+        case 4: return spmv_vi_jagged<4, UNR>;
+        case 8: return spmv_vi_jagged<8, UNR>;
+        case 16: return spmv_vi_jagged<16, UNR>;
+        case 32: return spmv_vi_jagged<32, UNR>;
+        default: return spmv_vi_jagged<64, UNR>;
+    }
+}
+
+SweepFn pick_jagged(int nap, int u, int* u_used) {
+    if (u <= 2) {
+        *u_used = 2;
+        return pick_jagged_nap<2>(nap);
+    }
+    if (u <= 4) {
+        *u_used = 4;
+        return pick_jagged_nap<4>(nap);
+    }
+    *u_used = 8;
+    return pick_jagged_nap<8>(nap);
+}
+
+// LDS value window of the tiled sweep: at most this many floats (32 KB)
+constexpr uint32_t kMaxBand = 8192;
+
+// Sweep kernels (0-based; the C ABI numbers them from 1):
+//   0 chunked, compensated float   (spmv_vi_sweep<.., 0>)
+//   1 chunked, fma float           (spmv_vi_sweep<.., 1>; does not meet 1e-5 over long horizons)
+//   2 lean streaming, float64      (spmv_vi_lean; the default)
+//   3 tiled with an LDS value band (spmv_vi_tile)
+//   4 jagged-diagonal, float64     (spmv_vi_jagged)
+constexpr int kDefaultVariant = 2;
+constexpr int kLeanBlocksPerCU = 5;  // fastest resident grid of the lean sweep (100x100x120 model)
+bool is_lean_variant(int variant) { return variant == 2; }
+bool is_tile_variant(int variant) { return variant == 3; }
+bool is_jagged_variant(int variant) { return variant == 4; }
 
 SweepFn pick_sweep(int na, int variant, int u, int* u_used) {
     const int nap = next_pow2(na);
     switch (variant) {
+        case 0: return pick_u<0>(nap, u, u_used);
         case 1: return pick_u<1>(nap, u, u_used);
-        case 2: *u_used = 4; return pick_rows(nap);
-        case 3: return pick_u<3>(nap, u, u_used);
-        case 4: return pick_stream<0>(nap, u, u_used);
-        case 5: return pick_stream<3>(nap, u, u_used);
-        default: return pick_u<0>(nap, u, u_used);
+        case 2: return pick_lean(nap, u, u_used);
+        case 3: return pick_tile(nap, u, u_used);
+        case 4: return pick_jagged(nap, u, u_used);
+        default: return nullptr;
     }
 }
 
@@ -783,31 +1082,105 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
             set_error("vi_solve: device allocation failed");
             return E2E_ERR_NOMEM;
         }
+        m->vi_part_cap = kVIMaxBlocks;
     }
     const int nap = next_pow2((int)m->na);
     const int grp = 64 / nap;
     const int variant = opts.kernel_variant > 0 ? opts.kernel_variant - 1 : kDefaultVariant;
-    const bool stream = is_stream_variant(variant);
+    const bool lean = is_lean_variant(variant);
+    const bool tiled = is_tile_variant(variant);
+    const bool jagged = is_jagged_variant(variant);
     // loads in flight per lane. Chunked kernels: a chunk holds about 64/NAP states of
-    // nnz/row * na entries. Streaming kernel: 4 (two windows of 256 entries in flight).
+    // nnz/row * na entries; the other kernels: 4 (windows of 256 entries).
     const double per_chunk = (m->rows ? (double)m->nnz / (double)m->rows : 1.0) * m->na * grp;
     int u_req = opts.loads_in_flight > 0 ? opts.loads_in_flight
-                : stream ? 4 : (per_chunk > 192 ? 8 : per_chunk > 96 ? 4 : 2);
+                : (lean || tiled || jagged) ? 4 : (per_chunk > 192 ? 8 : per_chunk > 96 ? 4 : 2);
     int lpr = 0;
     SweepFn fn = pick_sweep((int)m->na, variant, u_req, &lpr);
-    const int threads = stream ? kStreamThreads : kVIThreads;
+    E2E_CHECK(fn != nullptr, E2E_ERR_INVALID, "vi_solve: unknown kernel_variant %d", opts.kernel_variant);
+    const int threads = (lean || tiled || jagged) ? kStreamThreads : kVIThreads;
+    if (jagged) {
+        E2E_CHECK(ns * (int64_t)sizeof(float) < (1ll << 32) && (m->rows + 1) * 4 < (1ll << 32) &&
+                      (m->nnz + kNzPad) * 8 < (1ll << 32),
+                  E2E_ERR_CAPACITY, "vi_solve: the jagged sweep addresses entries with 32-bit byte offsets");
+        E2E_CHECK(m->na <= 64 && m->info.max_row_nnz < 65536, E2E_ERR_CAPACITY,
+                  "vi_solve: the jagged sweep needs at most 64 actions and rows shorter than 65536");
+        if (!m->vi_jnz) {
+            if (hipMalloc(&m->vi_jnz, sizeof(int2) * (m->nnz + kNzPad)) != hipSuccess ||
+                hipMalloc(&m->vi_jmeta, sizeof(uint32_t) * m->rows) != hipSuccess ||
+                hipMalloc(&m->vi_jR, sizeof(float) * m->rows) != hipSuccess) {
+                set_error("vi_solve: device allocation of the jagged layout failed");
+                return E2E_ERR_NOMEM;
+            }
+            E2E_HIP(hipMemsetAsync(m->vi_jnz + m->nnz, 0, sizeof(int2) * kNzPad, s));
+            hipLaunchKernelGGL(jds_convert, dim3((unsigned)ceil_div(ns, 4)), dim3(256), 0, s, m->row_ptr, m->nz,
+                               m->R, ns, (int)m->na, m->vi_jnz, m->vi_jmeta, m->vi_jR);
+            E2E_HIP(hipGetLastError());
+        }
+    }
+    // tiled sweep: one block per tile of 16 state groups per wave, an LDS window sized to
+    // the widest column band (once per model)
+    uint32_t vwin = 0, tile_states = 0;
+    int64_t ntiles = 0;
+    if (tiled) {
+        E2E_CHECK(ns * (int64_t)sizeof(float) < (1ll << 32) && (m->rows + 1) * 4 < (1ll << 32), E2E_ERR_CAPACITY,
+                  "vi_solve: the tiled sweep addresses states and rows with 32-bit byte offsets");
+        tile_states = (uint32_t)(16 * kStreamWaves * grp);
+        ntiles = ceil_div(ns, (int64_t)tile_states);
+        if (m->vi_tile_states != tile_states) {
+            if (m->vi_tdesc) (void)hipFree(m->vi_tdesc);
+            m->vi_tdesc = nullptr;
+            m->vi_tile_states = 0;
+            if (hipMalloc(&m->vi_tdesc, sizeof(uint2) * ntiles) != hipSuccess) {
+                m->vi_tdesc = nullptr;
+                set_error("vi_solve: device allocation failed");
+                return E2E_ERR_NOMEM;
+            }
+            DevBuf mx;
+            E2E_TRY(dmalloc(mx, sizeof(unsigned)));
+            E2E_HIP(hipMemsetAsync(mx.p, 0, sizeof(unsigned), s));
+            hipLaunchKernelGGL(tile_band, dim3((unsigned)ntiles), dim3(256), 0, s, m->row_ptr, m->nz, ns, (int)m->na,
+                               tile_states, m->vi_tdesc, mx.as<unsigned>());
+            E2E_HIP(hipGetLastError());
+            unsigned hmax = 0;
+            E2E_HIP(hipMemcpyAsync(&hmax, mx.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+            E2E_HIP(hipStreamSynchronize(s));
+            m->vi_maxspan = hmax;
+            m->vi_tile_states = tile_states;
+        }
+        vwin = (uint32_t)std::min<uint64_t>((m->vi_maxspan + 63u) / 64u * 64u, kMaxBand);
+        if (vwin == 0) vwin = 64;
+        if (m->vi_part_cap < ntiles) {
+            (void)hipFree(m->vi_part);
+            m->vi_part = nullptr;
+            m->vi_part_cap = 0;
+            if (hipMalloc(&m->vi_part, sizeof(float) * ntiles) != hipSuccess) {
+                m->vi_part = nullptr;
+                set_error("vi_solve: device allocation failed");
+                return E2E_ERR_NOMEM;
+            }
+            m->vi_part_cap = ntiles;
+        }
+    }
+    const size_t shmem = (size_t)vwin * sizeof(float);
     // persistent grid: as many blocks as can be resident at once (grid-stride over chunks,
     // or one contiguous share per wave), so that no block starts after the others finished
     const int64_t states_per_block = (threads / 64) * grp;
     int bpc = opts.blocks_per_cu;
-    if (bpc <= 0) E2E_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void*)fn, threads, 0));
+    if (bpc <= 0) {
+        E2E_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void*)fn, threads, shmem));
+        if (lean) bpc = std::min(bpc, kLeanBlocksPerCU);
+    }
     int ncu = 0, dev = 0;
     E2E_HIP(hipGetDevice(&dev));
     E2E_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     const int64_t resident = (int64_t)std::max(bpc, 1) * std::max(ncu, 1);
     const int64_t nblocks =
-        std::min<int64_t>(std::min<int64_t>(ceil_div(ns, states_per_block), resident), kVIMaxBlocks);
-    if (stream) {
+        tiled ? ntiles
+              : std::min<int64_t>(std::min<int64_t>(ceil_div(ns, states_per_block), resident), kVIMaxBlocks);
+    if (lean) {
+        E2E_CHECK(ns * (int64_t)sizeof(float) < (1ll << 32) && (m->rows + 1) * 4 < (1ll << 32), E2E_ERR_CAPACITY,
+                  "vi_solve: the lean sweep addresses states and rows with 32-bit byte offsets");
         const int64_t W = nblocks * kStreamWaves;
         if (m->vi_wpart_waves != W || m->vi_wpart_spc != grp) {
             if (m->vi_wpart) (void)hipFree(m->vi_wpart);
@@ -825,15 +1198,35 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
             m->vi_wpart_spc = grp;
         }
     }
-    VIArgs A{m->row_ptr, m->nz, m->R, m->V[0], m->V[1], m->pi, m->st, m->vi_part, ns, (int)m->na,
-             opts.thresh, opts.discount, stream ? m->vi_wpart : nullptr};
+    VIArgs A;
+    std::memset(&A, 0, sizeof(A));  // padding too: the graph cache compares the bytes
+    A.rp = m->row_ptr;
+    A.nz = m->nz;
+    A.R = m->R;
+    A.V0 = m->V[0];
+    A.V1 = m->V[1];
+    A.pi = m->pi;
+    A.st = m->st;
+    A.part = m->vi_part;
+    A.ns = ns;
+    A.na = (int)m->na;
+    A.thresh = opts.thresh;
+    A.discount = opts.discount;
+    A.wpart = lean ? m->vi_wpart : nullptr;
+    A.tdesc = tiled ? m->vi_tdesc : nullptr;
+    A.tile_states = tile_states;
+    A.vwin = vwin;
+    A.jnz = jagged ? m->vi_jnz : nullptr;
+    A.jmeta = jagged ? m->vi_jmeta : nullptr;
+    A.jR = jagged ? m->vi_jR : nullptr;
 
     hipLaunchKernelGGL(vi_init, dim3((unsigned)ceil_div(ns, 256)), dim3(256), 0, s, m->V[0], ns, m->st);
     E2E_HIP(hipGetLastError());
 
     if (opts.use_graph) {
+        static_assert(sizeof(VIArgs) <= sizeof(m->graph_args), "graph key");
         if (!m->graph || m->graph_chunk != spc || m->graph_fn != (void*)fn || m->graph_blocks != nblocks ||
-            m->graph_thresh != opts.thresh || m->graph_discount != opts.discount) {
+            std::memcmp(m->graph_args, &A, sizeof(A)) != 0) {
             if (m->graph) {
                 (void)hipGraphExecDestroy(m->graph);
                 m->graph = nullptr;
@@ -841,7 +1234,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
             hipGraph_t gr;
             E2E_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             for (int i = 0; i < spc; i++)
-                hipLaunchKernelGGL(fn, dim3((unsigned)nblocks), dim3(threads), 0, s, A);
+                hipLaunchKernelGGL(fn, dim3((unsigned)nblocks), dim3(threads), shmem, s, A);
             E2E_HIP(hipStreamEndCapture(s, &gr));
             hipError_t e = hipGraphInstantiate(&m->graph, gr, nullptr, nullptr, 0);
             (void)hipGraphDestroy(gr);
@@ -853,6 +1246,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
             m->graph_chunk = spc;
             m->graph_lpr = lpr;
             m->graph_blocks = nblocks;
+            std::memcpy(m->graph_args, &A, sizeof(A));
             m->graph_thresh = opts.thresh;
             m->graph_discount = opts.discount;
             m->graph_fn = (void*)fn;
@@ -872,7 +1266,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
         if (opts.use_graph && n == spc) {
             E2E_HIP(hipGraphLaunch(m->graph, s));
         } else {
-            for (int i = 0; i < n; i++) hipLaunchKernelGGL(fn, dim3((unsigned)nblocks), dim3(threads), 0, s, A);
+            for (int i = 0; i < n; i++) hipLaunchKernelGGL(fn, dim3((unsigned)nblocks), dim3(threads), shmem, s, A);
             E2E_HIP(hipGetLastError());
         }
         E2E_HIP(hipEventRecord(e1, s));

@@ -128,9 +128,11 @@ typedef struct e2e_solve_opts {
     int32_t use_graph;         /* 1: replay the sweep chain as a captured hipGraph */
     int32_t loads_in_flight;   /* nnz loads per lane per window: 2, 4 or 8 (0: chosen from nnz per state) */
     double timeout_ms;         /* 0: none */
-    int32_t kernel_variant;    /* 0: default; 1 staged + compensated, 2 staged + fma, 3 row per lane + fma,
-                                  4 staged + float64 products and sums */
-    int32_t blocks_per_cu;     /* resident sweep blocks per compute unit (0: the occupancy limit) */
+    int32_t kernel_variant;    /* sweep kernel; 0: default (3). 1 chunked + compensated float, 2 chunked +
+                                  fma float (faster; exceeds 1e-5 over long horizons), 3 lean streaming +
+                                  float64, 4 tiled with an LDS value band + float64, 5 jagged-diagonal +
+                                  float64; other values are an error */
+    int32_t blocks_per_cu;     /* resident sweep blocks per compute unit (0: chosen by the library) */
 } e2e_solve_opts;
 
 typedef struct e2e_solve_stats {

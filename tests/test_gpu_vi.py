@@ -89,9 +89,11 @@ def test_vi_not_converged_status(gpu):
         assert st["status"] == 4 and st["sweeps"] == 3 and not st["converged"]
 
 
-# every sweep kernel, with the loads-in-flight settings it supports; the fma variants
-# (2, 3) round each product, so they get a wider bound on these short horizons
-VARIANTS = [(1, 2), (1, 8), (2, 4), (3, 0), (4, 4), (5, 2), (5, 4), (5, 8), (6, 2), (6, 4), (6, 8)]
+# every sweep kernel with the loads-in-flight settings it supports: 1 chunked +
+# compensated float, 2 chunked + fma float, 3 lean streaming, 4 tiled with an LDS value
+# band, 5 jagged-diagonal (3-5 in float64). Variant 2 rounds each product, so it gets a
+# wider bound on these short horizons.
+VARIANTS = [(1, 2), (1, 8), (2, 4), (3, 2), (3, 4), (3, 8), (4, 2), (4, 4), (4, 8), (5, 2), (5, 4), (5, 8)]
 
 
 @pytest.mark.parametrize("variant,lif", VARIANTS)
@@ -102,7 +104,7 @@ def test_vi_kernel_variants(gpu, variant, lif):
     with e2e.build_model(w.params, f) as m:
         pol, val, st = m.solve(thresh=1e-6, kernel_variant=variant, loads_in_flight=lif)
         assert st["kernel_variant"] == variant and st["converged"]
-        _check_solve(ref, pol, val, st, tol=TOL if variant not in (2, 3) else 4 * TOL)
+        _check_solve(ref, pol, val, st, tol=TOL if variant != 2 else 4 * TOL)
 
 
 def _random_coo(ns, na, seed, empty_frac=0.2):
@@ -127,25 +129,29 @@ def _random_coo(ns, na, seed, empty_frac=0.2):
     return (np.array(S1, np.int32), np.array(S2, np.int32), np.array(val, np.float32), R)
 
 
-@pytest.mark.parametrize("na", [1, 3, 16, 33])
-def test_vi_streaming_matches_chunked_bits(gpu, na):
-    """The streaming kernels add the same products in the same order as the chunked
-    ones, so their values and policies are bit-identical; empty rows and dead states
-    (every Q below -1e6: value -1e6, policy -1, as in spvi_2.cu:93-122) included."""
-    ns = 3001
+@pytest.mark.parametrize("ns,na", [(3001, 1), (3001, 3), (3001, 16), (3001, 33), (20011, 3)])
+def test_vi_float64_kernels_agree_bitwise(gpu, ns, na):
+    """The lean, tiled and jagged sweeps form the same exact float64 products and add
+    each row's in its entry order, so their values and policies are bit-identical;
+    empty rows and dead states (every Q below -1e6: value -1e6, policy -1, as in
+    spvi_2.cu:93-122) included. Columns are uniform over all states, so with 20011
+    states the tiles' column bands exceed the LDS window and the tiled kernel gathers
+    from global memory."""
     S1, S2, val, R = _random_coo(ns, na, seed=60 + na)
     with e2e.import_coo(ns, na, S1, S2, val, R) as m:
-        for a_, b_ in ((1, 5), (4, 6)):
-            pa, va, sa = m.solve(thresh=1e-3, max_sweeps=6, kernel_variant=a_, raise_on_nonconv=False)
+        pa, va, sa = m.solve(thresh=1e-3, max_sweeps=6, kernel_variant=3, raise_on_nonconv=False)
+        for b_ in (4, 5):
             pb, vb, sb = m.solve(thresh=1e-3, max_sweeps=6, kernel_variant=b_, raise_on_nonconv=False)
             assert sa["sweeps"] == sb["sweeps"]
-            assert np.array_equal(va.view(np.uint32), vb.view(np.uint32)), (a_, b_)
-            assert np.array_equal(pa, pb), (a_, b_)
+            assert np.array_equal(va.view(np.uint32), vb.view(np.uint32)), b_
+            assert np.array_equal(pa, pb), b_
+        pc, vc, sc = m.solve(thresh=1e-3, max_sweeps=6, kernel_variant=1, raise_on_nonconv=False)
+        assert rel_err(vc, va.astype(np.float64)).max() <= TOL
         V64, pi64, it64, _ = oracle.vi_sweeps_f64(ns, na, S1, S2, val, R, 1e-3, 6)
-        assert rel_err(vb, V64).max() <= TOL
+        assert rel_err(va, V64).max() <= TOL
         assert (pi64 == -1).sum() >= ns // 10
-        assert (pb.astype(np.int64)[pi64 == -1] == 0xFFFFFFFF).all()
-        assert (vb[pi64 == -1] == -1e6).all()
+        assert (pa.astype(np.int64)[pi64 == -1] == 0xFFFFFFFF).all()
+        assert (va[pi64 == -1] == -1e6).all()
 
 
 def test_vi_c2_accuracy(gpu):
@@ -165,3 +171,11 @@ def test_vi_c2_accuracy(gpu):
     assert err.max() <= TOL
     Q64 = q_values_f64(ns, na, S1, S2, pv, R, V64)
     policy_ties_ok(pol, pi64, Q64, 4 * TOL)
+
+
+def test_vi_unknown_variant_is_an_error(gpu):
+    w = small(g=6, nt=4, nr=50, nm=1, seed=71)
+    with e2e.build_model(w.params, w.fields()) as m:
+        with pytest.raises(e2e.E2EError) as ei:
+            m.solve(kernel_variant=6)
+        assert ei.value.code == 1
