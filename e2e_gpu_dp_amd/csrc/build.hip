@@ -49,6 +49,7 @@ struct BuildArgs {
     int g, nc, nt, nm, nr, na;
     int i_term, j_term, tsg, reward_type, m;
     float dt, r_out, r_term, alpha;
+    float int_lim;             // integer reward-sum path: |reward| < int_lim, so nr of them fit int32
     int t0, K, radix_bits, R, tc;
     int stage_y;
     uint2* staging;
@@ -128,26 +129,19 @@ template <bool UNIT>
 __device__ __forceinline__ void lane_move(const BuildArgs& A, float xc, float yc, float fc,
                                           float fs, float vx, float vy, int T, int& pi,
                                           int& pj, float& r) {
+    // Written with selects instead of the reference's if/else-if ladders; every float
+    // operation and its operands are the same, so the results are the same bits.
     const int n = A.g - 1;
-    float vnetx = fc + vx;
-    float vnety = fs + vy;
+    const float vnetx = fc + vx;
+    const float vnety = fs + vy;
     float xnew = xc + (vnetx * A.dt);
     float ynew = yc + (vnety * A.dt);
-    r = 0.0f;
-    if (xnew > A.xsn) {
-        xnew = A.xsn;
-        r += A.r_out;
-    } else if (xnew < A.xs0) {
-        xnew = A.xs0;
-        r += A.r_out;
-    }
-    if (ynew > A.ysn) {
-        ynew = A.ysn;
-        r += A.r_out;
-    } else if (ynew < A.ys0) {
-        ynew = A.ys0;
-        r += A.r_out;
-    }
+    const bool xh = xnew > A.xsn, xl = xnew < A.xs0;  // :296-303
+    xnew = xh ? A.xsn : (xl ? A.xs0 : xnew);
+    r = (xh || xl) ? 0.0f + A.r_out : 0.0f;
+    const bool yh = ynew > A.ysn, yl = ynew < A.ys0;  // :304-311
+    ynew = yh ? A.ysn : (yl ? A.ys0 : ynew);
+    r = (yh || yl) ? r + A.r_out : r;
     const float ax = xnew - A.xs0;
     const float ay = -(ynew - A.ysn);
     int xind, yind;
@@ -168,27 +162,33 @@ __device__ __forceinline__ void lane_move(const BuildArgs& A, float xc, float yc
     }
     pi = yind;
     pj = xind;
-    if (pi == 0 || pi == n || pj == 0 || pj == n) r += A.r_out;
-    if (is_terminal(A, pi, pj))
-        r += A.r_term;
-    else if (T > A.nt - 20)
-        r = (float)((double)r + A.late_pen);
+    const bool edge = pi == 0 || pi == n || pj == 0 || pj == n;  // :339-341
+    r = edge ? r + A.r_out : r;
+    const float late = (float)((double)r + A.late_pen);           // :343-354
+    r = is_terminal(A, pi, pj) ? r + A.r_term : (T > A.nt - 20 ? late : r);
 }
 
+// Wave reductions with DPP: row shifts 1, 2, 4, 8 leave each row's total in its lane 15,
+// row_bcast:15 / row_bcast:31 carry rows 0 and 2 into 1 and 3, so lane 63 holds the
+// total; readlane makes it wave-uniform. Lanes without a source take the identity.
+template <typename Op>
+__device__ __forceinline__ int wave_reduce(int x, int id, Op op) {
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x111, 0xf, 0xf, false));
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x112, 0xf, 0xf, false));
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x114, 0xf, 0xf, false));
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x118, 0xf, 0xf, false));
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x142, 0xa, 0xf, false));
+    x = op(x, __builtin_amdgcn_update_dpp(id, x, 0x143, 0xc, 0xf, false));
+    return __builtin_amdgcn_readlane(x, 63);
+}
 __device__ __forceinline__ int wave_min(int x) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
-    return x;
+    return wave_reduce(x, 0x7fffffff, [](int a, int b) { return min(a, b); });
 }
 __device__ __forceinline__ int wave_max(int x) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
-    return x;
+    return wave_reduce(x, (int)0x80000000, [](int a, int b) { return max(a, b); });
 }
 __device__ __forceinline__ int wave_sum(int x) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
-    return x;
+    return wave_reduce(x, 0, [](int a, int b) { return a + b; });
 }
 
 // Exclusive scan over the NW waves of a block. lds: >= NW ints. Two barriers.
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
                     rwl[w] = rw;
                     // integer-valued rewards sum exactly in int in any order, which equals
                     // the canonical double sum
-                    if (rw == rintf(rw) && fabsf(rw) < 1048576.0f) ipart += (int)rw;
+                    if (rw == rintf(rw) && fabsf(rw) < A.int_lim) ipart += (int)rw;
                     else allint = false;
                 }
             }
@@ -420,33 +420,24 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
             const uint32_t Wj = (uint32_t)(djmax - djmin + 1);
             const uint64_t nkeys = (uint64_t)(dimax - dimin + 1) * Wj;
             const int nbits = nkeys <= 1 ? 0 : 64 - __clzll((long long)(nkeys - 1));
-            for (int r = 0; r < R; r++) {
-                const int w = r * WG + tid;
-                if (w < nr) {
-                    const uint32_t k0 = keysA[w];
-                    const int di = (int)(k0 >> 16) - 32768, dj = (int)(k0 & 0xffffu) - 32768;
-                    keysA[w] = (uint32_t)(di - dimin) * Wj + (uint32_t)(dj - djmin);
-                }
-            }
             const size_t row = mid * A.na + a;
             int nuq = 0;
             unsigned long long arena_off = 0;
             if (nbits <= D) {
-                // single counting pass: per-wave digit histogram by ballot matching
+                // single counting pass: the block's key histogram by LDS atomics (the
+                // counts are integers, so the order of the adds does not matter); the
+                // histogram of the compressed keys is the count-by-key in key order
                 for (int r = 0; r < R; r++) {
                     const int w = r * WG + tid;
-                    const bool valid = w < nr;
-                    const uint32_t key = valid ? keysA[w] : 0u;
-                    const unsigned long long peers = match_peers(key, nbits, valid);
-                    if (valid && lane == __ffsll((long long)peers) - 1)
-                        cnt[wv * NBMAX + key] += (uint32_t)__popcll(peers);
+                    if (w < nr) {
+                        const uint32_t k0 = keysA[w];
+                        const int di = (int)(k0 >> 16) - 32768, dj = (int)(k0 & 0xffffu) - 32768;
+                        atomicAdd(&cnt[(uint32_t)(di - dimin) * Wj + (uint32_t)(dj - djmin)], 1u);
+                    }
                 }
                 __syncthreads();
                 uint32_t tot = 0;
-                for (int d = tid; d < NBMAX; d += WG) {  // NBMAX <= 256 <= WG: one digit per thread
-#pragma unroll
-                    for (int w = 0; w < NW; w++) tot += cnt[w * NBMAX + d];
-                }
+                if (tid < NBMAX) tot = cnt[tid];  // NBMAX <= 256 <= WG: one digit per thread
                 int total;
                 const int u = block_excl_scan_int<NW>(tot > 0 ? 1 : 0, misc + 128, &total);
                 nuq = total;
@@ -460,6 +451,14 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
                     put_entry(A, row, u, nuq, arena_off, (uint32_t)((j1 + dj) + g * (i1 + di)), tot);
                 }
             } else {
+                for (int r = 0; r < R; r++) {  // range-compress the keys in place
+                    const int w = r * WG + tid;
+                    if (w < nr) {
+                        const uint32_t k0 = keysA[w];
+                        const int di = (int)(k0 >> 16) - 32768, dj = (int)(k0 & 0xffffu) - 32768;
+                        keysA[w] = (uint32_t)(di - dimin) * Wj + (uint32_t)(dj - djmin);
+                    }
+                }
                 // multi-pass LSD radix sort of the nr keys in LDS; element (wave, round,
                 // lane) of a pass is keysA[r*WG + tid] in pass 0 (any order is fine for the
                 // first pass) and src[wv*R*64 + r*64 + lane] afterwards (the stable order)
@@ -891,6 +890,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     A.i_term = p->i_term; A.j_term = p->j_term; A.tsg = p->term_subgrid_size;
     A.reward_type = p->reward_type; A.m = p->neighb_gsize;
     A.dt = p->dt; A.r_out = p->r_outbound; A.r_term = p->r_terminal; A.alpha = p->alpha;
+    A.int_lim = (float)std::min<int64_t>(1 << 20, 2147483647ll / std::max<int64_t>(p->nrzns, 1));
     A.K = K;
     A.radix_bits = D;
     A.stats = dStats.as<unsigned long long>();
