@@ -67,7 +67,8 @@ struct BuildArgs {
 };
 
 __device__ __forceinline__ bool is_terminal(const BuildArgs& A, int i, int j) {
-    return (i >= A.i_term && i < A.i_term + A.tsg) && (j >= A.j_term && j < A.j_term + A.tsg);
+    // unsigned compares: i - i_term in [0, tsg) and j - j_term in [0, tsg); no branches
+    return ((unsigned)(i - A.i_term) < (unsigned)A.tsg) & ((unsigned)(j - A.j_term) < (unsigned)A.tsg);
 }
 
 // calculate_one_step_reward :231-265
@@ -292,6 +293,8 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
         __syncthreads();
     }
     const bool obs_slice = A.slice_obs && (A.slice_obs[t] | A.slice_obs[t + 1]);
+    // the scalar field enters only the radiation rewards (methods 2 and 3, :249-255)
+    const bool uses_rad = A.s && (A.reward_type == 2 || A.reward_type == 3);
     unsigned long long oow_acc = 0, multipass_acc = 0, overflow_acc = 0;
     uint32_t maxnuq_acc = 0;
     bool bad = false, bad_index = false;
@@ -321,7 +324,7 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
         }
         const bool term1 = is_terminal(A, i1, j1);
         const bool obs1 = A.mask && A.mask[mid] == 1;
-        const float rad1 = A.s ? A.s[mid] : 0.0f;
+        const float rad1 = uses_rad ? A.s[mid] : 0.0f;
         const float xc = A.xs[j1], yc = A.ys[g - 1 - i1];
 
         for (int a = 0; a < A.na; a++) {
@@ -340,13 +343,13 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
                         if (!obs1) {
                             const float2 v = vxy[w];
                             lane_move<UNIT>(A, xc, yc, fc, fs, v.x, v.y, t, pi, pj, rw);
-                            if (pi < 0 || pi >= g || pj < 0 || pj >= g) {  // impossible for dx, dy > 0;
-                                bad_index = true;                          // guards the gathers below
-                                pi = min(max(pi, 0), g - 1);
-                                pj = min(max(pj, 0), g - 1);
-                            }
+                            // impossible for dx, dy > 0; guards the gathers below
+                            const int ci = min(max(pi, 0), g - 1), cj = min(max(pj, 0), g - 1);
+                            bad_index |= (ci != pi) | (cj != pj);
+                            pi = ci;
+                            pj = cj;
                             const int sp2 = pj + g * pi;
-                            const float rad2 = A.s ? A.s[tslice + nc + sp2] : 0.0f;
+                            const float rad2 = uses_rad ? A.s[tslice + nc + sp2] : 0.0f;
                             rw += step_reward(A, speed, rad1, rad2);
                             if (OBS && obs_slice && crosses_obstacle(A, i1, j1, pi, pj, t)) rw = A.r_out;
                         } else {
@@ -358,7 +361,7 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
                     dimax = max(dimax, di);
                     djmin = min(djmin, dj);
                     djmax = max(djmax, dj);
-                    if (di + del < 0 || di + del >= A.m || dj + del < 0 || dj + del >= A.m) oow++;
+                    oow += ((unsigned)(di + del) >= (unsigned)A.m) | ((unsigned)(dj + del) >= (unsigned)A.m);
                     keysA[w] = ((uint32_t)(di + 32768) << 16) | (uint32_t)(dj + 32768);
                 }
                 if (valid) {
