@@ -51,6 +51,9 @@ struct BuildArgs {
     float dt, r_out, r_term, alpha;
     float int_lim;             // integer reward-sum path: |reward| < int_lim, so nr of them fit int32
     int t0, K, radix_bits, R, tc;
+    const uint32_t* seg_list;  // block-level kernel in list mode: one (s, a) row per block
+    uint32_t* defer_list;      // wave kernel: rows handed to the block-level kernel
+    unsigned* defer_count;
     int stage_y;
     uint2* staging;
     uint32_t* row_nnz;
@@ -282,7 +285,24 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     const int nchunks = (nr + 63) / 64;
 
-    const int t = A.t0 + blockIdx.y;
+    // grid mode: blockIdx.y = slice - t0, cells blockIdx.x*tc .. +tc, all actions;
+    // list mode: the one row seg_list[blockIdx.x] = ((t*nc + c)*na + a)
+    int t, c_begin, c_end, a_begin, a_end;
+    if (A.seg_list) {
+        const uint32_t lrow = A.seg_list[blockIdx.x];
+        const uint32_t lmid = lrow / (uint32_t)A.na;
+        a_begin = (int)(lrow - lmid * (uint32_t)A.na);
+        a_end = a_begin + 1;
+        t = (int)(lmid / (uint32_t)nc);
+        c_begin = (int)(lmid - (uint32_t)t * (uint32_t)nc);
+        c_end = c_begin + 1;
+    } else {
+        t = A.t0 + blockIdx.y;
+        c_begin = blockIdx.x * A.tc;
+        c_end = min(c_begin + A.tc, nc);
+        a_begin = 0;
+        a_end = A.na;
+    }
     const size_t tslice = (size_t)t * nc;
     if (A.stage_y) {
         const float* src = A.yi + (size_t)t * nr * nm;
@@ -300,9 +320,7 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
     bool bad = false, bad_index = false;
     const int del = (A.m - 1) / 2;
 
-    for (int ci = 0; ci < A.tc; ci++) {
-        const int c = blockIdx.x * A.tc + ci;
-        if (c >= nc) break;
+    for (int c = c_begin; c < c_end; c++) {
         const int i1 = c / g, j1 = c - i1 * g;
         const size_t mid = tslice + c;
         // extract_velocity :361-404: mode sum from 0 in k order, then added to the mean
@@ -327,7 +345,7 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
         const float rad1 = uses_rad ? A.s[mid] : 0.0f;
         const float xc = A.xs[j1], yc = A.ys[g - 1 - i1];
 
-        for (int a = 0; a < A.na; a++) {
+        for (int a = a_begin; a < a_end; a++) {
             const float speed = A.act_speed[a], fc = A.act_fcos[a], fs = A.act_fsin[a];
             int dimin = 0x7fffffff, dimax = -0x7fffffff, djmin = 0x7fffffff, djmax = -0x7fffffff;
             int oow = 0, ipart = 0;
@@ -576,6 +594,234 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
     }
 }
 
+// Wave-level DPP scans and reductions for the wave kernel (see wave_reduce).
+__device__ __forceinline__ int wave_incl_scan(int x) {
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
+    return x;
+}
+
+constexpr int kWaveBins = 256;  // key histogram bins per wave (nkeys above: deferred)
+
+// LDS bytes of mc_wave<NW>.
+__host__ __device__ inline size_t wave_lds_bytes(int NW, int nr, int nm, bool stage_y) {
+    const size_t nr2 = ((size_t)nr + 1) & ~(size_t)1;
+    return sizeof(float2) * (size_t)nr + NW * (sizeof(uint16_t) * nr2 + sizeof(uint32_t) * kWaveBins) +
+           (stage_y ? sizeof(float) * (size_t)nr * nm : 0);
+}
+
+// Wave kernel: one wavefront per (cell, action) segment. A block of NW waves takes tc
+// consecutive cells of one slice; for each cell its threads form the velocity sample of
+// every realization once (extract_velocity, kept in LDS), then wave wv moves actions
+// wv, wv + NW, ... over all realizations in rounds of 64 (one round = one reward chunk):
+//   pass 1: move, S2 offset (di, dj) kept as a 16-bit key in the wave's LDS slice,
+//           wave min/max of di and dj, window misses, and the row reward in the
+//           canonical order (chunk tree sums added in ascending chunk order; while every
+//           chunk is integer-valued the exact int32 sum stands in for them);
+//   pass 2: compressed key (di - dimin) * Wj + (dj - djmin), which orders S2 ascending,
+//           into the wave's LDS histogram (atomics: the counts are integers);
+//   emit:   nonzero bins in key order via a wave scan (the count-by-key), to the row's K
+//           staging slots or the overflow arena.
+// No block barrier separates the actions. A segment whose offsets leave [-128, 127] or
+// whose key range exceeds kWaveBins is appended to defer_list, and the block-level
+// kernel (list mode) builds that row exactly as it builds every row in grid mode.
+template <int NW, bool OBS, bool UNIT>
+__global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
+    constexpr int WG = NW * 64;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nr = A.nr, nm = A.nm, g = A.g, nc = A.nc, na = A.na;
+    const int nr2 = (nr + 1) & ~1;
+    float2* vxy = reinterpret_cast<float2*>(smem_raw);                             // [nr]
+    uint16_t* keys = reinterpret_cast<uint16_t*>(vxy + nr) + (size_t)wv * nr2;      // [nr2] per wave
+    uint32_t* hist = reinterpret_cast<uint32_t*>(reinterpret_cast<uint16_t*>(vxy + nr) + (size_t)NW * nr2) +
+                     (size_t)wv * kWaveBins;                                        // [kWaveBins] per wave
+    float* ylds = reinterpret_cast<float*>(reinterpret_cast<uint32_t*>(reinterpret_cast<uint16_t*>(vxy + nr) +
+                                                                       (size_t)NW * nr2) +
+                                           (size_t)NW * kWaveBins);                // [nm][nr] when staged
+    const int t = A.t0 + blockIdx.y;
+    const size_t tslice = (size_t)t * nc;
+    for (int i = lane; i < kWaveBins; i += 64) hist[i] = 0u;
+    if (A.stage_y) {
+        const float* src = A.yi + (size_t)t * nr * nm;
+        for (int i = tid; i < nr * nm; i += WG) {
+            const int w = i / nm, k = i - w * nm;
+            ylds[k * nr + w] = src[i];
+        }
+    }
+    const bool obs_slice = A.slice_obs && (A.slice_obs[t] | A.slice_obs[t + 1]);
+    const bool uses_rad = A.s && (A.reward_type == 2 || A.reward_type == 3);
+    const int del = (A.m - 1) / 2;
+    const int nrounds = (nr + 63) / 64;
+    unsigned long long oow_acc = 0, overflow_acc = 0;
+    uint32_t maxnuq_acc = 0;
+    bool bad = false, bad_index = false;
+
+    const int c_begin = blockIdx.x * A.tc, c_end = min(c_begin + A.tc, nc);
+    for (int c = c_begin; c < c_end; c++) {
+        const int i1 = c / g, j1 = c - i1 * g;
+        const size_t mid = tslice + c;
+        __syncthreads();  // the previous cell's actions are done with vxy (and ylds is staged)
+        const float u0 = A.u[mid], v0 = A.v[mid];
+        for (int w = tid; w < nr; w += WG) {  // extract_velocity :361-404
+            float sx = 0.0f, sy = 0.0f;
+            for (int k = 0; k < nm; k++) {
+                const size_t o = ((size_t)t * nm + k) * nc + c;
+                const float y = A.stage_y ? ylds[k * nr + w] : A.yi[((size_t)t * nr + w) * nm + k];
+                sx = sx + A.ui[o] * y;
+                sy = sy + A.vi[o] * y;
+            }
+            const float vx = u0 + sx, vy = v0 + sy;
+            bad |= !(isfinite(vx) && isfinite(vy));
+            vxy[w] = make_float2(vx, vy);
+        }
+        __syncthreads();
+        const bool term1 = is_terminal(A, i1, j1);
+        const bool obs1 = A.mask && A.mask[mid] == 1;
+        const float rad1 = uses_rad ? A.s[mid] : 0.0f;
+        const float xc = A.xs[j1], yc = A.ys[g - 1 - i1];
+
+        for (int a = wv; a < na; a += NW) {
+            const float speed = A.act_speed[a], fc = A.act_fcos[a], fs = A.act_fsin[a];
+            int dimin = 0x7fffffff, dimax = -0x7fffffff, djmin = 0x7fffffff, djmax = -0x7fffffff;
+            int oow = 0, ip = 0;
+            bool wide = false, mixed = false;
+            double rsum = 0.0;
+            // ---- pass 1: transition_calc :522-545 for the realizations of each round
+            for (int r = 0; r < nrounds; r++) {
+                const int w = r * 64 + lane;
+                const bool valid = w < nr;
+                float rw = 0.0f;
+                int pi = i1, pj = j1;
+                if (valid && !term1) {
+                    if (!obs1) {
+                        const float2 v = vxy[w];
+                        lane_move<UNIT>(A, xc, yc, fc, fs, v.x, v.y, t, pi, pj, rw);
+                        const int ci = min(max(pi, 0), g - 1), cj = min(max(pj, 0), g - 1);
+                        bad_index |= (ci != pi) | (cj != pj);
+                        pi = ci;
+                        pj = cj;
+                        const float rad2 = uses_rad ? A.s[tslice + nc + pj + g * pi] : 0.0f;
+                        rw += step_reward(A, speed, rad1, rad2);
+                        if (OBS && obs_slice && crosses_obstacle(A, i1, j1, pi, pj, t)) rw = A.r_out;
+                    } else {
+                        rw = A.r_out;
+                    }
+                }
+                const int di = pi - i1, dj = pj - j1;
+                if (valid) {
+                    dimin = min(dimin, di);
+                    dimax = max(dimax, di);
+                    djmin = min(djmin, dj);
+                    djmax = max(djmax, dj);
+                    oow += ((unsigned)(di + del) >= (unsigned)A.m) | ((unsigned)(dj + del) >= (unsigned)A.m);
+                    wide |= ((unsigned)(di + 128) > 255u) | ((unsigned)(dj + 128) > 255u);
+                    keys[w] = (uint16_t)((((unsigned)(di + 128) & 255u) << 8) | ((unsigned)(dj + 128) & 255u));
+                }
+                // reward of the row: compute_mean's sum in the canonical order
+                const bool okint = !valid || (rw == rintf(rw) && fabsf(rw) < A.int_lim);
+                const bool round_int = __all(okint);
+                if (!mixed && round_int) {
+                    ip += valid ? (int)rw : 0;
+                } else {
+                    if (!mixed) {
+                        rsum = (double)wave_sum(ip);  // exact: all earlier chunks are integers
+                        mixed = true;
+                    }
+                    double cs;
+                    if (round_int) {
+                        cs = (double)wave_sum(valid ? (int)rw : 0);
+                    } else {
+                        double p = valid ? (double)rw : 0.0;
+#pragma unroll
+                        for (int o = 32; o >= 1; o >>= 1) {
+                            const double q = __shfl_down(p, o, 64);
+                            p = p + q;
+                        }
+                        cs = __shfl(p, 0, 64);
+                    }
+                    rsum = rsum + cs;
+                }
+            }
+            if (!mixed) rsum = (double)wave_sum(ip);
+            dimin = wave_min(dimin);
+            dimax = wave_max(dimax);
+            djmin = wave_min(djmin);
+            djmax = wave_max(djmax);
+            oow = wave_sum(oow);
+            const bool wave_wide = __any(wide);
+            const uint32_t Wj = (uint32_t)(djmax - djmin + 1);
+            const uint64_t nkeys = (uint64_t)(dimax - dimin + 1) * Wj;
+            const size_t row = mid * na + a;
+            if (wave_wide || nkeys > (uint64_t)kWaveBins) {
+                if (lane == 0) A.defer_list[atomicAdd(A.defer_count, 1u)] = (uint32_t)row;
+                continue;  // the block-level kernel writes this row
+            }
+            // ---- pass 2: key histogram (count-by-key)
+            for (int r = 0; r < nrounds; r++) {
+                const int w = r * 64 + lane;
+                if (w < nr) {
+                    const uint32_t k = keys[w];
+                    const int di = (int)(k >> 8) - 128, dj = (int)(k & 255u) - 128;
+                    atomicAdd(&hist[(uint32_t)(di - dimin) * Wj + (uint32_t)(dj - djmin)], 1u);
+                }
+            }
+            // ---- emit the nonzero bins in key order; lane l owns bins 4l .. 4l+3
+            uint32_t cnt4[4];
+            int mine = 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                cnt4[q] = hist[lane * 4 + q];
+                mine += cnt4[q] != 0u;
+            }
+            const int incl = wave_incl_scan(mine);
+            const int nuq = __builtin_amdgcn_readlane(incl, 63);
+            int u = incl - mine;
+            unsigned long long arena_off = 0;
+            if (nuq > A.K) {
+                unsigned long long slot = 0;
+                if (lane == 0) slot = atomicAdd(A.arena_top, (unsigned long long)nuq);
+                arena_off = __shfl(slot, 0, 64);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (cnt4[q] != 0u) {
+                    const uint32_t kk = (uint32_t)(lane * 4 + q);
+                    const int di = (int)(kk / Wj) + dimin, dj = (int)(kk % Wj) + djmin;
+                    put_entry(A, row, u, nuq, arena_off, (uint32_t)((j1 + dj) + g * (i1 + di)), cnt4[q]);
+                    u++;
+                    hist[kk] = 0u;  // the histogram is all zero again for the next segment
+                }
+            }
+            if (lane == 0) {
+                A.row_nnz[row] = (uint32_t)nuq;
+                const float fs_ = (float)rsum;  // compute_mean :563-570
+                A.Rout[row] = fs_ / nr;
+                if (A.Rsum) A.Rsum[row] = fs_;
+                oow_acc += (unsigned long long)oow;
+                if ((uint32_t)nuq > maxnuq_acc) maxnuq_acc = (uint32_t)nuq;
+                if (nuq > A.K) {
+                    overflow_acc++;
+                    if (arena_off + (unsigned long long)nuq > A.arena_cap) atomicOr(A.stats + 4, 1ull);
+                    else A.staging[row * A.K] = make_uint2((uint32_t)arena_off, (uint32_t)(arena_off >> 32));
+                }
+            }
+        }
+    }
+    if (__any(bad) && lane == 0) atomicOr(A.stats + 3, 1ull);
+    if (__any(bad_index) && lane == 0) atomicOr(A.stats + 6, 1ull);
+    if (lane == 0) {
+        if (oow_acc) atomicAdd(A.stats + 0, oow_acc);
+        if (overflow_acc) atomicAdd(A.stats + 1, overflow_acc);
+        if (maxnuq_acc) atomicMax(A.stats + 2, (unsigned long long)maxnuq_acc);
+    }
+}
+
 // (c) CSR scatter: one thread per (row, slot). col = (t+1)*nc + sp2, val = count/nr
 // (reduce_kernel :635-637: S2 = state1D_from_spid(t+1, sp_id2), prob = count/nrzns).
 __global__ __launch_bounds__(256) void csr_scatter(const uint2* __restrict__ staging,
@@ -694,6 +940,47 @@ BuildFn pick_build(int wg, bool obs, bool unit) {
     if (wg == 256) return pick_build<256>(obs, unit);
     if (wg == 512) return pick_build<512>(obs, unit);
     return pick_build<1024>(obs, unit);
+}
+
+template <int NW>
+BuildFn pick_wave(bool obs, bool unit) {
+    if (obs) return unit ? mc_wave<NW, true, true> : mc_wave<NW, true, false>;
+    return unit ? mc_wave<NW, false, true> : mc_wave<NW, false, false>;
+}
+
+BuildFn pick_wave(int nw, bool obs, bool unit) {
+    if (nw == 4) return pick_wave<4>(obs, unit);
+    if (nw == 8) return pick_wave<8>(obs, unit);
+    return pick_wave<16>(obs, unit);
+}
+
+// Wavefronts per block of the per-row kernel: the most resident waves per CU (LDS,
+// VGPRs) that have a row to work on (na rows per cell shared by the block's waves).
+int choose_nw(int nr, int nm, int na, bool stage_y, bool obs, bool unit) {
+    int best = 0;
+    double best_score = -1.0;
+    for (int nw : {4, 8, 16}) {
+        const size_t lds = wave_lds_bytes(nw, nr, nm, stage_y);
+        if (lds > 160 * 1024) continue;
+        int blocks = (int)((160 * 1024) / lds);
+        hipFuncAttributes fa{};
+        int per_simd = 8;
+        if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(pick_wave(nw, obs, unit))) == hipSuccess) {
+            if (fa.numRegs > 0) per_simd = std::min(8, 512 / fa.numRegs);
+            if (fa.localSizeBytes > 0) per_simd = std::max(1, per_simd / 2);
+        } else {
+            (void)hipGetLastError();
+        }
+        blocks = std::min(blocks, (4 * per_simd) / nw);  // nw / 4 waves per SIMD per block
+        const int rounds = (na + nw - 1) / nw;
+        const double busy = (double)na / (double)(rounds * nw);  // waves with a row
+        const double score = blocks * nw * busy;
+        if (score > best_score) {
+            best_score = score;
+            best = nw;
+        }
+    }
+    return best;  // 0: the per-row kernel does not fit
 }
 
 e2e_status dispatch_build(const BuildArgs& A, int wg, bool obs, bool unit, dim3 grid, size_t lds,
@@ -942,13 +1229,24 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     }
     m->has_obstacles = any_obs;
     const int wg = choose_wg(nr, D, nm, stage, any_obs, unit, opts.block_threads);
+    const int nw = opts.count_kernel == 1 ? 0 : choose_nw(nr, nm, (int)na, stage, any_obs, unit);
+    const size_t wave_lds = nw ? wave_lds_bytes(nw, nr, nm, stage) : 0;
     m->info.block_threads = wg;
+    m->info.wave_blocks = nw;
     m->info.unit_cells = unit ? 1 : 0;
     A.R = (int)ceil_div(nr, wg);
     const size_t lds = build_lds_bytes(wg, A.R, D, nm, nr, stage);
     E2E_CHECK(lds <= 160 * 1024, E2E_ERR_CAPACITY, "build kernel needs %zu B of LDS (nrzns too large)", lds);
     m->info.n_multipass_rows = 0;
 
+    DevBuf dDefer, dDcnt;
+    if (nw) {
+        if ((st = dmalloc(dDefer, sizeof(uint32_t) * std::max<int64_t>(rows, 1))) ||
+            (st = dmalloc(dDcnt, sizeof(unsigned))))
+            return fail(st);
+        A.defer_list = dDefer.as<uint32_t>();
+        A.defer_count = dDcnt.as<unsigned>();
+    }
     phase("row buffers allocated");
     const int64_t nslices = nt - 1;
     const int64_t spl = opts.slices_per_launch > 0 ? opts.slices_per_launch : nslices;
@@ -966,12 +1264,22 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
         A.arena_cap = arena_cap;
         move_ms = 0.0;
         launches = 0;
+        unsigned ndefer = 0;
+        if (nw) E2E_HIP(hipMemsetAsync(dDcnt.p, 0, sizeof(unsigned), s));
         for (int64_t t0 = 0; t0 < nslices; t0 += spl) {
             const int64_t cnt = std::min(spl, nslices - t0);
             A.t0 = (int)t0;
             dim3 grid((unsigned)ceil_div(nc, A.tc), (unsigned)cnt);
             E2E_HIP(hipEventRecord(e0, s));
-            if ((st = dispatch_build(A, wg, any_obs, unit, grid, lds, s))) break;
+            if (nw) {
+                BuildFn wf = pick_wave(nw, any_obs, unit);
+                if (wave_lds > 64 * 1024)
+                    E2E_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(wf),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)wave_lds));
+                hipLaunchKernelGGL(wf, grid, dim3(nw * 64), wave_lds, s, A);
+            } else if ((st = dispatch_build(A, wg, any_obs, unit, grid, lds, s))) {
+                break;
+            }
             E2E_HIP(hipGetLastError());
             E2E_HIP(hipEventRecord(e1, s));
             E2E_HIP(hipEventSynchronize(e1));
@@ -981,6 +1289,25 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
             launches++;
         }
         if (st) break;
+        if (nw) {  // rows the per-row kernel handed over: the block-level kernel in list mode
+            E2E_HIP(hipMemcpyAsync(&ndefer, dDcnt.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+            E2E_HIP(hipStreamSynchronize(s));
+            if (ndefer) {
+                A.seg_list = dDefer.as<uint32_t>();
+                E2E_HIP(hipEventRecord(e0, s));
+                st = dispatch_build(A, wg, any_obs, unit, dim3(ndefer), lds, s);
+                A.seg_list = nullptr;
+                if (st) break;
+                E2E_HIP(hipGetLastError());
+                E2E_HIP(hipEventRecord(e1, s));
+                E2E_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                E2E_HIP(hipEventElapsedTime(&ms, e0, e1));
+                move_ms += ms;
+                launches++;
+            }
+        }
+        m->info.n_deferred_rows = (int32_t)ndefer;
         unsigned long long hs[8];
         E2E_HIP(hipMemcpyAsync(hs, dStats.p, sizeof(hs), hipMemcpyDeviceToHost, s));
         E2E_HIP(hipStreamSynchronize(s));

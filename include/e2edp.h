@@ -94,8 +94,12 @@ typedef struct e2e_build_opts {
     int32_t slot_k;            /* staging entries per row before the overflow arena (0: 16) */
     int32_t radix_bits;        /* digit width of the LDS segmented radix sort, 1..8 (0: 8) */
     int32_t slices_per_launch; /* time slices per build-kernel launch (0: all) */
-    int32_t block_threads;     /* build workgroup size 256/512/1024 (0: chosen from the LDS footprint) */
+    int32_t block_threads;     /* block-level kernel workgroup size 256/512/1024 (0: chosen from the LDS footprint) */
     void* stream;              /* hipStream_t to run on; NULL: the library creates one */
+    int32_t count_kernel;      /* 0: one wavefront per (state, action) row, rows it cannot take (S2 offsets
+                                  beyond +-127 cells or more than 256 candidate keys) go to the block-level
+                                  kernel; 1: the block-level kernel (one workgroup per row) for every row */
+    int32_t reserved;
 } e2e_build_opts;
 
 typedef struct e2e_model_info {
@@ -118,6 +122,8 @@ typedef struct e2e_model_info {
     int32_t move_launches;
     int32_t block_threads;     /* workgroup size the build kernel ran with */
     int32_t unit_cells;        /* 1: the unit-cell (dx = dy = 1) move was used */
+    int32_t wave_blocks;       /* wavefronts per block of the per-row kernel (0: not used) */
+    int32_t n_deferred_rows;   /* rows the per-row kernel handed to the block-level kernel */
 } e2e_model_info;
 
 typedef struct e2e_solve_opts {

@@ -70,37 +70,46 @@ CASES = ["tiny", "small", "obstacles_tsg2", "reward0", "reward1", "reward2", "re
          "cells_0p7", "cells_2_wg512", "wg1024"]
 
 
+# count_kernel 0: one wavefront per row, rows with wide S2 spreads handed to the
+# block-level kernel; 1: the block-level kernel (radix sort, forced block sizes) for all
+@pytest.mark.parametrize("kernel", [0, 1])
 @pytest.mark.parametrize("name", CASES)
-def test_build_bit_exact(gpu, name):
+def test_build_bit_exact(gpu, name, kernel):
     p, f, opts = _case(name)
-    print(f"[case {name}] g={p.gsize} nt={p.nt} nr={p.nrzns} opts={opts}", flush=True)
+    print(f"[case {name} kernel {kernel}] g={p.gsize} nt={p.nt} nr={p.nrzns} opts={opts}", flush=True)
     ref = oracle_model(p, f)
-    with e2e.build_model(p, f, **opts) as m:
+    with e2e.build_model(p, f, count_kernel=kernel, **opts) as m:
         assert_same_model(m.export_coo(), ref, name)
         info = m.info
         assert info["n_out_of_window"] == ref["oow"]
         assert info["max_row_nnz"] == ref["max_row_nnz"]
-        if name == "multipass":
+        assert (info["wave_blocks"] > 0) == (kernel == 0)
+        if kernel == 1 and name == "multipass":
             assert info["n_multipass_rows"] > 0
         if name in ("radix2_overflow", "overflow_k1"):
             assert info["n_overflow_rows"] > 0
         if name == "fast_flow_oow":
             assert info["n_out_of_window"] > 0
+            if kernel == 0:
+                assert info["n_deferred_rows"] > 0  # spreads beyond the per-row kernel
         assert info["unit_cells"] == (p.dx == 1.0 and p.dy == 1.0)
-        if "block_threads" in opts:
+        if kernel == 1 and "block_threads" in opts:
             assert info["block_threads"] == opts["block_threads"]
         # the int32 export (src/convert_model_datatype.py) carries the same values
         S1, S2, val, R = m.export_coo(np.int32)
         assert (S1 == ref["S1"]).all() and (S2 == ref["S2"]).all()
 
 
-def test_build_c1_bit_exact(gpu):
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_build_c1_bit_exact(gpu, kernel):
     w = workload("C1")
     f = w.fields()
     ref = oracle_model(w.params, f)
-    with e2e.build_model(w.params, f) as m:
+    with e2e.build_model(w.params, f, count_kernel=kernel) as m:
         assert_same_model(m.export_coo(), ref, "C1")
         assert m.info["lanes"] == w.lanes
+        if kernel == 0:
+            assert m.info["n_deferred_rows"] == 0
 
 
 def test_build_deterministic_and_device_fields(gpu):
