@@ -131,7 +131,7 @@ __device__ bool crosses_obstacle(const BuildArgs& A, int i1, int j1, int i2, int
 // bits as the general path at a fraction of its cost.
 template <bool UNIT>
 __device__ __forceinline__ void lane_move(const BuildArgs& A, float xc, float yc, float fc,
-                                          float fs, float vx, float vy, int T, int& pi,
+                                          float fs, float vx, float vy, bool late_slice, int& pi,
                                           int& pj, float& r) {
     // Written with selects instead of the reference's if/else-if ladders; every float
     // operation and its operands are the same, so the results are the same bits.
@@ -166,10 +166,15 @@ __device__ __forceinline__ void lane_move(const BuildArgs& A, float xc, float yc
     }
     pi = yind;
     pj = xind;
-    const bool edge = pi == 0 || pi == n || pj == 0 || pj == n;  // :339-341
+    const bool edge = (pi == 0) | (pi == n) | (pj == 0) | (pj == n);  // :339-341
     r = edge ? r + A.r_out : r;
-    const float late = (float)((double)r + A.late_pen);           // :343-354
-    r = is_terminal(A, pi, pj) ? r + A.r_term : (T > A.nt - 20 ? late : r);
+    const bool term = is_terminal(A, pi, pj);                          // :343-354
+    if (late_slice) {  // T > nt - 20: uniform over the slice
+        const float late = (float)((double)r + A.late_pen);
+        r = term ? r + A.r_term : late;
+    } else {
+        r = term ? r + A.r_term : r;
+    }
 }
 
 // Wave reductions with DPP: row shifts 1, 2, 4, 8 leave each row's total in its lane 15,
@@ -360,7 +365,7 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
                     if (!term1) {
                         if (!obs1) {
                             const float2 v = vxy[w];
-                            lane_move<UNIT>(A, xc, yc, fc, fs, v.x, v.y, t, pi, pj, rw);
+                            lane_move<UNIT>(A, xc, yc, fc, fs, v.x, v.y, t > A.nt - 20, pi, pj, rw);
                             // impossible for dx, dy > 0; guards the gathers below
                             const int ci = min(max(pi, 0), g - 1), cj = min(max(pj, 0), g - 1);
                             bad_index |= (ci != pi) | (cj != pj);
@@ -656,6 +661,7 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
     }
     const bool obs_slice = A.slice_obs && (A.slice_obs[t] | A.slice_obs[t + 1]);
     const bool uses_rad = A.s && (A.reward_type == 2 || A.reward_type == 3);
+    const bool late_slice = t > A.nt - 20;
     const int del = (A.m - 1) / 2;
     const int nrounds = (nr + 63) / 64;
     unsigned long long oow_acc = 0, overflow_acc = 0;
@@ -688,42 +694,15 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
 
         for (int a = wv; a < na; a += NW) {
             const float speed = A.act_speed[a], fc = A.act_fcos[a], fs = A.act_fsin[a];
+            // step reward of the methods that do not read the scalar field: one per action
+            const float rstep = uses_rad ? 0.0f : step_reward(A, speed, 0.0f, 0.0f);
             int dimin = 0x7fffffff, dimax = -0x7fffffff, djmin = 0x7fffffff, djmax = -0x7fffffff;
             int oow = 0, ip = 0;
             bool wide = false, mixed = false;
             double rsum = 0.0;
             // ---- pass 1: transition_calc :522-545 for the realizations of each round
-            for (int r = 0; r < nrounds; r++) {
-                const int w = r * 64 + lane;
-                const bool valid = w < nr;
-                float rw = 0.0f;
-                int pi = i1, pj = j1;
-                if (valid && !term1) {
-                    if (!obs1) {
-                        const float2 v = vxy[w];
-                        lane_move<UNIT>(A, xc, yc, fc, fs, v.x, v.y, t, pi, pj, rw);
-                        const int ci = min(max(pi, 0), g - 1), cj = min(max(pj, 0), g - 1);
-                        bad_index |= (ci != pi) | (cj != pj);
-                        pi = ci;
-                        pj = cj;
-                        const float rad2 = uses_rad ? A.s[tslice + nc + pj + g * pi] : 0.0f;
-                        rw += step_reward(A, speed, rad1, rad2);
-                        if (OBS && obs_slice && crosses_obstacle(A, i1, j1, pi, pj, t)) rw = A.r_out;
-                    } else {
-                        rw = A.r_out;
-                    }
-                }
-                const int di = pi - i1, dj = pj - j1;
-                if (valid) {
-                    dimin = min(dimin, di);
-                    dimax = max(dimax, di);
-                    djmin = min(djmin, dj);
-                    djmax = max(djmax, dj);
-                    oow += ((unsigned)(di + del) >= (unsigned)A.m) | ((unsigned)(dj + del) >= (unsigned)A.m);
-                    wide |= ((unsigned)(di + 128) > 255u) | ((unsigned)(dj + 128) > 255u);
-                    keys[w] = (uint16_t)((((unsigned)(di + 128) & 255u) << 8) | ((unsigned)(dj + 128) & 255u));
-                }
-                // reward of the row: compute_mean's sum in the canonical order
+            // reward of the row: compute_mean's sum in the canonical order
+            auto add_reward = [&](bool valid, float rw) {
                 const bool okint = !valid || (rw == rintf(rw) && fabsf(rw) < A.int_lim);
                 const bool round_int = __all(okint);
                 if (!mixed && round_int) {
@@ -746,6 +725,47 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                         cs = __shfl(p, 0, 64);
                     }
                     rsum = rsum + cs;
+                }
+            };
+            auto track = [&](int w, int di, int dj) {
+                dimin = min(dimin, di);
+                dimax = max(dimax, di);
+                djmin = min(djmin, dj);
+                djmax = max(djmax, dj);
+                oow += ((unsigned)(di + del) >= (unsigned)A.m) | ((unsigned)(dj + del) >= (unsigned)A.m);
+                wide |= ((unsigned)(di + 128) > 255u) | ((unsigned)(dj + 128) > 255u);
+                keys[w] = (uint16_t)((((unsigned)(di + 128) & 255u) << 8) | ((unsigned)(dj + 128) & 255u));
+            };
+            if (!term1 && !obs1) {
+                // the common cell: every lane moves. Full rounds carry no validity test.
+                auto move_round = [&](int w, bool valid) {
+                    const float2 v = vxy[valid ? w : 0];
+                    int pi, pj;
+                    float rw;
+                    lane_move<UNIT>(A, xc, yc, fc, fs, v.x, v.y, late_slice, pi, pj, rw);
+                    if (!UNIT) {  // impossible for unit cells; guards the gathers below
+                        const int ci = min(max(pi, 0), g - 1), cj = min(max(pj, 0), g - 1);
+                        bad_index |= valid & ((ci != pi) | (cj != pj));
+                        pi = ci;
+                        pj = cj;
+                    }
+                    if (uses_rad) rw += step_reward(A, speed, rad1, A.s[tslice + nc + pj + g * pi]);
+                    else rw += rstep;
+                    if (OBS && obs_slice && crosses_obstacle(A, i1, j1, pi, pj, t)) rw = A.r_out;
+                    if (valid) track(w, pi - i1, pj - j1);
+                    add_reward(valid, rw);
+                };
+                const int nfull = nr / 64;
+                for (int r = 0; r < nfull; r++) move_round(r * 64 + lane, true);
+                if (nfull < nrounds) move_round(nfull * 64 + lane, nfull * 64 + lane < nr);
+            } else {
+                // terminal cell (stays, reward 0) or obstacle cell (stays, r_outbound)
+                const float rw = term1 ? 0.0f : A.r_out;
+                for (int r = 0; r < nrounds; r++) {
+                    const int w = r * 64 + lane;
+                    const bool valid = w < nr;
+                    if (valid) track(w, 0, 0);
+                    add_reward(valid, rw);
                 }
             }
             if (!mixed) rsum = (double)wave_sum(ip);
