@@ -611,6 +611,9 @@ __device__ __forceinline__ int wave_incl_scan(int x) {
 }
 
 constexpr int kWaveBins = 256;  // key histogram bins per wave (nkeys above: deferred)
+constexpr int kFixHalf = 7;     // fixed offset window +-7 cells, counted during the move
+constexpr int kFixSide = 2 * kFixHalf + 1;
+static_assert(kFixSide * kFixSide <= kWaveBins, "fixed window fits the histogram");
 
 // LDS bytes of mc_wave<NW>.
 __host__ __device__ inline size_t wave_lds_bytes(int NW, int nr, int nm, bool stage_y) {
@@ -698,7 +701,7 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             const float rstep = uses_rad ? 0.0f : step_reward(A, speed, 0.0f, 0.0f);
             int dimin = 0x7fffffff, dimax = -0x7fffffff, djmin = 0x7fffffff, djmax = -0x7fffffff;
             int oow = 0, ip = 0;
-            bool wide = false, mixed = false;
+            bool mixed = false;
             double rsum = 0.0;
             // ---- pass 1: transition_calc :522-545 for the realizations of each round
             // reward of the row: compute_mean's sum in the canonical order
@@ -727,14 +730,17 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                     rsum = rsum + cs;
                 }
             };
+            // per lane: bounding box of the offsets, the 16-bit key, and the count in the
+            // fixed 15 x 15 offset histogram (bin (di+7)*15 + (dj+7): ascending S2 order
+            // too) when the offset is within +-7 cells
             auto track = [&](int w, int di, int dj) {
                 dimin = min(dimin, di);
                 dimax = max(dimax, di);
                 djmin = min(djmin, dj);
                 djmax = max(djmax, dj);
-                oow += ((unsigned)(di + del) >= (unsigned)A.m) | ((unsigned)(dj + del) >= (unsigned)A.m);
-                wide |= ((unsigned)(di + 128) > 255u) | ((unsigned)(dj + 128) > 255u);
                 keys[w] = (uint16_t)((((unsigned)(di + 128) & 255u) << 8) | ((unsigned)(dj + 128) & 255u));
+                const unsigned fi = (unsigned)(di + kFixHalf), fj = (unsigned)(dj + kFixHalf);
+                if ((fi < (unsigned)kFixSide) & (fj < (unsigned)kFixSide)) atomicAdd(&hist[fi * kFixSide + fj], 1u);
             };
             if (!term1 && !obs1) {
                 // the common cell: every lane moves. Full rounds carry no validity test.
@@ -773,23 +779,41 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             dimax = wave_max(dimax);
             djmin = wave_min(djmin);
             djmax = wave_max(djmax);
-            oow = wave_sum(oow);
-            const bool wave_wide = __any(wide);
-            const uint32_t Wj = (uint32_t)(djmax - djmin + 1);
-            const uint64_t nkeys = (uint64_t)(dimax - dimin + 1) * Wj;
+            const bool wave_wide = dimin < -128 || dimax > 127 || djmin < -128 || djmax > 127;
+            const bool fixed = dimin >= -kFixHalf && dimax <= kFixHalf && djmin >= -kFixHalf && djmax <= kFixHalf;
+            // key decode: fixed window, or the range-compressed key of the bounding box
+            const uint32_t Wj = fixed ? (uint32_t)kFixSide : (uint32_t)(djmax - djmin + 1);
+            const int di0 = fixed ? -kFixHalf : dimin, dj0 = fixed ? -kFixHalf : djmin;
+            const uint64_t nkeys = fixed ? (uint64_t)kFixSide * kFixSide : (uint64_t)(dimax - dimin + 1) * Wj;
             const size_t row = mid * na + a;
             if (wave_wide || nkeys > (uint64_t)kWaveBins) {
+                for (int i = lane; i < kFixSide * kFixSide; i += 64) hist[i] = 0u;  // partial fixed counts
                 if (lane == 0) A.defer_list[atomicAdd(A.defer_count, 1u)] = (uint32_t)row;
                 continue;  // the block-level kernel writes this row
             }
-            // ---- pass 2: key histogram (count-by-key)
-            for (int r = 0; r < nrounds; r++) {
-                const int w = r * 64 + lane;
-                if (w < nr) {
-                    const uint32_t k = keys[w];
-                    const int di = (int)(k >> 8) - 128, dj = (int)(k & 255u) - 128;
-                    atomicAdd(&hist[(uint32_t)(di - dimin) * Wj + (uint32_t)(dj - djmin)], 1u);
+            if (!fixed) {
+                // ---- pass 2: histogram of the range-compressed keys (count-by-key)
+                for (int i = lane; i < kFixSide * kFixSide; i += 64) hist[i] = 0u;
+                for (int r = 0; r < nrounds; r++) {
+                    const int w = r * 64 + lane;
+                    if (w < nr) {
+                        const uint32_t k = keys[w];
+                        const int di = (int)(k >> 8) - 128, dj = (int)(k & 255u) - 128;
+                        atomicAdd(&hist[(uint32_t)(di - dimin) * Wj + (uint32_t)(dj - djmin)], 1u);
+                    }
                 }
+            }
+            // lanes whose S2 left the neighb_gsize window (statistics; none when the box fits)
+            if (dimin + del < 0 || dimax + del >= A.m || djmin + del < 0 || djmax + del >= A.m) {
+                for (int r = 0; r < nrounds; r++) {
+                    const int w = r * 64 + lane;
+                    if (w < nr) {
+                        const uint32_t k = keys[w];
+                        const int di = (int)(k >> 8) - 128, dj = (int)(k & 255u) - 128;
+                        oow += ((unsigned)(di + del) >= (unsigned)A.m) | ((unsigned)(dj + del) >= (unsigned)A.m);
+                    }
+                }
+                oow = wave_sum(oow);
             }
             // ---- emit the nonzero bins in key order; lane l owns bins 4l .. 4l+3
             uint32_t cnt4[4];
@@ -812,7 +836,7 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             for (int q = 0; q < 4; q++) {
                 if (cnt4[q] != 0u) {
                     const uint32_t kk = (uint32_t)(lane * 4 + q);
-                    const int di = (int)(kk / Wj) + dimin, dj = (int)(kk % Wj) + djmin;
+                    const int di = (int)(kk / Wj) + di0, dj = (int)(kk % Wj) + dj0;
                     put_entry(A, row, u, nuq, arena_off, (uint32_t)((j1 + dj) + g * (i1 + di)), cnt4[q]);
                     u++;
                     hist[kk] = 0u;  // the histogram is all zero again for the next segment
