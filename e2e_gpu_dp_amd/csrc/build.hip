@@ -617,8 +617,7 @@ static_assert(kFixSide * kFixSide <= kWaveBins, "fixed window fits the histogram
 
 // LDS bytes of mc_wave<NW>.
 __host__ __device__ inline size_t wave_lds_bytes(int NW, int nr, int nm, bool stage_y) {
-    const size_t nr2 = ((size_t)nr + 1) & ~(size_t)1;
-    return sizeof(float2) * (size_t)nr + NW * (sizeof(uint16_t) * nr2 + sizeof(uint32_t) * kWaveBins) +
+    return sizeof(float2) * (size_t)nr + NW * sizeof(uint32_t) * kWaveBins +
            (stage_y ? sizeof(float) * (size_t)nr * nm : 0);
 }
 
@@ -626,17 +625,19 @@ __host__ __device__ inline size_t wave_lds_bytes(int NW, int nr, int nm, bool st
 // consecutive cells of one slice; for each cell its threads form the velocity sample of
 // every realization once (extract_velocity, kept in LDS), then wave wv moves actions
 // wv, wv + NW, ... over all realizations in rounds of 64 (one round = one reward chunk):
-//   pass 1: move, S2 offset (di, dj) kept as a 16-bit key in the wave's LDS slice,
-//           wave min/max of di and dj, window misses, and the row reward in the
-//           canonical order (chunk tree sums added in ascending chunk order; while every
-//           chunk is integer-valued the exact int32 sum stands in for them);
-//   pass 2: compressed key (di - dimin) * Wj + (dj - djmin), which orders S2 ascending,
-//           into the wave's LDS histogram (atomics: the counts are integers);
+//   pass 1: move; S2 offset (di, dj) counted in the wave's LDS histogram over the fixed
+//           window |di|, |dj| <= 7 (bin order = ascending S2; atomics, as the counts
+//           are integers); wave min/max of di and dj; the row reward in the canonical
+//           order (chunk tree sums added in ascending chunk order; while every chunk is
+//           integer-valued the exact int32 sum stands in for them);
+//   pass 2: only when the offsets leave the fixed window: the moves again, counted
+//           by the range-compressed key (di - dimin) * Wj + (dj - djmin), which also
+//           orders S2 ascending; and the lanes outside the neighb_gsize window;
 //   emit:   nonzero bins in key order via a wave scan (the count-by-key), to the row's K
 //           staging slots or the overflow arena.
-// No block barrier separates the actions. A segment whose offsets leave [-128, 127] or
-// whose key range exceeds kWaveBins is appended to defer_list, and the block-level
-// kernel (list mode) builds that row exactly as it builds every row in grid mode.
+// No block barrier separates the actions. A segment whose key range exceeds kWaveBins
+// is appended to defer_list, and the block-level kernel (list mode) builds that row
+// exactly as it builds every row in grid mode.
 template <int NW, bool OBS, bool UNIT>
 __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
     constexpr int WG = NW * 64;
@@ -644,14 +645,9 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nr = A.nr, nm = A.nm, g = A.g, nc = A.nc, na = A.na;
-    const int nr2 = (nr + 1) & ~1;
-    float2* vxy = reinterpret_cast<float2*>(smem_raw);                             // [nr]
-    uint16_t* keys = reinterpret_cast<uint16_t*>(vxy + nr) + (size_t)wv * nr2;      // [nr2] per wave
-    uint32_t* hist = reinterpret_cast<uint32_t*>(reinterpret_cast<uint16_t*>(vxy + nr) + (size_t)NW * nr2) +
-                     (size_t)wv * kWaveBins;                                        // [kWaveBins] per wave
-    float* ylds = reinterpret_cast<float*>(reinterpret_cast<uint32_t*>(reinterpret_cast<uint16_t*>(vxy + nr) +
-                                                                       (size_t)NW * nr2) +
-                                           (size_t)NW * kWaveBins);                // [nm][nr] when staged
+    float2* vxy = reinterpret_cast<float2*>(smem_raw);                                  // [nr]
+    uint32_t* hist = reinterpret_cast<uint32_t*>(vxy + nr) + (size_t)wv * kWaveBins;   // [kWaveBins] per wave
+    float* ylds = reinterpret_cast<float*>(reinterpret_cast<uint32_t*>(vxy + nr) + (size_t)NW * kWaveBins);  // [nm][nr]
     const int t = A.t0 + blockIdx.y;
     const size_t tslice = (size_t)t * nc;
     for (int i = lane; i < kWaveBins; i += 64) hist[i] = 0u;
@@ -703,7 +699,30 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             int oow = 0, ip = 0;
             bool mixed = false;
             double rsum = 0.0;
-            // ---- pass 1: transition_calc :522-545 for the realizations of each round
+            // S2 offset and reward of realization w: transition_calc :522-545 (terminal
+            // cells stay with reward 0, obstacle cells stay with r_outbound)
+            auto lane_offset = [&](int w, bool valid, int& di, int& dj, float& rw) {
+                if (term1 || obs1) {
+                    di = 0;
+                    dj = 0;
+                    rw = term1 ? 0.0f : A.r_out;
+                    return;
+                }
+                const float2 v = vxy[valid ? w : 0];
+                int pi, pj;
+                lane_move<UNIT>(A, xc, yc, fc, fs, v.x, v.y, late_slice, pi, pj, rw);
+                if (!UNIT) {  // impossible for unit cells; guards the gathers below
+                    const int ci = min(max(pi, 0), g - 1), cj = min(max(pj, 0), g - 1);
+                    bad_index |= valid & ((ci != pi) | (cj != pj));
+                    pi = ci;
+                    pj = cj;
+                }
+                if (uses_rad) rw += step_reward(A, speed, rad1, A.s[tslice + nc + pj + g * pi]);
+                else rw += rstep;
+                if (OBS && obs_slice && crosses_obstacle(A, i1, j1, pi, pj, t)) rw = A.r_out;
+                di = pi - i1;
+                dj = pj - j1;
+            };
             // reward of the row: compute_mean's sum in the canonical order
             auto add_reward = [&](bool valid, float rw) {
                 const bool okint = !valid || (rw == rintf(rw) && fabsf(rw) < A.int_lim);
@@ -730,56 +749,34 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                     rsum = rsum + cs;
                 }
             };
-            // per lane: bounding box of the offsets, the 16-bit key, and the count in the
-            // fixed 15 x 15 offset histogram (bin (di+7)*15 + (dj+7): ascending S2 order
-            // too) when the offset is within +-7 cells
-            auto track = [&](int w, int di, int dj) {
-                dimin = min(dimin, di);
-                dimax = max(dimax, di);
-                djmin = min(djmin, dj);
-                djmax = max(djmax, dj);
-                keys[w] = (uint16_t)((((unsigned)(di + 128) & 255u) << 8) | ((unsigned)(dj + 128) & 255u));
-                const unsigned fi = (unsigned)(di + kFixHalf), fj = (unsigned)(dj + kFixHalf);
-                if ((fi < (unsigned)kFixSide) & (fj < (unsigned)kFixSide)) atomicAdd(&hist[fi * kFixSide + fj], 1u);
-            };
-            if (!term1 && !obs1) {
-                // the common cell: every lane moves. Full rounds carry no validity test.
-                auto move_round = [&](int w, bool valid) {
-                    const float2 v = vxy[valid ? w : 0];
-                    int pi, pj;
-                    float rw;
-                    lane_move<UNIT>(A, xc, yc, fc, fs, v.x, v.y, late_slice, pi, pj, rw);
-                    if (!UNIT) {  // impossible for unit cells; guards the gathers below
-                        const int ci = min(max(pi, 0), g - 1), cj = min(max(pj, 0), g - 1);
-                        bad_index |= valid & ((ci != pi) | (cj != pj));
-                        pi = ci;
-                        pj = cj;
-                    }
-                    if (uses_rad) rw += step_reward(A, speed, rad1, A.s[tslice + nc + pj + g * pi]);
-                    else rw += rstep;
-                    if (OBS && obs_slice && crosses_obstacle(A, i1, j1, pi, pj, t)) rw = A.r_out;
-                    if (valid) track(w, pi - i1, pj - j1);
-                    add_reward(valid, rw);
-                };
-                const int nfull = nr / 64;
-                for (int r = 0; r < nfull; r++) move_round(r * 64 + lane, true);
-                if (nfull < nrounds) move_round(nfull * 64 + lane, nfull * 64 + lane < nr);
-            } else {
-                // terminal cell (stays, reward 0) or obstacle cell (stays, r_outbound)
-                const float rw = term1 ? 0.0f : A.r_out;
-                for (int r = 0; r < nrounds; r++) {
-                    const int w = r * 64 + lane;
-                    const bool valid = w < nr;
-                    if (valid) track(w, 0, 0);
-                    add_reward(valid, rw);
+            // ---- pass 1, one round = one reward chunk of 64 realizations: bounding box
+            // of the offsets, and the count in the fixed 15 x 15 offset histogram (bin
+            // (di+7)*15 + (dj+7): ascending S2 order too) when within +-7 cells. Full
+            // rounds carry no validity test.
+            auto round1 = [&](int w, bool valid) {
+                int di, dj;
+                float rw;
+                lane_offset(w, valid, di, dj, rw);
+                if (valid) {
+                    dimin = min(dimin, di);
+                    dimax = max(dimax, di);
+                    djmin = min(djmin, dj);
+                    djmax = max(djmax, dj);
+                    const unsigned fi = (unsigned)(di + kFixHalf), fj = (unsigned)(dj + kFixHalf);
+                    if ((fi < (unsigned)kFixSide) & (fj < (unsigned)kFixSide))
+                        atomicAdd(&hist[fi * kFixSide + fj], 1u);
                 }
-            }
+                add_reward(valid, rw);
+            };
+            const int nfull = nr / 64;
+            for (int r = 0; r < nfull; r++) round1(r * 64 + lane, true);
+            if (nfull < nrounds) round1(nfull * 64 + lane, nfull * 64 + lane < nr);
             if (!mixed) rsum = (double)wave_sum(ip);
             dimin = wave_min(dimin);
             dimax = wave_max(dimax);
             djmin = wave_min(djmin);
             djmax = wave_max(djmax);
-            const bool wave_wide = dimin < -128 || dimax > 127 || djmin < -128 || djmax > 127;
+            const bool wave_wide = (int64_t)dimax - dimin >= 65536 || (int64_t)djmax - djmin >= 65536;
             const bool fixed = dimin >= -kFixHalf && dimax <= kFixHalf && djmin >= -kFixHalf && djmax <= kFixHalf;
             // key decode: fixed window, or the range-compressed key of the bounding box
             const uint32_t Wj = fixed ? (uint32_t)kFixSide : (uint32_t)(djmax - djmin + 1);
@@ -791,25 +788,19 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                 if (lane == 0) A.defer_list[atomicAdd(A.defer_count, 1u)] = (uint32_t)row;
                 continue;  // the block-level kernel writes this row
             }
-            if (!fixed) {
-                // ---- pass 2: histogram of the range-compressed keys (count-by-key)
-                for (int i = lane; i < kFixSide * kFixSide; i += 64) hist[i] = 0u;
+            const bool window_miss = dimin + del < 0 || dimax + del >= A.m || djmin + del < 0 || djmax + del >= A.m;
+            if (!fixed || window_miss) {
+                // ---- pass 2 (spreads beyond +-7 cells): the moves again, for the histogram
+                // of the range-compressed keys and for the lanes outside the window
+                if (!fixed)
+                    for (int i = lane; i < kFixSide * kFixSide; i += 64) hist[i] = 0u;
                 for (int r = 0; r < nrounds; r++) {
                     const int w = r * 64 + lane;
                     if (w < nr) {
-                        const uint32_t k = keys[w];
-                        const int di = (int)(k >> 8) - 128, dj = (int)(k & 255u) - 128;
-                        atomicAdd(&hist[(uint32_t)(di - dimin) * Wj + (uint32_t)(dj - djmin)], 1u);
-                    }
-                }
-            }
-            // lanes whose S2 left the neighb_gsize window (statistics; none when the box fits)
-            if (dimin + del < 0 || dimax + del >= A.m || djmin + del < 0 || djmax + del >= A.m) {
-                for (int r = 0; r < nrounds; r++) {
-                    const int w = r * 64 + lane;
-                    if (w < nr) {
-                        const uint32_t k = keys[w];
-                        const int di = (int)(k >> 8) - 128, dj = (int)(k & 255u) - 128;
+                        int di, dj;
+                        float rw;
+                        lane_offset(w, true, di, dj, rw);
+                        if (!fixed) atomicAdd(&hist[(uint32_t)(di - dimin) * Wj + (uint32_t)(dj - djmin)], 1u);
                         oow += ((unsigned)(di + del) >= (unsigned)A.m) | ((unsigned)(dj + del) >= (unsigned)A.m);
                     }
                 }
