@@ -753,6 +753,21 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             // of the offsets, and the count in the fixed 15 x 15 offset histogram (bin
             // (di+7)*15 + (dj+7): ascending S2 order too) when within +-7 cells. Full
             // rounds carry no validity test.
+            // Every reward a lane can get here is one of: rstep + r_k (+ r_terminal, or the
+            // late-slice penalty), r_k = k * r_outbound summed one at a time, k = 0..3
+            // (x clamp, y clamp, edge), or r_outbound (obstacles), or 0. If all of them are
+            // integers within int_lim the row sum is an exact int32 sum: no per-lane test.
+            bool seg_int = !uses_rad;
+            if (seg_int) {
+                auto ok = [&](float v) { return v == rintf(v) && fabsf(v) < A.int_lim; };
+                float rk = 0.0f;
+                seg_int = ok(A.r_out) && ok(0.0f);
+                for (int kk = 0; kk < 4; kk++) {
+                    const float other = late_slice ? (float)((double)rk + A.late_pen) : rk;
+                    seg_int = seg_int && ok((rk + A.r_term) + rstep) && ok(other + rstep);
+                    rk = rk + A.r_out;
+                }
+            }
             auto round1 = [&](int w, bool valid) {
                 int di, dj;
                 float rw;
@@ -762,11 +777,13 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                     dimax = max(dimax, di);
                     djmin = min(djmin, dj);
                     djmax = max(djmax, dj);
-                    const unsigned fi = (unsigned)(di + kFixHalf), fj = (unsigned)(dj + kFixHalf);
-                    if ((fi < (unsigned)kFixSide) & (fj < (unsigned)kFixSide))
-                        atomicAdd(&hist[fi * kFixSide + fj], 1u);
+                    // bin of the fixed window; an offset outside it lands in some bin, but
+                    // then the bounding box leaves the window and the counts are rebuilt
+                    const unsigned bin = (unsigned)(di + kFixHalf) * kFixSide + (unsigned)(dj + kFixHalf);
+                    atomicAdd(&hist[min(bin, (unsigned)kWaveBins - 1u)], 1u);
                 }
-                add_reward(valid, rw);
+                if (seg_int) ip += valid ? (int)rw : 0;
+                else add_reward(valid, rw);
             };
             const int nfull = nr / 64;
             for (int r = 0; r < nfull; r++) round1(r * 64 + lane, true);
@@ -784,7 +801,7 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             const uint64_t nkeys = fixed ? (uint64_t)kFixSide * kFixSide : (uint64_t)(dimax - dimin + 1) * Wj;
             const size_t row = mid * na + a;
             if (wave_wide || nkeys > (uint64_t)kWaveBins) {
-                for (int i = lane; i < kFixSide * kFixSide; i += 64) hist[i] = 0u;  // partial fixed counts
+                for (int i = lane; i < kWaveBins; i += 64) hist[i] = 0u;  // partial fixed counts
                 if (lane == 0) A.defer_list[atomicAdd(A.defer_count, 1u)] = (uint32_t)row;
                 continue;  // the block-level kernel writes this row
             }
@@ -793,7 +810,7 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                 // ---- pass 2 (spreads beyond +-7 cells): the moves again, for the histogram
                 // of the range-compressed keys and for the lanes outside the window
                 if (!fixed)
-                    for (int i = lane; i < kFixSide * kFixSide; i += 64) hist[i] = 0u;
+                    for (int i = lane; i < kWaveBins; i += 64) hist[i] = 0u;
                 for (int r = 0; r < nrounds; r++) {
                     const int w = r * 64 + lane;
                     if (w < nr) {
