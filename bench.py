@@ -231,6 +231,8 @@ def main(argv=None):
     lanes = infos[0][0]["lanes"]
 
     move_ms = float(np.mean([i["move_kernel_ms"] / max(i["move_launches"], 1) for i, _ in infos]))
+    # the per-row wavefront kernel, or the block-level kernel when the build used only that
+    build_kernel = "mc_wave" if infos[0][0]["wave_blocks"] > 0 else "mc_move_count"
     flops_per_lane = 4 * p.nmodes + 18
     build_tflops = lanes * flops_per_lane / (move_ms * 1e-3) / 1e12
     build_ms = float(np.mean([i["build_ms"] for i, _ in infos]))
@@ -248,10 +250,10 @@ def main(argv=None):
         "build": {"transitions_per_s": lanes / (build_ms * 1e-3), "build_ms": build_ms,
                   "move_kernel_ms": move_ms, "scan_ms": infos[0][0]["scan_ms"],
                   "csr_ms": infos[0][0]["csr_ms"]},
-        "roofline": {"kernel": "mc_move_count", "bound": "valu", "achieved": build_tflops,
+        "roofline": {"kernel": build_kernel, "bound": "valu", "achieved": build_tflops,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": build_tflops / PEAK_FP32_TFLOPS,
-                     "traffic": _pmc_traffic(args.workload, "mc_move_count"),
+                     "traffic": _pmc_traffic(args.workload, build_kernel),
                      "flops_per_lane": flops_per_lane},
     }
     if not args.skip_vi:

@@ -1006,34 +1006,11 @@ BuildFn pick_wave(int nw, bool obs, bool unit) {
     return pick_wave<16>(obs, unit);
 }
 
-// Wavefronts per block of the per-row kernel: the most resident waves per CU (LDS,
-// VGPRs) that have a row to work on (na rows per cell shared by the block's waves).
-int choose_nw(int nr, int nm, int na, bool stage_y, bool obs, bool unit) {
-    int best = 0;
-    double best_score = -1.0;
-    for (int nw : {4, 8, 16}) {
-        const size_t lds = wave_lds_bytes(nw, nr, nm, stage_y);
-        if (lds > 160 * 1024) continue;
-        int blocks = (int)((160 * 1024) / lds);
-        hipFuncAttributes fa{};
-        int per_simd = 8;
-        if (hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(pick_wave(nw, obs, unit))) == hipSuccess) {
-            if (fa.numRegs > 0) per_simd = std::min(8, 512 / fa.numRegs);
-            if (fa.localSizeBytes > 0) per_simd = std::max(1, per_simd / 2);
-        } else {
-            (void)hipGetLastError();
-        }
-        blocks = std::min(blocks, (4 * per_simd) / nw);  // nw / 4 waves per SIMD per block
-        const int rounds = (na + nw - 1) / nw;
-        const double busy = (double)na / (double)(rounds * nw);  // waves with a row
-        const double score = blocks * nw * busy;
-        if (score > best_score) {
-            best_score = score;
-            best = nw;
-        }
-    }
-    return best;  // 0: the per-row kernel does not fit
-}
+// Wavefronts per block of the per-row kernel: 4. Per-cell barriers cost less with small
+// blocks, and 4-wave blocks measured fastest on both the 25x25 and the 100x100 workloads
+// (100x100 x nt=120: 108 ms against 121 ms with 8 waves and 254 ms with 16). 0 when even
+// that does not fit the LDS (velocity samples of nrzns realizations).
+int choose_nw(int nr, int nm, bool stage_y) { return wave_lds_bytes(4, nr, nm, stage_y) <= 160 * 1024 ? 4 : 0; }
 
 e2e_status dispatch_build(const BuildArgs& A, int wg, bool obs, bool unit, dim3 grid, size_t lds,
                           hipStream_t s) {
@@ -1281,7 +1258,9 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     }
     m->has_obstacles = any_obs;
     const int wg = choose_wg(nr, D, nm, stage, any_obs, unit, opts.block_threads);
-    const int nw = opts.count_kernel == 1 ? 0 : choose_nw(nr, nm, (int)na, stage, any_obs, unit);
+    int nw = opts.count_kernel == 1 ? 0 : choose_nw(nr, nm, stage);
+    if (nw && opts.block_threads && wave_lds_bytes(opts.block_threads / 64, nr, nm, stage) <= 160 * 1024)
+        nw = opts.block_threads / 64;  // a forced workgroup size applies to the per-row kernel too
     const size_t wave_lds = nw ? wave_lds_bytes(nw, nr, nm, stage) : 0;
     m->info.block_threads = wg;
     m->info.wave_blocks = nw;

@@ -94,7 +94,8 @@ typedef struct e2e_build_opts {
     int32_t slot_k;            /* staging entries per row before the overflow arena (0: 16) */
     int32_t radix_bits;        /* digit width of the LDS segmented radix sort, 1..8 (0: 8) */
     int32_t slices_per_launch; /* time slices per build-kernel launch (0: all) */
-    int32_t block_threads;     /* block-level kernel workgroup size 256/512/1024 (0: chosen from the LDS footprint) */
+    int32_t block_threads;     /* build workgroup size 256/512/1024 of both counting kernels (0: chosen from
+                                  the LDS footprint and register use) */
     void* stream;              /* hipStream_t to run on; NULL: the library creates one */
     int32_t count_kernel;      /* 0: one wavefront per (state, action) row, rows it cannot take (S2 offsets
                                   beyond +-127 cells or more than 256 candidate keys) go to the block-level

@@ -93,8 +93,11 @@ def test_build_bit_exact(gpu, name, kernel):
             if kernel == 0:
                 assert info["n_deferred_rows"] > 0  # spreads beyond the per-row kernel
         assert info["unit_cells"] == (p.dx == 1.0 and p.dy == 1.0)
-        if kernel == 1 and "block_threads" in opts:
-            assert info["block_threads"] == opts["block_threads"]
+        if "block_threads" in opts:
+            if kernel == 1:
+                assert info["block_threads"] == opts["block_threads"]
+            else:
+                assert info["wave_blocks"] == opts["block_threads"] // 64
         # the int32 export (src/convert_model_datatype.py) carries the same values
         S1, S2, val, R = m.export_coo(np.int32)
         assert (S1 == ref["S1"]).all() and (S2 == ref["S2"]).all()
