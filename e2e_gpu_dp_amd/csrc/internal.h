@@ -79,6 +79,7 @@ struct Model {
 
     // plain solve buffers
     float* V[2] = {nullptr, nullptr};
+    double* W[2] = {nullptr, nullptr};  // float64 values of the float64 sweeps
     int32_t* pi = nullptr;
     VIState* st = nullptr;
     float* vi_part = nullptr;     // per-block residual partials of the running sweep

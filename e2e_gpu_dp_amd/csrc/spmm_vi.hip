@@ -15,10 +15,15 @@
 //   * rows of the goal cell are self-loops with R = 0 in the per-goal model, so
 //     V_j = 0 and pi_j = 0 there at every t; the kernel writes that directly.
 //
-// Layout: V[2][ns][KP] (KP = padded column count, columns contiguous per state), pi
-// [ns][KP]. Variant (CPL, SPW): a wave takes SPW states, 64/SPW lanes per state, CPL
-// adjacent columns per lane (float/float2/float4 gathers of V), so a wave covers
-// KB = 64/SPW*CPL columns of each state; grid.y walks the column blocks.
+// Layout: V[2][ns][KP] float64 (KP = padded column count, columns contiguous per
+// state), pi [ns][KP]. Variant (CPL, SPW): a wave takes SPW states, 64/SPW lanes per
+// state, CPL adjacent columns per lane (one or two 16-byte gathers of V), so a wave
+// covers KB = 64/SPW*CPL columns of each state; grid.y walks the column blocks.
+//
+// Arithmetic is that of the plain float64 sweeps: values carried in float64 between
+// sweeps, each row's products (double)P * V added in ascending entry order, Q = R +
+// discount * sum, first argmax, residual |V_new - V_old| rounded up to float. A column
+// therefore reproduces its plain solve bit for bit.
 //
 // Each column freezes at the first sweep whose own residual is below thresh, exactly
 // where its plain solve would stop; every column's arithmetic is independent of which
@@ -36,8 +41,8 @@ struct BatchArgs {
     const int2* nz;
     const float* R;
     const float* Rsum;
-    float* V0;
-    float* V1;
+    double* V0;
+    double* V1;
     int32_t* pi;
     const int32_t* goal;   // [KP] goal cell per column (-1: none)
     int32_t* conv;         // [KP] sweep at which the column converged, -1 running
@@ -52,34 +57,36 @@ struct BatchArgs {
 
 namespace {
 
-__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
-    s = a + b;
-    const float bp = s - a;
-    e = (a - (s - bp)) + (b - bp);
+// float residual never below the double one r >= 0 (round to nearest, then one ulp up
+// if that fell below r), the same rounding as the plain sweeps
+__device__ __forceinline__ float res_up(double r) {
+    const float f = (float)r;
+    return (double)f < r ? __uint_as_float(__float_as_uint(f) + 1u) : f;
 }
 
 template <int CPL>
 struct VecLoad;
 template <>
 struct VecLoad<1> {
-    __device__ static void ld(const float* p, float* x) { x[0] = *p; }
+    __device__ static void ld(const double* p, double* x) { x[0] = *p; }
 };
 template <>
 struct VecLoad<2> {
-    __device__ static void ld(const float* p, float* x) {
-        float2 v = *reinterpret_cast<const float2*>(p);
+    __device__ static void ld(const double* p, double* x) {
+        const double2 v = *reinterpret_cast<const double2*>(p);
         x[0] = v.x;
         x[1] = v.y;
     }
 };
 template <>
 struct VecLoad<4> {
-    __device__ static void ld(const float* p, float* x) {
-        float4 v = *reinterpret_cast<const float4*>(p);
+    __device__ static void ld(const double* p, double* x) {
+        const double2 v = reinterpret_cast<const double2*>(p)[0];
+        const double2 w = reinterpret_cast<const double2*>(p)[1];
         x[0] = v.x;
         x[1] = v.y;
-        x[2] = v.z;
-        x[3] = v.w;
+        x[2] = w.x;
+        x[3] = w.y;
     }
 };
 
@@ -97,8 +104,8 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_sweep(const BatchArgs A) {
     BatchState* st = A.st;
     if (st->all_conv) return;
     const uint32_t k = st->k;
-    const float* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
-    float* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
+    const double* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
+    double* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int grp = lane / LPS, l = lane % LPS;
     const int j0 = blockIdx.y * KB + l * CPL;
@@ -118,7 +125,7 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_sweep(const BatchArgs A) {
         const int cell = (int)(s - (int64_t)t * A.nc);
         const int colbase = (t + 1) * A.nc;
         const float delta = (float)((double)A.r_term - (t > A.nt - 20 ? A.late_pen : 0.0));
-        float best[CPL];
+        double best[CPL];
         int bi[CPL];
 #pragma unroll
         for (int c = 0; c < CPL; c++) {
@@ -128,28 +135,22 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_sweep(const BatchArgs A) {
         for (int a = 0; a < A.na; a++) {
             const int64_t row = s * A.na + a;
             const uint32_t beg = A.rp[row], end = A.rp[row + 1];
-            float sh[CPL], sl[CPL];
+            double sd[CPL];
             int hit[CPL];
 #pragma unroll
             for (int c = 0; c < CPL; c++) {
-                sh[c] = 0.0f;
-                sl[c] = 0.0f;
+                sd[c] = 0.0;
                 hit[c] = -1;
             }
             for (uint32_t j = beg; j < end; j++) {
                 const int2 e = A.nz[j];
                 const float v = __int_as_float(e.y);
-                float x[CPL];
+                double x[CPL];
                 VecLoad<CPL>::ld(Vin + (int64_t)e.x * KP + j0, x);
                 const int cc = e.x - colbase;
 #pragma unroll
                 for (int c = 0; c < CPL; c++) {
-                    const float p = v * x[c];
-                    const float pe = fmaf(v, x[c], -p);
-                    float tt, err;
-                    two_sum(sh[c], p, tt, err);
-                    sh[c] = tt;
-                    sl[c] = sl[c] + (err + pe);
+                    sd[c] = sd[c] + (double)v * x[c];
                     if (cc == goal[c]) hit[c] = (int)rintf(v * (float)A.nr);
                 }
             }
@@ -157,32 +158,26 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_sweep(const BatchArgs A) {
 #pragma unroll
             for (int c = 0; c < CPL; c++) {
                 const float rj = hit[c] >= 0 ? (A.Rsum[row] + (float)hit[c] * delta) / A.nr : rr;
-                float h = sh[c], lo = sl[c];
-                if (A.discount != 1.0f) {
-                    h *= A.discount;
-                    lo *= A.discount;
-                }
-                float tt, err;
-                two_sum(rj, h, tt, err);
-                const float q = tt + (err + lo);
+                double q = (double)rj + (double)A.discount * sd[c];
+                if (q != q) q = -INFINITY;
                 if (q > best[c]) {  // strict '>' in action order (spvi_2.cu:112)
                     best[c] = q;
                     bi[c] = a;
                 }
             }
         }
-        float vo[CPL];
+        double vo[CPL];
         int po[CPL];
 #pragma unroll
         for (int c = 0; c < CPL; c++) {
             if (cell == goal[c]) {
-                vo[c] = 0.0f;
+                vo[c] = 0.0;
                 po[c] = 0;
-            } else if (best[c] > -1e6f) {
+            } else if (best[c] > -1e6) {
                 vo[c] = best[c];
                 po[c] = bi[c];
             } else {
-                vo[c] = -1e6f;
+                vo[c] = -1e6;
                 po[c] = -1;
             }
         }
@@ -192,7 +187,7 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_sweep(const BatchArgs A) {
             if (live[c]) {
                 Vout[o + c] = vo[c];
                 A.pi[o + c] = po[c];
-                rmax[c] = fmaxf(rmax[c], fabsf(vo[c] - Vin[o + c]));
+                rmax[c] = fmaxf(rmax[c], res_up(fabs(vo[c] - Vin[o + c])));
             }
         }
     }
@@ -249,10 +244,10 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_sweep(const BatchArgs A) {
     }
 }
 
-__global__ void batch_init(float* V0, int64_t n, const int32_t* goal, int32_t* conv, int32_t* start,
+__global__ void batch_init(double* V0, int64_t n, const int32_t* goal, int32_t* conv, int32_t* start,
                            float* last_res, int kp, int k, BatchState* st) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) V0[i] = 0.0f;
+    if (i < n) V0[i] = 0.0;
     if (i < kp) {
         conv[i] = i < k ? -1 : 0;  // padding columns never run
         start[i] = 1;
@@ -265,16 +260,17 @@ __global__ void batch_init(float* V0, int64_t n, const int32_t* goal, int32_t* c
     }
 }
 
-// out[j][s] = V[conv_j parity][s][j]; pol[j][s] likewise from pi (written only while live)
-__global__ void batch_gather(const float* V0, const float* V1, const int32_t* pi, const int32_t* conv,
+// out[j][s] = V[conv_j parity][s][j] rounded to float once; pol[j][s] likewise from pi
+// (written only while live)
+__global__ void batch_gather(const double* V0, const double* V1, const int32_t* pi, const int32_t* conv,
                              int64_t ns, int kp, int k, int last_k, float* vout, int32_t* pout) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ns * k) return;
     const int j = (int)(i / ns);
     const int64_t s = i - (int64_t)j * ns;
     const int cj = conv[j] >= 0 ? conv[j] : last_k;
-    const float* V = (cj & 1) ? V1 : V0;
-    vout[i] = V[s * kp + j];
+    const double* V = (cj & 1) ? V1 : V0;
+    vout[i] = (float)V[s * kp + j];
     pout[i] = pi[s * kp + j];
 }
 
@@ -297,7 +293,7 @@ int kb_of(const Variant& v) { return (64 / v.spw) * v.cpl; }
 struct BatchBufs {
     int kp = 0;
     int64_t ns = 0;
-    float* V[2] = {nullptr, nullptr};
+    double* V[2] = {nullptr, nullptr};
     int32_t* pi = nullptr;
     int32_t* goal = nullptr;
     int32_t* conv = nullptr;
@@ -335,7 +331,7 @@ void batch_cache_free(BatchCache* c) {
 
 static e2e_status alloc_bufs(const Model* m, BatchBufs& b, int kp) {
     const size_t n = (size_t)m->ns * kp;
-    if (hipMalloc(&b.V[0], n * 4) != hipSuccess || hipMalloc(&b.V[1], n * 4) != hipSuccess ||
+    if (hipMalloc(&b.V[0], n * 8) != hipSuccess || hipMalloc(&b.V[1], n * 8) != hipSuccess ||
         hipMalloc(&b.pi, n * 4) != hipSuccess || hipMalloc(&b.goal, kp * 4) != hipSuccess ||
         hipMalloc(&b.conv, kp * 4) != hipSuccess || hipMalloc(&b.start, kp * 4) != hipSuccess ||
         hipMalloc(&b.part, (size_t)kBMaxBlocks * kp * 4) != hipSuccess ||
@@ -546,7 +542,8 @@ e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve
         stats->solve_ms = now_ms() - t_start;
         stats->sweep_kernel_ms = kms;
         stats->tune_ms = tune_ms;
-        stats->bytes_per_sweep = 8.0 * m->nnz + 8.0 * m->rows + 4.0 + 12.0 * (double)k * m->ns;
+        // entries, row pointers + R, and per column: V read, V written (8 B each), policy
+        stats->bytes_per_sweep = 8.0 * m->nnz + 8.0 * m->rows + 4.0 + 20.0 * (double)k * m->ns;
         stats->launches = launched;
     }
     if (nconv < k) {
@@ -567,16 +564,16 @@ namespace {
 
 // admit n columns: goal[col] = g, conv = -1, start = k, zero the column in the buffer
 // the next sweep reads; clears all_conv so launches do work again.
-__global__ void live_admit_kernel(float* V0, float* V1, int64_t ns, int kp, const int32_t* cols,
+__global__ void live_admit_kernel(double* V0, double* V1, int64_t ns, int kp, const int32_t* cols,
                                   const int32_t* goals, int n, int32_t* goal, int32_t* conv, int32_t* start,
                                   float* last_res, BatchState* st) {
     const uint32_t k = st->k;
-    float* Vin = (k & 1u) ? V0 : V1;
+    double* Vin = (k & 1u) ? V0 : V1;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ns * n) {
         const int c = (int)(i / ns);
         const int64_t s = i - (int64_t)c * ns;
-        Vin[s * kp + cols[c]] = 0.0f;
+        Vin[s * kp + cols[c]] = 0.0;
     }
     if (i < n) {
         goal[cols[i]] = goals[i];
@@ -587,22 +584,22 @@ __global__ void live_admit_kernel(float* V0, float* V1, int64_t ns, int kp, cons
     if (i == 0) st->all_conv = 0;
 }
 
-__global__ void live_column(const float* V0, const float* V1, const int32_t* pi, const int32_t* conv,
+__global__ void live_column(const double* V0, const double* V1, const int32_t* pi, const int32_t* conv,
                             int64_t ns, int kp, int col, float* vout, int32_t* pout) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
     const int cj = conv[col];
-    const float* V = (cj & 1) ? V1 : V0;
-    vout[s] = V[s * kp + col];
+    const double* V = (cj & 1) ? V1 : V0;
+    vout[s] = (float)V[s * kp + col];
     pout[s] = pi[s * kp + col];
 }
 
-__global__ void live_init(float* V0, float* V1, int64_t n, int32_t* goal, int32_t* conv, int32_t* start,
+__global__ void live_init(double* V0, double* V1, int64_t n, int32_t* goal, int32_t* conv, int32_t* start,
                           float* last_res, int kp, BatchState* st) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
-        V0[i] = 0.0f;
-        V1[i] = 0.0f;
+        V0[i] = 0.0;
+        V1[i] = 0.0;
     }
     if (i < kp) {
         goal[i] = -1;
@@ -631,7 +628,8 @@ struct LiveBatch {
 };
 
 int64_t live_batch_bytes(const Model* m, int kp) {
-    return (int64_t)m->ns * kp * 4 * 5 + (int64_t)kp * (40 + 4 * kBMaxBlocks) + 4096;
+    // V[2] float64, pi, and the float value / policy staging of a harvest
+    return (int64_t)m->ns * kp * (2 * 8 + 3 * 4) + (int64_t)kp * (40 + 4 * kBMaxBlocks) + 4096;
 }
 
 e2e_status live_create(Model* m, int kp, float thresh, hipStream_t s, LiveBatch** out) {

@@ -39,6 +39,8 @@ struct VIArgs {
     const float* R;
     float* V0;
     float* V1;
+    double* W0;  // float64 value buffers of the float64 sweeps (lean, tiled, jagged)
+    double* W1;
     int32_t* pi;
     VIState* st;
     float* part;   // [gridDim.x] per-block residual maxima of the running sweep
@@ -309,6 +311,20 @@ __device__ __forceinline__ int dpp_row_shr(int v, int n, int id) {
 __device__ __forceinline__ float dpp_row_shr(float v, int n, float id) {
     return __int_as_float(dpp_row_shr(__float_as_int(v), n, __float_as_int(id)));
 }
+__device__ __forceinline__ double dpp_row_shr(double v, int n, double id) {
+    const long long b = __double_as_longlong(v), ib = __double_as_longlong(id);
+    const int lo = dpp_row_shr((int)(b & 0xffffffffll), n, (int)(ib & 0xffffffffll));
+    const int hi = dpp_row_shr((int)(b >> 32), n, (int)(ib >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// float residual that is never below the double one r >= 0 (so res < thresh in float
+// implies the double residual is below thresh too): round to nearest, then one ulp up
+// if that fell below r (for a non-negative finite float the next one up is bits + 1)
+__device__ __forceinline__ float res_up(double r) {
+    const float f = (float)r;
+    return (double)f < r ? __uint_as_float(__float_as_uint(f) + 1u) : f;
+}
 
 // A lane adds its row's products from LDS kRowReads at a time: all reads issued, one
 // wait, then the in-range ones added in ascending order (the same order, and so the
@@ -374,8 +390,8 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
     VIState* st = A.st;
     if (st->conv >= 0) return;  // converged at an earlier sweep
     const uint32_t k = st->k;
-    const float* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
-    float* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
+    const double* __restrict__ Vin = (k & 1u) ? A.W0 : A.W1;  // float64 values between sweeps
+    double* __restrict__ Vout = (k & 1u) ? A.W1 : A.W0;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int a = lane % NAP, sl = lane / NAP;
@@ -395,7 +411,8 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
         const uint32_t nwin = (Pend - P0 + WIN - 1) / WIN;
         struct Meta {
             uint32_t rb, re;
-            float rr, vold;
+            float rr;
+            double vold;
         };
         // row data of group g (clamped to the wave's last group, so always in range)
         auto load_meta = [&](uint32_t g, Meta& M) {
@@ -415,7 +432,7 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
         double sd = 0.0;
 
         auto finish_group = [&]() {
-            float q = valid ? (float)((double)cur.rr + (double)A.discount * sd) : -INFINITY;
+            double q = valid ? (double)cur.rr + (double)A.discount * sd : -INFINITY;
             if (q != q) q = -INFINITY;
             int ba = valid ? a : 0x7fffffff;
             // max / first argmax over the NAP lanes of each state; the combine is
@@ -425,7 +442,7 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
                 // (a = NAP-1) has combined exactly its own NAP lanes
 #pragma unroll
                 for (int o = 1; o < NAP; o <<= 1) {
-                    const float q2 = dpp_row_shr(q, o, -INFINITY);
+                    const double q2 = dpp_row_shr(q, o, (double)-INFINITY);
                     const int a2 = dpp_row_shr(ba, o, 0x7fffffff);
                     const bool take = q2 > q || (q2 == q && a2 < ba);
                     q = take ? q2 : q;
@@ -434,7 +451,7 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
             } else {
 #pragma unroll
                 for (int o = 1; o < NAP; o <<= 1) {
-                    const float q2 = __shfl_xor(q, o, 64);
+                    const double q2 = __shfl_xor(q, o, 64);
                     const int a2 = __shfl_xor(ba, o, 64);
                     if (q2 > q || (q2 == q && a2 < ba)) {
                         q = q2;
@@ -445,18 +462,18 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
             const bool writer = (NAP <= 16) ? (a == NAP - 1) : (a == 0);
             if (writer && g < ngroups && s_begin + g * SPC + (uint32_t)sl < s_end) {
                 const uint32_t s = s_begin + g * SPC + (uint32_t)sl;
-                float vnew;
+                double vnew;
                 int32_t pol;
-                if (q > -1e6f) {
+                if (q > -1e6) {
                     vnew = q;
                     pol = ba;
                 } else {
-                    vnew = -1e6f;
+                    vnew = -1e6;
                     pol = -1;
                 }
                 Vout[s] = vnew;
                 A.pi[s] = pol;
-                res = fmaxf(res, fabsf(vnew - cur.vold));
+                res = fmaxf(res, res_up(fabs(vnew - cur.vold)));
             }
             sd = 0.0;
             g++;
@@ -473,11 +490,11 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
             int2 e[U];
 #pragma unroll
             for (int u = 0; u < U; u++) e[u] = nzp[kw * WIN + u * 64];
-            float x[U];
+            double x[U];
 #pragma unroll
             for (int u = 0; u < U; u++) x[u] = ld_off(Vin, (uint32_t)e[u].x);
 #pragma unroll
-            for (int u = 0; u < U; u++) wsd[u * 64 + lane] = (double)__int_as_float(e[u].y) * (double)x[u];
+            for (int u = 0; u < U; u++) wsd[u * 64 + lane] = (double)__int_as_float(e[u].y) * x[u];
             __builtin_amdgcn_wave_barrier();
             for (;;) {
                 if (valid) row_part(wsd - w0, max(cur.rb, w0), min(cur.re, wend), sd);
@@ -540,15 +557,15 @@ template <int NAP, int U>
 __global__ __launch_bounds__(kStreamThreads) void spmv_vi_tile(const VIArgs A) {
     constexpr int SPC = 64 / NAP;
     constexpr int WIN = 64 * U;
-    extern __shared__ float vband[];  // [vwin]
+    extern __shared__ double vband[];  // [vwin]
     __shared__ float wmax[kStreamWaves];
     __shared__ int is_last;
     __shared__ double stage[kStreamWaves][WIN + kRowReads];
     VIState* st = A.st;
     if (st->conv >= 0) return;  // converged at an earlier sweep
     const uint32_t k = st->k;
-    const float* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
-    float* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
+    const double* __restrict__ Vin = (k & 1u) ? A.W0 : A.W1;  // float64 values between sweeps
+    double* __restrict__ Vout = (k & 1u) ? A.W1 : A.W0;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int a = lane % NAP, sl = lane / NAP;
@@ -575,7 +592,8 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_tile(const VIArgs A) {
         const uint32_t nwin = (Pend - P0 + WIN - 1) / WIN;
         struct Meta {
             uint32_t rb, re;
-            float rr, vold;
+            float rr;
+            double vold;
         };
         auto load_meta = [&](uint32_t g, Meta& M) {
             const uint32_t s = min(s_begin + g * SPC + (uint32_t)sl, s_end - 1u);
@@ -594,13 +612,13 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_tile(const VIArgs A) {
         double sd = 0.0;
 
         auto finish_group = [&]() {
-            float q = valid ? (float)((double)cur.rr + (double)A.discount * sd) : -INFINITY;
+            double q = valid ? (double)cur.rr + (double)A.discount * sd : -INFINITY;
             if (q != q) q = -INFINITY;
             int ba = valid ? a : 0x7fffffff;
             if (NAP <= 16) {
 #pragma unroll
                 for (int o = 1; o < NAP; o <<= 1) {
-                    const float q2 = dpp_row_shr(q, o, -INFINITY);
+                    const double q2 = dpp_row_shr(q, o, (double)-INFINITY);
                     const int a2 = dpp_row_shr(ba, o, 0x7fffffff);
                     const bool take = q2 > q || (q2 == q && a2 < ba);
                     q = take ? q2 : q;
@@ -609,7 +627,7 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_tile(const VIArgs A) {
             } else {
 #pragma unroll
                 for (int o = 1; o < NAP; o <<= 1) {
-                    const float q2 = __shfl_xor(q, o, 64);
+                    const double q2 = __shfl_xor(q, o, 64);
                     const int a2 = __shfl_xor(ba, o, 64);
                     if (q2 > q || (q2 == q && a2 < ba)) {
                         q = q2;
@@ -620,18 +638,18 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_tile(const VIArgs A) {
             const bool writer = (NAP <= 16) ? (a == NAP - 1) : (a == 0);
             if (writer && g < ngroups && s_begin + g * SPC + (uint32_t)sl < s_end) {
                 const uint32_t s = s_begin + g * SPC + (uint32_t)sl;
-                float vnew;
+                double vnew;
                 int32_t pol;
-                if (q > -1e6f) {
+                if (q > -1e6) {
                     vnew = q;
                     pol = ba;
                 } else {
-                    vnew = -1e6f;
+                    vnew = -1e6;
                     pol = -1;
                 }
                 Vout[s] = vnew;
                 A.pi[s] = pol;
-                res = fmaxf(res, fabsf(vnew - cur.vold));
+                res = fmaxf(res, res_up(fabs(vnew - cur.vold)));
             }
             sd = 0.0;
             g++;
@@ -648,7 +666,7 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_tile(const VIArgs A) {
             int2 e[U];
 #pragma unroll
             for (int u = 0; u < U; u++) e[u] = nzp[kw * WIN + u * 64];
-            float x[U];
+            double x[U];
             if (in_lds) {
 #pragma unroll
                 for (int u = 0; u < U; u++) x[u] = vband[min((uint32_t)e[u].x - cmin, span - 1u)];
@@ -657,7 +675,7 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_tile(const VIArgs A) {
                 for (int u = 0; u < U; u++) x[u] = ld_off(Vin, (uint32_t)e[u].x);
             }
 #pragma unroll
-            for (int u = 0; u < U; u++) wsd[u * 64 + lane] = (double)__int_as_float(e[u].y) * (double)x[u];
+            for (int u = 0; u < U; u++) wsd[u * 64 + lane] = (double)__int_as_float(e[u].y) * x[u];
             __builtin_amdgcn_wave_barrier();
             for (;;) {
                 if (valid) row_part(wsd - w0, max(cur.rb, w0), min(cur.re, wend), sd);
@@ -757,8 +775,8 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_jagged(const VIArgs A)
     VIState* st = A.st;
     if (st->conv >= 0) return;  // converged at an earlier sweep
     const uint32_t k = st->k;
-    const float* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
-    float* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
+    const double* __restrict__ Vin = (k & 1u) ? A.W0 : A.W1;  // float64 values between sweeps
+    double* __restrict__ Vout = (k & 1u) ? A.W1 : A.W0;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int rank = lane % NAP, sl = lane / NAP;
@@ -776,7 +794,7 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_jagged(const VIArgs A)
         const uint32_t len = meta & 0xffffu, act = meta >> 16;
         const float rr = vr ? ld_off(A.jR, row) : 0.0f;
         const uint32_t base = vs ? ld_off(A.rp, s * na) : 0u;
-        const float vold = vs ? ld_off(Vin, s) : 0.0f;
+        const double vold = vs ? ld_off(Vin, s) : 0.0;
         double acc = 0.0;
         uint32_t off = 0;
         for (uint32_t kd = 0;; kd += UNR) {
@@ -793,20 +811,20 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_jagged(const VIArgs A)
             int2 e[UNR];
 #pragma unroll
             for (int u = 0; u < UNR; u++) e[u] = on[u] ? ld_off(A.jnz, pos[u]) : make_int2(0, 0);
-            float x[UNR];
+            double x[UNR];
 #pragma unroll
-            for (int u = 0; u < UNR; u++) x[u] = on[u] ? ld_off(Vin, (uint32_t)e[u].x) : 0.0f;
+            for (int u = 0; u < UNR; u++) x[u] = on[u] ? ld_off(Vin, (uint32_t)e[u].x) : 0.0;
 #pragma unroll
             for (int u = 0; u < UNR; u++)
-                if (on[u]) acc += (double)__int_as_float(e[u].y) * (double)x[u];  // exact product
+                if (on[u]) acc += (double)__int_as_float(e[u].y) * x[u];
         }
-        float q = vr ? (float)((double)rr + (double)A.discount * acc) : -INFINITY;
+        double q = vr ? (double)rr + (double)A.discount * acc : -INFINITY;
         if (q != q) q = -INFINITY;
         int ba = vr ? (int)act : 0x7fffffff;
         if (NAP <= 16) {
 #pragma unroll
             for (int o = 1; o < NAP; o <<= 1) {
-                const float q2 = dpp_row_shr(q, o, -INFINITY);
+                const double q2 = dpp_row_shr(q, o, (double)-INFINITY);
                 const int a2 = dpp_row_shr(ba, o, 0x7fffffff);
                 const bool take = q2 > q || (q2 == q && a2 < ba);
                 q = take ? q2 : q;
@@ -815,7 +833,7 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_jagged(const VIArgs A)
         } else {
 #pragma unroll
             for (int o = 1; o < NAP; o <<= 1) {
-                const float q2 = __shfl_xor(q, o, 64);
+                const double q2 = __shfl_xor(q, o, 64);
                 const int a2 = __shfl_xor(ba, o, 64);
                 if (q2 > q || (q2 == q && a2 < ba)) {
                     q = q2;
@@ -825,18 +843,18 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_jagged(const VIArgs A)
         }
         const bool writer = (NAP <= 16) ? (rank == NAP - 1) : (rank == 0);
         if (writer && vs) {
-            float vnew;
+            double vnew;
             int32_t pol;
-            if (q > -1e6f) {
+            if (q > -1e6) {
                 vnew = q;
                 pol = ba;
             } else {
-                vnew = -1e6f;
+                vnew = -1e6;
                 pol = -1;
             }
             Vout[s] = vnew;
             A.pi[s] = pol;
-            res = fmaxf(res, fabsf(vnew - vold));
+            res = fmaxf(res, res_up(fabs(vnew - vold)));
         }
     }
 
@@ -908,9 +926,17 @@ __global__ __launch_bounds__(256) void jds_convert(const uint32_t* __restrict__ 
     }
 }
 
-__global__ void vi_init(float* V0, int64_t ns, VIState* st) {
+__global__ void vi_to_f32(const double* __restrict__ W, float* __restrict__ V, int64_t ns) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ns) V0[i] = 0.0f;  // PV starts at 0 (spvi_2.cu:433-434,470)
+    if (i < ns) V[i] = (float)W[i];  // the f32 value function, rounded once
+}
+
+__global__ void vi_init(float* V0, double* W0, int64_t ns, VIState* st) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ns) {  // PV starts at 0 (spvi_2.cu:433-434,470)
+        V0[i] = 0.0f;
+        W0[i] = 0.0;
+    }
     if (i == 0) {
         st->k = 1u;
         st->conv = -1;
@@ -1033,8 +1059,8 @@ SweepFn pick_jagged(int nap, int u, int* u_used) {
     return pick_jagged_nap<8>(nap);
 }
 
-// LDS value window of the tiled sweep: at most this many floats (32 KB)
-constexpr uint32_t kMaxBand = 8192;
+// LDS value window of the tiled sweep: at most this many values
+constexpr uint32_t kMaxBand = 4096;  // doubles: 32 KB
 
 // Sweep kernels (0-based; the C ABI numbers them from 1):
 //   0 chunked, compensated float   (spmv_vi_sweep<.., 0>)
@@ -1075,10 +1101,18 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     hipStream_t s = m->stream;
     if (!m->V[0]) {
         if (hipMalloc(&m->V[0], sizeof(float) * ns) != hipSuccess ||
+            hipMalloc(&m->W[0], sizeof(double) * ns) != hipSuccess ||
+            hipMalloc(&m->W[1], sizeof(double) * ns) != hipSuccess ||
             hipMalloc(&m->V[1], sizeof(float) * ns) != hipSuccess ||
             hipMalloc(&m->pi, sizeof(int32_t) * ns) != hipSuccess ||
             hipMalloc(&m->st, sizeof(VIState)) != hipSuccess ||
             hipMalloc(&m->vi_part, sizeof(float) * kVIMaxBlocks) != hipSuccess) {
+            void** bufs[] = {(void**)&m->V[0], (void**)&m->W[0], (void**)&m->W[1], (void**)&m->V[1],
+                             (void**)&m->pi,   (void**)&m->st,   (void**)&m->vi_part};
+            for (void** b : bufs) {  // all or nothing, so a later call allocates again
+                if (*b) (void)hipFree(*b);
+                *b = nullptr;
+            }
             set_error("vi_solve: device allocation failed");
             return E2E_ERR_NOMEM;
         }
@@ -1100,7 +1134,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     E2E_CHECK(fn != nullptr, E2E_ERR_INVALID, "vi_solve: unknown kernel_variant %d", opts.kernel_variant);
     const int threads = (lean || tiled || jagged) ? kStreamThreads : kVIThreads;
     if (jagged) {
-        E2E_CHECK(ns * (int64_t)sizeof(float) < (1ll << 32) && (m->rows + 1) * 4 < (1ll << 32) &&
+        E2E_CHECK(ns * (int64_t)sizeof(double) < (1ll << 32) && (m->rows + 1) * 4 < (1ll << 32) &&
                       (m->nnz + kNzPad) * 8 < (1ll << 32),
                   E2E_ERR_CAPACITY, "vi_solve: the jagged sweep addresses entries with 32-bit byte offsets");
         E2E_CHECK(m->na <= 64 && m->info.max_row_nnz < 65536, E2E_ERR_CAPACITY,
@@ -1123,7 +1157,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     uint32_t vwin = 0, tile_states = 0;
     int64_t ntiles = 0;
     if (tiled) {
-        E2E_CHECK(ns * (int64_t)sizeof(float) < (1ll << 32) && (m->rows + 1) * 4 < (1ll << 32), E2E_ERR_CAPACITY,
+        E2E_CHECK(ns * (int64_t)sizeof(double) < (1ll << 32) && (m->rows + 1) * 4 < (1ll << 32), E2E_ERR_CAPACITY,
                   "vi_solve: the tiled sweep addresses states and rows with 32-bit byte offsets");
         tile_states = (uint32_t)(16 * kStreamWaves * grp);
         ntiles = ceil_div(ns, (int64_t)tile_states);
@@ -1162,7 +1196,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
             m->vi_part_cap = ntiles;
         }
     }
-    const size_t shmem = (size_t)vwin * sizeof(float);
+    const size_t shmem = (size_t)vwin * sizeof(double);
     // persistent grid: as many blocks as can be resident at once (grid-stride over chunks,
     // or one contiguous share per wave), so that no block starts after the others finished
     const int64_t states_per_block = (threads / 64) * grp;
@@ -1179,7 +1213,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
         tiled ? ntiles
               : std::min<int64_t>(std::min<int64_t>(ceil_div(ns, states_per_block), resident), kVIMaxBlocks);
     if (lean) {
-        E2E_CHECK(ns * (int64_t)sizeof(float) < (1ll << 32) && (m->rows + 1) * 4 < (1ll << 32), E2E_ERR_CAPACITY,
+        E2E_CHECK(ns * (int64_t)sizeof(double) < (1ll << 32) && (m->rows + 1) * 4 < (1ll << 32), E2E_ERR_CAPACITY,
                   "vi_solve: the lean sweep addresses states and rows with 32-bit byte offsets");
         const int64_t W = nblocks * kStreamWaves;
         if (m->vi_wpart_waves != W || m->vi_wpart_spc != grp) {
@@ -1205,6 +1239,8 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     A.R = m->R;
     A.V0 = m->V[0];
     A.V1 = m->V[1];
+    A.W0 = m->W[0];
+    A.W1 = m->W[1];
     A.pi = m->pi;
     A.st = m->st;
     A.part = m->vi_part;
@@ -1220,7 +1256,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     A.jmeta = jagged ? m->vi_jmeta : nullptr;
     A.jR = jagged ? m->vi_jR : nullptr;
 
-    hipLaunchKernelGGL(vi_init, dim3((unsigned)ceil_div(ns, 256)), dim3(256), 0, s, m->V[0], ns, m->st);
+    hipLaunchKernelGGL(vi_init, dim3((unsigned)ceil_div(ns, 256)), dim3(256), 0, s, m->V[0], m->W[0], ns, m->st);
     E2E_HIP(hipGetLastError());
 
     if (opts.use_graph) {
@@ -1288,7 +1324,14 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     if (rc == E2E_OK && hst.conv < 0) rc = E2E_ERR_NOT_CONVERGED;
     if (rc == E2E_ERR_NOT_CONVERGED) set_error("not converged after %d sweeps (residual %g)", sweeps, hst.last_res);
     if (rc == E2E_ERR_TIMEOUT) set_error("timeout after %d sweeps", sweeps);
+    const bool f64 = lean || tiled || jagged;  // values carried in float64 between sweeps
     const float* Vfin = (sweeps & 1) ? m->V[1] : m->V[0];
+    if (f64) {
+        hipLaunchKernelGGL(vi_to_f32, dim3((unsigned)ceil_div(ns, 256)), dim3(256), 0, s,
+                           (sweeps & 1) ? m->W[1] : m->W[0], m->V[0], ns);
+        E2E_HIP(hipGetLastError());
+        Vfin = m->V[0];
+    }
     if (value) E2E_HIP(hipMemcpyAsync(value, Vfin, sizeof(float) * ns, hipMemcpyDeviceToHost, s));
     if (policy) E2E_HIP(hipMemcpyAsync(policy, m->pi, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
     E2E_HIP(hipStreamSynchronize(s));
@@ -1302,7 +1345,10 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
         stats->solve_ms = now_ms() - t_start;
         stats->sweep_kernel_ms = kms;
         stats->sweep_kernel_ms_per_sweep = sweeps > 0 ? kms / sweeps : 0.0;
-        stats->bytes_per_sweep = 8.0 * m->nnz + 4.0 * (m->rows + 1) + 4.0 * m->rows + 12.0 * ns;
+        // entries, row pointers, R, V read + V write + policy write (8-byte V for the
+        // float64 sweeps)
+        stats->bytes_per_sweep =
+            8.0 * m->nnz + 4.0 * (m->rows + 1) + 4.0 * m->rows + (f64 ? 20.0 : 12.0) * ns;
         stats->launches = launched;
     }
     return rc;
