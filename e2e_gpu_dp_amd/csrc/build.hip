@@ -611,13 +611,23 @@ __device__ __forceinline__ int wave_incl_scan(int x) {
 }
 
 constexpr int kWaveBins = 256;  // key histogram bins per wave (nkeys above: deferred)
+// Copies of each wave's histogram: lane l counts into copy l % kHistCopies (word
+// bin * kHistCopies + copy, so a bin's copies sit in adjacent banks). Lanes of a wave
+// mostly land in a few bins, and LDS atomics to one word are serialized; the copies cut
+// those collisions by kHistCopies. The emit adds the copies.
+#ifndef E2E_HIST_COPIES
+#define E2E_HIST_COPIES 4
+#endif
+constexpr int kHistCopies = E2E_HIST_COPIES;
+static_assert((kHistCopies & (kHistCopies - 1)) == 0, "power of two");
+constexpr int kHistWords = kWaveBins * kHistCopies;
 constexpr int kFixHalf = 7;     // fixed offset window +-7 cells, counted during the move
 constexpr int kFixSide = 2 * kFixHalf + 1;
 static_assert(kFixSide * kFixSide <= kWaveBins, "fixed window fits the histogram");
 
 // LDS bytes of mc_wave<NW>.
 __host__ __device__ inline size_t wave_lds_bytes(int NW, int nr, int nm, bool stage_y) {
-    return sizeof(float2) * (size_t)nr + NW * sizeof(uint32_t) * kWaveBins +
+    return sizeof(float2) * (size_t)nr + NW * sizeof(uint32_t) * kHistWords +
            (stage_y ? sizeof(float) * (size_t)nr * nm : 0);
 }
 
@@ -646,11 +656,12 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nr = A.nr, nm = A.nm, g = A.g, nc = A.nc, na = A.na;
     float2* vxy = reinterpret_cast<float2*>(smem_raw);                                  // [nr]
-    uint32_t* hist = reinterpret_cast<uint32_t*>(vxy + nr) + (size_t)wv * kWaveBins;   // [kWaveBins] per wave
-    float* ylds = reinterpret_cast<float*>(reinterpret_cast<uint32_t*>(vxy + nr) + (size_t)NW * kWaveBins);  // [nm][nr]
+    uint32_t* hist = reinterpret_cast<uint32_t*>(vxy + nr) + (size_t)wv * kHistWords;  // [kWaveBins][kHistCopies] per wave
+    uint32_t* hcopy = hist + (lane & (kHistCopies - 1));  // this lane's copy: hcopy[bin * kHistCopies]
+    float* ylds = reinterpret_cast<float*>(reinterpret_cast<uint32_t*>(vxy + nr) + (size_t)NW * kHistWords);  // [nm][nr]
     const int t = A.t0 + blockIdx.y;
     const size_t tslice = (size_t)t * nc;
-    for (int i = lane; i < kWaveBins; i += 64) hist[i] = 0u;
+    for (int i = lane; i < kHistWords; i += 64) hist[i] = 0u;
     if (A.stage_y) {
         const float* src = A.yi + (size_t)t * nr * nm;
         for (int i = tid; i < nr * nm; i += WG) {
@@ -758,16 +769,44 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             // (x clamp, y clamp, edge), or r_outbound (obstacles), or 0. If all of them are
             // integers within int_lim the row sum is an exact int32 sum: no per-lane test.
             bool seg_int = !uses_rad;
+            // the reward of a lane with k of those penalties that lands on a terminal cell
+            // or not: tt0 / tt1 and tn0 / tn1 for k = 0 / 1
+            int tt0 = 0, tt1 = 0, tn0 = 0, tn1 = 0;
+            bool seg_lin = seg_int;
             if (seg_int) {
                 auto ok = [&](float v) { return v == rintf(v) && fabsf(v) < A.int_lim; };
                 float rk = 0.0f;
                 seg_int = ok(A.r_out) && ok(0.0f);
+                // linear in k with slope r_outbound: then a lane's reward is the k = edge
+                // value plus r_outbound times its clamp count
+                seg_lin = seg_int;
                 for (int kk = 0; kk < 4; kk++) {
                     const float other = late_slice ? (float)((double)rk + A.late_pen) : rk;
-                    seg_int = seg_int && ok((rk + A.r_term) + rstep) && ok(other + rstep);
+                    const float rt = (rk + A.r_term) + rstep, ro = other + rstep;
+                    seg_int = seg_int && ok(rt) && ok(ro);
+                    if (seg_int) {
+                        if (kk == 0) {
+                            tt0 = (int)rt;
+                            tn0 = (int)ro;
+                        } else if (kk == 1) {
+                            tt1 = (int)rt;
+                            tn1 = (int)ro;
+                        }
+                        const int64_t slope = (int64_t)kk * (int64_t)A.r_out;
+                        seg_lin = seg_lin && (int64_t)(int)rt == (int64_t)tt0 + slope &&
+                                  (int64_t)(int)ro == (int64_t)tn0 + slope;
+                    }
                     rk = rk + A.r_out;
                 }
+                seg_lin = seg_lin && seg_int;
             }
+            // Fast pass 1: unit cells, a start cell that moves (neither terminal nor an
+            // obstacle), no scalar-field reward, no obstacle test, integer rewards linear
+            // in the penalty count. The edge and terminal terms of the reward depend on
+            // the landing cell only, so they are added per histogram bin at the emit; a
+            // lane only counts its clamps (x, y).
+            const bool fast = UNIT && !uses_rad && !(OBS && obs_slice) && !term1 && !obs1 && seg_lin;
+            int kcl = 0;
             auto round1 = [&](int w, bool valid) {
                 int di, dj;
                 float rw;
@@ -780,14 +819,47 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                     // bin of the fixed window; an offset outside it lands in some bin, but
                     // then the bounding box leaves the window and the counts are rebuilt
                     const unsigned bin = (unsigned)(di + kFixHalf) * kFixSide + (unsigned)(dj + kFixHalf);
-                    atomicAdd(&hist[min(bin, (unsigned)kWaveBins - 1u)], 1u);
+                    atomicAdd(&hcopy[min(bin, (unsigned)kWaveBins - 1u) * kHistCopies], 1u);
                 }
                 if (seg_int) ip += valid ? (int)rw : 0;
                 else add_reward(valid, rw);
             };
+            // the move of lane_move<true> for a moving lane, cut to what the fast pass
+            // needs: clamp with med3 (= the if/else-if clamp for ordered bounds; its
+            // penalty is taken when the clamp changed the value; a non-finite velocity
+            // fails the build anyway), then the index of lane_move's rint / remainder
+            // rounding, written as floor(a) + (fract(a) >= 0.5): equal for every a >= 0
+            // (a = k + f: f < 0.5 gives k, f > 0.5 and f == 0.5 give k + 1 in both)
+            auto round_fast = [&](int w, bool valid) {
+                const float2 v = vxy[valid ? w : 0];
+                const float vnetx = fc + v.x;
+                const float vnety = fs + v.y;
+                const float xnew = xc + (vnetx * A.dt);
+                const float ynew = yc + (vnety * A.dt);
+                const float xcl = __builtin_amdgcn_fmed3f(xnew, A.xs0, A.xsn);
+                const float ycl = __builtin_amdgcn_fmed3f(ynew, A.ys0, A.ysn);
+                const float ax = xcl - A.xs0;
+                const float ay = -(ycl - A.ysn);
+                const int dj = (int)floorf(ax) + (__builtin_amdgcn_fractf(ax) >= 0.5f ? 1 : 0) - j1;
+                const int di = (int)floorf(ay) + (__builtin_amdgcn_fractf(ay) >= 0.5f ? 1 : 0) - i1;
+                if (valid) {
+                    kcl += (xcl != xnew ? 1 : 0) + (ycl != ynew ? 1 : 0);
+                    dimin = min(dimin, di);
+                    dimax = max(dimax, di);
+                    djmin = min(djmin, dj);
+                    djmax = max(djmax, dj);
+                    const unsigned bin = (unsigned)(__mul24(di, kFixSide) + dj + (kFixHalf * kFixSide + kFixHalf));
+                    atomicAdd(&hcopy[min(bin, (unsigned)kWaveBins - 1u) * kHistCopies], 1u);
+                }
+            };
             const int nfull = nr / 64;
-            for (int r = 0; r < nfull; r++) round1(r * 64 + lane, true);
-            if (nfull < nrounds) round1(nfull * 64 + lane, nfull * 64 + lane < nr);
+            if (fast) {
+                for (int r = 0; r < nfull; r++) round_fast(r * 64 + lane, true);
+                if (nfull < nrounds) round_fast(nfull * 64 + lane, nfull * 64 + lane < nr);
+            } else {
+                for (int r = 0; r < nfull; r++) round1(r * 64 + lane, true);
+                if (nfull < nrounds) round1(nfull * 64 + lane, nfull * 64 + lane < nr);
+            }
             if (!mixed) rsum = (double)wave_sum(ip);
             dimin = wave_min(dimin);
             dimax = wave_max(dimax);
@@ -801,7 +873,7 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             const uint64_t nkeys = fixed ? (uint64_t)kFixSide * kFixSide : (uint64_t)(dimax - dimin + 1) * Wj;
             const size_t row = mid * na + a;
             if (wave_wide || nkeys > (uint64_t)kWaveBins) {
-                for (int i = lane; i < kWaveBins; i += 64) hist[i] = 0u;  // partial fixed counts
+                for (int i = lane; i < kHistWords; i += 64) hist[i] = 0u;  // partial fixed counts
                 if (lane == 0) A.defer_list[atomicAdd(A.defer_count, 1u)] = (uint32_t)row;
                 continue;  // the block-level kernel writes this row
             }
@@ -810,14 +882,15 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                 // ---- pass 2 (spreads beyond +-7 cells): the moves again, for the histogram
                 // of the range-compressed keys and for the lanes outside the window
                 if (!fixed)
-                    for (int i = lane; i < kWaveBins; i += 64) hist[i] = 0u;
+                    for (int i = lane; i < kHistWords; i += 64) hist[i] = 0u;
                 for (int r = 0; r < nrounds; r++) {
                     const int w = r * 64 + lane;
                     if (w < nr) {
                         int di, dj;
                         float rw;
                         lane_offset(w, true, di, dj, rw);
-                        if (!fixed) atomicAdd(&hist[(uint32_t)(di - dimin) * Wj + (uint32_t)(dj - djmin)], 1u);
+                        if (!fixed)
+                            atomicAdd(&hcopy[((uint32_t)(di - dimin) * Wj + (uint32_t)(dj - djmin)) * kHistCopies], 1u);
                         oow += ((unsigned)(di + del) >= (unsigned)A.m) | ((unsigned)(dj + del) >= (unsigned)A.m);
                     }
                 }
@@ -826,10 +899,29 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             // ---- emit the nonzero bins in key order; lane l owns bins 4l .. 4l+3
             uint32_t cnt4[4];
             int mine = 0;
+            int rbin = 0;  // fast rows: sum over the lane's bins of count * (the k = edge reward)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                cnt4[q] = hist[lane * 4 + q];
+                uint32_t cq = 0u;
+#pragma unroll
+                for (int c = 0; c < kHistCopies; c++) cq += hist[(lane * 4 + q) * kHistCopies + c];
+                cnt4[q] = cq;
                 mine += cnt4[q] != 0u;
+                if (fast && cnt4[q] != 0u) {
+                    const uint32_t kk = (uint32_t)(lane * 4 + q);
+                    const int pi = i1 + (int)(kk / Wj) + di0, pj = j1 + (int)(kk % Wj) + dj0;
+                    const int n1 = g - 1;
+                    const bool edge = (pi == 0) | (pi == n1) | (pj == 0) | (pj == n1);  // :339-341
+                    const bool term = is_terminal(A, pi, pj);                             // :343-354
+                    rbin += (int)cnt4[q] * (term ? (edge ? tt1 : tt0) : (edge ? tn1 : tn0));
+                }
+            }
+            if (fast) {
+                // every partial sum is a sum of lane rewards tk[t][edge] (each below
+                // int_lim, nr of them), so rbin's wave sum fits int32; the clamp part
+                // r_outbound * (clamp count) is summed in 64 bits
+                const int64_t tot = (int64_t)wave_sum(rbin) + (int64_t)A.r_out * (int64_t)wave_sum(kcl);
+                rsum = (double)tot;
             }
             const int incl = wave_incl_scan(mine);
             const int nuq = __builtin_amdgcn_readlane(incl, 63);
@@ -847,7 +939,9 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                     const int di = (int)(kk / Wj) + di0, dj = (int)(kk % Wj) + dj0;
                     put_entry(A, row, u, nuq, arena_off, (uint32_t)((j1 + dj) + g * (i1 + di)), cnt4[q]);
                     u++;
-                    hist[kk] = 0u;  // the histogram is all zero again for the next segment
+#pragma unroll
+                    for (int c = 0; c < kHistCopies; c++)  // all zero again for the next segment
+                        hist[kk * kHistCopies + c] = 0u;
                 }
             }
             if (lane == 0) {
