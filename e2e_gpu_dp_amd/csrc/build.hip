@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 #include "internal.h"
 
@@ -599,6 +600,13 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
     }
 }
 
+// floor(x) converted to int in one instruction (x finite, within int range)
+__device__ __forceinline__ int cvt_flr_i32(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 // Wave-level DPP scans and reductions for the wave kernel (see wave_reduce).
 __device__ __forceinline__ int wave_incl_scan(int x) {
     x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
@@ -830,32 +838,55 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             // fails the build anyway), then the index of lane_move's rint / remainder
             // rounding, written as floor(a) + (fract(a) >= 0.5): equal for every a >= 0
             // (a = k + f: f < 0.5 gives k, f > 0.5 and f == 0.5 give k + 1 in both)
-            auto round_fast = [&](int w, bool valid) {
+            // The clamp bounds in VGPRs (med3 takes one scalar operand); the column offset
+            // carries the fixed window's bias, so dj + bias is formed by the rounding's
+            // carry add and bin = di * 15 + (dj + bias).
+            constexpr int kBias = kFixHalf * kFixSide + kFixHalf;
+            float vxs0 = A.xs0, vxsn = A.xsn, vys0 = A.ys0, vysn = A.ysn;
+            int mi1 = -i1, mj1 = kBias - j1;
+            float vxc = xc, vyc = yc;  // loaded before the loop, not waited for inside it
+            asm volatile("" : "+v"(vxs0), "+v"(vxsn), "+v"(vys0), "+v"(vysn), "+v"(mi1), "+v"(mj1), "+v"(vxc),
+                         "+v"(vyc));
+            auto round_fast = [&](int w, bool valid, auto dt_one) {
                 const float2 v = vxy[valid ? w : 0];
                 const float vnetx = fc + v.x;
                 const float vnety = fs + v.y;
-                const float xnew = xc + (vnetx * A.dt);
-                const float ynew = yc + (vnety * A.dt);
-                const float xcl = __builtin_amdgcn_fmed3f(xnew, A.xs0, A.xsn);
-                const float ycl = __builtin_amdgcn_fmed3f(ynew, A.ys0, A.ysn);
-                const float ax = xcl - A.xs0;
-                const float ay = -(ycl - A.ysn);
-                const int dj = (int)floorf(ax) + (__builtin_amdgcn_fractf(ax) >= 0.5f ? 1 : 0) - j1;
-                const int di = (int)floorf(ay) + (__builtin_amdgcn_fractf(ay) >= 0.5f ? 1 : 0) - i1;
+                float xnew, ynew;
+                if constexpr (decltype(dt_one)::value) {  // vnet * 1.0f == vnet exactly
+                    xnew = vxc + vnetx;
+                    ynew = vyc + vnety;
+                } else {
+                    xnew = vxc + (vnetx * A.dt);
+                    ynew = vyc + (vnety * A.dt);
+                }
+                const float xcl = __builtin_amdgcn_fmed3f(xnew, vxs0, vxsn);
+                const float ycl = __builtin_amdgcn_fmed3f(ynew, vys0, vysn);
+                const float ax = xcl - vxs0;
+                // -(ycl - ysn) up to the sign of a zero, which neither floor nor fract
+                // below can tell apart
+                const float ay = vysn - ycl;
+                const int djb = cvt_flr_i32(ax) + mj1 + (__builtin_amdgcn_fractf(ax) >= 0.5f ? 1 : 0);
+                const int di = cvt_flr_i32(ay) + mi1 + (__builtin_amdgcn_fractf(ay) >= 0.5f ? 1 : 0);
                 if (valid) {
-                    kcl += (xcl != xnew ? 1 : 0) + (ycl != ynew ? 1 : 0);
+                    kcl += xcl != xnew ? 1 : 0;
+                    kcl += ycl != ynew ? 1 : 0;
                     dimin = min(dimin, di);
                     dimax = max(dimax, di);
-                    djmin = min(djmin, dj);
-                    djmax = max(djmax, dj);
-                    const unsigned bin = (unsigned)(__mul24(di, kFixSide) + dj + (kFixHalf * kFixSide + kFixHalf));
+                    djmin = min(djmin, djb);
+                    djmax = max(djmax, djb);
+                    const unsigned bin = (unsigned)(__mul24(di, kFixSide) + djb);
                     atomicAdd(&hcopy[min(bin, (unsigned)kWaveBins - 1u) * kHistCopies], 1u);
                 }
             };
             const int nfull = nr / 64;
             if (fast) {
-                for (int r = 0; r < nfull; r++) round_fast(r * 64 + lane, true);
-                if (nfull < nrounds) round_fast(nfull * 64 + lane, nfull * 64 + lane < nr);
+                if (A.dt == 1.0f) {
+                    for (int r = 0; r < nfull; r++) round_fast(r * 64 + lane, true, std::true_type{});
+                    if (nfull < nrounds) round_fast(nfull * 64 + lane, nfull * 64 + lane < nr, std::true_type{});
+                } else {
+                    for (int r = 0; r < nfull; r++) round_fast(r * 64 + lane, true, std::false_type{});
+                    if (nfull < nrounds) round_fast(nfull * 64 + lane, nfull * 64 + lane < nr, std::false_type{});
+                }
             } else {
                 for (int r = 0; r < nfull; r++) round1(r * 64 + lane, true);
                 if (nfull < nrounds) round1(nfull * 64 + lane, nfull * 64 + lane < nr);
@@ -865,6 +896,10 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             dimax = wave_max(dimax);
             djmin = wave_min(djmin);
             djmax = wave_max(djmax);
+            if (fast) {  // every wave has a valid lane, so these are real offsets plus the bias
+                djmin -= kBias;
+                djmax -= kBias;
+            }
             const bool wave_wide = (int64_t)dimax - dimin >= 65536 || (int64_t)djmax - djmin >= 65536;
             const bool fixed = dimin >= -kFixHalf && dimax <= kFixHalf && djmin >= -kFixHalf && djmax <= kFixHalf;
             // key decode: fixed window, or the range-compressed key of the bounding box
