@@ -178,7 +178,7 @@ class Model:
 
 def build_model(params, fields: dict, keep_rsum: bool = False, slot_k: int = 0,
                 radix_bits: int = 0, slices_per_launch: int = 0, block_threads: int = 0,
-                stream: int = 0, count_kernel: int = 0) -> Model:
+                stream: int = 0, count_kernel: int = 0, paired_rounds: int = 0) -> Model:
     """Monte Carlo build on the GPU. `fields` holds u, v, ui, vi, yi and optionally s,
     mask (numpy arrays, or contiguous torch CUDA tensors)."""
     keep = []
@@ -195,7 +195,7 @@ def build_model(params, fields: dict, keep_rsum: bool = False, slot_k: int = 0,
                 raise ValueError("fields must be all on the host or all on the device")
     f = _abi.Fields(**ptrs)
     o = _abi.BuildOpts(int(bool(on_dev)), int(keep_rsum), slot_k, radix_bits, slices_per_launch,
-                       block_threads, stream or None, count_kernel)
+                       block_threads, stream or None, count_kernel, paired_rounds)
     h = C.c_void_p()
     check(lib().e2e_model_build(C.byref(c_params(params)), C.byref(f), C.byref(o), C.byref(h)))
     return Model(h.value, params)

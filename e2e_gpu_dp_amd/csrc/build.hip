@@ -65,6 +65,7 @@ struct BuildArgs {
     unsigned long long arena_cap;
     unsigned long long* stats;  // 0 oow lanes, 1 overflow rows, 2 max nuq, 3 nonfinite,
                                 // 4 arena exhausted, 5 multipass rows
+    int32_t pipe_rounds;  // per-row kernel: pair its rounds (few waves resident per SIMD)
     float act_speed[kMaxActions];
     float act_fcos[kMaxActions];
     float act_fsin[kMaxActions];
@@ -847,8 +848,7 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             float vxc = xc, vyc = yc;  // loaded before the loop, not waited for inside it
             asm volatile("" : "+v"(vxs0), "+v"(vxsn), "+v"(vys0), "+v"(vysn), "+v"(mi1), "+v"(mj1), "+v"(vxc),
                          "+v"(vyc));
-            auto round_fast = [&](int w, bool valid, auto dt_one) {
-                const float2 v = vxy[valid ? w : 0];
+            auto move_fast = [&](float2 v, bool valid, auto dt_one) {
                 const float vnetx = fc + v.x;
                 const float vnety = fs + v.y;
                 float xnew, ynew;
@@ -879,13 +879,47 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                 }
             };
             const int nfull = nr / 64;
+            // rounds in pairs, each round's velocity read issued before the previous
+            // round's histogram atomic (LDS operations complete in order, so a read
+            // issued after an atomic would wait for it); va / vb alternate, no copies
+            // Reads past realization nr - 1 (at most 63 entries, into the histograms that
+            // follow vxy in LDS) only feed lanes of a round that is not run or not valid.
+            auto rounds_fast = [&](auto dt_one) {
+                float2 va = vxy[lane];
+                int r = 0;
+                for (; r + 2 <= nfull; r += 2) {
+                    const float2 vb = vxy[(r + 1) * 64 + lane];
+                    move_fast(va, true, dt_one);
+                    va = vxy[(r + 2) * 64 + lane];
+                    move_fast(vb, true, dt_one);
+                }
+                if (r < nfull) {
+                    const float2 vb = vxy[(r + 1) * 64 + lane];
+                    move_fast(va, true, dt_one);
+                    va = vb;
+                    r++;
+                }
+                if (r < nrounds) move_fast(va, r * 64 + lane < nr, dt_one);
+            };
+            // one round at a time (enough waves resident to hide the LDS latency)
+            auto rounds_plain = [&](auto dt_one) {
+                for (int r = 0; r < nfull; r++) move_fast(vxy[r * 64 + lane], true, dt_one);
+                if (nfull < nrounds) {
+                    const int w = nfull * 64 + lane;
+                    move_fast(vxy[w < nr ? w : 0], w < nr, dt_one);
+                }
+            };
             if (fast) {
-                if (A.dt == 1.0f) {
-                    for (int r = 0; r < nfull; r++) round_fast(r * 64 + lane, true, std::true_type{});
-                    if (nfull < nrounds) round_fast(nfull * 64 + lane, nfull * 64 + lane < nr, std::true_type{});
+                if (A.pipe_rounds) {
+                    if (A.dt == 1.0f)
+                        rounds_fast(std::true_type{});
+                    else
+                        rounds_fast(std::false_type{});
                 } else {
-                    for (int r = 0; r < nfull; r++) round_fast(r * 64 + lane, true, std::false_type{});
-                    if (nfull < nrounds) round_fast(nfull * 64 + lane, nfull * 64 + lane < nr, std::false_type{});
+                    if (A.dt == 1.0f)
+                        rounds_plain(std::true_type{});
+                    else
+                        rounds_plain(std::false_type{});
                 }
             } else {
                 for (int r = 0; r < nfull; r++) round1(r * 64 + lane, true);
@@ -1391,6 +1425,14 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     if (nw && opts.block_threads && wave_lds_bytes(opts.block_threads / 64, nr, nm, stage) <= 160 * 1024)
         nw = opts.block_threads / 64;  // a forced workgroup size applies to the per-row kernel too
     const size_t wave_lds = nw ? wave_lds_bytes(nw, nr, nm, stage) : 0;
+    // waves of the per-row kernel resident per SIMD, as its LDS allows (4 SIMDs per CU);
+    // below 4 its rounds are paired so that LDS latency overlaps the next round
+    E2E_CHECK(opts.paired_rounds >= 0 && opts.paired_rounds <= 2, E2E_ERR_INVALID,
+              "paired_rounds must be 0, 1 or 2 (got %d)", opts.paired_rounds);
+    A.pipe_rounds = opts.paired_rounds == 2 ? 1
+                    : opts.paired_rounds == 1 ? 0
+                    : (nw && (int)((160 * 1024) / wave_lds) * nw / 4 < 4 ? 1 : 0);
+    m->info.paired_rounds = nw ? A.pipe_rounds : 0;
     m->info.block_threads = wg;
     m->info.wave_blocks = nw;
     m->info.unit_cells = unit ? 1 : 0;

@@ -100,7 +100,9 @@ typedef struct e2e_build_opts {
     int32_t count_kernel;      /* 0: one wavefront per (state, action) row, rows it cannot take (S2 offsets
                                   beyond +-127 cells or more than 256 candidate keys) go to the block-level
                                   kernel; 1: the block-level kernel (one workgroup per row) for every row */
-    int32_t reserved;
+    int32_t paired_rounds;     /* rounds of 64 realizations in the per-row kernel: 0 automatic (paired when
+                                  its LDS leaves fewer than 4 wavefronts per SIMD), 1 one at a time, 2 in
+                                  pairs (the next round's LDS read issued ahead); other values are an error */
 } e2e_build_opts;
 
 typedef struct e2e_model_info {
@@ -125,6 +127,8 @@ typedef struct e2e_model_info {
     int32_t unit_cells;        /* 1: the unit-cell (dx = dy = 1) move was used */
     int32_t wave_blocks;       /* wavefronts per block of the per-row kernel (0: not used) */
     int32_t n_deferred_rows;   /* rows the per-row kernel handed to the block-level kernel */
+    int32_t paired_rounds;     /* 1: the per-row kernel ran its rounds in pairs */
+    int32_t reserved;
 } e2e_model_info;
 
 typedef struct e2e_solve_opts {
@@ -153,7 +157,7 @@ typedef struct e2e_solve_stats {
     double bytes_per_sweep;    /* algorithmic bytes of one sweep */
     int32_t launches;
     int32_t kernel_variant;    /* variant used (1-based, as in e2e_solve_opts) */
-    int32_t blocks;            /* sweep grid size (blocks of 512 threads) */
+    int32_t blocks;            /* sweep grid size in blocks */
     int32_t reserved;
 } e2e_solve_stats;
 

@@ -71,19 +71,25 @@ CASES = ["tiny", "small", "obstacles_tsg2", "reward0", "reward1", "reward2", "re
 
 
 # count_kernel 0: one wavefront per row, rows with wide S2 spreads handed to the
-# block-level kernel; 1: the block-level kernel (radix sort, forced block sizes) for all
-@pytest.mark.parametrize("kernel", [0, 1])
+# block-level kernel, its rounds one at a time (paired_rounds 1) or in pairs (2);
+# count_kernel 1: the block-level kernel (radix sort, forced block sizes) for all
+KERNELS = {"row": (0, 1), "row_paired": (0, 2), "block": (1, 0)}
+
+
+@pytest.mark.parametrize("kname", list(KERNELS))
 @pytest.mark.parametrize("name", CASES)
-def test_build_bit_exact(gpu, name, kernel):
+def test_build_bit_exact(gpu, name, kname):
+    kernel, paired = KERNELS[kname]
     p, f, opts = _case(name)
-    print(f"[case {name} kernel {kernel}] g={p.gsize} nt={p.nt} nr={p.nrzns} opts={opts}", flush=True)
+    print(f"[case {name} kernel {kname}] g={p.gsize} nt={p.nt} nr={p.nrzns} opts={opts}", flush=True)
     ref = oracle_model(p, f)
-    with e2e.build_model(p, f, count_kernel=kernel, **opts) as m:
+    with e2e.build_model(p, f, count_kernel=kernel, paired_rounds=paired, **opts) as m:
         assert_same_model(m.export_coo(), ref, name)
         info = m.info
         assert info["n_out_of_window"] == ref["oow"]
         assert info["max_row_nnz"] == ref["max_row_nnz"]
         assert (info["wave_blocks"] > 0) == (kernel == 0)
+        assert info["paired_rounds"] == (1 if paired == 2 else 0)
         if kernel == 1 and name == "multipass":
             assert info["n_multipass_rows"] > 0
         if name in ("radix2_overflow", "overflow_k1"):
