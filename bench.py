@@ -265,8 +265,13 @@ def main(argv=None):
                      "gbps": gbps, "loads_in_flight": st["loads_in_flight"]}
         out["vi"]["kernel_variant"] = st["kernel_variant"]
         out["vi"]["blocks"] = st["blocks"]
+        # the bytes the variant's layout moves (compressed entries, float64 values), and
+        # the rate that makes; bytes_per_sweep / gbps are in the reference's CSR terms
+        out["vi"]["moved_bytes_per_sweep"] = st["moved_bytes_per_sweep"]
+        out["vi"]["moved_gbps"] = st["moved_bytes_per_sweep"] / (per * 1e-3) / 1e9
+        out["vi"]["compressed_windows"] = st["compressed_windows"]
         kname = {1: "spmv_vi_sweep", 2: "spmv_vi_sweep", 3: "spmv_vi_lean", 4: "spmv_vi_tile",
-                 5: "spmv_vi_jagged"}.get(st["kernel_variant"], "?")
+                 5: "spmv_vi_jagged", 6: "spmv_vi_lean"}.get(st["kernel_variant"], "?")
         out["roofline_vi"] = {"kernel": kname, "bound": "hbm", "achieved": gbps,
                               "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": gbps / PEAK_HBM_GBPS,
                               "traffic": _pmc_traffic(args.workload, kname)}

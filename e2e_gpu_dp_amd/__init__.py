@@ -137,11 +137,11 @@ class Model:
     def solve(self, thresh: float = 0.1, discount: float = 1.0, max_sweeps: int = 0,
               sweeps_per_check: int = 0, use_graph: bool = False, loads_in_flight: int = 0,
               timeout_ms: float = 0.0, raise_on_nonconv: bool = True, kernel_variant: int = 0,
-              blocks_per_cu: int = 0):
+              blocks_per_cu: int = 0, comp_span: int = 0):
         """Sparse VI. Returns (policy uint32[Ns], value float32[Ns], stats dict)."""
         ns = self.info["ns"]
         o = _abi.SolveOpts(thresh, discount, max_sweeps, sweeps_per_check, int(use_graph),
-                           loads_in_flight, timeout_ms, kernel_variant, blocks_per_cu)
+                           loads_in_flight, timeout_ms, kernel_variant, blocks_per_cu, comp_span)
         st = _abi.SolveStats()
         pol = np.empty(ns, np.uint32)
         val = np.empty(ns, np.float32)

@@ -64,6 +64,7 @@ class SolveOpts(C.Structure):
         ("thresh", C.c_float), ("discount", C.c_float), ("max_sweeps", C.c_int32),
         ("sweeps_per_check", C.c_int32), ("use_graph", C.c_int32), ("loads_in_flight", C.c_int32),
         ("timeout_ms", C.c_double), ("kernel_variant", C.c_int32), ("blocks_per_cu", C.c_int32),
+        ("comp_span", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -72,8 +73,9 @@ class SolveStats(C.Structure):
         ("sweeps", C.c_int32), ("converged", C.c_int32), ("final_residual", C.c_float),
         ("loads_in_flight", C.c_int32), ("solve_ms", C.c_double), ("sweep_kernel_ms", C.c_double),
         ("sweep_kernel_ms_per_sweep", C.c_double), ("bytes_per_sweep", C.c_double),
+        ("moved_bytes_per_sweep", C.c_double),
         ("launches", C.c_int32), ("kernel_variant", C.c_int32), ("blocks", C.c_int32),
-        ("reserved", C.c_int32),
+        ("compressed_windows", C.c_int32),
     ]
 
 

@@ -45,7 +45,8 @@ e2e_status free_model(Model* m) {
     if (m->graph) (void)hipGraphExecDestroy(m->graph);
     batch_cache_free(m->batch);
     void* ptrs[] = {m->row_ptr, m->nz,      m->R,        m->Rsum,     m->V[0],   m->V[1],     m->W[0], m->W[1],
-                    m->pi,      m->st,      m->vi_part,  m->vi_wpart, m->vi_tdesc, m->vi_jnz, m->vi_jmeta, m->vi_jR};
+                    m->pi,      m->st,      m->vi_part,  m->vi_wpart, m->vi_tdesc, m->vi_jnz, m->vi_jmeta, m->vi_jR,
+                    m->vi_cz,   m->vi_cbase};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
@@ -88,6 +89,7 @@ static int64_t model_device_bytes(const Model* m) {
     if (m->Rsum) b += sizeof(float) * m->rows;
     if (m->V[0]) b += (2 * sizeof(float) + 2 * sizeof(double) + sizeof(int32_t)) * m->ns + sizeof(VIState);
     if (m->vi_jnz) b += sizeof(int2) * (m->nnz + kNzPad) + (sizeof(uint32_t) + sizeof(float)) * m->rows;
+    if (m->vi_cz) b += (int64_t)(sizeof(uint32_t) * 256 + sizeof(int32_t)) * m->vi_cwin;
     return b;
 }
 

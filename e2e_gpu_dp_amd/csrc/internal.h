@@ -101,6 +101,13 @@ struct Model {
     uint2* vi_tdesc = nullptr;    // tiled sweep: per tile (first column, column span)
     uint32_t vi_tile_states = 0, vi_maxspan = 0;
     int64_t vi_part_cap = 0;      // entries of vi_part
+    // compressed stream of the default sweep: per entry count << 16 | (column - base),
+    // per 256-entry window its base column (-1: that window keeps its 8-byte entries)
+    uint32_t* vi_cz = nullptr;    // [vi_cwin * 256]
+    int32_t* vi_cbase = nullptr;  // [vi_cwin]
+    int64_t vi_cwin = 0;
+    int64_t vi_cwin_comp = 0;     // windows kept at 4 bytes per entry
+    int vi_comp_span = 0;         // column span limit the stream was converted with
 
     bool has_obstacles = false;
     BatchCache* batch = nullptr;  // buffers + tuning of the batched multi-goal solve

@@ -55,6 +55,9 @@ struct VIArgs {
     const int2* jnz;        // jagged sweep: entries in per-state jagged-diagonal order
     const uint32_t* jmeta;  // jagged sweep: length | (action << 16) per sorted row
     const float* jR;        // jagged sweep: R per sorted row
+    const uint32_t* cz;     // compressed sweep: count << 16 | (column - window base) per entry
+    const int32_t* cbase;   // compressed sweep: base column per 256-entry window, -1: kept 8-byte
+    double inv_nr;          // compressed sweep: 1 / nrzns
 };
 
 constexpr int kVIThreads = 512;   // 8 waves per block
@@ -345,6 +348,7 @@ __device__ __forceinline__ void row_part(const double* q, uint32_t jb, uint32_t 
 }
 
 constexpr int kStreamThreads = 256;
+constexpr int kCompWin = 256;  // entries per window of the compressed stream
 constexpr int kStreamWaves = kStreamThreads / 64;
 
 // wpart[w] = first state of wave w: the smallest multiple of spc whose weight
@@ -380,10 +384,11 @@ __global__ void vi_partition(const uint32_t* __restrict__ rp, int64_t ns, int na
 // about 1.5x faster at 8 waves per SIMD than at 4). Wave-uniform values live in SGPRs
 // (readfirstlane, scalar loads of the read-only partition and row pointers). The next
 // group's row data is requested when a group starts. Products and sums in float64.
-template <int NAP, int U>
+template <int NAP, int U, bool COMP>
 __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
     constexpr int SPC = 64 / NAP;
     constexpr int WIN = 64 * U;
+    static_assert(!COMP || WIN == kCompWin, "compressed windows are 256 entries");
     __shared__ float wmax[kStreamWaves];
     __shared__ int is_last;
     __shared__ double stage[kStreamWaves][WIN + kRowReads];
@@ -407,8 +412,11 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
 
     if (s_begin < s_end) {
         const uint32_t ngroups = (s_end - s_begin + SPC - 1) / SPC;
-        const uint32_t P0 = rps[s_begin * na], Pend = rps[s_end * na];
-        const uint32_t nwin = (Pend - P0 + WIN - 1) / WIN;
+        const uint32_t Pb = rps[s_begin * na], Pend = rps[s_end * na];
+        // COMP: windows are the stream's aligned 256-entry blocks (one base column each);
+        // entries before Pb or from Pend on are staged but belong to no row of this wave
+        const uint32_t P0 = COMP ? Pb / WIN * WIN : Pb;
+        const uint32_t nwin = Pend > Pb ? (Pend - P0 + WIN - 1) / WIN : 0u;
         struct Meta {
             uint32_t rb, re;
             float rr;
@@ -485,16 +493,36 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
         };
 
         const int2* __restrict__ nzp = A.nz + P0 + lane;
+        const uint32_t* __restrict__ czp = A.cz + P0 + lane;
+        const CPtr<int32_t> cbs = (CPtr<int32_t>)A.cbase;
         for (uint32_t kw = 0; kw < nwin; kw++) {
             const uint32_t w0 = P0 + kw * WIN, wend = w0 + WIN;
-            int2 e[U];
+            double x[U], pv[U];
+            const int32_t base = COMP ? cbs[w0 / WIN] : -1;
+            if (COMP && base >= 0) {
+                // 4-byte entries: the probability count / nrzns is rounded to float from
+                // the double count * (1 / nrzns) (the same float as the division: that
+                // product is within 2^-52 of count / nrzns, which for count < 2^16 lies
+                // at least 2^-41 away from any rounding midpoint of float); the window
+                // was kept compressed only if every entry decodes to its stored bits
+                uint32_t e[U];
 #pragma unroll
-            for (int u = 0; u < U; u++) e[u] = nzp[kw * WIN + u * 64];
-            double x[U];
+                for (int u = 0; u < U; u++) e[u] = czp[kw * WIN + u * 64];
 #pragma unroll
-            for (int u = 0; u < U; u++) x[u] = ld_off(Vin, (uint32_t)e[u].x);
+                for (int u = 0; u < U; u++) x[u] = ld_off(Vin, (uint32_t)base + (e[u] & 0xffffu));
 #pragma unroll
-            for (int u = 0; u < U; u++) wsd[u * 64 + lane] = (double)__int_as_float(e[u].y) * x[u];
+                for (int u = 0; u < U; u++) pv[u] = (double)(float)((double)(e[u] >> 16) * A.inv_nr);
+            } else {
+                int2 e[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) e[u] = nzp[kw * WIN + u * 64];
+#pragma unroll
+                for (int u = 0; u < U; u++) x[u] = ld_off(Vin, (uint32_t)e[u].x);
+#pragma unroll
+                for (int u = 0; u < U; u++) pv[u] = (double)__int_as_float(e[u].y);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) wsd[u * 64 + lane] = pv[u] * x[u];
             __builtin_amdgcn_wave_barrier();
             for (;;) {
                 if (valid) row_part(wsd - w0, max(cur.rb, w0), min(cur.re, wend), sd);
@@ -543,7 +571,7 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
 }
 
 // ---------------------------------------------------------------------------------
-// Tiled sweep (the default). One block per tile of tile_states consecutive states;
+// Tiled sweep. One block per tile of tile_states consecutive states;
 // each of its 4 waves streams the entries of a quarter of the tile exactly like
 // spmv_vi_lean. The difference is where the value gathers go: a 64-lane gather from
 // global memory costs the texture addresser about 5x the cycles of a coalesced load of
@@ -926,6 +954,53 @@ __global__ __launch_bounds__(256) void jds_convert(const uint32_t* __restrict__ 
     }
 }
 
+// Compressed stream: one wavefront per 256-entry window of the CSR entries. A window is
+// kept at 4 bytes per entry when its columns span less than 2^16 and every probability
+// is count / nr for a count < 2^16 that the sweep's decode reproduces bit for bit;
+// otherwise its base is -1 and the sweep reads the 8-byte entries.
+__global__ __launch_bounds__(256) void comp_convert(const int2* __restrict__ nz, int64_t nnz, int64_t nwin,
+                                                   int nr, double inv_nr, uint32_t* __restrict__ cz,
+                                                   int32_t* __restrict__ cbase, int span,
+                                                   unsigned long long* ncomp) {
+    const int lane = threadIdx.x & 63;
+    const int64_t win = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (win >= nwin) return;  // whole wavefronts
+    int32_t col[4];
+    uint32_t cnt[4];
+    bool ok = true;
+    int32_t cmin = 0x7fffffff, cmax = -1;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int64_t j = win * kCompWin + u * 64 + lane;
+        const int2 e = nz[j];  // padded past nnz with zeros
+        const bool real = j < nnz;
+        const float pf = __int_as_float(e.y);
+        const double c = rint((double)pf * (double)nr);
+        const bool in16 = c >= 0.0 && c < 65536.0;
+        cnt[u] = real && in16 ? (uint32_t)c : 0u;
+        ok = ok && (!real || (in16 && __float_as_int((float)((double)cnt[u] * inv_nr)) == e.y));
+        col[u] = real ? e.x : -1;
+        if (real) {
+            cmin = min(cmin, e.x);
+            cmax = max(cmax, e.x);
+        }
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        cmin = min(cmin, __shfl_xor(cmin, o, 64));
+        cmax = max(cmax, __shfl_xor(cmax, o, 64));
+    }
+    ok = __all(ok) && cmax >= 0 && (int64_t)cmax - cmin < span;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int64_t j = win * kCompWin + u * 64 + lane;
+        cz[j] = ok && col[u] >= 0 ? (cnt[u] << 16) | (uint32_t)(col[u] - cmin) : 0u;
+    }
+    if (lane == 0) {
+        cbase[win] = ok ? cmin : -1;
+        if (ok) atomicAdd(ncomp, 1ull);
+    }
+}
+
 __global__ void vi_to_f32(const double* __restrict__ W, float* __restrict__ V, int64_t ns) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < ns) V[i] = (float)W[i];  // the f32 value function, rounded once
@@ -981,16 +1056,16 @@ SweepFn pick_u(int nap, int u, int* u_used) {
     return pick_nap<8, ACC>(nap);
 }
 
-template <int U>
+template <int U, bool COMP = false>
 SweepFn pick_lean_nap(int nap) {
     switch (nap) {
-        case 1: return spmv_vi_lean<1, U>;
-        case 2: return spmv_vi_lean<2, U>;
-        case 4: return spmv_vi_lean<4, U>;
-        case 8: return spmv_vi_lean<8, U>;
-        case 16: return spmv_vi_lean<16, U>;
-        case 32: return spmv_vi_lean<32, U>;
-        default: return spmv_vi_lean<64, U>;
+        case 1: return spmv_vi_lean<1, U, COMP>;
+        case 2: return spmv_vi_lean<2, U, COMP>;
+        case 4: return spmv_vi_lean<4, U, COMP>;
+        case 8: return spmv_vi_lean<8, U, COMP>;
+        case 16: return spmv_vi_lean<16, U, COMP>;
+        case 32: return spmv_vi_lean<32, U, COMP>;
+        default: return spmv_vi_lean<64, U, COMP>;
     }
 }
 
@@ -1068,9 +1143,13 @@ constexpr uint32_t kMaxBand = 4096;  // doubles: 32 KB
 //   2 lean streaming, float64      (spmv_vi_lean; the default)
 //   3 tiled with an LDS value band (spmv_vi_tile)
 //   4 jagged-diagonal, float64     (spmv_vi_jagged)
-constexpr int kDefaultVariant = 2;
+//   5 lean streaming over the compressed 4-byte entries, float64 (spmv_vi_lean<.., true>;
+//     the default for built models whose probabilities are counts over nrzns < 2^16)
+constexpr int kDefaultVariant = 5;
+constexpr int kDefaultImported = 2;  // imported models: their probabilities are arbitrary
 constexpr int kLeanBlocksPerCU = 5;  // fastest resident grid of the lean sweep (100x100x120 model)
-bool is_lean_variant(int variant) { return variant == 2; }
+bool is_comp_variant(int variant) { return variant == 5; }
+bool is_lean_variant(int variant) { return variant == 2 || variant == 5; }
 bool is_tile_variant(int variant) { return variant == 3; }
 bool is_jagged_variant(int variant) { return variant == 4; }
 
@@ -1082,6 +1161,9 @@ SweepFn pick_sweep(int na, int variant, int u, int* u_used) {
         case 2: return pick_lean(nap, u, u_used);
         case 3: return pick_tile(nap, u, u_used);
         case 4: return pick_jagged(nap, u, u_used);
+        case 5:
+            *u_used = 4;
+            return pick_lean_nap<4, true>(nap);
         default: return nullptr;
     }
 }
@@ -1120,7 +1202,12 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     }
     const int nap = next_pow2((int)m->na);
     const int grp = 64 / nap;
-    const int variant = opts.kernel_variant > 0 ? opts.kernel_variant - 1 : kDefaultVariant;
+    const bool comp_ok = m->built && m->p.nrzns > 0 && m->p.nrzns < 65536;
+    const int variant = opts.kernel_variant > 0 ? opts.kernel_variant - 1
+                        : (comp_ok ? kDefaultVariant : kDefaultImported);
+    const bool comp = is_comp_variant(variant);
+    E2E_CHECK(!comp || comp_ok, E2E_ERR_INVALID,
+              "vi_solve: the compressed sweep needs a built model with nrzns below 65536");
     const bool lean = is_lean_variant(variant);
     const bool tiled = is_tile_variant(variant);
     const bool jagged = is_jagged_variant(variant);
@@ -1212,9 +1299,45 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     const int64_t nblocks =
         tiled ? ntiles
               : std::min<int64_t>(std::min<int64_t>(ceil_div(ns, states_per_block), resident), kVIMaxBlocks);
+    const int span = opts.comp_span > 0 ? opts.comp_span : 65536;
+    E2E_CHECK(span <= 65536, E2E_ERR_INVALID, "vi_solve: comp_span must be at most 65536 (got %d)", opts.comp_span);
+    if (comp && m->vi_cz && m->vi_comp_span != span) {  // converted with another span
+        (void)hipFree(m->vi_cz);
+        (void)hipFree(m->vi_cbase);
+        m->vi_cz = nullptr;
+        m->vi_cbase = nullptr;
+    }
+    if (comp && !m->vi_cz) {  // the compressed stream, once per model
+        const int64_t nwin = ceil_div(m->nnz, (int64_t)kCompWin);
+        if (hipMalloc(&m->vi_cz, sizeof(uint32_t) * (size_t)std::max<int64_t>(nwin, 1) * kCompWin) != hipSuccess ||
+            hipMalloc(&m->vi_cbase, sizeof(int32_t) * (size_t)std::max<int64_t>(nwin, 1)) != hipSuccess) {
+            if (m->vi_cz) (void)hipFree(m->vi_cz);
+            m->vi_cz = nullptr;
+            m->vi_cbase = nullptr;
+            set_error("vi_solve: device allocation of the compressed stream failed");
+            return E2E_ERR_NOMEM;
+        }
+        m->vi_cwin = nwin;
+        m->vi_cwin_comp = 0;
+        m->vi_comp_span = span;
+        if (nwin > 0) {
+            DevBuf cnt;
+            E2E_TRY(dmalloc(cnt, sizeof(unsigned long long)));
+            E2E_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), s));
+            hipLaunchKernelGGL(comp_convert, dim3((unsigned)ceil_div(nwin, 4)), dim3(256), 0, s, m->nz, m->nnz, nwin,
+                               m->p.nrzns, 1.0 / (double)m->p.nrzns, m->vi_cz, m->vi_cbase, span,
+                               cnt.as<unsigned long long>());
+            E2E_HIP(hipGetLastError());
+            unsigned long long h = 0;
+            E2E_HIP(hipMemcpyAsync(&h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
+            E2E_HIP(hipStreamSynchronize(s));
+            m->vi_cwin_comp = (int64_t)h;
+        }
+    }
     if (lean) {
-        E2E_CHECK(ns * (int64_t)sizeof(double) < (1ll << 32) && (m->rows + 1) * 4 < (1ll << 32), E2E_ERR_CAPACITY,
-                  "vi_solve: the lean sweep addresses states and rows with 32-bit byte offsets");
+        E2E_CHECK(ns * (int64_t)sizeof(double) < (1ll << 32) && (m->rows + 1) * 4 < (1ll << 32) &&
+                      (m->nnz + kNzPad) * 8 < (1ll << 32),
+                  E2E_ERR_CAPACITY, "vi_solve: the lean sweep addresses states and rows with 32-bit byte offsets");
         const int64_t W = nblocks * kStreamWaves;
         if (m->vi_wpart_waves != W || m->vi_wpart_spc != grp) {
             if (m->vi_wpart) (void)hipFree(m->vi_wpart);
@@ -1255,6 +1378,9 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     A.jnz = jagged ? m->vi_jnz : nullptr;
     A.jmeta = jagged ? m->vi_jmeta : nullptr;
     A.jR = jagged ? m->vi_jR : nullptr;
+    A.cz = comp ? m->vi_cz : nullptr;
+    A.cbase = comp ? m->vi_cbase : nullptr;
+    A.inv_nr = comp ? 1.0 / (double)m->p.nrzns : 0.0;
 
     hipLaunchKernelGGL(vi_init, dim3((unsigned)ceil_div(ns, 256)), dim3(256), 0, s, m->V[0], m->W[0], ns, m->st);
     E2E_HIP(hipGetLastError());
@@ -1345,10 +1471,16 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
         stats->solve_ms = now_ms() - t_start;
         stats->sweep_kernel_ms = kms;
         stats->sweep_kernel_ms_per_sweep = sweeps > 0 ? kms / sweeps : 0.0;
-        // entries, row pointers, R, V read + V write + policy write (8-byte V for the
-        // float64 sweeps)
-        stats->bytes_per_sweep =
-            8.0 * m->nnz + 4.0 * (m->rows + 1) + 4.0 * m->rows + (f64 ? 20.0 : 12.0) * ns;
+        // entries, row pointers, R, V read + V write + policy write
+        const double meta = 4.0 * (m->rows + 1) + 4.0 * m->rows;
+        stats->bytes_per_sweep = 8.0 * m->nnz + meta + 12.0 * ns;
+        double entries = 8.0 * m->nnz;
+        if (comp) {  // whole aligned windows: 4 bytes per entry + base, 8 for wide windows
+            const double cw = (double)m->vi_cwin_comp, ww = (double)(m->vi_cwin - m->vi_cwin_comp);
+            entries = cw * (4.0 * kCompWin + 4.0) + ww * (8.0 * kCompWin + 4.0);
+        }
+        stats->moved_bytes_per_sweep = entries + meta + (f64 ? 20.0 : 12.0) * ns;
+        stats->compressed_windows = comp ? (int32_t)std::min<int64_t>(m->vi_cwin_comp, INT32_MAX) : 0;
         stats->launches = launched;
     }
     return rc;

@@ -139,11 +139,16 @@ typedef struct e2e_solve_opts {
     int32_t use_graph;         /* 1: replay the sweep chain as a captured hipGraph */
     int32_t loads_in_flight;   /* nnz loads per lane per window: 2, 4 or 8 (0: chosen from nnz per state) */
     double timeout_ms;         /* 0: none */
-    int32_t kernel_variant;    /* sweep kernel; 0: default (3). 1 chunked + compensated float, 2 chunked +
-                                  fma float (faster; exceeds 1e-5 over long horizons), 3 lean streaming +
-                                  float64, 4 tiled with an LDS value band + float64, 5 jagged-diagonal +
-                                  float64; other values are an error */
+    int32_t kernel_variant;    /* sweep kernel; 0: default (6 for a built model with nrzns < 65536, else 3).
+                                  1 chunked + compensated float, 2 chunked + fma float (faster; exceeds 1e-5
+                                  over long horizons), 3 lean streaming + float64, 4 tiled with an LDS value
+                                  band + float64, 5 jagged-diagonal + float64, 6 lean streaming over 4-byte
+                                  compressed entries + float64 (built models only); other values are an
+                                  error */
     int32_t blocks_per_cu;     /* resident sweep blocks per compute unit (0: chosen by the library) */
+    int32_t comp_span;         /* variant 6: a 256-entry window whose columns span at least this many keeps
+                                  its 8-byte entries (0: 65536, the most a 16-bit offset reaches; 1..65536) */
+    int32_t reserved;
 } e2e_solve_opts;
 
 typedef struct e2e_solve_stats {
@@ -154,11 +159,15 @@ typedef struct e2e_solve_stats {
     double solve_ms;           /* wall time of the solve */
     double sweep_kernel_ms;    /* HIP-event time of all sweep launches (incl. early-exit ones) */
     double sweep_kernel_ms_per_sweep; /* sweep_kernel_ms / launches that did work */
-    double bytes_per_sweep;    /* algorithmic bytes of one sweep */
+    double bytes_per_sweep;    /* algorithmic bytes of one sweep in the reference's CSR terms:
+                                  8 nnz + 4 (rows+1) + 4 rows + 12 ns (entries, row pointers, R, V read,
+                                  V write, policy write) */
+    double moved_bytes_per_sweep; /* the same traffic in the layout the variant reads: 4-byte compressed
+                                  entries (8 for windows kept uncompressed) and float64 values */
     int32_t launches;
     int32_t kernel_variant;    /* variant used (1-based, as in e2e_solve_opts) */
     int32_t blocks;            /* sweep grid size in blocks */
-    int32_t reserved;
+    int32_t compressed_windows; /* 256-entry windows the compressed variant reads at 4 bytes per entry */
 } e2e_solve_stats;
 
 typedef struct e2e_batch_opts {

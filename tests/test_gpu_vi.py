@@ -93,7 +93,8 @@ def test_vi_not_converged_status(gpu):
 # compensated float, 2 chunked + fma float, 3 lean streaming, 4 tiled with an LDS value
 # band, 5 jagged-diagonal (3-5 in float64). Variant 2 rounds each product, so it gets a
 # wider bound on these short horizons.
-VARIANTS = [(1, 2), (1, 8), (2, 4), (3, 2), (3, 4), (3, 8), (4, 2), (4, 4), (4, 8), (5, 2), (5, 4), (5, 8)]
+VARIANTS = [(1, 2), (1, 8), (2, 4), (3, 2), (3, 4), (3, 8), (4, 2), (4, 4), (4, 8), (5, 2), (5, 4), (5, 8),
+            (6, 4)]
 
 
 @pytest.mark.parametrize("variant,lif", VARIANTS)
@@ -176,6 +177,53 @@ def test_vi_c2_accuracy(gpu):
 def test_vi_unknown_variant_is_an_error(gpu):
     w = small(g=6, nt=4, nr=50, nm=1, seed=71)
     with e2e.build_model(w.params, w.fields()) as m:
+        for bad in ({"kernel_variant": 7}, {"comp_span": 65537}):
+            with pytest.raises(e2e.E2EError) as ei:
+                m.solve(**bad)
+            assert ei.value.code == 1
+
+
+def test_vi_compressed_needs_a_built_model(gpu):
+    S1, S2, val, R = _random_coo(301, 2, seed=77)
+    with e2e.import_coo(301, 2, S1, S2, val, R) as m:
+        _, _, st = m.solve(thresh=1e-3, max_sweeps=4, raise_on_nonconv=False)
+        assert st["kernel_variant"] == 3  # the default for an imported model
         with pytest.raises(e2e.E2EError) as ei:
             m.solve(kernel_variant=6)
         assert ei.value.code == 1
+
+
+@pytest.mark.parametrize("case,span", [("small", 0), ("small", 24), ("c1", 0), ("c1", 200),
+                                       ("wide_flow", 0), ("wide_flow", 64)])
+def test_vi_compressed_matches_lean_bitwise(gpu, case, span):
+    """The compressed sweep (4-byte entries: count and column offset) decodes every
+    probability to the stored float and adds the same float64 products in the same
+    order as the lean sweep, so values and policies are bit-identical. A small column
+    span limit keeps part of the windows at 8 bytes, so both window kinds mix."""
+    if case == "small":
+        w = small(g=11, nt=14, nr=400, nm=2, na=12, seed=53)
+    elif case == "c1":
+        w = workload("C1")
+    else:
+        w = small(g=30, nt=4, nr=400, nm=2, seed=13)
+    f = w.fields()
+    if case == "wide_flow":
+        from e2e_gpu_dp_amd.fields import make_fields
+        p = w.params
+        f = make_fields(p.gsize, p.nt, p.nmodes, p.nrzns, 13, F=p.F, mean_speed_frac=9.0, mode_rms_frac=6.0)
+    with e2e.build_model(w.params, f) as m:
+        pa, va, sa = m.solve(thresh=1e-6, kernel_variant=3)
+        pb, vb, sb = m.solve(thresh=1e-6, kernel_variant=6, comp_span=span)
+        print(f"[{case} span {span}] windows compressed {sb['compressed_windows']}", flush=True)
+        assert sb["kernel_variant"] == 6 and sa["sweeps"] == sb["sweeps"]
+        assert np.array_equal(va.view(np.uint32), vb.view(np.uint32))
+        assert np.array_equal(pa, pb)
+        nwin = -(-m.info["nnz"] // 256)
+        if span == 0:
+            assert sb["compressed_windows"] == nwin  # every window of a built model fits
+        else:
+            assert 0 <= sb["compressed_windows"] < nwin
+        assert sb["moved_bytes_per_sweep"] < sa["moved_bytes_per_sweep"] or span != 0
+        # the default for a built model is the compressed sweep
+        pc, vc, sc = m.solve(thresh=1e-6)
+        assert sc["kernel_variant"] == 6 and np.array_equal(vc.view(np.uint32), va.view(np.uint32))
