@@ -954,51 +954,52 @@ __global__ __launch_bounds__(256) void jds_convert(const uint32_t* __restrict__ 
     }
 }
 
-// Compressed stream: one wavefront per 256-entry window of the CSR entries. A window is
-// kept at 4 bytes per entry when its columns span less than 2^16 and every probability
-// is count / nr for a count < 2^16 that the sweep's decode reproduces bit for bit;
-// otherwise its base is -1 and the sweep reads the 8-byte entries.
+// Compressed stream: wavefronts stride over the 256-entry windows of the CSR entries. A
+// window is kept at 4 bytes per entry when its columns span less than `span` (at most
+// 2^16) and every probability is count / nr for a count < 2^16 that the sweep's decode
+// reproduces bit for bit; otherwise its base is -1 and the sweep reads the 8-byte
+// entries. Each wavefront adds its count of compressed windows once.
 __global__ __launch_bounds__(256) void comp_convert(const int2* __restrict__ nz, int64_t nnz, int64_t nwin,
                                                    int nr, double inv_nr, uint32_t* __restrict__ cz,
                                                    int32_t* __restrict__ cbase, int span,
                                                    unsigned long long* ncomp) {
     const int lane = threadIdx.x & 63;
-    const int64_t win = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (win >= nwin) return;  // whole wavefronts
-    int32_t col[4];
-    uint32_t cnt[4];
-    bool ok = true;
-    int32_t cmin = 0x7fffffff, cmax = -1;
+    unsigned long long kept = 0;
+    for (int64_t win = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); win < nwin; win += (int64_t)gridDim.x * 4) {
+        int32_t col[4];
+        uint32_t cnt[4];
+        bool ok = true;
+        int32_t cmin = 0x7fffffff, cmax = -1;
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const int64_t j = win * kCompWin + u * 64 + lane;
-        const int2 e = nz[j];  // padded past nnz with zeros
-        const bool real = j < nnz;
-        const float pf = __int_as_float(e.y);
-        const double c = rint((double)pf * (double)nr);
-        const bool in16 = c >= 0.0 && c < 65536.0;
-        cnt[u] = real && in16 ? (uint32_t)c : 0u;
-        ok = ok && (!real || (in16 && __float_as_int((float)((double)cnt[u] * inv_nr)) == e.y));
-        col[u] = real ? e.x : -1;
-        if (real) {
-            cmin = min(cmin, e.x);
-            cmax = max(cmax, e.x);
+        for (int u = 0; u < 4; u++) {
+            const int64_t j = win * kCompWin + u * 64 + lane;
+            const int2 e = nz[j];  // padded past nnz with zeros
+            const bool real = j < nnz;
+            const float pf = __int_as_float(e.y);
+            const double c = rint((double)pf * (double)nr);
+            const bool in16 = c >= 0.0 && c < 65536.0;
+            cnt[u] = real && in16 ? (uint32_t)c : 0u;
+            ok = ok && (!real || (in16 && __float_as_int((float)((double)cnt[u] * inv_nr)) == e.y));
+            col[u] = real ? e.x : -1;
+            if (real) {
+                cmin = min(cmin, e.x);
+                cmax = max(cmax, e.x);
+            }
         }
-    }
-    for (int o = 32; o >= 1; o >>= 1) {
-        cmin = min(cmin, __shfl_xor(cmin, o, 64));
-        cmax = max(cmax, __shfl_xor(cmax, o, 64));
-    }
-    ok = __all(ok) && cmax >= 0 && (int64_t)cmax - cmin < span;
+        for (int o = 32; o >= 1; o >>= 1) {
+            cmin = min(cmin, __shfl_xor(cmin, o, 64));
+            cmax = max(cmax, __shfl_xor(cmax, o, 64));
+        }
+        ok = __all(ok) && cmax >= 0 && (int64_t)cmax - cmin < span;
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const int64_t j = win * kCompWin + u * 64 + lane;
-        cz[j] = ok && col[u] >= 0 ? (cnt[u] << 16) | (uint32_t)(col[u] - cmin) : 0u;
+        for (int u = 0; u < 4; u++) {
+            const int64_t j = win * kCompWin + u * 64 + lane;
+            cz[j] = ok && col[u] >= 0 ? (cnt[u] << 16) | (uint32_t)(col[u] - cmin) : 0u;
+        }
+        if (lane == 0) cbase[win] = ok ? cmin : -1;
+        kept += ok ? 1ull : 0ull;
     }
-    if (lane == 0) {
-        cbase[win] = ok ? cmin : -1;
-        if (ok) atomicAdd(ncomp, 1ull);
-    }
+    if (lane == 0 && kept) atomicAdd(ncomp, kept);
 }
 
 __global__ void vi_to_f32(const double* __restrict__ W, float* __restrict__ V, int64_t ns) {
@@ -1324,7 +1325,8 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
             DevBuf cnt;
             E2E_TRY(dmalloc(cnt, sizeof(unsigned long long)));
             E2E_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long), s));
-            hipLaunchKernelGGL(comp_convert, dim3((unsigned)ceil_div(nwin, 4)), dim3(256), 0, s, m->nz, m->nnz, nwin,
+            const unsigned cblocks = (unsigned)std::min<int64_t>(ceil_div(nwin, 4), 4096);
+            hipLaunchKernelGGL(comp_convert, dim3(cblocks), dim3(256), 0, s, m->nz, m->nnz, nwin,
                                m->p.nrzns, 1.0 / (double)m->p.nrzns, m->vi_cz, m->vi_cbase, span,
                                cnt.as<unsigned long long>());
             E2E_HIP(hipGetLastError());
