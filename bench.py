@@ -160,6 +160,32 @@ def _serving(e2e, np, n_requests: int, seed: int = 5678):
     }
 
 
+def _paths(e2e, np, torch):
+    """Build cost of the paths C2 leaves unexercised, on the same grid, horizon and
+    sample: obstacle islands (the per-lane obstacle test) and reward method 2 over a
+    scalar field (non-integer rewards: the canonical float64 chunk sums)."""
+    from e2e_gpu_dp_amd.configs import workload
+    res = {}
+    for name, key in (("C2-obs", "obstacles"), ("C2-rad", "scalar_reward")):
+        w = workload(name)
+        f = w.fields()
+        dev = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in f.items()}
+        torch.cuda.synchronize()
+        infos = []
+        for _ in range(3):  # the first is a warmup
+            m = e2e.build_model(w.params, dev)
+            infos.append(m.info)
+            m.close()
+        i = infos[-1]
+        mv = float(np.mean([x["move_kernel_ms"] for x in infos[1:]]))
+        res[key] = {"workload": name, "build_ms": float(np.mean([x["build_ms"] for x in infos[1:]])),
+                    "move_kernel_ms": mv, "transitions_per_s": w.lanes / (mv * 1e-3),
+                    "nnz": i["nnz"], "deferred_rows": i["n_deferred_rows"],
+                    "obstacle_cells": int(f["mask"].sum()) if name == "C2-obs" else 0}
+        del dev
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +199,8 @@ def main(argv=None):
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=120)
+    ap.add_argument("--no-paths", action="store_true",
+                    help="skip the obstacle and scalar-reward build variants")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 3 device-to-device copies of BYTES (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -211,7 +239,7 @@ def main(argv=None):
         m = e2e.build_model(p, dev)
         st = None
         if not args.skip_vi:
-            _, _, st = m.solve(thresh=args.thresh, sweeps_per_check=1, max_sweeps=p.nt + 2)
+            _, _, st = m.solve(thresh=args.thresh, max_sweeps=p.nt + 2)
         info = m.info
         m.close()
         return info, st
@@ -275,6 +303,9 @@ def main(argv=None):
         out["roofline_vi"] = {"kernel": kname, "bound": "hbm", "achieved": gbps,
                               "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": gbps / PEAK_HBM_GBPS,
                               "traffic": _pmc_traffic(args.workload, kname)}
+    if not args.no_paths and args.workload == "C2":
+        print("[bench] obstacle and scalar-reward builds", file=sys.stderr, flush=True)
+        out["paths"] = _paths(e2e, np, torch)
     if not args.no_batched:
         print("[bench] batched goals", file=sys.stderr, flush=True)
         out["batched"] = _batched(e2e, np, max(args.steps, 3))

@@ -66,10 +66,13 @@ class Workload:
     params: Params
     seed: int
     sigmas: list
+    obstacles: list = None   # (t0, t1, i0, i1, j0, j1) boxes of obstacle_mask
+    with_scalar: bool = False  # a scalar field (all_s_mat) for reward methods 2 and 3
 
     def fields(self):
         p = self.params
-        return make_fields(p.gsize, p.nt, p.nmodes, p.nrzns, self.seed, self.sigmas, F=p.F)
+        return make_fields(p.gsize, p.nt, p.nmodes, p.nrzns, self.seed, self.sigmas, F=p.F,
+                           with_scalar=self.with_scalar, obstacles=self.obstacles)
 
     @property
     def lanes(self) -> int:
@@ -77,9 +80,36 @@ class Workload:
         return (p.nt - 1) * p.nc * p.na * p.nrzns
 
 
+def _islands(g: int, nt: int, seed: int):
+    """Obstacle boxes for the obstacle variants: three fixed islands of about g/10 cells
+    a side, and one that is present only in the second half of the horizon, away from
+    the goal."""
+    rng = np.random.default_rng(seed + 91)
+    side = max(2, g // 10)
+    boxes = []
+    for k in range(4):
+        i0 = int(rng.integers(1, g - side - 1))
+        j0 = int(rng.integers(1, g - side - 1))
+        t0, t1 = (0, nt) if k < 3 else (nt // 2, nt)
+        boxes.append((t0, t1, i0, i0 + side, j0, j0 + side))
+    return boxes
+
+
 def workload(name: str) -> Workload:
     """C1 (small, CPU-runnable), C2 (100^2 x 120, the single-GPU bench case),
-    C4 (200^2 x 200, large); C3 and C5 reuse these builders."""
+    C4 (200^2 x 200, large); C3 and C5 reuse these builders. "C2-obs" is C2 with obstacle
+    islands (the goes_through_obstacle path, src/build_model.cu:444-477) and "C2-rad" is
+    C2 with reward method 2 over a scalar field (:244-250): the two build paths the
+    named workloads leave unexercised."""
+    if name in ("C2-obs", "C2-rad"):
+        w = workload("C2")
+        p = w.params
+        if name == "C2-obs":
+            boxes = _islands(p.gsize, p.nt, w.seed)
+            gi, gj = p.i_term, p.j_term
+            boxes = [b for b in boxes if not (b[2] <= gi < b[3] and b[4] <= gj < b[5])]
+            return Workload(name, p, w.seed, w.sigmas, obstacles=boxes)
+        return Workload(name, p.replace(reward_type=2), w.seed, w.sigmas, with_scalar=True)
     if name == "C1":
         g, nt, na, nr, nm, seed = 25, 20, 8, 1000, 4, 1234
         sig = [1.0, 0.5, 0.25, 0.125]

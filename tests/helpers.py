@@ -50,3 +50,22 @@ def policy_ties_ok(pol_gpu, pi64, Q64, tol):
     gap = np.abs(qg - qr) / np.maximum(np.abs(qr), 1.0)
     assert (gap <= tol).all(), f"{(gap > tol).sum()} policy differences beyond ties (max gap {gap.max()})"
     return diff.size
+
+
+def assert_slice_matches(params, fields, t, S1, S2, val, R):
+    """Rows of time slice t in an exported reference-order model (S1, S2, val, R as
+    int32/float32 arrays) equal, bit for bit, the oracle's build of that slice alone
+    (src/build_model.cu:1142-1165 builds slice by slice, so a slice's rows do not depend
+    on the others). Returns the number of entries compared."""
+    import oracle
+    p = params
+    ns, na, nc = p.ns, p.na, p.nc
+    ref = oracle.build(p, fields, t, t + 1)
+    sel = (S1 % ns) // nc == t
+    assert sel.sum() == len(ref["S1"]), t
+    assert np.array_equal(S1[sel], ref["S1"].astype(np.int32)), t
+    assert np.array_equal(S2[sel], ref["S2"].astype(np.int32)), t
+    assert np.array_equal(val[sel].view(np.uint32), ref["val"].view(np.uint32)), t
+    rows = (np.arange(na)[:, None] * ns + t * nc + np.arange(nc)[None, :]).ravel()
+    assert np.array_equal(R[rows].view(np.uint32), ref["R"][rows].view(np.uint32)), t
+    return int(sel.sum())
