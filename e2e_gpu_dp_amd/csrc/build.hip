@@ -705,7 +705,19 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             bad |= !(isfinite(vx) && isfinite(vy));
             vxy[w] = make_float2(vx, vy);
         }
-        __syncthreads();
+        // obstacle slices: is any cell within +-kFixHalf of this one an obstacle at t or
+        // t+1? If not, no lane whose S2 stays in that window can cross one
+        // (goes_through_obstacle :444-477 only looks inside the S1-S2 bounding box)
+        bool near_obs = false;
+        if (OBS && obs_slice) {
+            const int32_t* m0 = A.mask + tslice;
+            for (int k = tid; k < kFixSide * kFixSide; k += WG) {
+                const int i = i1 + k / kFixSide - kFixHalf, j = j1 + k % kFixSide - kFixHalf;
+                if (i >= 0 && i < g && j >= 0 && j < g)
+                    near_obs = near_obs || m0[j + g * i] == 1 || m0[nc + j + g * i] == 1;
+            }
+        }
+        near_obs = __syncthreads_or(near_obs);
         const bool term1 = is_terminal(A, i1, j1);
         const bool obs1 = A.mask && A.mask[mid] == 1;
         const float rad1 = uses_rad ? A.s[mid] : 0.0f;
@@ -810,11 +822,13 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                 seg_lin = seg_lin && seg_int;
             }
             // Fast pass 1: unit cells, a start cell that moves (neither terminal nor an
-            // obstacle), no scalar-field reward, no obstacle test, integer rewards linear
-            // in the penalty count. The edge and terminal terms of the reward depend on
-            // the landing cell only, so they are added per histogram bin at the emit; a
-            // lane only counts its clamps (x, y).
-            const bool fast = UNIT && !uses_rad && !(OBS && obs_slice) && !term1 && !obs1 && seg_lin;
+            // obstacle), no scalar-field reward, integer rewards linear in the penalty
+            // count, and no obstacle test to make: not an obstacle slice, or no obstacle
+            // within the fixed window (a row whose lanes then leave that window goes to
+            // the block-level kernel). The edge and terminal terms of the reward depend
+            // on the landing cell only, so they are added per histogram bin at the emit;
+            // a lane only counts its clamps (x, y).
+            const bool fast = UNIT && !uses_rad && !(OBS && obs_slice && near_obs) && !term1 && !obs1 && seg_lin;
             int kcl = 0;
             auto round1 = [&](int w, bool valid) {
                 int di, dj;
@@ -941,7 +955,7 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             const int di0 = fixed ? -kFixHalf : dimin, dj0 = fixed ? -kFixHalf : djmin;
             const uint64_t nkeys = fixed ? (uint64_t)kFixSide * kFixSide : (uint64_t)(dimax - dimin + 1) * Wj;
             const size_t row = mid * na + a;
-            if (wave_wide || nkeys > (uint64_t)kWaveBins) {
+            if (wave_wide || nkeys > (uint64_t)kWaveBins || (fast && OBS && obs_slice && !fixed)) {
                 for (int i = lane; i < kHistWords; i += 64) hist[i] = 0u;  // partial fixed counts
                 if (lane == 0) A.defer_list[atomicAdd(A.defer_count, 1u)] = (uint32_t)row;
                 continue;  // the block-level kernel writes this row
