@@ -1336,10 +1336,9 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
             m->vi_cwin_comp = (int64_t)h;
         }
     }
-    if (lean) {
-        E2E_CHECK(ns * (int64_t)sizeof(double) < (1ll << 32) && (m->rows + 1) * 4 < (1ll << 32) &&
-                      (m->nnz + kNzPad) * 8 < (1ll << 32),
-                  E2E_ERR_CAPACITY, "vi_solve: the lean sweep addresses states and rows with 32-bit byte offsets");
+    if (lean) {  // (entries are addressed through 64-bit pointers)
+        E2E_CHECK(ns * (int64_t)sizeof(double) < (1ll << 32) && (m->rows + 1) * 4 < (1ll << 32), E2E_ERR_CAPACITY,
+                  "vi_solve: the lean sweep addresses states and rows with 32-bit byte offsets");
         const int64_t W = nblocks * kStreamWaves;
         if (m->vi_wpart_waves != W || m->vi_wpart_spc != grp) {
             if (m->vi_wpart) (void)hipFree(m->vi_wpart);
