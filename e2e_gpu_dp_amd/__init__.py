@@ -78,6 +78,7 @@ class Model:
     def __init__(self, handle: int, params=None):
         self._h = C.c_void_p(handle)
         self.params = params
+        self.n_alpha = 0  # alpha columns built with the model
 
     # -- lifetime
     def close(self):
@@ -175,12 +176,43 @@ class Model:
             check(code)
         return pol, val, sw, stats
 
+    def export_R_alpha(self, j: int) -> np.ndarray:
+        """R of alpha column j in the reference order (float32[na*ns])."""
+        R = np.empty(self.info["na"] * self.info["ns"], np.float32)
+        check(lib().e2e_model_export_R_alpha(self._h, j, R.ctypes.data))
+        return R
+
+    def solve_alphas(self, thresh: float = 0.1, discount: float = 1.0, max_sweeps: int = 0,
+                     sweeps_per_check: int = 0, variant: int = -1, use_graph: bool = False,
+                     timeout_ms: float = 0.0, raise_on_nonconv: bool = True):
+        """Alpha-sweep VI over the model's alpha columns (one SpMM per sweep). Returns
+        (policy uint32[n_alpha, Ns], value float32[n_alpha, Ns], per-column sweeps, stats)."""
+        k = self.n_alpha
+        if k <= 0:
+            raise ValueError("the model was built without alpha columns")
+        ns = self.info["ns"]
+        o = _abi.SolveOpts(thresh, discount, max_sweeps, sweeps_per_check, 0, 0, timeout_ms)
+        bo = _abi.BatchOpts(variant, int(use_graph))
+        st = _abi.BatchStats()
+        pol = np.empty((k, ns), np.uint32)
+        val = np.empty((k, ns), np.float32)
+        sw = np.empty(k, np.int32)
+        code = lib().e2e_vi_solve_alphas(self._h, C.byref(o), C.byref(bo), pol.ctypes.data, val.ctypes.data,
+                                         sw.ctypes.data, C.byref(st))
+        stats = {kk: getattr(st, kk) for kk, _ in st._fields_}
+        stats["status"] = code
+        if code != 0 and (raise_on_nonconv or code != 4):
+            check(code)
+        return pol, val, sw, stats
+
 
 def build_model(params, fields: dict, keep_rsum: bool = False, slot_k: int = 0,
                 radix_bits: int = 0, slices_per_launch: int = 0, block_threads: int = 0,
-                stream: int = 0, count_kernel: int = 0, paired_rounds: int = 0) -> Model:
+                stream: int = 0, count_kernel: int = 0, paired_rounds: int = 0, alphas=None) -> Model:
     """Monte Carlo build on the GPU. `fields` holds u, v, ui, vi, yi and optionally s,
-    mask (numpy arrays, or contiguous torch CUDA tensors)."""
+    mask (numpy arrays, or contiguous torch CUDA tensors). `alphas` (reward method 4):
+    also build the reward vector of each of these alpha values (float32, at most 64),
+    for export_R_alpha and solve_alphas."""
     keep = []
     ptrs = {}
     on_dev = None
@@ -194,11 +226,15 @@ def build_model(params, fields: dict, keep_rsum: bool = False, slot_k: int = 0,
             elif on_dev != dev:
                 raise ValueError("fields must be all on the host or all on the device")
     f = _abi.Fields(**ptrs)
+    al = None if alphas is None else np.ascontiguousarray(alphas, np.float32).ravel()
     o = _abi.BuildOpts(int(bool(on_dev)), int(keep_rsum), slot_k, radix_bits, slices_per_launch,
-                       block_threads, stream or None, count_kernel, paired_rounds)
+                       block_threads, stream or None, count_kernel, paired_rounds,
+                       0 if al is None else len(al), 0, None if al is None else al.ctypes.data)
     h = C.c_void_p()
     check(lib().e2e_model_build(C.byref(c_params(params)), C.byref(f), C.byref(o), C.byref(h)))
-    return Model(h.value, params)
+    m = Model(h.value, params)
+    m.n_alpha = 0 if al is None else len(al)
+    return m
 
 
 def import_coo(ns: int, na: int, S1, S2, val, R, stream: int = 0) -> Model:

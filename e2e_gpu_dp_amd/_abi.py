@@ -46,6 +46,7 @@ class BuildOpts(C.Structure):
         ("fields_on_device", C.c_int32), ("keep_rsum", C.c_int32), ("slot_k", C.c_int32),
         ("radix_bits", C.c_int32), ("slices_per_launch", C.c_int32), ("block_threads", C.c_int32),
         ("stream", C.c_void_p), ("count_kernel", C.c_int32), ("paired_rounds", C.c_int32),
+        ("n_alpha", C.c_int32), ("reserved", C.c_int32), ("alphas", C.c_void_p),
     ]
 
 
@@ -131,6 +132,8 @@ SIGNATURES = {
     "e2e_model_export_coo_i32": (C.c_int, [VP, VP, VP, VP, VP]),
     "e2e_model_export_csr": (C.c_int, [VP, VP, VP, VP, VP]),
     "e2e_vi_solve": (C.c_int, [VP, P(SolveOpts), VP, VP, P(SolveStats)]),
+    "e2e_model_export_R_alpha": (C.c_int, [VP, C.c_int32, VP]),
+    "e2e_vi_solve_alphas": (C.c_int, [VP, P(SolveOpts), P(BatchOpts), VP, VP, VP, P(BatchStats)]),
     "e2e_vi_solve_goals": (C.c_int, [VP, VP, C.c_int32, P(SolveOpts), P(BatchOpts), VP, VP, VP,
                                      P(BatchStats)]),
     "e2e_server_create": (C.c_int, [P(ServerOpts), P(VP)]),

@@ -46,7 +46,7 @@ e2e_status free_model(Model* m) {
     batch_cache_free(m->batch);
     void* ptrs[] = {m->row_ptr, m->nz,      m->R,        m->Rsum,     m->V[0],   m->V[1],     m->W[0], m->W[1],
                     m->pi,      m->st,      m->vi_part,  m->vi_wpart, m->vi_tdesc, m->vi_jnz, m->vi_jmeta, m->vi_jR,
-                    m->vi_cz,   m->vi_cbase};
+                    m->vi_cz,   m->vi_cbase, m->Ralpha};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
@@ -67,6 +67,9 @@ void actions_table(const e2e_params* p, float* speed, float* angle);
 e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve_opts* o,
                           const e2e_batch_opts* bo, uint32_t* policy, float* value, int32_t* col_sweeps,
                           e2e_batch_stats* stats);
+e2e_status vi_solve_alphas(Model* m, const e2e_solve_opts* o, const e2e_batch_opts* bo, uint32_t* policy,
+                           float* value, int32_t* col_sweeps, e2e_batch_stats* stats);
+e2e_status model_export_R_alpha(const Model* m, int32_t j, float* R);
 
 template <typename F>
 static e2e_status guarded(F&& f) {
@@ -90,6 +93,7 @@ static int64_t model_device_bytes(const Model* m) {
     if (m->V[0]) b += (2 * sizeof(float) + 2 * sizeof(double) + sizeof(int32_t)) * m->ns + sizeof(VIState);
     if (m->vi_jnz) b += sizeof(int2) * (m->nnz + kNzPad) + (sizeof(uint32_t) + sizeof(float)) * m->rows;
     if (m->vi_cz) b += (int64_t)(sizeof(uint32_t) * 256 + sizeof(int32_t)) * m->vi_cwin;
+    if (m->Ralpha) b += (int64_t)sizeof(float) * m->n_alpha * m->rows;
     return b;
 }
 
@@ -250,6 +254,25 @@ e2e_status e2e_model_export_csr(const void* model, uint32_t* row_ptr, int32_t* c
         return E2E_ERR_INVALID;
     }
     return guarded([&] { return model_export_csr(static_cast<const Model*>(model), row_ptr, col, val, R); });
+}
+
+e2e_status e2e_model_export_R_alpha(const void* model, int32_t j, float* R) {
+    if (!model) {
+        set_error("e2e_model_export_R_alpha: model is NULL");
+        return E2E_ERR_INVALID;
+    }
+    return guarded([&] { return model_export_R_alpha(static_cast<const Model*>(model), j, R); });
+}
+
+e2e_status e2e_vi_solve_alphas(void* model, const e2e_solve_opts* opts, const e2e_batch_opts* bopts,
+                               uint32_t* policy, float* value, int32_t* col_sweeps, e2e_batch_stats* stats) {
+    if (!model) {
+        set_error("e2e_vi_solve_alphas: model is NULL");
+        return E2E_ERR_INVALID;
+    }
+    return guarded([&] {
+        return vi_solve_alphas(static_cast<Model*>(model), opts, bopts, policy, value, col_sweeps, stats);
+    });
 }
 
 e2e_status e2e_vi_solve(void* model, const e2e_solve_opts* opts, uint32_t* policy, float* value,

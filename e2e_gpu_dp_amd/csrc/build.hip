@@ -66,6 +66,10 @@ struct BuildArgs {
     unsigned long long* stats;  // 0 oow lanes, 1 overflow rows, 2 max nuq, 3 nonfinite,
                                 // 4 arena exhausted, 5 multipass rows
     int32_t pipe_rounds;  // per-row kernel: pair its rounds (few waves resident per SIMD)
+    int32_t n_alpha;      // alpha columns (reward method 4): rewards for each alpha too
+    float* Ralpha;        // [n_alpha][rows_total]
+    int64_t rows_total;
+    float alphas[64];
     float act_speed[kMaxActions];
     float act_fcos[kMaxActions];
     float act_fsin[kMaxActions];
@@ -134,7 +138,7 @@ __device__ bool crosses_obstacle(const BuildArgs& A, int i1, int j1, int i2, int
 template <bool UNIT>
 __device__ __forceinline__ void lane_move(const BuildArgs& A, float xc, float yc, float fc,
                                           float fs, float vx, float vy, bool late_slice, int& pi,
-                                          int& pj, float& r) {
+                                          int& pj, float& r, int* cls = nullptr) {
     // Written with selects instead of the reference's if/else-if ladders; every float
     // operation and its operands are the same, so the results are the same bits.
     const int n = A.g - 1;
@@ -171,6 +175,8 @@ __device__ __forceinline__ void lane_move(const BuildArgs& A, float xc, float yc
     const bool edge = (pi == 0) | (pi == n) | (pj == 0) | (pj == n);  // :339-341
     r = edge ? r + A.r_out : r;
     const bool term = is_terminal(A, pi, pj);                          // :343-354
+    // the reward class: how many r_outbound were added (x clamp, y clamp, edge), terminal
+    if (cls) *cls = (((xh || xl) ? 1 : 0) + ((yh || yl) ? 1 : 0) + (edge ? 1 : 0)) * 2 + (term ? 1 : 0);
     if (late_slice) {  // T > nt - 20: uniform over the slice
         const float late = (float)((double)r + A.late_pen);
         r = term ? r + A.r_term : late;
@@ -634,6 +640,55 @@ constexpr int kFixHalf = 7;     // fixed offset window +-7 cells, counted during
 constexpr int kFixSide = 2 * kFixHalf + 1;
 static_assert(kFixSide * kFixSide <= kWaveBins, "fixed window fits the histogram");
 
+// Alpha columns (reward method 4): the row's reward for every alpha, from the counts of
+// its lanes per reward class. A lane's reward is the class value of
+// calculate_one_step_reward :256-260 and move :268-356: rk = k penalties of r_outbound
+// added one at a time, then + r_terminal (terminal) or the late penalty (T > nt - 20,
+// in double), then + the step reward alpha*(-F^2 dt) + (1-alpha)*(-dt); an obstacle
+// crossing is r_outbound. The sum over classes, n_c * v_c in double, is exact, and so
+// is every partial sum of the canonical chunk order, when the values' exponents span
+// at most 53 - 24 - ceil(log2 nr) binades; then both give the same float, as the
+// canonical sum of a build with that alpha. Otherwise stats[7] bit 0 is set (the build
+// fails). Rows of a terminal or obstacle start cell have no step reward: every column
+// takes the row's own reward.
+__device__ __forceinline__ void alpha_rewards(const BuildArgs& A, size_t row, float r_own, bool fixed_start,
+                                           unsigned long long acls, float speed, bool late_slice, int nr) {
+    const int lane = threadIdx.x & 63;
+    if (fixed_start) {
+        for (int j = lane; j < A.n_alpha; j += 64) A.Ralpha[(size_t)j * A.rows_total + row] = r_own;
+        return;
+    }
+    int n[9];
+#pragma unroll
+    for (int c = 0; c < 9; c++) n[c] = wave_sum((int)((acls >> (7 * c)) & 127ull));
+    int lg = 0;
+    while ((1 << lg) < nr) lg++;
+    for (int j = lane; j < A.n_alpha; j += 64) {
+        const float al = A.alphas[j];
+        const float energy_reward = -(speed * speed) * A.dt, time_reward = -A.dt;
+        const float rs = al * energy_reward + (1 - al) * time_reward;
+        double sum = 0.0;
+        int emin = 1 << 20, emax = -(1 << 20);
+        auto add = [&](int cnt, float v) {
+            if (cnt == 0 || v == 0.0f) return;
+            const int e = ilogbf(v);
+            emin = min(emin, e);
+            emax = max(emax, e);
+            sum += (double)cnt * (double)v;
+        };
+        float rk = 0.0f;
+        for (int k = 0; k < 4; k++) {
+            const float other = late_slice ? (float)((double)rk + A.late_pen) : rk;
+            add(n[2 * k], other + rs);
+            add(n[2 * k + 1], (rk + A.r_term) + rs);
+            rk = rk + A.r_out;
+        }
+        add(n[8], A.r_out);
+        if (emax >= emin && (emax - emin) + 24 + lg > 53) atomicOr(A.stats + 7, 1ull);
+        A.Ralpha[(size_t)j * A.rows_total + row] = (float)sum / nr;  // compute_mean :563-570
+    }
+}
+
 // LDS bytes of mc_wave<NW>.
 __host__ __device__ inline size_t wave_lds_bytes(int NW, int nr, int nm, bool stage_y) {
     return sizeof(float2) * (size_t)nr + NW * sizeof(uint32_t) * kHistWords +
@@ -657,7 +712,7 @@ __host__ __device__ inline size_t wave_lds_bytes(int NW, int nr, int nm, bool st
 // No block barrier separates the actions. A segment whose key range exceeds kWaveBins
 // is appended to defer_list, and the block-level kernel (list mode) builds that row
 // exactly as it builds every row in grid mode.
-template <int NW, bool OBS, bool UNIT>
+template <int NW, bool OBS, bool UNIT, bool ALPHA>
 __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
     constexpr int WG = NW * 64;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -681,6 +736,7 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
     const bool obs_slice = A.slice_obs && (A.slice_obs[t] | A.slice_obs[t + 1]);
     const bool uses_rad = A.s && (A.reward_type == 2 || A.reward_type == 3);
     const bool late_slice = t > A.nt - 20;
+    constexpr bool alpha_rows = ALPHA;  // alpha columns: separate instances carry their code
     const int del = (A.m - 1) / 2;
     const int nrounds = (nr + 63) / 64;
     unsigned long long oow_acc = 0, overflow_acc = 0;
@@ -733,16 +789,19 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             double rsum = 0.0;
             // S2 offset and reward of realization w: transition_calc :522-545 (terminal
             // cells stay with reward 0, obstacle cells stay with r_outbound)
+            int lcls = -1;  // alpha columns: the reward class of the lane just moved
             auto lane_offset = [&](int w, bool valid, int& di, int& dj, float& rw) {
                 if (term1 || obs1) {
                     di = 0;
                     dj = 0;
                     rw = term1 ? 0.0f : A.r_out;
+                    lcls = -1;
                     return;
                 }
                 const float2 v = vxy[valid ? w : 0];
                 int pi, pj;
-                lane_move<UNIT>(A, xc, yc, fc, fs, v.x, v.y, late_slice, pi, pj, rw);
+                lane_move<UNIT>(A, xc, yc, fc, fs, v.x, v.y, late_slice, pi, pj, rw,
+                                alpha_rows ? &lcls : nullptr);
                 if (!UNIT) {  // impossible for unit cells; guards the gathers below
                     const int ci = min(max(pi, 0), g - 1), cj = min(max(pj, 0), g - 1);
                     bad_index |= valid & ((ci != pi) | (cj != pj));
@@ -751,7 +810,10 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                 }
                 if (uses_rad) rw += step_reward(A, speed, rad1, A.s[tslice + nc + pj + g * pi]);
                 else rw += rstep;
-                if (OBS && obs_slice && crosses_obstacle(A, i1, j1, pi, pj, t)) rw = A.r_out;
+                if (OBS && obs_slice && crosses_obstacle(A, i1, j1, pi, pj, t)) {
+                    rw = A.r_out;
+                    lcls = 8;
+                }
                 di = pi - i1;
                 dj = pj - j1;
             };
@@ -828,8 +890,12 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             // the block-level kernel). The edge and terminal terms of the reward depend
             // on the landing cell only, so they are added per histogram bin at the emit;
             // a lane only counts its clamps (x, y).
-            const bool fast = UNIT && !uses_rad && !(OBS && obs_slice && near_obs) && !term1 && !obs1 && seg_lin;
+            const bool fast = UNIT && !uses_rad && !(OBS && obs_slice && near_obs) && !term1 && !obs1 && seg_lin &&
+                              !alpha_rows;
             int kcl = 0;
+            // alpha columns: per-lane lane counts of the 9 reward classes (class c = 2 *
+            // (penalties added) + terminal, 8 = obstacle crossing), 7 bits each
+            unsigned long long acls = 0ull;
             auto round1 = [&](int w, bool valid) {
                 int di, dj;
                 float rw;
@@ -846,6 +912,7 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                 }
                 if (seg_int) ip += valid ? (int)rw : 0;
                 else add_reward(valid, rw);
+                if (alpha_rows && valid && lcls >= 0) acls += 1ull << (7 * lcls);
             };
             // the move of lane_move<true> for a moving lane, cut to what the fast pass
             // needs: clamp with med3 (= the if/else-if clamp for ordered bounds; its
@@ -956,6 +1023,7 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             const uint64_t nkeys = fixed ? (uint64_t)kFixSide * kFixSide : (uint64_t)(dimax - dimin + 1) * Wj;
             const size_t row = mid * na + a;
             if (wave_wide || nkeys > (uint64_t)kWaveBins || (fast && OBS && obs_slice && !fixed)) {
+                if (alpha_rows && lane == 0) atomicOr(A.stats + 7, 2ull);  // no alpha rewards there
                 for (int i = lane; i < kHistWords; i += 64) hist[i] = 0u;  // partial fixed counts
                 if (lane == 0) A.defer_list[atomicAdd(A.defer_count, 1u)] = (uint32_t)row;
                 continue;  // the block-level kernel writes this row
@@ -1027,6 +1095,7 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                         hist[kk * kHistCopies + c] = 0u;
                 }
             }
+            if (alpha_rows) alpha_rewards(A, row, (float)rsum / nr, term1 || obs1, acls, speed, late_slice, nr);
             if (lane == 0) {
                 A.row_nnz[row] = (uint32_t)nuq;
                 const float fs_ = (float)rsum;  // compute_mean :563-570
@@ -1129,6 +1198,14 @@ __global__ void export_rows(const uint32_t* __restrict__ row_ptr, const int2* __
     if (Ram) Ram[am] = R[r];
 }
 
+// ---- state-major R -> the reference's action-major order (rows a*ns + s)
+__global__ void r_action_major(const float* __restrict__ R, float* __restrict__ out, int64_t ns, int64_t na) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // action-major row
+    if (i >= ns * na) return;
+    const int64_t a = i / ns, s = i - a * ns;
+    out[i] = R[s * na + a];
+}
+
 // ---- import from the reference COO order
 __global__ void counts_state_major(const uint32_t* __restrict__ rp_am, uint32_t* __restrict__ out,
                                    int64_t ns, int64_t na) {
@@ -1172,15 +1249,19 @@ BuildFn pick_build(int wg, bool obs, bool unit) {
 }
 
 template <int NW>
-BuildFn pick_wave(bool obs, bool unit) {
-    if (obs) return unit ? mc_wave<NW, true, true> : mc_wave<NW, true, false>;
-    return unit ? mc_wave<NW, false, true> : mc_wave<NW, false, false>;
+BuildFn pick_wave(bool obs, bool unit, bool alpha) {
+    if (alpha) {
+        if (obs) return unit ? mc_wave<NW, true, true, true> : mc_wave<NW, true, false, true>;
+        return unit ? mc_wave<NW, false, true, true> : mc_wave<NW, false, false, true>;
+    }
+    if (obs) return unit ? mc_wave<NW, true, true, false> : mc_wave<NW, true, false, false>;
+    return unit ? mc_wave<NW, false, true, false> : mc_wave<NW, false, false, false>;
 }
 
-BuildFn pick_wave(int nw, bool obs, bool unit) {
-    if (nw == 4) return pick_wave<4>(obs, unit);
-    if (nw == 8) return pick_wave<8>(obs, unit);
-    return pick_wave<16>(obs, unit);
+BuildFn pick_wave(int nw, bool obs, bool unit, bool alpha) {
+    if (nw == 4) return pick_wave<4>(obs, unit, alpha);
+    if (nw == 8) return pick_wave<8>(obs, unit, alpha);
+    return pick_wave<16>(obs, unit, alpha);
 }
 
 // Wavefronts per block of the per-row kernel: 4. Per-cell barriers cost less with small
@@ -1447,6 +1528,32 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
                     : opts.paired_rounds == 1 ? 0
                     : (nw && (int)((160 * 1024) / wave_lds) * nw / 4 < 4 ? 1 : 0);
     m->info.paired_rounds = nw ? A.pipe_rounds : 0;
+    // alpha columns: rewards for n_alpha values of alpha from this one build
+    E2E_CHECK(opts.n_alpha >= 0 && opts.n_alpha <= 64, E2E_ERR_INVALID, "n_alpha=%d not in 0..64", opts.n_alpha);
+    A.n_alpha = 0;
+    A.Ralpha = nullptr;
+    A.rows_total = rows;
+    if (opts.n_alpha > 0) {
+        E2E_CHECK(opts.alphas, E2E_ERR_INVALID, "n_alpha > 0 needs the alphas array");
+        E2E_CHECK(p->reward_type == 4, E2E_ERR_INVALID,
+                  "alpha columns need reward_type 4 (the only reward that depends on alpha)");
+        E2E_CHECK(nw > 0 && opts.count_kernel == 0, E2E_ERR_CAPACITY,
+                  "alpha columns need the per-row kernel (count_kernel 0, velocity samples within its LDS)");
+        E2E_CHECK(nr <= 127 * 64, E2E_ERR_CAPACITY, "alpha columns count at most 8128 realizations per row");
+        if (hipMalloc(&m->Ralpha, sizeof(float) * (size_t)opts.n_alpha * rows) != hipSuccess) {
+            m->Ralpha = nullptr;
+            set_error("hipMalloc of the alpha rewards failed");
+            return fail(E2E_ERR_NOMEM);
+        }
+        E2E_HIP(hipMemsetAsync(m->Ralpha, 0, sizeof(float) * (size_t)opts.n_alpha * rows, s));  // last slice: 0
+        m->n_alpha = opts.n_alpha;
+        A.n_alpha = opts.n_alpha;
+        A.Ralpha = m->Ralpha;
+        for (int j = 0; j < opts.n_alpha; j++) {
+            m->alphas[j] = opts.alphas[j];
+            A.alphas[j] = opts.alphas[j];
+        }
+    }
     m->info.block_threads = wg;
     m->info.wave_blocks = nw;
     m->info.unit_cells = unit ? 1 : 0;
@@ -1488,7 +1595,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
             dim3 grid((unsigned)ceil_div(nc, A.tc), (unsigned)cnt);
             E2E_HIP(hipEventRecord(e0, s));
             if (nw) {
-                BuildFn wf = pick_wave(nw, any_obs, unit);
+                BuildFn wf = pick_wave(nw, any_obs, unit, A.n_alpha > 0);
                 if (wave_lds > 64 * 1024)
                     E2E_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(wf),
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)wave_lds));
@@ -1535,6 +1642,17 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
         if (hs[6]) {
             set_error("a lane left the grid (check dx, dy, x0, y0)");
             st = E2E_ERR_INVALID;
+            break;
+        }
+        if (hs[7] & 2ull) {
+            set_error("alpha columns: a row's S2 spread exceeds the per-row kernel; build each alpha separately");
+            st = E2E_ERR_CAPACITY;
+            break;
+        }
+        if (hs[7] & 1ull) {
+            set_error("alpha columns: reward values too far apart in magnitude for an exact sum over "
+                      "reward classes; build each alpha separately");
+            st = E2E_ERR_CAPACITY;
             break;
         }
         m->info.n_out_of_window = (int64_t)hs[0];
@@ -1746,6 +1864,21 @@ e2e_status model_export_csr(const Model* m, uint32_t* row_ptr, int32_t* col, flo
             }
         }
     }
+    E2E_HIP(hipStreamSynchronize(s));
+    return E2E_OK;
+}
+
+e2e_status model_export_R_alpha(const Model* m, int32_t j, float* R) {
+    E2E_CHECK(m->Ralpha && j >= 0 && j < m->n_alpha, E2E_ERR_INVALID,
+              "alpha column %d: the model has %d alpha columns", j, m->n_alpha);
+    E2E_CHECK(R, E2E_ERR_INVALID, "R is NULL");
+    hipStream_t s = m->stream;
+    DevBuf d;
+    E2E_TRY(dmalloc(d, sizeof(float) * m->rows));
+    hipLaunchKernelGGL(r_action_major, dim3((unsigned)ceil_div(m->rows, 256)), dim3(256), 0, s,
+                       m->Ralpha + (size_t)j * m->rows, d.as<float>(), m->ns, m->na);
+    E2E_HIP(hipGetLastError());
+    E2E_HIP(hipMemcpyAsync(R, d.p, sizeof(float) * m->rows, hipMemcpyDeviceToHost, s));
     E2E_HIP(hipStreamSynchronize(s));
     return E2E_OK;
 }

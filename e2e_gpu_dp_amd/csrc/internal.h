@@ -112,6 +112,11 @@ struct Model {
     bool has_obstacles = false;
     BatchCache* batch = nullptr;  // buffers + tuning of the batched multi-goal solve
 
+    // alpha columns: R of each alpha, state-major like R
+    float* Ralpha = nullptr;      // [n_alpha][rows]
+    int32_t n_alpha = 0;
+    float alphas[64] = {};
+
     e2e_model_info info{};
 };
 

@@ -53,6 +53,9 @@ struct BatchArgs {
     int nc, na, nt, kp, nr;
     float thresh, r_term, discount;
     double late_pen;       // 0.05 * r_outbound
+    const float* Rcol;     // alpha columns: R of column j at Rcol[j * rows + row] (NULL: goal columns)
+    int64_t rows;
+    int rcol_n;            // columns Rcol holds (padding columns read the last one)
 };
 
 namespace {
@@ -157,7 +160,9 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_sweep(const BatchArgs A) {
             const float rr = A.R[row];
 #pragma unroll
             for (int c = 0; c < CPL; c++) {
-                const float rj = hit[c] >= 0 ? (A.Rsum[row] + (float)hit[c] * delta) / A.nr : rr;
+                const float rj = A.Rcol ? A.Rcol[(int64_t)min(j0 + c, A.rcol_n - 1) * A.rows + row]
+                                 : hit[c] >= 0 ? (A.Rsum[row] + (float)hit[c] * delta) / A.nr
+                                               : rr;
                 double q = (double)rj + (double)A.discount * sd[c];
                 if (q != q) q = -INFINITY;
                 if (q > best[c]) {  // strict '>' in action order (spvi_2.cu:112)
@@ -320,6 +325,7 @@ struct BatchCache {
     hipGraphExec_t graph = nullptr;
     int graph_variant = -1, graph_chunk = 0, graph_grid = 0;
     float graph_thresh = 0.f, graph_discount = 0.f;
+    const float* graph_rcol = nullptr;  // goal columns (NULL) or the alpha rewards
 };
 
 void batch_cache_free(BatchCache* c) {
@@ -382,6 +388,9 @@ static BatchArgs make_args(const Model* m, const BatchBufs& b, const e2e_solve_o
     A.r_term = m->p.r_terminal;
     A.discount = o.discount;
     A.late_pen = 0.05 * (double)m->p.r_outbound;
+    A.Rcol = nullptr;
+    A.rows = m->rows;
+    A.rcol_n = 1;
     return A;
 }
 
@@ -392,18 +401,25 @@ static dim3 grid_of(const Model* m, const Variant& v, int kp) {
     return dim3((unsigned)gx, (unsigned)(kp / kb_of(v)));
 }
 
-e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve_opts* o,
-                          const e2e_batch_opts* bo, uint32_t* policy, float* value, int32_t* col_sweeps,
-                          e2e_batch_stats* stats) {
+// Goal columns (goals != NULL) or alpha columns (goals == NULL: the model's n_alpha
+// reward vectors, the model's own terminal cells).
+static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2e_solve_opts* o,
+                                 const e2e_batch_opts* bo, uint32_t* policy, float* value, int32_t* col_sweeps,
+                                 e2e_batch_stats* stats) {
     const double t_start = now_ms();
-    E2E_CHECK(m->built, E2E_ERR_INVALID, "batched goals need a model built by e2e_model_build");
-    E2E_CHECK(m->Rsum, E2E_ERR_INVALID, "batched goals need keep_rsum=1 at build time");
-    E2E_CHECK(m->p.term_subgrid_size == 0, E2E_ERR_INVALID,
-              "batched goals need the shared model built without a terminal cell (term_subgrid_size 0)");
-    E2E_CHECK(!m->has_obstacles, E2E_ERR_INVALID, "batched goals are not defined for models with obstacles");
-    E2E_CHECK(k >= 1 && k <= 64 && goals, E2E_ERR_INVALID, "k=%d goals must be in 1..64", k);
-    for (int j = 0; j < k; j++)
-        E2E_CHECK(goals[j] >= 0 && goals[j] < m->nc, E2E_ERR_INVALID, "goal %d out of range", goals[j]);
+    if (goals) {
+        E2E_CHECK(m->built, E2E_ERR_INVALID, "batched goals need a model built by e2e_model_build");
+        E2E_CHECK(m->Rsum, E2E_ERR_INVALID, "batched goals need keep_rsum=1 at build time");
+        E2E_CHECK(m->p.term_subgrid_size == 0, E2E_ERR_INVALID,
+                  "batched goals need the shared model built without a terminal cell (term_subgrid_size 0)");
+        E2E_CHECK(!m->has_obstacles, E2E_ERR_INVALID, "batched goals are not defined for models with obstacles");
+        E2E_CHECK(k >= 1 && k <= 64, E2E_ERR_INVALID, "k=%d goals must be in 1..64", k);
+        for (int j = 0; j < k; j++)
+            E2E_CHECK(goals[j] >= 0 && goals[j] < m->nc, E2E_ERR_INVALID, "goal %d out of range", goals[j]);
+    } else {
+        E2E_CHECK(m->Ralpha && m->n_alpha >= 1 && k == m->n_alpha, E2E_ERR_INVALID,
+                  "alpha columns need a model built with n_alpha > 0");
+    }
     e2e_solve_opts opts{};
     opts.thresh = 0.1f;
     opts.discount = 1.0f;
@@ -418,9 +434,13 @@ e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve
     BatchBufs& b = c->b;
     hipStream_t s = m->stream;
     std::vector<int32_t> hg(kp, -1);
-    for (int j = 0; j < k; j++) hg[j] = goals[j];
+    for (int j = 0; goals && j < k; j++) hg[j] = goals[j];
     E2E_HIP(hipMemcpyAsync(b.goal, hg.data(), kp * 4, hipMemcpyHostToDevice, s));
     BatchArgs A = make_args(m, b, opts);
+    if (!goals) {
+        A.Rcol = m->Ralpha;
+        A.rcol_n = m->n_alpha;
+    }
     const int64_t nV = m->ns * kp;
     auto init = [&]() {
         hipLaunchKernelGGL(batch_init, dim3((unsigned)ceil_div(nV, 256)), dim3(256), 0, s, b.V[0], nV, b.goal,
@@ -467,7 +487,7 @@ e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve
 
     if (bopt.use_graph &&
         (!c->graph || c->graph_variant != variant || c->graph_chunk != spc || c->graph_grid != (int)grid.x ||
-         c->graph_thresh != opts.thresh || c->graph_discount != opts.discount)) {
+         c->graph_thresh != opts.thresh || c->graph_discount != opts.discount || c->graph_rcol != A.Rcol)) {
         if (c->graph) {
             (void)hipGraphExecDestroy(c->graph);
             c->graph = nullptr;
@@ -488,6 +508,7 @@ e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve
         c->graph_grid = (int)grid.x;
         c->graph_thresh = opts.thresh;
         c->graph_discount = opts.discount;
+        c->graph_rcol = A.Rcol;
     }
 
     init();
@@ -551,6 +572,18 @@ e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve
         return E2E_ERR_NOT_CONVERGED;
     }
     return E2E_OK;
+}
+
+e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve_opts* o,
+                          const e2e_batch_opts* bo, uint32_t* policy, float* value, int32_t* col_sweeps,
+                          e2e_batch_stats* stats) {
+    E2E_CHECK(goals, E2E_ERR_INVALID, "goal_cells is NULL");
+    return vi_solve_batch(m, goals, k, o, bo, policy, value, col_sweeps, stats);
+}
+
+e2e_status vi_solve_alphas(Model* m, const e2e_solve_opts* o, const e2e_batch_opts* bo, uint32_t* policy,
+                           float* value, int32_t* col_sweeps, e2e_batch_stats* stats) {
+    return vi_solve_batch(m, nullptr, m->n_alpha, o, bo, policy, value, col_sweeps, stats);
 }
 
 // ------------------------------------------------------------------ live batches

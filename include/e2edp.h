@@ -103,6 +103,12 @@ typedef struct e2e_build_opts {
     int32_t paired_rounds;     /* rounds of 64 realizations in the per-row kernel: 0 automatic (paired when
                                   its LDS leaves fewer than 4 wavefronts per SIMD), 1 one at a time, 2 in
                                   pairs (the next round's LDS read issued ahead); other values are an error */
+    int32_t n_alpha;           /* > 0: also build the rewards for these n_alpha values of alpha (at most 64;
+                                  reward_type 4 only): the alpha sweep of src/my_script.sh:10-19 from one
+                                  build, its transitions being the same for every alpha */
+    int32_t reserved;
+    const float* alphas;       /* [n_alpha] host array, each alpha as the reference forms it
+                                  (stof(argv[1]) / 100, src/build_model.cu:937-939) */
 } e2e_build_opts;
 
 typedef struct e2e_model_info {
@@ -272,6 +278,13 @@ e2e_status e2e_model_export_coo_i32(const void* model, int32_t* S1, int32_t* S2,
 e2e_status e2e_model_export_csr(const void* model, uint32_t* row_ptr, int32_t* col,
                                 float* val, float* R);
 
+/* The reward vector of alpha column j of a model built with n_alpha > 0, in the
+ * reference's order (R of length Na*Ns, rows a*Ns + s, the last slice 0): the
+ * master_R that src/build_model.cu:1246-1314 writes for a build with that alpha
+ * (src/my_script.sh:10-19 runs one build per alpha; its transitions do not depend on
+ * alpha). Host buffer of na*ns floats. */
+e2e_status e2e_model_export_R_alpha(const void* model, int32_t j, float* R);
+
 /* Sparse value iteration: replaces solver_spvi_solve (src/spvi_2.cu:650-726).
  * policy/value are host arrays of ns elements (either may be NULL). */
 e2e_status e2e_vi_solve(void* model, const e2e_solve_opts* opts,
@@ -288,6 +301,16 @@ e2e_status e2e_vi_solve_goals(void* model, const int32_t* goal_cells, int32_t k,
                               const e2e_solve_opts* opts, const e2e_batch_opts* bopts,
                               uint32_t* policy, float* value, int32_t* col_sweeps,
                               e2e_batch_stats* stats);
+
+/* Alpha-sweep solve: replaces the per-alpha spvi runs of src/my_script.sh:10-19
+ * (src/Makefile:7-19, solver_spvi_solve src/spvi_2.cu:650-726 once per alpha). Column j
+ * is the solve of the model built with alpha = alphas[j]; all columns share the one set
+ * of transitions. The model must come from e2e_model_build with n_alpha > 0.
+ * policy/value are host arrays [n_alpha][ns] (either may be NULL); col_sweeps
+ * [n_alpha] gets each column's sweep count (or NULL). */
+e2e_status e2e_vi_solve_alphas(void* model, const e2e_solve_opts* opts, const e2e_batch_opts* bopts,
+                               uint32_t* policy, float* value, int32_t* col_sweeps,
+                               e2e_batch_stats* stats);
 
 /* Continuous-batching planning server (no counterpart in the reference, whose pipeline
  * is one build + one solve per goal: src/Makefile:7-19). Environments (fields + params,
