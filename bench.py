@@ -186,6 +186,44 @@ def _paths(e2e, np, torch):
     return res
 
 
+def _alpha_sweep(e2e, np, torch):
+    """The reference's alpha sweep (src/my_script.sh:10-19: build + solve for alpha =
+    0, 0.1, .., 1) on the C2 grid with reward method 4 and 2 speeds x 8 headings: one build with 11 alpha
+    columns + one batched solve, against 11 builds + 11 plain solves, every column
+    compared bit for bit."""
+    from e2e_gpu_dp_amd.configs import workload
+    w = workload("C2-alpha")
+    p = w.params
+    alphas = (np.arange(11, dtype=np.float32) * np.float32(10)) / np.float32(100)
+    f = w.fields()
+    dev = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in f.items()}
+    torch.cuda.synchronize()
+
+    def batched():
+        t0 = time.perf_counter()
+        m = e2e.build_model(p, dev, alphas=alphas)
+        b_ms = m.info["build_ms"]
+        pol, val, sw, st = m.solve_alphas(thresh=1e-6)
+        m.close()
+        return (time.perf_counter() - t0) * 1e3, b_ms, st, pol, val
+
+    batched()  # warmup
+    total_ms, build_ms, st, pol, val = batched()
+    t0 = time.perf_counter()
+    same = 0
+    for j, a in enumerate(alphas):
+        m = e2e.build_model(p.replace(alpha=float(a)), dev)
+        pj, vj, _ = m.solve(thresh=1e-6)
+        m.close()
+        same += int(np.array_equal(vj.view(np.uint32), val[j].view(np.uint32)) and np.array_equal(pj, pol[j]))
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    return {"workload": "C2-alpha: 100x100 x nt=120, 2 speeds x 8 headings, reward method 4, 11 alphas",
+            "sweep_ms": total_ms, "build_ms": build_ms, "solve_ms": st["solve_ms"],
+            "per_alpha_builds_and_solves_ms": plain_ms, "speedup": plain_ms / total_ms,
+            "columns_bit_identical": f"{same}/{len(alphas)}", "sweeps": st["sweeps"],
+            "variant": st["variant"], "cols_per_lane": st["cols_per_lane"]}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -306,6 +344,8 @@ def main(argv=None):
     if not args.no_paths and args.workload == "C2":
         print("[bench] obstacle and scalar-reward builds", file=sys.stderr, flush=True)
         out["paths"] = _paths(e2e, np, torch)
+        print("[bench] alpha sweep", file=sys.stderr, flush=True)
+        out["alpha_sweep"] = _alpha_sweep(e2e, np, torch)
     if not args.no_batched:
         print("[bench] batched goals", file=sys.stderr, flush=True)
         out["batched"] = _batched(e2e, np, max(args.steps, 3))

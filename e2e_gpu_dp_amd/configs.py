@@ -100,7 +100,12 @@ def workload(name: str) -> Workload:
     C4 (200^2 x 200, large); C3 and C5 reuse these builders. "C2-obs" is C2 with obstacle
     islands (the goes_through_obstacle path, src/build_model.cu:444-477) and "C2-rad" is
     C2 with reward method 2 over a scalar field (:244-250): the two build paths the
-    named workloads leave unexercised."""
+    named workloads leave unexercised. "C2-alpha" is C2 with reward method 4 and 2 speeds
+    x 8 headings (16 actions: F/2 and F), for the alpha sweep of src/my_script.sh:10-19."""
+    if name == "C2-alpha":  # the alpha sweep: reward method 4, 2 speeds x 8 headings
+        w = workload("C2")
+        return Workload(name, w.params.replace(reward_type=4, num_ac_speeds=2, num_ac_angles=8), w.seed,
+                        w.sigmas)
     if name in ("C2-obs", "C2-rad"):
         w = workload("C2")
         p = w.params

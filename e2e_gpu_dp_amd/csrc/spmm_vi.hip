@@ -403,6 +403,50 @@ static dim3 grid_of(const Model* m, const Variant& v, int kp) {
 
 // Goal columns (goals != NULL) or alpha columns (goals == NULL: the model's n_alpha
 // reward vectors, the model's own terminal cells).
+e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* value, e2e_solve_stats* stats);
+
+// Alpha columns one at a time: the plain solve (the default sweep kernel) with R set to
+// each column's rewards. Every column is the plain solve of the model built with that
+// alpha, bit for bit.
+static e2e_status vi_solve_alpha_columns(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* value,
+                                         int32_t* col_sweeps, e2e_batch_stats* stats) {
+    const double t_start = now_ms();
+    float* const R_own = m->R;
+    e2e_status rc = E2E_OK;
+    int conv = 0, maxsw = 0, launches = 0;
+    double kms = 0.0, bytes = 0.0;
+    for (int j = 0; j < m->n_alpha; j++) {
+        m->R = m->Ralpha + (size_t)j * m->rows;
+        e2e_solve_stats st{};
+        const e2e_status r = vi_solve(m, o, policy ? policy + (size_t)j * m->ns : nullptr,
+                                      value ? value + (size_t)j * m->ns : nullptr, &st);
+        m->R = R_own;
+        if (r != E2E_OK && r != E2E_ERR_NOT_CONVERGED && r != E2E_ERR_TIMEOUT) return r;
+        if (r != E2E_OK && rc == E2E_OK) rc = r;
+        conv += st.converged ? 1 : 0;
+        maxsw = std::max(maxsw, (int)st.sweeps);
+        if (col_sweeps) col_sweeps[j] = st.converged ? st.sweeps : -st.sweeps;
+        kms += st.sweep_kernel_ms;
+        launches += st.launches;
+        bytes = st.bytes_per_sweep;
+    }
+    if (stats) {
+        *stats = e2e_batch_stats{};
+        stats->sweeps = maxsw;
+        stats->converged = conv;
+        stats->variant = -2;
+        stats->cols_per_lane = 1;
+        stats->states_per_wave = 0;
+        stats->col_block = 1;
+        stats->solve_ms = now_ms() - t_start;
+        stats->sweep_kernel_ms = kms;
+        stats->bytes_per_sweep = bytes;
+        stats->launches = launches;
+    }
+    if (rc == E2E_ERR_NOT_CONVERGED) set_error("%d of %d alpha columns did not converge", m->n_alpha - conv, m->n_alpha);
+    return rc;
+}
+
 static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2e_solve_opts* o,
                                  const e2e_batch_opts* bo, uint32_t* policy, float* value, int32_t* col_sweeps,
                                  e2e_batch_stats* stats) {
@@ -419,6 +463,10 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
     } else {
         E2E_CHECK(m->Ralpha && m->n_alpha >= 1 && k == m->n_alpha, E2E_ERR_INVALID,
                   "alpha columns need a model built with n_alpha > 0");
+        const int v = bo ? bo->variant : -1;
+        const double vblock = (double)m->ns * (double)std::max(16, (int)ceil_div(k, 16) * 16) * 8.0;
+        if (v == -2 || (v == -1 && vblock > 96.0 * 1024 * 1024))
+            return vi_solve_alpha_columns(m, o, policy, value, col_sweeps, stats);
     }
     e2e_solve_opts opts{};
     opts.thresh = 0.1f;

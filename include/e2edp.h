@@ -177,7 +177,10 @@ typedef struct e2e_solve_stats {
 } e2e_solve_stats;
 
 typedef struct e2e_batch_opts {
-    int32_t variant;           /* SpMM kernel variant; -1: autotune on this model */
+    int32_t variant;           /* SpMM kernel variant; -1: autotune on this model. Alpha columns also take
+                                  -2: one plain sweep chain per column over the shared model (chosen
+                                  automatically when the value block, ns x 16-column groups x 8 bytes,
+                                  exceeds 96 MB) */
     int32_t use_graph;         /* 1: replay the sweep chain as a captured hipGraph */
     int32_t reserved[2];
 } e2e_batch_opts;
