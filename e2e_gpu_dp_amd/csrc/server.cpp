@@ -40,7 +40,7 @@ int64_t live_batch_bytes(const Model* m, int kp);
 e2e_status live_create(Model* m, int kp, float thresh, hipStream_t s, LiveBatch** out);
 void live_free(LiveBatch* L);
 e2e_status live_admit(LiveBatch* L, const int32_t* cols, const int32_t* goals, int n);
-e2e_status live_step(LiveBatch* L, int nsweeps);
+e2e_status live_step(LiveBatch* L, int nsweeps, int ncols);
 e2e_status live_state(LiveBatch* L, BatchState* st, int32_t* conv, int32_t* start);
 e2e_status live_harvest(LiveBatch* L, int col, int64_t start_state, float* v_start, int32_t* a_start,
                         float* vcol, uint32_t* pcol);
@@ -269,7 +269,10 @@ struct Server {
                     e->adm_cols.clear();
                     e->adm_goals.clear();
                 }
-                live_step(e->batch, spc);
+                int hi = 0;  // highest occupied slot + 1: the columns a sweep must cover
+                for (int c = 0; c < (int)e->slot.size(); c++)
+                    if (e->slot[c]) hi = c + 1;
+                live_step(e->batch, spc, hi);
                 for (int c = 0; c < (int)e->slot.size(); c++)
                     if (e->slot[c]) e->steps_since_admit[c] += spc;
             }

@@ -733,9 +733,9 @@ e2e_status live_create(Model* m, int kp, float thresh, hipStream_t s, LiveBatch*
     L->A = make_args(m, L->b, o);
     // variant: the model's tuned choice for this width if one exists, else columns across
     // the whole wave (one state per wave)
-    L->variant = 0;
-    if (m->batch && m->batch->tuned_kp == kp && m->batch->tuned_variant >= 0) L->variant = m->batch->tuned_variant;
-    if (kp % kb_of(kVariants[L->variant])) L->variant = kp % 32 == 0 ? 3 : 6;
+    // 16 columns per block (one lane per column, 4 states per wave), so that a sweep
+    // launches only the column blocks that hold live columns (live_step)
+    L->variant = 6;
     L->grid = grid_of(m, kVariants[L->variant], kp);
     const int64_t n = m->ns * kp;
     hipLaunchKernelGGL(live_init, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, L->b.V[0], L->b.V[1], n,
@@ -776,9 +776,15 @@ e2e_status live_admit(LiveBatch* L, const int32_t* cols, const int32_t* goals, i
     return E2E_OK;
 }
 
-e2e_status live_step(LiveBatch* L, int nsweeps) {
+// Sweeps over the column blocks that hold slots 0 .. ncols-1 (the caller's highest live
+// slot + 1): free slots beyond are not computed. The residual and convergence logic
+// of the sweep only acts on running columns, all of which are in the launched blocks.
+e2e_status live_step(LiveBatch* L, int nsweeps, int ncols) {
+    dim3 grid = L->grid;
+    const int kb = kb_of(kVariants[L->variant]);
+    grid.y = (unsigned)std::min<int64_t>(grid.y, std::max<int64_t>(1, ceil_div((int64_t)ncols, (int64_t)kb)));
     for (int i = 0; i < nsweeps; i++)
-        hipLaunchKernelGGL(kVariants[L->variant].fn, L->grid, dim3(kBThreads), 0, L->s, L->A);
+        hipLaunchKernelGGL(kVariants[L->variant].fn, grid, dim3(kBThreads), 0, L->s, L->A);
     E2E_HIP(hipGetLastError());
     return E2E_OK;
 }
