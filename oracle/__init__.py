@@ -17,8 +17,11 @@ SO = HERE / "liboracle.so"
 _lib = None
 
 
-def build():
-    r = subprocess.run(["make", "-C", str(HERE), "-s"], capture_output=True, text=True)
+def compile_lib(force: bool = False):
+    """Compile liboracle.so from oracle.c (make in this folder; force: rebuild even when
+    up to date)."""
+    r = subprocess.run(["make", "-C", str(HERE), "-s"] + (["-B"] if force else []), capture_output=True,
+                       text=True)
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
 
@@ -27,7 +30,7 @@ def lib():
     global _lib
     if _lib is None:
         if not SO.exists():
-            build()
+            compile_lib()
         L = C.CDLL(str(SO))
         VP = C.c_void_p
         i64 = C.c_int64

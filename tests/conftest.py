@@ -21,7 +21,7 @@ def _ensure_built():
         build(verbose=False)
     import oracle
     if not oracle.SO.exists():
-        oracle.build()
+        oracle.compile_lib()
 
 
 _ensure_built()

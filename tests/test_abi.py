@@ -80,3 +80,15 @@ def test_bad_arguments_are_errors_not_crashes():
     f = _abi.Fields()
     assert L.e2e_model_build(C.byref(bad), C.byref(f), None, C.byref(h)) == 1
     assert h.value is None
+
+
+def test_graft_build_entry_compiles_the_oracle():
+    """build() compiles the oracle through oracle.compile_lib; oracle.build is the
+    oracle's model builder (params, fields), a different function."""
+    import inspect
+
+    import oracle
+    assert inspect.signature(oracle.compile_lib).parameters.keys() == {"force"}
+    assert list(inspect.signature(oracle.build).parameters)[:2] == ["params", "fields"]
+    src = (Path(__file__).resolve().parents[1] / "__graft_entry__.py").read_text()
+    assert "oracle.compile_lib(" in src
