@@ -64,6 +64,10 @@ struct BatchState {
 struct BatchCache;
 void batch_cache_free(BatchCache* c);
 
+// wpart[0..W]: first state of each of W waves, balanced by entries + rows, group-aligned
+void launch_wave_partition(const uint32_t* rp, int64_t ns, int na, int64_t W, int spc, uint32_t* wpart,
+                           hipStream_t s);
+
 struct Model {
     e2e_params p{};
     bool built = false;       // false: imported from COO

@@ -16,9 +16,15 @@
 //     V_j = 0 and pi_j = 0 there at every t; the kernel writes that directly.
 //
 // Layout: V[2][ns][KP] float64 (KP = padded column count, columns contiguous per
-// state), pi [ns][KP]. Variant (CPL, SPW): a wave takes SPW states, 64/SPW lanes per
-// state, CPL adjacent columns per lane (one or two 16-byte gathers of V), so a wave
-// covers KB = 64/SPW*CPL columns of each state; grid.y walks the column blocks.
+// state), pi [ns][KP]. grid.y walks blocks of KB columns. Two kernel families:
+//   * spmm_vi_sweep<CPL, SPW>: a wave takes SPW states, 64/SPW lanes per state, CPL
+//     adjacent columns per lane; each lane walks all na rows of its state one entry at
+//     a time. KB = 64/SPW*CPL.
+//   * spmm_vi_rows<LC, CPL, U>: a wave takes 64/(LC*NAP) states (NAP = na rounded up to
+//     a power of two); lane (state, action, column lane) walks one row, U entries at a
+//     time (all U entry loads, then all U gathers, then the in-range products added in
+//     ascending order), so each lane keeps U gathers in flight and the LC lanes of a row
+//     read each gathered line of V together. KB = LC*CPL.
 //
 // Arithmetic is that of the plain float64 sweeps: values carried in float64 between
 // sweeps, each row's products (double)P * V added in ascending entry order, Q = R +
@@ -33,6 +39,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "lanes.h"
 
 namespace e2e {
 
@@ -51,14 +58,19 @@ struct BatchArgs {
     BatchState* st;
     int64_t ns;
     int nc, na, nt, kp, nr;
+    int nap;               // na rounded up to a power of two (spmm_vi_rows lane layout)
     float thresh, r_term, discount;
     double late_pen;       // 0.05 * r_outbound
     const float* Rcol;     // alpha columns: R of column j at Rcol[j * rows + row] (NULL: goal columns)
     int64_t rows;
     int rcol_n;            // columns Rcol holds (padding columns read the last one)
+    const uint32_t* wpart; // spmm_vi_stream: [waves+1] first state of each wave
+    uint32_t* colmax;      // spmm_vi_stream: [KP] running column residual maxima (float bits)
 };
 
 namespace {
+
+using BatchFn = void (*)(const BatchArgs);
 
 // float residual never below the double one r >= 0 (round to nearest, then one ulp up
 // if that fell below r), the same rounding as the plain sweeps
@@ -93,9 +105,75 @@ struct VecLoad<4> {
     }
 };
 
+template <>
+struct VecLoad<8> {
+    __device__ static void ld(const double* p, double* x) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const double2 v = reinterpret_cast<const double2*>(p)[i];
+            x[2 * i] = v.x;
+            x[2 * i + 1] = v.y;
+        }
+    }
+};
+
 constexpr int kBThreads = 1024;  // 16 waves per block
 constexpr int kBWaves = kBThreads / 64;
 constexpr int kBMaxBlocks = 512;
+
+// End of a sweep, after each wave has put its per-column residual maxima in red[wave]:
+// per-block column maxima to part[block x][KP]; the last block of the grid takes the
+// maxima over the x-blocks, freezes the columns below thresh and publishes the sweep.
+template <int KB>
+__device__ __forceinline__ void sweep_tail(const BatchArgs& A, float (*red)[KB], float* fin, int& is_last,
+                                           uint32_t k) {
+    BatchState* st = A.st;
+    const int tid = threadIdx.x;
+    const int KP = A.kp;
+    __syncthreads();
+    if (tid < KB) {
+        float m = 0.0f;
+#pragma unroll
+        for (int w = 0; w < kBWaves; w++) m = fmaxf(m, red[w][tid]);
+        A.part[(int64_t)blockIdx.x * KP + blockIdx.y * KB + tid] = m;
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) is_last = atomicAdd(&st->done, 1u) == gridDim.x * gridDim.y - 1;
+    __syncthreads();
+    if (is_last) {
+        __threadfence();
+        // column j's residual = max over the x-blocks of part[bx][j]; thread t takes
+        // column t % KP and every (kBThreads/KP)-th x-block, then LDS combines
+        const int groups = kBThreads / KP;
+        {
+            const int j = tid % KP, g0 = tid / KP;
+            float r = 0.0f;
+#pragma unroll 8
+            for (unsigned bx = g0; bx < gridDim.x; bx += groups)
+                r = fmaxf(r, __hip_atomic_load(A.part + (int64_t)bx * KP + j, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT));
+            fin[tid] = r;
+        }
+        __syncthreads();
+        int running = 0;
+        for (int j = tid; j < KP; j += kBThreads) {
+            float r = 0.0f;
+            for (int g2 = 0; g2 < groups; g2++) r = fmaxf(r, fin[g2 * KP + j]);
+            if (A.conv[j] < 0) {
+                A.last_res[j] = r;
+                if (r < A.thresh) A.conv[j] = (int32_t)k;
+                else running++;
+            }
+        }
+        const int n_running = __syncthreads_count(running > 0);
+        if (tid == 0) {
+            st->all_conv = n_running == 0;
+            st->done = 0u;
+            st->k = k + 1u;
+        }
+    }
+}
 
 template <int CPL, int SPW>
 __global__ __launch_bounds__(kBThreads) void spmm_vi_sweep(const BatchArgs A) {
@@ -204,12 +282,414 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_sweep(const BatchArgs A) {
 #pragma unroll
         for (int c = 0; c < CPL; c++) red[wv][l * CPL + c] = rmax[c];
     }
+    sweep_tail<KB>(A, red, fin, is_last, k);
+}
+
+// Row-parallel sweep. Lane (sl, a, cl) = lane / (LC*NAP), (lane / LC) % NAP, lane % LC
+// takes row a of state sl of the wave's current group and columns j0 .. j0+CPL-1. Each
+// block takes a contiguous range of groups (blocks remapped so that the 8 XCDs each
+// get a contiguous eighth of the states, keeping the gathers of V(t+1) in one XCD's L2);
+// its waves interleave over that range.
+template <int LC, int CPL, int U>
+__global__ __launch_bounds__(kBThreads) void spmm_vi_rows(const BatchArgs A) {
+    constexpr int KB = LC * CPL;
+    __shared__ float red[kBWaves][KB];
+    __shared__ float fin[kBThreads];
+    __shared__ int is_last;
+    BatchState* st = A.st;
+    if (st->all_conv) return;
+    const uint32_t k = st->k;
+    const double* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
+    double* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nap = A.nap;
+    const int cl = lane % LC;
+    const int a = (lane / LC) & (nap - 1);
+    const int sl = lane / (LC * nap);
+    const int spw = 64 / (LC * nap);
+    const int j0 = blockIdx.y * KB + cl * CPL;
+    const int KP = A.kp;
+    const uint32_t ns = (uint32_t)A.ns, na = (uint32_t)A.na, nc = (uint32_t)A.nc;
+    int goal[CPL];
+    bool live[CPL];
+    float rmax[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; c++) {
+        goal[c] = A.goal[j0 + c];
+        live[c] = A.conv[j0 + c] < 0;
+        rmax[c] = 0.0f;
+    }
+    const uint32_t G = (ns + spw - 1) / spw;
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    const unsigned pb = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;
+    const uint32_t g_beg = (uint32_t)((uint64_t)G * pb / nb), g_end = (uint32_t)((uint64_t)G * (pb + 1) / nb);
+    for (uint32_t g = g_beg + wv; g < g_end; g += kBWaves) {
+        const uint32_t s = g * spw + sl;
+        const bool sval = s < ns;
+        const bool valid = sval && (uint32_t)a < na;
+        const uint32_t sc = sval ? s : ns - 1u;
+        const uint32_t t = sc / nc;
+        const int cell = (int)(sc - t * nc);
+        const int colbase = (int)((t + 1) * nc);
+        const float delta = (float)((double)A.r_term - ((int)t > A.nt - 20 ? A.late_pen : 0.0));
+        const int64_t row = (int64_t)sc * na + min((uint32_t)a, na - 1u);
+        uint32_t jb = 0, je = 0;
+        if (valid) {
+            jb = A.rp[row];
+            je = A.rp[row + 1];
+        }
+        double sd[CPL];
+        int hit[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; c++) {
+            sd[c] = 0.0;
+            hit[c] = -1;
+        }
+        // entries jb .. jb+U-1 may run past the row: they are loaded (the entry array
+        // has a zeroed tail of kNzPad entries, and every column is a state < ns, so the
+        // gather stays inside V) but not added
+        while (jb < je) {
+            int2 e[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) e[u] = A.nz[jb + u];
+            double x[U][CPL];
+#pragma unroll
+            for (int u = 0; u < U; u++) VecLoad<CPL>::ld(Vin + (int64_t)e[u].x * KP + j0, x[u]);
+            const uint32_t cnt = je - jb;
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if ((uint32_t)u < cnt) {
+                    const float v = __int_as_float(e[u].y);
+                    const int cc = e[u].x - colbase;
+#pragma unroll
+                    for (int c = 0; c < CPL; c++) {
+                        sd[c] = sd[c] + (double)v * x[u][c];
+                        if (cc == goal[c]) hit[c] = (int)rintf(v * (float)A.nr);
+                    }
+                }
+            }
+            jb += U;
+        }
+        double q[CPL];
+        int ba[CPL];
+        const float rr = valid ? A.R[row] : 0.0f;
+        const float rs = (valid && A.Rsum) ? A.Rsum[row] : 0.0f;
+#pragma unroll
+        for (int c = 0; c < CPL; c++) {
+            const float rj = A.Rcol ? (valid ? A.Rcol[(int64_t)min(j0 + c, A.rcol_n - 1) * A.rows + row] : 0.0f)
+                             : hit[c] >= 0 ? (rs + (float)hit[c] * delta) / A.nr
+                                           : rr;
+            double qq = valid ? (double)rj + (double)A.discount * sd[c] : -INFINITY;
+            if (qq != qq) qq = -INFINITY;
+            q[c] = qq;
+            ba[c] = valid ? a : 0x7fffffff;
+        }
+        // max / first argmax over the NAP lanes of the state (strict '>' in action
+        // order, spvi_2.cu:112: the larger value, on a tie the lower action); the
+        // combine is associative and commutative, so the butterfly gives every lane
+        // the result of the sequential scan
+        for (int o = LC; o < LC * nap; o <<= 1) {
+#pragma unroll
+            for (int c = 0; c < CPL; c++) {
+                const double q2 = __shfl_xor(q[c], o, 64);
+                const int a2 = __shfl_xor(ba[c], o, 64);
+                if (q2 > q[c] || (q2 == q[c] && a2 < ba[c])) {
+                    q[c] = q2;
+                    ba[c] = a2;
+                }
+            }
+        }
+        if (a == 0 && sval) {
+            const int64_t o = (int64_t)s * KP + j0;
+#pragma unroll
+            for (int c = 0; c < CPL; c++) {
+                double vo;
+                int po;
+                if (cell == goal[c]) {
+                    vo = 0.0;
+                    po = 0;
+                } else if (q[c] > -1e6) {
+                    vo = q[c];
+                    po = ba[c];
+                } else {
+                    vo = -1e6;
+                    po = -1;
+                }
+                if (live[c]) {
+                    Vout[o + c] = vo;
+                    A.pi[o + c] = po;
+                    rmax[c] = fmaxf(rmax[c], res_up(fabs(vo - Vin[o + c])));
+                }
+            }
+        }
+    }
+    // per-column residual: the lanes of one column lane cl, then the waves
+#pragma unroll
+    for (int c = 0; c < CPL; c++)
+        for (int off = LC; off < 64; off <<= 1) rmax[c] = fmaxf(rmax[c], __shfl_xor(rmax[c], off, 64));
+    if (lane < LC) {
+#pragma unroll
+        for (int c = 0; c < CPL; c++) red[wv][cl * CPL + c] = rmax[c];
+    }
+    sweep_tail<KB>(A, red, fin, is_last, k);
+}
+
+// Streaming sweep with transposed gathers. The state partition and the entry stream
+// are those of the plain lean sweep: each wave takes a contiguous state range (wpart,
+// balanced by entries + rows), lanes (a, sl) = (action, state) of groups of SPC = 64/NAP
+// states, and the entries are read as one stream in windows of WIN = 64*U. Per window:
+//   * gathers: lane (q, c) = (lane / KB, lane % KB) takes entry q of each run of
+//     E = 64/KB entries and column c, so the KB lanes of an entry read its KB adjacent
+//     values of V (one line) together: one cache line per entry, as in the plain sweep,
+//     whatever KB is. Products P*V(c) go to LDS [entry][column];
+//   * rows: each lane adds its row's products of the window, per column in ascending
+//     entry order (the order, and so the bits, of the plain sweep);
+//   * groups end as in the plain sweep: max / first argmax over the NAP lanes of a state
+//     per column, one writer lane per state and column.
+// The residual of each column goes through a per-column atomic maximum (colmax, float
+// bits, all residuals >= 0); the last block of the grid reads and clears it.
+constexpr int kSThreads = 256;
+constexpr int kSWaves = kSThreads / 64;
+constexpr int kSRowReads = 2;  // entries a row lane reads per step (the LDS pad of a window)
+
+template <int NAP, int KB, int U>
+__global__ __launch_bounds__(kSThreads) void spmm_vi_stream(const BatchArgs A) {
+    constexpr int SPC = 64 / NAP;
+    constexpr int WIN = 64 * U;
+    constexpr int E = 64 / KB;    // entries per gather instruction
+    constexpr int NIT = WIN / E;  // gather instructions per window
+    constexpr int BAT = NIT < 8 ? NIT : 8;
+    static_assert(NIT % BAT == 0, "gather batches");
+    __shared__ __align__(16) double stage[kSWaves][(WIN + kSRowReads) * KB];
+    __shared__ int2 hitc[kSWaves][WIN + kSRowReads];  // goal columns: (column, count) of each entry
+    __shared__ float red[kSWaves][KB];
+    __shared__ int sgoal[KB];
+    __shared__ int slive[KB];
+    __shared__ int is_last;
+    BatchState* st = A.st;
+    if (st->all_conv) return;
+    const uint32_t k = st->k;
+    const double* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
+    double* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int a = lane % NAP, sl = lane / NAP;
+    const int gq = lane / KB, gc = lane % KB;
+    const uint32_t na = (uint32_t)A.na, nc = (uint32_t)A.nc;
+    const int KP = A.kp;
+    const int jb0 = blockIdx.y * KB;  // first column of this block
+    const bool goals = A.Rcol == nullptr;
+    if (tid < KB) {
+        sgoal[tid] = A.goal[jb0 + tid];
+        slive[tid] = A.conv[jb0 + tid] < 0;
+    }
+    __syncthreads();
+    int goal[KB];
+    float rmax[KB];
+#pragma unroll
+    for (int c = 0; c < KB; c++) {
+        goal[c] = sgoal[c];
+        rmax[c] = 0.0f;
+    }
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    const unsigned pb = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;
+    const unsigned w = pb * kSWaves + wv;
+    const uint32_t s_begin = A.wpart[w], s_end = A.wpart[w + 1];
+    double* wst = stage[wv];
+    int2* whc = hitc[wv];
+
+    if (s_begin < s_end) {
+        const uint32_t ngroups = (s_end - s_begin + SPC - 1) / SPC;
+        const uint32_t Pb = A.rp[s_begin * na], Pend = A.rp[s_end * na];
+        const uint32_t nwin = Pend > Pb ? (Pend - Pb + WIN - 1) / WIN : 0u;
+        struct Meta {
+            uint32_t rb, re;
+            float rr, rs;
+        };
+        auto load_meta = [&](uint32_t g, Meta& M) {
+            const uint32_t s = min(s_begin + g * SPC + (uint32_t)sl, s_end - 1u);
+            const uint32_t row = s * na + min((uint32_t)a, na - 1u);
+            M.rb = A.rp[row];
+            M.re = A.rp[row + 1];
+            M.rr = A.R[row];
+            M.rs = goals ? A.Rsum[row] : 0.0f;
+        };
+        Meta cur, nxt;
+        uint32_t g = 0;
+        load_meta(0, cur);
+        load_meta(min(1u, ngroups - 1u), nxt);
+        uint32_t ce = A.rp[min(s_begin + SPC, s_end) * na];
+        bool valid = (s_begin + (uint32_t)sl < s_end) && ((uint32_t)a < na);
+        double sd[KB];
+        int hit[KB];
+#pragma unroll
+        for (int c = 0; c < KB; c++) {
+            sd[c] = 0.0;
+            hit[c] = -1;
+        }
+        // the lane's state in group g, its slice's next-slice base, and per column the
+        // goal column index of that base (hits are entries with column gbase + goal[c])
+        auto group_base = [&](uint32_t gg) {
+            const uint32_t s = min(s_begin + gg * SPC + (uint32_t)sl, s_end - 1u);
+            return (s / nc + 1u) * nc;
+        };
+        uint32_t colbase = group_base(0);
+
+        // products of entries [jb, je) of the window starting at entry w0, ascending
+        // (absolute entry indices: a row that ended before the window gives je < jb)
+        auto row_part = [&](uint32_t w0, uint32_t jb, uint32_t je) {
+            while (jb < je) {
+                const uint32_t r = jb - w0;
+                double x[kSRowReads][KB];
+                int2 h[kSRowReads];
+#pragma unroll
+                for (int i = 0; i < kSRowReads; i++) {
+                    if (KB == 1) {
+                        x[i][0] = wst[r + i];
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < KB; c += 2) {
+                            const double2 v = *reinterpret_cast<const double2*>(wst + (r + i) * KB + c);
+                            x[i][c] = v.x;
+                            x[i][c + 1] = v.y;
+                        }
+                    }
+                    if (goals) h[i] = whc[r + i];
+                }
+                const uint32_t cnt = je - jb;
+#pragma unroll
+                for (int i = 0; i < kSRowReads; i++) {
+                    if ((uint32_t)i < cnt) {
+#pragma unroll
+                        for (int c = 0; c < KB; c++) sd[c] = sd[c] + x[i][c];
+                        if (goals) {
+                            const int cc = h[i].x - (int)colbase;
+#pragma unroll
+                            for (int c = 0; c < KB; c++)
+                                if (cc == goal[c]) hit[c] = h[i].y;
+                        }
+                    }
+                }
+                jb += kSRowReads;
+            }
+        };
+
+        auto finish_group = [&]() {
+            const uint32_t s = s_begin + g * SPC + (uint32_t)sl;
+            const bool own = g < ngroups && s < s_end;
+            const uint32_t sc = min(s, s_end - 1u);
+            const uint32_t t = sc / nc;
+            const int cell = (int)(sc - t * nc);
+            const float delta = (float)((double)A.r_term - ((int)t > A.nt - 20 ? A.late_pen : 0.0));
+            const uint32_t row = sc * na + min((uint32_t)a, na - 1u);
+            const bool writer = (NAP <= 16) ? (a == NAP - 1) : (a == 0);
+#pragma unroll
+            for (int c = 0; c < KB; c++) {
+                const float rj = !goals ? A.Rcol[(int64_t)min(jb0 + c, A.rcol_n - 1) * A.rows + row]
+                                 : hit[c] >= 0 ? (cur.rs + (float)hit[c] * delta) / A.nr
+                                               : cur.rr;
+                double q = valid ? (double)rj + (double)A.discount * sd[c] : -INFINITY;
+                if (q != q) q = -INFINITY;
+                int ba = valid ? a : 0x7fffffff;
+                if (NAP <= 16) {
+#pragma unroll
+                    for (int o = 1; o < NAP; o <<= 1) {
+                        const double q2 = dpp_row_shr(q, o, (double)-INFINITY);
+                        const int a2 = dpp_row_shr(ba, o, 0x7fffffff);
+                        const bool take = q2 > q || (q2 == q && a2 < ba);
+                        q = take ? q2 : q;
+                        ba = take ? a2 : ba;
+                    }
+                } else {
+#pragma unroll
+                    for (int o = 1; o < NAP; o <<= 1) {
+                        const double q2 = __shfl_xor(q, o, 64);
+                        const int a2 = __shfl_xor(ba, o, 64);
+                        if (q2 > q || (q2 == q && a2 < ba)) {
+                            q = q2;
+                            ba = a2;
+                        }
+                    }
+                }
+                if (writer && own && slive[c]) {
+                    double vo;
+                    int po;
+                    if (cell == goal[c]) {
+                        vo = 0.0;
+                        po = 0;
+                    } else if (q > -1e6) {
+                        vo = q;
+                        po = ba;
+                    } else {
+                        vo = -1e6;
+                        po = -1;
+                    }
+                    const int64_t o = (int64_t)s * KP + jb0 + c;
+                    Vout[o] = vo;
+                    A.pi[o] = po;
+                    rmax[c] = fmaxf(rmax[c], res_up(fabs(vo - Vin[o])));
+                }
+                sd[c] = 0.0;
+                hit[c] = -1;
+            }
+            g++;
+            cur = nxt;
+            load_meta(min(g + 1u, ngroups - 1u), nxt);
+            const uint32_t s0 = s_begin + g * SPC;
+            ce = A.rp[min(s0 + SPC, s_end) * na];
+            valid = (g < ngroups) && (s0 + (uint32_t)sl < s_end) && ((uint32_t)a < na);
+            colbase = group_base(min(g, ngroups - 1u));
+        };
+
+        for (uint32_t kw = 0; kw < nwin; kw++) {
+            const uint32_t w0 = Pb + kw * WIN, wend = w0 + WIN;
+            // entries w0 .. wend-1: those past the wave's range are loaded (the entry
+            // array has a zeroed tail of kNzPad >= WIN entries and every column is a
+            // state < ns) but no row of this wave reads them
+#pragma unroll
+            for (int i0 = 0; i0 < NIT; i0 += BAT) {
+                int2 e[BAT];
+                double x[BAT];
+#pragma unroll
+                for (int i = 0; i < BAT; i++) e[i] = A.nz[w0 + (uint32_t)((i0 + i) * E + gq)];
+#pragma unroll
+                for (int i = 0; i < BAT; i++) x[i] = Vin[(int64_t)e[i].x * KP + jb0 + gc];
+#pragma unroll
+                for (int i = 0; i < BAT; i++) {
+                    const float v = __int_as_float(e[i].y);
+                    wst[((i0 + i) * E + gq) * KB + gc] = (double)v * x[i];
+                    if (goals && gc == 0) whc[(i0 + i) * E + gq] = make_int2(e[i].x, (int)rintf(v * (float)A.nr));
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (;;) {
+                if (valid) row_part(w0, max(cur.rb, w0), min(cur.re, wend));
+                if (ce > wend) break;  // the group continues in the next window
+                finish_group();
+                if (g >= ngroups) break;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        while (g < ngroups) finish_group();  // trailing groups whose rows are all empty
+    }
+
+    // per-column residual: the wave's lanes, then the block's waves, then one atomic
+    // maximum per column and block
+#pragma unroll
+    for (int c = 0; c < KB; c++) {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) rmax[c] = fmaxf(rmax[c], __shfl_xor(rmax[c], o, 64));
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int c = 0; c < KB; c++) red[wv][c] = rmax[c];
+    }
     __syncthreads();
     if (tid < KB) {
         float m = 0.0f;
 #pragma unroll
-        for (int w = 0; w < kBWaves; w++) m = fmaxf(m, red[w][tid]);
-        A.part[(int64_t)blockIdx.x * KP + blockIdx.y * KB + tid] = m;
+        for (int i = 0; i < kSWaves; i++) m = fmaxf(m, red[i][tid]);
+        atomicMax(A.colmax + jb0 + tid, __float_as_uint(m));
     }
     __threadfence();
     __syncthreads();
@@ -217,23 +697,9 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_sweep(const BatchArgs A) {
     __syncthreads();
     if (is_last) {
         __threadfence();
-        // column j's residual = max over the x-blocks of part[bx][j]; thread t takes
-        // column t % KP and every (kBThreads/KP)-th x-block, then LDS combines
-        const int groups = kBThreads / KP;
-        {
-            const int j = tid % KP, g0 = tid / KP;
-            float r = 0.0f;
-#pragma unroll 8
-            for (unsigned bx = g0; bx < gridDim.x; bx += groups)
-                r = fmaxf(r, __hip_atomic_load(A.part + (int64_t)bx * KP + j, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT));
-            fin[tid] = r;
-        }
-        __syncthreads();
         int running = 0;
-        for (int j = tid; j < KP; j += kBThreads) {
-            float r = 0.0f;
-            for (int g2 = 0; g2 < groups; g2++) r = fmaxf(r, fin[g2 * KP + j]);
+        for (int j = tid; j < KP; j += kSThreads) {
+            const float r = __uint_as_float(atomicExch(A.colmax + j, 0u));
             if (A.conv[j] < 0) {
                 A.last_res[j] = r;
                 if (r < A.thresh) A.conv[j] = (int32_t)k;
@@ -249,11 +715,37 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_sweep(const BatchArgs A) {
     }
 }
 
+// stream kernel for (NAP, KB); nullptr when not instantiated
+template <int KB, int U>
+BatchFn stream_fn_kb(int nap) {
+    switch (nap) {
+        case 1: return spmm_vi_stream<1, KB, U>;
+        case 2: return spmm_vi_stream<2, KB, U>;
+        case 4: return spmm_vi_stream<4, KB, U>;
+        case 8: return spmm_vi_stream<8, KB, U>;
+        case 16: return spmm_vi_stream<16, KB, U>;
+        case 32: return spmm_vi_stream<32, KB, U>;
+        case 64: return spmm_vi_stream<64, KB, U>;
+        default: return nullptr;
+    }
+}
+
+BatchFn stream_fn(int kb, int u, int nap) {
+    if (kb == 1 && u == 4) return stream_fn_kb<1, 4>(nap);
+    if (kb == 2 && u == 4) return stream_fn_kb<2, 4>(nap);
+    if (kb == 4 && u == 2) return stream_fn_kb<4, 2>(nap);
+    if (kb == 4 && u == 4) return stream_fn_kb<4, 4>(nap);
+    if (kb == 8 && u == 1) return stream_fn_kb<8, 1>(nap);
+    if (kb == 8 && u == 2) return stream_fn_kb<8, 2>(nap);
+    return nullptr;
+}
+
 __global__ void batch_init(double* V0, int64_t n, const int32_t* goal, int32_t* conv, int32_t* start,
-                           float* last_res, int kp, int k, BatchState* st) {
+                           float* last_res, uint32_t* colmax, int kp, int k, BatchState* st) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) V0[i] = 0.0;
     if (i < kp) {
+        colmax[i] = 0u;
         conv[i] = i < k ? -1 : 0;  // padding columns never run
         start[i] = 1;
         last_res[i] = 0.0f;
@@ -279,21 +771,56 @@ __global__ void batch_gather(const double* V0, const double* V1, const int32_t* 
     pout[i] = pi[s * kp + j];
 }
 
-using BatchFn = void (*)(const BatchArgs);
-
+// kind 0: spmm_vi_sweep (lc = 64/spw lanes per state); kind 1: spmm_vi_rows (lc lanes
+// per row); kind 2: spmm_vi_stream (cpl = KB columns per block, u = entries per lane
+// and window). Kinds 1 and 2 take a number of states per wave that depends on na.
 struct Variant {
-    int cpl, spw;
-    BatchFn fn;
+    int cpl, lc;
+    int kind;
+    int u;
+    BatchFn fn;  // kinds 0 and 1
 };
 
 const Variant kVariants[] = {
-    {1, 1, spmm_vi_sweep<1, 1>}, {2, 2, spmm_vi_sweep<2, 2>}, {4, 4, spmm_vi_sweep<4, 4>},
-    {1, 2, spmm_vi_sweep<1, 2>}, {2, 4, spmm_vi_sweep<2, 4>}, {4, 8, spmm_vi_sweep<4, 8>},
-    {1, 4, spmm_vi_sweep<1, 4>}, {2, 8, spmm_vi_sweep<2, 8>}, {4, 16, spmm_vi_sweep<4, 16>},
+    {1, 64, 0, 0, spmm_vi_sweep<1, 1>},  {2, 32, 0, 0, spmm_vi_sweep<2, 2>},
+    {4, 16, 0, 0, spmm_vi_sweep<4, 4>},  {1, 32, 0, 0, spmm_vi_sweep<1, 2>},
+    {2, 16, 0, 0, spmm_vi_sweep<2, 4>},  {4, 8, 0, 0, spmm_vi_sweep<4, 8>},
+    {1, 16, 0, 0, spmm_vi_sweep<1, 4>},  {2, 8, 0, 0, spmm_vi_sweep<2, 8>},
+    {4, 4, 0, 0, spmm_vi_sweep<4, 16>},  {1, 4, 1, 0, spmm_vi_rows<4, 1, 4>},
+    {1, 8, 1, 0, spmm_vi_rows<8, 1, 4>},  {2, 4, 1, 0, spmm_vi_rows<4, 2, 4>},
+    {2, 8, 1, 0, spmm_vi_rows<8, 2, 4>},  {4, 4, 1, 0, spmm_vi_rows<4, 4, 4>},
+    {1, 16, 1, 0, spmm_vi_rows<16, 1, 4>}, {4, 8, 1, 0, spmm_vi_rows<8, 4, 4>},
+    {2, 16, 1, 0, spmm_vi_rows<16, 2, 4>}, {4, 1, 1, 0, spmm_vi_rows<1, 4, 4>},
+    {8, 1, 1, 0, spmm_vi_rows<1, 8, 2>},  {4, 2, 1, 0, spmm_vi_rows<2, 4, 4>},
+    {8, 2, 1, 0, spmm_vi_rows<2, 8, 2>},  {4, 1, 2, 2, nullptr},
+    {4, 1, 2, 4, nullptr},                {8, 1, 2, 1, nullptr},
+    {8, 1, 2, 2, nullptr},                {1, 1, 2, 4, nullptr},
+    {2, 1, 2, 4, nullptr},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
-int kb_of(const Variant& v) { return (64 / v.spw) * v.cpl; }
+int kb_of(const Variant& v) { return v.lc * v.cpl; }
+
+int nap_of(int na) {
+    int p = 1;
+    while (p < na) p <<= 1;
+    return p;
+}
+
+// states per wavefront; 0 when the variant cannot take this many actions
+int spw_of(const Variant& v, int na) {
+    if (v.kind == 0) return 64 / v.lc;
+    if (v.kind == 2) return 64 / nap_of(na);
+    const int lanes = v.lc * nap_of(na);
+    return lanes <= 64 ? 64 / lanes : 0;
+}
+
+bool variant_fits(const Variant& v, int na, int kp) { return spw_of(v, na) > 0 && kp % kb_of(v) == 0; }
+
+// padded column count: 4 or 8 for small batches (the value block's stride per state is
+// then 32 or 64 bytes, so the gathers of a few columns touch fewer lines), else a
+// multiple of 16
+int padded_cols(int k) { return k <= 4 ? 4 : k <= 8 ? 8 : (int)ceil_div(k, 16) * 16; }
 
 struct BatchBufs {
     int kp = 0;
@@ -308,8 +835,12 @@ struct BatchBufs {
     BatchState* st = nullptr;
     float* vout = nullptr;
     int32_t* pout = nullptr;
+    uint32_t* colmax = nullptr;  // [kp] (spmm_vi_stream)
+    uint32_t* wpart = nullptr;   // [wpart_W + 1] state partition of spmm_vi_stream
+    int64_t wpart_W = 0;
+    int wpart_spc = 0;
     void release() {
-        void* ps[] = {V[0], V[1], pi, goal, conv, start, part, last_res, st, vout, pout};
+        void* ps[] = {V[0], V[1], pi, goal, conv, start, part, last_res, st, vout, pout, colmax, wpart};
         for (void* p : ps)
             if (p) (void)hipFree(p);
         *this = BatchBufs();
@@ -323,9 +854,10 @@ struct BatchCache {
     int tuned_kp = -1;
     int tuned_variant = -1;
     hipGraphExec_t graph = nullptr;
-    int graph_variant = -1, graph_chunk = 0, graph_grid = 0;
+    int graph_variant = -1, graph_chunk = 0, graph_grid = 0, graph_gy = 0;
     float graph_thresh = 0.f, graph_discount = 0.f;
     const float* graph_rcol = nullptr;  // goal columns (NULL) or the alpha rewards
+    const uint32_t* graph_wpart = nullptr;
 };
 
 void batch_cache_free(BatchCache* c) {
@@ -342,7 +874,8 @@ static e2e_status alloc_bufs(const Model* m, BatchBufs& b, int kp) {
         hipMalloc(&b.conv, kp * 4) != hipSuccess || hipMalloc(&b.start, kp * 4) != hipSuccess ||
         hipMalloc(&b.part, (size_t)kBMaxBlocks * kp * 4) != hipSuccess ||
         hipMalloc(&b.last_res, kp * 4) != hipSuccess || hipMalloc(&b.st, sizeof(BatchState)) != hipSuccess ||
-        hipMalloc(&b.vout, n * 4) != hipSuccess || hipMalloc(&b.pout, n * 4) != hipSuccess) {
+        hipMalloc(&b.vout, n * 4) != hipSuccess || hipMalloc(&b.pout, n * 4) != hipSuccess ||
+        hipMalloc(&b.colmax, kp * 4) != hipSuccess) {
         (void)hipGetLastError();
         b.release();
         set_error("batched solve: device allocation failed");
@@ -384,6 +917,7 @@ static BatchArgs make_args(const Model* m, const BatchBufs& b, const e2e_solve_o
     A.nt = (int)m->nt;
     A.kp = b.kp;
     A.nr = m->p.nrzns;
+    A.nap = nap_of((int)m->na);
     A.thresh = o.thresh;
     A.r_term = m->p.r_terminal;
     A.discount = o.discount;
@@ -391,14 +925,75 @@ static BatchArgs make_args(const Model* m, const BatchBufs& b, const e2e_solve_o
     A.Rcol = nullptr;
     A.rows = m->rows;
     A.rcol_n = 1;
+    A.colmax = b.colmax;
+    A.wpart = b.wpart;
     return A;
 }
 
-static dim3 grid_of(const Model* m, const Variant& v, int kp) {
-    const int64_t states_per_block = kBWaves * v.spw;
+// grid.y covers the column blocks that hold columns 0 .. ncols-1
+static dim3 grid_of(const Model* m, const Variant& v, int kp, int ncols) {
+    const int64_t states_per_block = (int64_t)kBWaves * spw_of(v, (int)m->na);
     int64_t gx = ceil_div(m->ns, states_per_block);
     gx = std::min<int64_t>(gx, kBMaxBlocks);  // grid-stride beyond this
-    return dim3((unsigned)gx, (unsigned)(kp / kb_of(v)));
+    const int64_t gy = std::min<int64_t>(kp / kb_of(v), ceil_div((int64_t)std::max(ncols, 1), (int64_t)kb_of(v)));
+    return dim3((unsigned)gx, (unsigned)gy);
+}
+
+// A variant's launch on a model: kernel, block size, grid (grid.y for all kp columns;
+// launches may cut it to the blocks that hold live columns). spmm_vi_stream needs the
+// state partition of its grid, computed here once per (waves, group size).
+struct Plan {
+    BatchFn fn = nullptr;
+    dim3 grid, block;
+    int kb = 0;
+};
+
+static e2e_status plan_of(const Model* m, BatchBufs& b, const Variant& v, hipStream_t s, Plan* p) {
+    p->kb = kb_of(v);
+    if (v.kind != 2) {
+        p->fn = v.fn;
+        p->block = dim3(kBThreads);
+        p->grid = grid_of(m, v, b.kp, b.kp);
+        return E2E_OK;
+    }
+    const int nap = nap_of((int)m->na);
+    p->fn = stream_fn(v.cpl, v.u, nap);
+    E2E_CHECK(p->fn, E2E_ERR_INVALID, "batched sweep: no streaming kernel for %d columns", v.cpl);
+    E2E_CHECK(m->ns * (int64_t)m->na < (1ll << 32), E2E_ERR_CAPACITY,
+              "batched sweep: the streaming kernel indexes rows with 32 bits");
+    int bpc = 0, ncu = 0, dev = 0;
+    E2E_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void*)p->fn, kSThreads, 0));
+    E2E_HIP(hipGetDevice(&dev));
+    E2E_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int spc = 64 / nap;
+    int64_t nblocks = (int64_t)std::max(bpc, 1) * std::max(ncu, 1);
+    nblocks = std::max<int64_t>(1, std::min<int64_t>(nblocks, ceil_div(m->ns, (int64_t)kSWaves * spc)));
+    const int64_t W = nblocks * kSWaves;
+    if (b.wpart_W != W || b.wpart_spc != spc) {
+        if (b.wpart) (void)hipFree(b.wpart);
+        b.wpart = nullptr;
+        b.wpart_W = 0;
+        if (hipMalloc(&b.wpart, sizeof(uint32_t) * (W + 1)) != hipSuccess) {
+            b.wpart = nullptr;
+            (void)hipGetLastError();
+            set_error("batched sweep: device allocation failed");
+            return E2E_ERR_NOMEM;
+        }
+        launch_wave_partition(m->row_ptr, m->ns, (int)m->na, W, spc, b.wpart, s);
+        E2E_HIP(hipGetLastError());
+        b.wpart_W = W;
+        b.wpart_spc = spc;
+    }
+    p->block = dim3(kSThreads);
+    p->grid = dim3((unsigned)nblocks, (unsigned)(b.kp / p->kb));
+    return E2E_OK;
+}
+
+// p's grid cut to the column blocks that hold columns 0 .. ncols-1
+static dim3 plan_grid(const Plan& p, int ncols) {
+    dim3 g = p.grid;
+    g.y = (unsigned)std::min<int64_t>(g.y, ceil_div((int64_t)std::max(ncols, 1), (int64_t)p.kb));
+    return g;
 }
 
 // Goal columns (goals != NULL) or alpha columns (goals == NULL: the model's n_alpha
@@ -464,7 +1059,7 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
         E2E_CHECK(m->Ralpha && m->n_alpha >= 1 && k == m->n_alpha, E2E_ERR_INVALID,
                   "alpha columns need a model built with n_alpha > 0");
         const int v = bo ? bo->variant : -1;
-        const double vblock = (double)m->ns * (double)std::max(16, (int)ceil_div(k, 16) * 16) * 8.0;
+        const double vblock = (double)m->ns * (double)padded_cols(k) * 8.0;
         if (v == -2 || (v == -1 && vblock > 96.0 * 1024 * 1024))
             return vi_solve_alpha_columns(m, o, policy, value, col_sweeps, stats);
     }
@@ -475,7 +1070,7 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
     e2e_batch_opts bopt{};
     bopt.variant = -1;
     if (bo) bopt = *bo;
-    const int kp = std::max(16, (int)ceil_div(k, 16) * 16);
+    const int kp = padded_cols(k);
     if (!m->batch) m->batch = new BatchCache();
     BatchCache* c = m->batch;
     E2E_TRY(ensure_bufs(m, c, kp));
@@ -492,7 +1087,7 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
     const int64_t nV = m->ns * kp;
     auto init = [&]() {
         hipLaunchKernelGGL(batch_init, dim3((unsigned)ceil_div(nV, 256)), dim3(256), 0, s, b.V[0], nV, b.goal,
-                           b.conv, b.start, b.last_res, kp, k, b.st);
+                           b.conv, b.start, b.last_res, b.colmax, kp, k, b.st);
     };
     const int max_sweeps = opts.max_sweeps > 0 ? opts.max_sweeps : (int)m->nt + 2;
     const int spc = opts.sweeps_per_check > 0 ? opts.sweeps_per_check : 8;
@@ -503,7 +1098,7 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
     hipEvent_t e0, e1;
     E2E_HIP(hipEventCreate(&e0));
     E2E_HIP(hipEventCreate(&e1));
-    if (variant < 0 || variant >= kNumVariants || kp % kb_of(kVariants[variant])) {
+    if (variant < 0 || variant >= kNumVariants || !variant_fits(kVariants[variant], (int)m->na, kp)) {
         if (c->tuned_kp == kp && c->tuned_variant >= 0) {
             variant = c->tuned_variant;
         } else {
@@ -512,10 +1107,13 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
             const int trial = std::min(3, max_sweeps);
             for (int vi = 0; vi < kNumVariants; vi++) {
                 const Variant& v = kVariants[vi];
-                if (kp % kb_of(v)) continue;
+                if (!variant_fits(v, (int)m->na, kp)) continue;
+                Plan pl;
+                E2E_TRY(plan_of(m, b, v, s, &pl));
+                A.wpart = b.wpart;
                 init();
                 E2E_HIP(hipEventRecord(e0, s));
-                for (int i = 0; i < trial; i++) hipLaunchKernelGGL(v.fn, grid_of(m, v, kp), dim3(kBThreads), 0, s, A);
+                for (int i = 0; i < trial; i++) hipLaunchKernelGGL(pl.fn, plan_grid(pl, k), pl.block, 0, s, A);
                 E2E_HIP(hipEventRecord(e1, s));
                 E2E_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;
@@ -531,18 +1129,22 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
         }
     }
     const Variant& V = kVariants[variant];
-    const dim3 grid = grid_of(m, V, kp);
+    Plan plan;
+    E2E_TRY(plan_of(m, b, V, s, &plan));
+    A.wpart = b.wpart;
+    const dim3 grid = plan_grid(plan, k);
 
     if (bopt.use_graph &&
-        (!c->graph || c->graph_variant != variant || c->graph_chunk != spc || c->graph_grid != (int)grid.x ||
-         c->graph_thresh != opts.thresh || c->graph_discount != opts.discount || c->graph_rcol != A.Rcol)) {
+        (!c->graph || c->graph_variant != variant || c->graph_chunk != spc || c->graph_grid != (int)grid.x || c->graph_gy != (int)grid.y ||
+         c->graph_thresh != opts.thresh || c->graph_discount != opts.discount || c->graph_rcol != A.Rcol ||
+         c->graph_wpart != A.wpart)) {
         if (c->graph) {
             (void)hipGraphExecDestroy(c->graph);
             c->graph = nullptr;
         }
         hipGraph_t gr;
         E2E_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < spc; i++) hipLaunchKernelGGL(V.fn, grid, dim3(kBThreads), 0, s, A);
+        for (int i = 0; i < spc; i++) hipLaunchKernelGGL(plan.fn, grid, plan.block, 0, s, A);
         E2E_HIP(hipStreamEndCapture(s, &gr));
         hipError_t e = hipGraphInstantiate(&c->graph, gr, nullptr, nullptr, 0);
         (void)hipGraphDestroy(gr);
@@ -554,9 +1156,11 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
         c->graph_variant = variant;
         c->graph_chunk = spc;
         c->graph_grid = (int)grid.x;
+        c->graph_gy = (int)grid.y;
         c->graph_thresh = opts.thresh;
         c->graph_discount = opts.discount;
         c->graph_rcol = A.Rcol;
+        c->graph_wpart = A.wpart;
     }
 
     init();
@@ -570,7 +1174,7 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
         if (bopt.use_graph && n == spc) {
             E2E_HIP(hipGraphLaunch(c->graph, s));
         } else {
-            for (int i = 0; i < n; i++) hipLaunchKernelGGL(V.fn, grid, dim3(kBThreads), 0, s, A);
+            for (int i = 0; i < n; i++) hipLaunchKernelGGL(plan.fn, grid, plan.block, 0, s, A);
             E2E_HIP(hipGetLastError());
         }
         E2E_HIP(hipEventRecord(e1, s));
@@ -606,7 +1210,7 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
         stats->converged = nconv;
         stats->variant = variant;
         stats->cols_per_lane = V.cpl;
-        stats->states_per_wave = V.spw;
+        stats->states_per_wave = spw_of(V, (int)m->na);
         stats->col_block = kb_of(V);
         stats->solve_ms = now_ms() - t_start;
         stats->sweep_kernel_ms = kms;
@@ -676,7 +1280,7 @@ __global__ void live_column(const double* V0, const double* V1, const int32_t* p
 }
 
 __global__ void live_init(double* V0, double* V1, int64_t n, int32_t* goal, int32_t* conv, int32_t* start,
-                          float* last_res, int kp, BatchState* st) {
+                          float* last_res, uint32_t* colmax, int kp, BatchState* st) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         V0[i] = 0.0;
@@ -684,6 +1288,7 @@ __global__ void live_init(double* V0, double* V1, int64_t n, int32_t* goal, int3
     }
     if (i < kp) {
         goal[i] = -1;
+        colmax[i] = 0u;
         conv[i] = 0;  // free slot
         start[i] = 1;
         last_res[i] = 0.0f;
@@ -697,11 +1302,21 @@ __global__ void live_init(double* V0, double* V1, int64_t n, int32_t* goal, int3
 
 }  // namespace
 
+// Live batches pick the sweep kernel by the width a sweep has to cover (highest live
+// slot + 1); all kernels share the layout and the sweep state, so the choice may change
+// from sweep to sweep. Measured on a 150x150 x 125 x 8-heading model (1.8e8 entries):
+// 2 columns 0.94 ms (spmm_vi_stream, 2 per block), 4 columns 1.57 ms (spmm_vi_rows<1,4,4>),
+// 16 columns 3.54 ms (spmm_vi_rows<2,8,2>), against 5.6 ms for spmm_vi_sweep<1,4> at any
+// width up to 16. A kernel that does not fit the model's action count is skipped.
+constexpr int kLiveWidths[] = {2, 4, 1 << 30};
+constexpr int kLiveVariants[] = {26, 17, 20};
+constexpr int kLiveFallback = 6;  // spmm_vi_sweep<1, 4>: fits every model
+constexpr int kLiveN = 3;
+
 struct LiveBatch {
     Model* m = nullptr;
     BatchBufs b;
-    int variant = 0;
-    dim3 grid;
+    Plan plans[kLiveN];  // by kLiveWidths
     hipStream_t s = nullptr;
     BatchArgs A{};
     int32_t* d_list = nullptr;  // admission list scratch: [2][kp]
@@ -710,8 +1325,10 @@ struct LiveBatch {
 
 int64_t live_batch_bytes(const Model* m, int kp) {
     // V[2] float64, pi, and the float value / policy staging of a harvest
-    return (int64_t)m->ns * kp * (2 * 8 + 3 * 4) + (int64_t)kp * (40 + 4 * kBMaxBlocks) + 4096;
+    return (int64_t)m->ns * kp * (2 * 8 + 3 * 4) + (int64_t)kp * (44 + 4 * kBMaxBlocks) + 65536;
 }
+
+void live_free(LiveBatch* L);
 
 e2e_status live_create(Model* m, int kp, float thresh, hipStream_t s, LiveBatch** out) {
     E2E_CHECK(m->built && m->Rsum && m->p.term_subgrid_size == 0 && !m->has_obstacles, E2E_ERR_INVALID,
@@ -730,16 +1347,18 @@ e2e_status live_create(Model* m, int kp, float thresh, hipStream_t s, LiveBatch*
     e2e_solve_opts o{};
     o.thresh = thresh;
     o.discount = 1.0f;
-    L->A = make_args(m, L->b, o);
-    // variant: the model's tuned choice for this width if one exists, else columns across
-    // the whole wave (one state per wave)
-    // 16 columns per block (one lane per column, 4 states per wave), so that a sweep
-    // launches only the column blocks that hold live columns (live_step)
-    L->variant = 6;
-    L->grid = grid_of(m, kVariants[L->variant], kp);
+    for (int i = 0; i < kLiveN && st == E2E_OK; i++) {
+        const int v = variant_fits(kVariants[kLiveVariants[i]], (int)m->na, kp) ? kLiveVariants[i] : kLiveFallback;
+        st = plan_of(m, L->b, kVariants[v], s, &L->plans[i]);
+    }
+    if (st != E2E_OK) {
+        live_free(L);
+        return st;
+    }
+    L->A = make_args(m, L->b, o);  // after plan_of: the streaming kernel's partition
     const int64_t n = m->ns * kp;
     hipLaunchKernelGGL(live_init, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, L->b.V[0], L->b.V[1], n,
-                       L->b.goal, L->b.conv, L->b.start, L->b.last_res, kp, L->b.st);
+                       L->b.goal, L->b.conv, L->b.start, L->b.last_res, L->b.colmax, kp, L->b.st);
     E2E_HIP(hipGetLastError());
     L->bytes = live_batch_bytes(m, kp);
     *out = L;
@@ -780,11 +1399,11 @@ e2e_status live_admit(LiveBatch* L, const int32_t* cols, const int32_t* goals, i
 // slot + 1): free slots beyond are not computed. The residual and convergence logic
 // of the sweep only acts on running columns, all of which are in the launched blocks.
 e2e_status live_step(LiveBatch* L, int nsweeps, int ncols) {
-    dim3 grid = L->grid;
-    const int kb = kb_of(kVariants[L->variant]);
-    grid.y = (unsigned)std::min<int64_t>(grid.y, std::max<int64_t>(1, ceil_div((int64_t)ncols, (int64_t)kb)));
-    for (int i = 0; i < nsweeps; i++)
-        hipLaunchKernelGGL(kVariants[L->variant].fn, grid, dim3(kBThreads), 0, L->s, L->A);
+    int i = 0;
+    while (i < kLiveN - 1 && ncols > kLiveWidths[i]) i++;
+    const Plan& p = L->plans[i];
+    const dim3 grid = plan_grid(p, ncols);
+    for (int n = 0; n < nsweeps; n++) hipLaunchKernelGGL(p.fn, grid, p.block, 0, L->s, L->A);
     E2E_HIP(hipGetLastError());
     return E2E_OK;
 }

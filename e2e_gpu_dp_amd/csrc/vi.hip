@@ -28,6 +28,7 @@
 #include <cstring>
 
 #include "internal.h"
+#include "lanes.h"
 
 namespace e2e {
 
@@ -300,25 +301,6 @@ using CPtr = const __attribute__((address_space(4))) T*;
 template <typename T>
 __device__ __forceinline__ T ld_off(const T* __restrict__ p, uint32_t i) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + (i * (uint32_t)sizeof(T)));
-}
-
-// lane l takes lane l-n of its row of 16 (row_shr:n); lanes l < n of the row keep `id`
-__device__ __forceinline__ int dpp_row_shr(int v, int n, int id) {
-    switch (n) {
-        case 1: return __builtin_amdgcn_update_dpp(id, v, 0x111, 0xf, 0xf, false);
-        case 2: return __builtin_amdgcn_update_dpp(id, v, 0x112, 0xf, 0xf, false);
-        case 4: return __builtin_amdgcn_update_dpp(id, v, 0x114, 0xf, 0xf, false);
-        default: return __builtin_amdgcn_update_dpp(id, v, 0x118, 0xf, 0xf, false);
-    }
-}
-__device__ __forceinline__ float dpp_row_shr(float v, int n, float id) {
-    return __int_as_float(dpp_row_shr(__float_as_int(v), n, __float_as_int(id)));
-}
-__device__ __forceinline__ double dpp_row_shr(double v, int n, double id) {
-    const long long b = __double_as_longlong(v), ib = __double_as_longlong(id);
-    const int lo = dpp_row_shr((int)(b & 0xffffffffll), n, (int)(ib & 0xffffffffll));
-    const int hi = dpp_row_shr((int)(b >> 32), n, (int)(ib >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // float residual that is never below the double one r >= 0 (so res < thresh in float
@@ -1485,6 +1467,14 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
         stats->launches = launched;
     }
     return rc;
+}
+
+// wpart[0..W] of the lean state partition (vi_partition) for W waves and groups of spc
+// states, on stream s; shared with the batched streaming sweep
+void launch_wave_partition(const uint32_t* rp, int64_t ns, int na, int64_t W, int spc, uint32_t* wpart,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(vi_partition, dim3((unsigned)ceil_div(W + 1, 256)), dim3(256), 0, s, rp, ns, na, W, spc,
+                       wpart);
 }
 
 }  // namespace e2e

@@ -62,7 +62,7 @@ def test_batched_columns_independent_of_batch_and_variant(gpu):
     goals = goal_cells(14, 40, 9)
     with _shared(w, f) as m:
         ref_pol, ref_val, ref_sw, st = m.solve_goals(goals, thresh=1e-6, variant=0)
-        for variant in range(1, 9):
+        for variant in range(1, N_VARIANTS):
             pol, val, sw, st = m.solve_goals(goals, thresh=1e-6, variant=variant)
             assert np.array_equal(val.view(np.uint32), ref_val.view(np.uint32)), variant
             assert np.array_equal(pol, ref_pol) and np.array_equal(sw, ref_sw)
@@ -74,6 +74,57 @@ def test_batched_columns_independent_of_batch_and_variant(gpu):
         # graph-replayed sweeps
         polg, valg, swg, _ = m.solve_goals(goals, thresh=1e-6, use_graph=True, sweeps_per_check=4)
         assert np.array_equal(valg.view(np.uint32), ref_val.view(np.uint32))
+
+
+# SpMM variants 0-8: one lane group per state walks every row; 9-20: one lane group per
+# row (spmm_vi_rows), as (column lanes, columns per lane)
+N_VARIANTS = 27
+ROWS_VARIANTS = {9: (4, 1), 10: (8, 1), 11: (4, 2), 12: (8, 2), 13: (4, 4), 14: (16, 1), 15: (8, 4),
+                 16: (16, 2), 17: (1, 4), 18: (1, 8), 19: (2, 4), 20: (2, 8)}
+
+
+# column block of variants 0-8 (the state-parallel kernel fits any action count) and of
+# 21-26 (spmm_vi_stream: one lane per row, transposed gathers; any action count)
+STATE_VARIANTS_KB = {0: 64, 1: 64, 2: 64, 3: 32, 4: 32, 5: 32, 6: 16, 7: 16, 8: 16}
+STREAM_VARIANTS_KB = {21: 4, 22: 4, 23: 8, 24: 8, 25: 1, 26: 2}
+
+
+@pytest.mark.parametrize("na,k", [(8, 5), (8, 16), (16, 40), (12, 32), (4, 64), (16, 3)])
+def test_batched_row_kernels_bit_identical(gpu, na, k):
+    """Every variant that fits the model (row variants: column lanes x actions rounded up
+    to a power of two <= 64; all: column block dividing the padded width, 4 or 8 columns
+    for k <= 4 or 8, else a multiple of 16) runs as asked and gives the same bits; na = 12
+    leaves padding action lanes."""
+    g = 12
+    w = small(g=g, nt=9, nr=300, nm=3, na=na, seed=91 + na)
+    f = w.fields()
+    goals = goal_cells(g, k, 17 + k)
+    kp = 4 if k <= 4 else 8 if k <= 8 else -(-k // 16) * 16
+    nap = 1 << (na - 1).bit_length()
+    with _shared(w, f) as m:
+        ref_pol, ref_val, ref_sw, ref_st = m.solve_goals(goals, thresh=1e-6, variant=9)
+        assert ref_st["variant"] == 9
+        ran = set()
+        for variant in range(N_VARIANTS):
+            if variant in ROWS_VARIANTS:
+                lc, cpl = ROWS_VARIANTS[variant]
+                fits = lc * nap <= 64 and kp % (lc * cpl) == 0
+            elif variant in STREAM_VARIANTS_KB:
+                fits = kp % STREAM_VARIANTS_KB[variant] == 0
+            else:
+                fits = kp % STATE_VARIANTS_KB[variant] == 0
+            pol, val, sw, st = m.solve_goals(goals, thresh=1e-6, variant=variant)
+            print(f"na={na} k={k} variant {variant} fits={fits} ran={st['variant']} "
+                  f"spw={st['states_per_wave']}", flush=True)
+            if fits:
+                assert st["variant"] == variant
+                ran.add(variant)
+            assert np.array_equal(val.view(np.uint32), ref_val.view(np.uint32)), variant
+            assert np.array_equal(pol, ref_pol) and np.array_equal(sw, ref_sw), variant
+        assert len(ran & set(ROWS_VARIANTS)) >= 2
+        assert len(ran & set(STREAM_VARIANTS_KB)) >= 2
+        if kp >= 16:
+            assert ran & set(STATE_VARIANTS_KB)
 
 
 def test_batched_rejects_models_it_cannot_serve(gpu):
