@@ -939,6 +939,33 @@ static dim3 grid_of(const Model* m, const Variant& v, int kp, int ncols) {
     return dim3((unsigned)gx, (unsigned)gy);
 }
 
+// Kernel by the width a sweep has to cover (live batches: highest live slot + 1; large
+// batched solves: the column count); all kernels share the layout and the sweep state, so the choice may change
+// from sweep to sweep. Measured on a 150x150 x 125 x 8-heading model (1.8e8 entries):
+// 2 columns 0.94 ms (spmm_vi_stream, 2 per block), 4 columns 1.57 ms (spmm_vi_rows<1,4,4>),
+// 16 columns 3.54 ms (spmm_vi_rows<2,8,2>), against 5.6 ms for spmm_vi_sweep<1,4> at any
+// width up to 16. A kernel that does not fit the model's action count is skipped.
+constexpr int kLiveWidths[] = {2, 4, 1 << 30};
+constexpr int kLiveVariants[] = {26, 17, 20};
+constexpr int kLiveFallback = 6;  // spmm_vi_sweep<1, 4>: fits every model
+constexpr int kLiveN = 3;
+
+// index into kLiveWidths for a sweep over columns 0 .. ncols-1
+int width_class(int ncols) {
+    int i = 0;
+    while (i < kLiveN - 1 && ncols > kLiveWidths[i]) i++;
+    return i;
+}
+
+// the variant of width class i on this model (the fallback where it does not fit)
+int width_variant(const Model* m, int kp, int i) {
+    return variant_fits(kVariants[kLiveVariants[i]], (int)m->na, kp) ? kLiveVariants[i] : kLiveFallback;
+}
+
+// batched solves on models of at least this many entries and at most 16 columns take the
+// width choice instead of timing every variant (the timing would cost more than it saves)
+constexpr int64_t kTuneMaxNnz = 16ll << 20;
+
 // A variant's launch on a model: kernel, block size, grid (grid.y for all kp columns;
 // launches may cut it to the blocks that hold live columns). spmm_vi_stream needs the
 // state partition of its grid, computed here once per (waves, group size).
@@ -1058,10 +1085,7 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
     } else {
         E2E_CHECK(m->Ralpha && m->n_alpha >= 1 && k == m->n_alpha, E2E_ERR_INVALID,
                   "alpha columns need a model built with n_alpha > 0");
-        const int v = bo ? bo->variant : -1;
-        const double vblock = (double)m->ns * (double)padded_cols(k) * 8.0;
-        if (v == -2 || (v == -1 && vblock > 96.0 * 1024 * 1024))
-            return vi_solve_alpha_columns(m, o, policy, value, col_sweeps, stats);
+        if (bo && bo->variant == -2) return vi_solve_alpha_columns(m, o, policy, value, col_sweeps, stats);
     }
     e2e_solve_opts opts{};
     opts.thresh = 0.1f;
@@ -1101,6 +1125,8 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
     if (variant < 0 || variant >= kNumVariants || !variant_fits(kVariants[variant], (int)m->na, kp)) {
         if (c->tuned_kp == kp && c->tuned_variant >= 0) {
             variant = c->tuned_variant;
+        } else if (m->nnz >= kTuneMaxNnz && kp <= 16) {
+            variant = width_variant(m, kp, width_class(k));
         } else {
             const double t0 = now_ms();
             float best = 1e30f;
@@ -1302,16 +1328,6 @@ __global__ void live_init(double* V0, double* V1, int64_t n, int32_t* goal, int3
 
 }  // namespace
 
-// Live batches pick the sweep kernel by the width a sweep has to cover (highest live
-// slot + 1); all kernels share the layout and the sweep state, so the choice may change
-// from sweep to sweep. Measured on a 150x150 x 125 x 8-heading model (1.8e8 entries):
-// 2 columns 0.94 ms (spmm_vi_stream, 2 per block), 4 columns 1.57 ms (spmm_vi_rows<1,4,4>),
-// 16 columns 3.54 ms (spmm_vi_rows<2,8,2>), against 5.6 ms for spmm_vi_sweep<1,4> at any
-// width up to 16. A kernel that does not fit the model's action count is skipped.
-constexpr int kLiveWidths[] = {2, 4, 1 << 30};
-constexpr int kLiveVariants[] = {26, 17, 20};
-constexpr int kLiveFallback = 6;  // spmm_vi_sweep<1, 4>: fits every model
-constexpr int kLiveN = 3;
 
 struct LiveBatch {
     Model* m = nullptr;
@@ -1347,10 +1363,7 @@ e2e_status live_create(Model* m, int kp, float thresh, hipStream_t s, LiveBatch*
     e2e_solve_opts o{};
     o.thresh = thresh;
     o.discount = 1.0f;
-    for (int i = 0; i < kLiveN && st == E2E_OK; i++) {
-        const int v = variant_fits(kVariants[kLiveVariants[i]], (int)m->na, kp) ? kLiveVariants[i] : kLiveFallback;
-        st = plan_of(m, L->b, kVariants[v], s, &L->plans[i]);
-    }
+    for (int i = 0; i < kLiveN && st == E2E_OK; i++) st = plan_of(m, L->b, kVariants[width_variant(m, kp, i)], s, &L->plans[i]);
     if (st != E2E_OK) {
         live_free(L);
         return st;
@@ -1399,9 +1412,7 @@ e2e_status live_admit(LiveBatch* L, const int32_t* cols, const int32_t* goals, i
 // slot + 1): free slots beyond are not computed. The residual and convergence logic
 // of the sweep only acts on running columns, all of which are in the launched blocks.
 e2e_status live_step(LiveBatch* L, int nsweeps, int ncols) {
-    int i = 0;
-    while (i < kLiveN - 1 && ncols > kLiveWidths[i]) i++;
-    const Plan& p = L->plans[i];
+    const Plan& p = L->plans[width_class(ncols)];
     const dim3 grid = plan_grid(p, ncols);
     for (int n = 0; n < nsweeps; n++) hipLaunchKernelGGL(p.fn, grid, p.block, 0, L->s, L->A);
     E2E_HIP(hipGetLastError());
