@@ -23,7 +23,7 @@ from ._abi import E2EError, check, lib
 from .configs import Params as PyParams
 
 __all__ = ["Model", "Server", "build_model", "import_coo", "c_params", "device_count", "E2EError",
-           "pack_params", "actions"]
+           "pack_params", "actions", "rollout", "rollout_metrics"]
 
 _PFIELDS = [f[0] for f in _abi.Params._fields_]
 
@@ -247,6 +247,62 @@ def import_coo(ns: int, na: int, S1, S2, val, R, stream: int = 0) -> Model:
     check(lib().e2e_model_import_coo(ns, na, len(S1), S1.ctypes.data, S2.ctypes.data, val.ctypes.data,
                                      R.ctypes.data, stream or None, C.byref(h)))
     return Model(h.value)
+
+
+def rollout(params, fields: dict, policy, starts, interp: bool = True, cf: float = 1.0, cr: float = 1.0,
+            want_xy: bool = False, stream: int = 0):
+    """Policy rollouts on the GPU: plot_exact_trajectory_set_DP's realization loop
+    (Post_processing/plot_e2eGPU_policy.py:436-516) for every start cell in `starts`
+    ((i, j) pairs) and every realization. `policy` is the solver's policy (nt*g*g, e.g.
+    policy.npy). Returns a dict: success (int8 [n]: 1 reached, 0 failed), travel (int32),
+    energy (float64 [n, 3]: energy_cons, energy_col, net_energy_cons), xy (float64
+    [n, nt+1, 2] or None), stats; trajectory n = k*nrzns + r."""
+    keep = []
+    ptrs = {}
+    on_dev = None
+    for k, dt in (("u", np.float32), ("v", np.float32), ("ui", np.float32), ("vi", np.float32),
+                  ("yi", np.float32), ("s", np.float32), ("mask", np.int32)):
+        ptr, dev = _ptr(fields.get(k), keep, dt)
+        ptrs[k] = ptr
+        if ptr is not None:
+            if on_dev is None:
+                on_dev = dev
+            elif on_dev != dev:
+                raise ValueError("fields must be all on the host or all on the device")
+    f = _abi.Fields(**ptrs)
+    cp = c_params(params)
+    pol = np.ascontiguousarray(np.asarray(policy).ravel().astype(np.int64).astype(np.int32))
+    if len(pol) != cp.nt * cp.gsize * cp.gsize:
+        raise ValueError(f"policy has {len(pol)} entries, expected nt*gsize^2 = {cp.nt * cp.gsize ** 2}")
+    st = np.ascontiguousarray(np.asarray(starts, np.int32).reshape(-1, 2))
+    n = len(st) * cp.nrzns
+    succ = np.empty(n, np.int8)
+    trav = np.empty(n, np.int32)
+    en = np.empty((n, 3), np.float64)
+    xy = np.empty((n, cp.nt + 1, 2), np.float64) if want_xy else None
+    o = _abi.RolloutOpts(int(interp), int(bool(on_dev)), float(cf), float(cr), stream or None)
+    stats = _abi.RolloutStats()
+    check(lib().e2e_rollout(C.byref(cp), C.byref(f), pol.ctypes.data, len(st), st.ctypes.data, C.byref(o),
+                            succ.ctypes.data, trav.ctypes.data, en.ctypes.data,
+                            xy.ctypes.data if want_xy else None, C.byref(stats)))
+    return {"success": succ, "travel": trav, "energy": en, "xy": xy,
+            "stats": {k: getattr(stats, k) for k, _ in stats._fields_}}
+
+
+def rollout_metrics(r: dict) -> dict:
+    """get_metrics (Post_processing/plot_e2eGPU_policy.py:776-797) and the statistics
+    write_log puts in log.csv (:799-824): success rate, then min / max / mean of travel
+    time, required energy, collected energy and net energy over the successful
+    trajectories."""
+    ok = r["success"] == 1
+    out = {"success_rate": float(np.count_nonzero(ok)) / len(ok)}
+    for name, col in (("time", r["travel"].astype(np.float64)), ("reqd_energy", r["energy"][:, 0]),
+                      ("coll_energy", r["energy"][:, 1]), ("net_energy", r["energy"][:, 2])):
+        v = col[ok]
+        out[f"{name}_min"] = float(v.min()) if len(v) else float("nan")
+        out[f"{name}_max"] = float(v.max()) if len(v) else float("nan")
+        out[f"{name}_mean"] = float(v.mean()) if len(v) else float("nan")
+    return out
 
 
 REQUEST_STATUS = {0: "queued", 1: "running", 2: "done", 3: "rejected", 4: "failed"}

@@ -110,8 +110,18 @@ class ServerStats(C.Structure):
         [(n, C.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "wall_ms", "solves_per_s")]
 
 
+class RolloutOpts(C.Structure):
+    _fields_ = [("interp", C.c_int32), ("fields_on_device", C.c_int32), ("cf", C.c_double),
+                ("cr", C.c_double), ("stream", C.c_void_p), ("reserved", C.c_int32 * 4)]
+
+
+class RolloutStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("trajectories", "reached", "rejected", "moves")] + \
+        [("kernel_ms", C.c_double)]
+
+
 STRUCTS = [Params, Fields, BuildOpts, ModelInfo, SolveOpts, SolveStats, BatchOpts, BatchStats,
-           ServerOpts, RequestResult, ServerStats]
+           ServerOpts, RequestResult, ServerStats, RolloutOpts, RolloutStats]
 
 VP = C.c_void_p
 P = C.POINTER
@@ -136,6 +146,8 @@ SIGNATURES = {
     "e2e_vi_solve_alphas": (C.c_int, [VP, P(SolveOpts), P(BatchOpts), VP, VP, VP, P(BatchStats)]),
     "e2e_vi_solve_goals": (C.c_int, [VP, VP, C.c_int32, P(SolveOpts), P(BatchOpts), VP, VP, VP,
                                      P(BatchStats)]),
+    "e2e_rollout": (C.c_int, [P(Params), P(Fields), VP, C.c_int32, VP, P(RolloutOpts), VP, VP, VP, VP,
+                              P(RolloutStats)]),
     "e2e_server_create": (C.c_int, [P(ServerOpts), P(VP)]),
     "e2e_server_add_env": (C.c_int, [VP, C.c_int64, P(Params), P(Fields)]),
     "e2e_server_submit": (C.c_int, [VP, C.c_int64, C.c_int32, C.c_int32, P(C.c_uint64)]),

@@ -70,6 +70,9 @@ e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve
 e2e_status vi_solve_alphas(Model* m, const e2e_solve_opts* o, const e2e_batch_opts* bo, uint32_t* policy,
                            float* value, int32_t* col_sweeps, e2e_batch_stats* stats);
 e2e_status model_export_R_alpha(const Model* m, int32_t j, float* R);
+e2e_status rollout(const e2e_params* p, const e2e_fields* f, const int32_t* policy, int32_t n_starts,
+                   const int32_t* starts, const e2e_rollout_opts* o, int8_t* success, int32_t* travel,
+                   double* energy, double* xy, e2e_rollout_stats* stats);
 
 template <typename F>
 static e2e_status guarded(F&& f) {
@@ -129,6 +132,8 @@ int32_t e2e_struct_size(int32_t which) {
         case 8: return (int32_t)sizeof(e2e_server_opts);
         case 9: return (int32_t)sizeof(e2e_request_result);
         case 10: return (int32_t)sizeof(e2e_server_stats);
+        case 11: return (int32_t)sizeof(e2e_rollout_opts);
+        case 12: return (int32_t)sizeof(e2e_rollout_stats);
         default: return -1;
     }
 }
@@ -295,6 +300,12 @@ e2e_status e2e_vi_solve_goals(void* model, const int32_t* goal_cells, int32_t k,
         return vi_solve_goals(static_cast<Model*>(model), goal_cells, k, opts, bopts, policy, value, col_sweeps,
                               stats);
     });
+}
+
+e2e_status e2e_rollout(const e2e_params* p, const e2e_fields* fields, const int32_t* policy, int32_t n_starts,
+                       const int32_t* starts, const e2e_rollout_opts* opts, int8_t* success, int32_t* travel,
+                       double* energy, double* xy, e2e_rollout_stats* stats) {
+    return guarded([&] { return rollout(p, fields, policy, n_starts, starts, opts, success, travel, energy, xy, stats); });
 }
 
 }  // extern "C"

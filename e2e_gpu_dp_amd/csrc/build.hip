@@ -1356,6 +1356,9 @@ e2e_status validate_params(const e2e_params* p) {
 
 }  // namespace
 
+// the build's parameter checks, for the other entry points that take e2e_params
+e2e_status check_params(const e2e_params* p) { return validate_params(p); }
+
 // ------------------------------------------------------------------ build
 
 e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build_opts* o,

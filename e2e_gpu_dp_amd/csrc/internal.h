@@ -147,6 +147,7 @@ inline e2e_status dmalloc(DevBuf& b, size_t bytes) {
     return E2E_OK;
 }
 
+e2e_status check_params(const e2e_params* p);  // the build's parameter checks
 e2e_status model_alloc_stream(Model* m, void* stream);
 // allocates m->nz with nnz entries plus a zeroed tail of kNzPad entries (on m->stream)
 e2e_status alloc_nz(Model* m, int64_t nnz);

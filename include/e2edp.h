@@ -315,6 +315,42 @@ e2e_status e2e_vi_solve_alphas(void* model, const e2e_solve_opts* opts, const e2
                                uint32_t* policy, float* value, int32_t* col_sweeps,
                                e2e_batch_stats* stats);
 
+/* Policy rollouts: the post-processing's trajectory simulation and metrics
+ * (Post_processing/plot_e2eGPU_policy.py:376-520 plot_exact_trajectory_set_DP, its
+ * vehicle step Post_processing/grid_world.py:160-230 Grid.move_exact, get_metrics
+ * :776-797), one GPU lane per trajectory. */
+typedef struct e2e_rollout_opts {
+    int32_t interp;            /* 1: actions after the first bilinearly interpolated from the four
+                                  nearest cells' actions (get_interpolated_action :174-198; the
+                                  run the reference logs, :949-952); 0: the cell's own action */
+    int32_t fields_on_device;  /* 1: e2e_fields pointers are device pointers */
+    double cf, cr;             /* energy coefficients Cf, Cr (:427-428); 0: 1 */
+    void* stream;              /* hipStream_t to run on; NULL: a stream of the call's own */
+    int32_t reserved[4];
+} e2e_rollout_opts;
+
+typedef struct e2e_rollout_stats {
+    int64_t trajectories;      /* n_starts x nrzns */
+    int64_t reached;           /* trajectories that ended in the terminal subgrid */
+    int64_t rejected;          /* interpolations at a support pattern the reference rejects
+                                  (:158-161); the call then returns E2E_ERR_INVALID */
+    int64_t moves;             /* vehicle steps over all trajectories */
+    double kernel_ms;          /* the rollout kernel (HIP events on its stream) */
+} e2e_rollout_stats;
+
+/* Replaces the realization loop of plot_exact_trajectory_set_DP (:436-516) for any number
+ * of start cells. policy: host int32 [nt][gsize][gsize] (policy.npy; -1 takes the last
+ * action, as a Python list index does). starts: host int32 [n_starts][2] = (row i,
+ * column j). Trajectory n = k*nrzns + r starts at start k under realization r. Outputs
+ * (host, caller-owned): success [n] (1: reached the terminal subgrid, 0: left the grid
+ * interior, entered an obstacle or ran out of time, -1: rejected interpolation), travel
+ * [n] (moves), energy [n][3] (energy_cons, energy_col, net_energy_cons), xy
+ * [n][nt+1][2] positions per step or NULL. opts may be NULL (interp 1). */
+e2e_status e2e_rollout(const e2e_params* p, const e2e_fields* fields, const int32_t* policy,
+                       int32_t n_starts, const int32_t* starts, const e2e_rollout_opts* opts,
+                       int8_t* success, int32_t* travel, double* energy, double* xy,
+                       e2e_rollout_stats* stats);
+
 /* Continuous-batching planning server (no counterpart in the reference, whose pipeline
  * is one build + one solve per goal: src/Makefile:7-19). Environments (fields + params,
  * keyed by env_id, e.g. the field seed) are registered once; requests (env, goal cell,

@@ -52,6 +52,10 @@ def lib():
                                         C.c_int, C.c_int, VP, VP, VP]
         L.orc_bellman_residual_f64.argtypes = [i64, i64, i64, VP, VP, VP, VP, C.c_double, VP]
         L.orc_bellman_residual_f64.restype = C.c_double
+        L.orc_rollout.argtypes = [VP, VP, VP, C.c_int, VP, C.c_int, C.c_double, C.c_double, C.c_int,
+                                  VP, VP, VP, VP]
+        L.orc_rollout.restype = C.c_int
+        L.orc_rollout_actions.argtypes = [VP, VP, VP, VP, VP]
         L.orc_set_mutation.argtypes = [C.c_int]
         L.orc_max_threads.restype = C.c_int
         _lib = L
@@ -155,6 +159,38 @@ def build(params, fields, t_begin=0, t_end=None, nthreads=0):
         return out
     finally:
         lib().orc_model_free(m)
+
+
+def rollout_actions(params):
+    """speed, angle, speed*cos(angle), speed*sin(angle) of the post-processing's action
+    table (float64)."""
+    p = _cp(params)
+    na = p.num_ac_speeds * p.num_ac_angles
+    out = [np.zeros(na, np.float64) for _ in range(4)]
+    lib().orc_rollout_actions(C.byref(p), *[o.ctypes.data for o in out])
+    return out
+
+
+def rollout(params, fields, policy, starts, interp=True, Cf=1.0, Cr=1.0, nthreads=0, want_xy=False):
+    """Trajectories n = k*nrzns + r from start cells starts[k] = (i, j): dict with success
+    (int8: 1, 0, -1 rejected interpolation), travel (int32), energy (float64 [n, 3]:
+    energy_cons, energy_col, net_energy_cons), xy (float64 [n, nt+1, 2] or None), bad."""
+    p = _cp(params)
+    h = _FieldHold(fields)
+    pol = np.ascontiguousarray(np.asarray(policy).ravel().astype(np.int64).astype(np.int32))
+    st = np.ascontiguousarray(np.asarray(starts, np.int32).reshape(-1, 2))
+    n = len(st) * p.nrzns
+    out = {"success": np.empty(n, np.int8), "travel": np.empty(n, np.int32),
+           "energy": np.empty((n, 3), np.float64),
+           "xy": np.empty((n, p.nt + 1, 2), np.float64) if want_xy else None}
+    bad = lib().orc_rollout(C.byref(p), C.byref(h.f), pol.ctypes.data, len(st), st.ctypes.data, int(interp),
+                            float(Cf), float(Cr), nthreads, out["success"].ctypes.data,
+                            out["travel"].ctypes.data, out["energy"].ctypes.data,
+                            out["xy"].ctypes.data if want_xy else None)
+    if bad < 0:
+        raise ValueError("oracle rollout: bad parameters")
+    out["bad"] = bad
+    return out
 
 
 def _coo(S1, S2, val, R):

@@ -16,6 +16,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -714,4 +715,291 @@ double orc_bellman_residual_f64(int64_t ns, int64_t na, int64_t nnz, const int64
     free(Vin);
     free(Vout);
     return res;
+}
+
+/* ---------------------------------------------------------------- rollouts */
+/* The post-processing's trajectory simulation, Post_processing/plot_e2eGPU_policy.py
+ * and Post_processing/grid_world.py. Python float arithmetic is float64; the float32
+ * inputs (fields, speeds, grid ticks) are widened where they enter it. */
+
+/* Python's a % b and a // b for floats (b > 0): the sign of the remainder follows
+ * the divisor, and the quotient is the floor of (a - a%b)/b rounded to an integer. */
+static double py_mod(double a, double b) {
+    double m = fmod(a, b);
+    if (m != 0.0) {
+        if ((b < 0) != (m < 0)) m += b;
+    } else {
+        m = copysign(0.0, b);
+    }
+    return m;
+}
+
+static double py_floordiv(double a, double b) {
+    double m = fmod(a, b);
+    double d = (a - m) / b;
+    if (m != 0.0 && ((b < 0) != (m < 0))) d -= 1.0;
+    if (d != 0.0) {
+        double fd = floor(d);
+        if (d - fd > 0.5) fd += 1.0;
+        return fd;
+    }
+    return copysign(0.0, a / b);
+}
+
+/* round(x, 14) (grid_world.py:352): the double nearest the decimal string of x
+ * correctly rounded to 14 places. */
+static double py_round14(double x) {
+    char buf[64];
+    snprintf(buf, sizeof buf, "%.14f", x);
+    return strtod(buf, NULL);
+}
+
+void orc_rollout_actions(const e2e_params* p, double* speed, double* angle, double* ax, double* ay) {
+    int ns = p->num_ac_speeds, na = p->num_ac_angles;
+    for (int i = 0; i < ns; i++) {
+        /* populate_ac_speeds :322-337 (float32: Fmax is the float32 prob_params entry) */
+        float sp = ns == 1 ? p->F : (float)(i + 1) * (p->F / (float)ns);
+        for (int j = 0; j < na; j++) {
+            double an = py_round14((double)(j * 2) * M_PI / (double)na); /* :351-352 */
+            int idx = j + na * i; /* itertools.product(speed_list, angle_list) :356 */
+            if (speed) speed[idx] = (double)sp;
+            if (angle) angle[idx] = an;
+            if (ax) ax[idx] = (double)sp * cos(an);
+            if (ay) ay[idx] = (double)sp * sin(an);
+        }
+    }
+}
+
+typedef struct ro_ctx {
+    const e2e_params* p;
+    const e2e_fields* f;
+    const int32_t* policy;
+    int g, nc, nm, nr, nt, na;
+    double xs[4096], ys[4096];
+    double dj, di;
+    const double *speed, *ax, *ay;
+} ro_ctx;
+
+/* is_terminal() :140-157 without a query: the terminal subgrid */
+static int ro_terminal(const e2e_params* p, int i, int j) {
+    int it = p->i_term, jt = p->j_term, ts = p->term_subgrid_size;
+    return i >= it && i < it + ts && j >= jt && j < jt + ts;
+}
+
+/* g.actions[policy_1d[s1_id]] (plot_e2eGPU_policy.py:236-239, s1_id :27-29); a policy
+ * of -1 indexes the last action, as a Python list does */
+static int ro_action(const ro_ctx* c, int t, int i, int j) {
+    int a = c->policy[(long long)t * c->nc + (long long)i * c->g + j];
+    return a < 0 ? a + c->na : a;
+}
+
+/* extract_velocity in the build's float32 order (the mode sum, then the mean) */
+static void ro_velocity(const ro_ctx* c, int t, int i, int j, int rzn, double* vx, double* vy) {
+    const e2e_fields* f = c->f;
+    long long nc = c->nc, mean_id = (long long)t * nc + (long long)i * c->g + j;
+    long long uvi = (long long)t * c->nm * nc + (long long)i * c->g + j;
+    long long yi = ((long long)t * c->nr + rzn) * c->nm;
+    float sx = 0, sy = 0;
+    for (int k = 0; k < c->nm; k++) sx += f->ui[uvi + k * nc] * f->yi[yi + k];
+    for (int k = 0; k < c->nm; k++) sy += f->vi[uvi + k * nc] * f->yi[yi + k];
+    *vx = (double)(f->u[mean_id] + sx);
+    *vy = (double)(f->v[mean_id] + sy);
+}
+
+/* get_interpolated_action :174-198 + interpolate_action_angles :113-171 +
+ * interpolate_as_per_term_status :84-110. Returns -1 for the support patterns the
+ * reference rejects (:158-161). */
+static int ro_interp(const ro_ctx* c, int t, int i, int j, double x, double y, double* v_out,
+                     double* th_out) {
+    const e2e_params* p = c->p;
+    int g = c->g;
+    double x0 = c->xs[j], y0 = c->ys[g - 1 - i];
+    double delx = x - x0, dely = y - y0;
+    int fx = delx < 0 ? -1 : 1, fy = dely < 0 ? -1 : 1; /* f() :31-35 */
+    int si[4] = {i, i - fy, i, i - fy}, sj[4] = {j, j, j + fx, j + fx};
+    int term[4];
+    double vx[4], vy[4];
+    for (int k = 0; k < 4; k++) {
+        term[k] = si[k] == p->i_term && sj[k] == p->j_term; /* is_terminal(query_s) :154-156 */
+        int a = ro_action(c, t, si[k], sj[k]);
+        vx[k] = c->ax[a];
+        vy[k] = c->ay[a];
+    }
+    double x1 = c->xs[sj[0]], y1 = c->ys[g - 1 - si[0]];
+    double x2 = c->xs[sj[3]], y2 = c->ys[g - 1 - si[3]];
+    int tri = 0;
+    if (!term[0] && !term[1] && !term[2] && !term[3]) {
+    } else if (!term[0] && !term[1] && term[2]) { /* [0,0,1,1], [0,0,1,0] */
+        vx[2] = vx[0]; vx[3] = vx[1];
+        vy[2] = vy[0]; vy[3] = vy[1];
+    } else if (!term[0] && term[1] && !term[2]) { /* [0,1,0,1], [0,1,0,0] */
+        vx[1] = vx[0]; vx[3] = vx[2];
+        vy[1] = vy[0]; vy[3] = vy[2];
+    } else if (!term[0] && !term[1] && !term[2] && term[3]) { /* [0,0,0,1] */
+        tri = 1;
+    } else {
+        return -1;
+    }
+    double ivx, ivy;
+    if (!tri) {
+        /* bilinear_interpolate :44-63: denom * x_vec @ (val_mat @ y_vec) */
+        double denom = 1.0 / ((x2 - x1) * (y2 - y1));
+        double r0x = vx[0] * (y2 - y) + vx[1] * (y - y1), r1x = vx[2] * (y2 - y) + vx[3] * (y - y1);
+        double r0y = vy[0] * (y2 - y) + vy[1] * (y - y1), r1y = vy[2] * (y2 - y) + vy[3] * (y - y1);
+        ivx = denom * ((x2 - x) * r0x + (x - x1) * r1x);
+        ivy = denom * ((x2 - x) * r0y + (x - x1) * r1y);
+    } else {
+        /* traingle_interpolate :67-79 on (x1,y1), (x1,y2), (x2,y1) */
+        double X1 = x1, Y1 = y1, X2 = x1, Y2 = y2, X3 = x2, Y3 = y1;
+        double den = (Y2 - Y3) * (X1 - X3) + (X3 - X2) * (Y1 - Y3);
+        double W0 = ((Y2 - Y3) * (x - X3) + (X3 - X2) * (y - Y3)) / den;
+        double W1 = ((Y3 - Y1) * (x - X3) + (X1 - X3) * (y - Y3)) / den;
+        double W2 = 1 - W0 - W1;
+        ivx = vx[0] * W0 + vx[1] * W1 + vx[2] * W2;
+        ivy = vy[0] * W0 + vy[1] * W1 + vy[2] * W2;
+    }
+    *th_out = py_mod(atan2(ivy, ivx) + 2 * M_PI, 2 * M_PI); /* get_angle_in_0_2pi */
+    *v_out = sqrt(ivx * ivx + ivy * ivy);
+    return 0;
+}
+
+/* One trajectory: plot_exact_trajectory_set_DP's realization loop body :436-516 */
+static int ro_one(const ro_ctx* c, int si0, int sj0, int rzn, int interp, double Cf, double Cr,
+                  int8_t* succ_out, int32_t* travel_out, double* en, double* xy) {
+    const e2e_params* p = c->p;
+    const e2e_fields* f = c->f;
+    int g = c->g, nt = c->nt;
+    int t = 0, i = si0, j = sj0;
+    double x = c->xs[j], y = c->ys[g - 1 - i]; /* set_state :119-129 */
+    int a = ro_action(c, t, i, j);
+    double thrust = c->speed[a], cth = c->ax[a], sth = c->ay[a]; /* cos/sin products of the table */
+    int table = 1;
+    double ang = 0.0;
+    int succ = 2; /* None */
+    int travel = 0, bad = 0;
+    double e_cons = 0, e_col = 0, e_net = 0;
+    if (xy) {
+        xy[0] = x;
+        xy[1] = y;
+    }
+    for (int it = 0; it < nt; it++) {
+        if (succ == 2) {
+            double Vx, Vy;
+            ro_velocity(c, t, i, j, rzn, &Vx, &Vy);
+            double rad1 = f->s ? (double)f->s[(long long)t * c->nc + (long long)i * g + j] : 0.0;
+            /* move_exact :160-230 (acts only while not terminal and t < nt-1) */
+            if (!ro_terminal(p, i, j) && t < nt - 1) {
+                double vnetx = (table ? cth : thrust * cos(ang)) + Vx;
+                double vnety = (table ? sth : thrust * sin(ang)) + Vy;
+                double xnew = x + (vnetx * (double)p->dt), ynew = y + (vnety * (double)p->dt);
+                if (xnew > c->xs[g - 1]) xnew = c->xs[g - 1];
+                else if (xnew < c->xs[0]) xnew = c->xs[0];
+                if (ynew > c->ys[g - 1]) ynew = c->ys[g - 1];
+                else if (ynew < c->ys[0]) ynew = c->ys[0];
+                x = xnew;
+                y = ynew;
+                double remx = py_mod(xnew - c->xs[0], c->dj), remy = py_mod(-(ynew - c->ys[g - 1]), c->di);
+                double xind = py_floordiv(xnew - c->xs[0], c->dj), yind = py_floordiv(-(ynew - c->ys[g - 1]), c->di);
+                if (remx >= 0.5 * c->dj && remy >= 0.5 * c->di) {
+                    xind += 1;
+                    yind += 1;
+                } else if (remx >= 0.5 * c->dj && remy < 0.5 * c->di) {
+                    xind += 1;
+                } else if (remx < 0.5 * c->dj && remy >= 0.5 * c->di) {
+                    yind += 1;
+                }
+                if (!(isnan(xind) || isnan(yind))) {
+                    /* the snap stays on the ticks; the clamp only guards the indexing */
+                    i = (int)yind < 0 ? 0 : (int)yind > g - 1 ? g - 1 : (int)yind;
+                    j = (int)xind < 0 ? 0 : (int)xind > g - 1 ? g - 1 : (int)xind;
+                }
+                t += 1;
+            }
+            double rad2 = f->s ? (double)f->s[(long long)t * c->nc + (long long)i * g + j] : 0.0;
+            travel += 1;
+            double rr = rad2 + rad1;
+            e_cons += (Cf * thrust) * (Cf * thrust);
+            e_col += (Cr * rr) / 2;
+            e_net += (Cf * (thrust * thrust)) - ((Cr * rr) / 2);
+        }
+        if (xy) {
+            xy[2 * (it + 1)] = x;
+            xy[2 * (it + 1) + 1] = y;
+        }
+        int in_obs = f->mask && f->mask[(long long)t * c->nc + (long long)i * g + j] == 1;
+        if (i == 0 || i == g - 1 || j == 0 || j == g - 1 || in_obs) succ = 0; /* :473-474 */
+        if (!ro_terminal(p, i, j) && t < nt - 1 && succ == 2) {
+            if (interp) {
+                if (ro_interp(c, t, i, j, x, y, &thrust, &ang) < 0) {
+                    bad = 1;
+                    succ = -1;
+                    for (int k = it + 1; xy && k < nt; k++) {
+                        xy[2 * (k + 1)] = x;
+                        xy[2 * (k + 1) + 1] = y;
+                    }
+                    break;
+                }
+                table = 0;
+            } else {
+                a = ro_action(c, t, i, j);
+                thrust = c->speed[a];
+                cth = c->ax[a];
+                sth = c->ay[a];
+                table = 1;
+            }
+        } else if (ro_terminal(p, i, j)) {
+            succ = 1;
+        } else if (succ != -1) {
+            succ = 0;
+        }
+    }
+    *succ_out = (int8_t)succ;
+    *travel_out = travel;
+    en[0] = e_cons;
+    en[1] = e_col;
+    en[2] = e_net;
+    return bad;
+}
+
+int orc_rollout(const e2e_params* p, const e2e_fields* f, const int32_t* policy, int n_starts,
+                const int32_t* starts, int interp, double Cf, double Cr, int nthreads,
+                int8_t* success, int32_t* travel, double* energy, double* xy) {
+    int g = p->gsize;
+    if (g < 2 || g > 4096) return -1;
+    ro_ctx* c = (ro_ctx*)malloc(sizeof(ro_ctx));
+    int na = p->num_ac_speeds * p->num_ac_angles;
+    double* tab = (double*)malloc(sizeof(double) * 3 * na);
+    orc_rollout_actions(p, tab, NULL, tab + na, tab + 2 * na);
+    c->p = p;
+    c->f = f;
+    c->policy = policy;
+    c->g = g;
+    c->nc = g * g;
+    c->nm = p->nmodes;
+    c->nr = p->nrzns;
+    c->nt = p->nt;
+    c->na = na;
+    c->speed = tab;
+    c->ax = tab + na;
+    c->ay = tab + 2 * na;
+    /* Test_grid ticks (Post_processing/utils/setup_grid.py:107-108): (dx/2) + (i*dx) in
+     * float32, dj = |xs[1] - xs[0]| (grid_world.py:16-17) */
+    for (int k = 0; k < g; k++) {
+        c->xs[k] = (double)(p->dx / 2 + (float)k * p->dx);
+        c->ys[k] = (double)(p->dy / 2 + (float)k * p->dy);
+    }
+    c->dj = (double)fabsf((float)c->xs[1] - (float)c->xs[0]);
+    c->di = (double)fabsf((float)c->ys[1] - (float)c->ys[0]);
+    long long n = (long long)n_starts * p->nrzns;
+    int bad = 0;
+    if (nthreads <= 0) nthreads = orc_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 64) reduction(+ : bad)
+    for (long long q = 0; q < n; q++) {
+        int k = (int)(q / p->nrzns), r = (int)(q % p->nrzns);
+        bad += ro_one(c, starts[2 * k], starts[2 * k + 1], r, interp, Cf, Cr, success + q, travel + q,
+                      energy + 3 * q, xy ? xy + q * 2 * (p->nt + 1) : NULL);
+    }
+    free(tab);
+    free(c);
+    return bad;
 }

@@ -73,6 +73,25 @@ int orc_vi_sweeps_f32(int64_t ns, int64_t na, int64_t nnz, const int64_t* S1,
                       float discount, float thresh, int max_sweeps, int nthreads,
                       float* V, int32_t* pi, float* last_residual);
 
+/* Policy rollouts: the post-processing's trajectory simulation
+ * (Post_processing/plot_e2eGPU_policy.py:376-520 plot_exact_trajectory_set_DP, with
+ * Grid.move_exact Post_processing/grid_world.py:160-230). Trajectory n = start k x
+ * realization r (n = k*nrzns + r) starts at starts[2k], starts[2k+1] (row i, column j).
+ * interp 1: actions after the first bilinearly interpolated (get_interpolated_action
+ * :174-198, the logged run :949-952); 0: the grid policy. Outputs per trajectory:
+ * success (1 reached the terminal subgrid, 0 not), travel (moves), energy[3n..3n+2]
+ * (energy_cons, energy_col, net_energy_cons), xy[(n*(nt+1) + step)*2 + {0,1}] (positions,
+ * NULL to skip; a position repeats once the trajectory has ended). Arithmetic in float64
+ * on the float32 inputs widened. Returns the number of trajectories whose interpolation
+ * met a support pattern the reference rejects (:158-161; success then -1), or -1. */
+int orc_rollout(const e2e_params* p, const e2e_fields* f, const int32_t* policy, int n_starts,
+                const int32_t* starts, int interp, double Cf, double Cr, int nthreads,
+                int8_t* success, int32_t* travel, double* energy, double* xy);
+
+/* The rollout's action table (Post_processing/grid_world.py:322-365): speed (float32
+ * widened), angle round(i*2*pi/na, 14), and speed*cos(angle), speed*sin(angle) (glibc). */
+void orc_rollout_actions(const e2e_params* p, double* speed, double* angle, double* ax, double* ay);
+
 /* max_s | (T V)(s) - V(s) | in FP64 for a float value vector V (the final residual). */
 double orc_bellman_residual_f64(int64_t ns, int64_t na, int64_t nnz, const int64_t* S1,
                                 const int64_t* S2, const float* val, const float* R,
