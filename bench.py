@@ -14,7 +14,13 @@ Also reported on the same line:
   * "serving": a Poisson(20/s) stream of planning requests over environments with
     grids {50^2, 100^2, 150^2}, nt in U[60,150], 8 or 16 headings, 1000-5000
     realizations: p50/p99 latency and solves/s of the continuous-batching server vs a
-    one-request-at-a-time queue.
+    one-request-at-a-time queue;
+  * "paths": build cost of C2 with obstacle islands and with reward method 2 over a
+    scalar field;
+  * "alpha_sweep": the 11-alpha sweep (src/my_script.sh:10-19) from one build and one
+    batched solve vs 11 builds and 11 solves;
+  * "rollout": the C2 policy's trajectories (100 start cells x 2500 realizations,
+    interpolated actions), with the oracle's CPU rate beside it.
 Per-kernel times come from HIP events recorded inside the library on the stream the
 kernels run on.
 """
@@ -186,6 +192,54 @@ def _paths(e2e, np, torch):
     return res
 
 
+ROLLOUT_STARTS = 100  # a 10 x 10 block of start cells next to the reference's startpos
+
+
+def _rollout_starts(p):
+    """10 x 10 start cells ending at the reference's startpos (int(0.5 g), int(0.95 g)),
+    Post_processing/plot_e2eGPU_policy.py:875."""
+    si, sj = int(0.5 * p.gsize), int(0.95 * p.gsize)
+    return [(i, j) for i in range(si - 5, si + 5) for j in range(sj - 10, sj)]
+
+
+def _rollouts(e2e, np, p, dev, pol):
+    """Policy rollouts (plot_exact_trajectory_set_DP with the interpolated policy, the
+    run the reference logs, :949-952) of the C2 policy: 100 start cells x all
+    realizations, fields resident in HBM."""
+    starts = _rollout_starts(p)
+    runs = []
+    for _ in range(3):  # the first is a warmup
+        runs.append(e2e.rollout(p, dev, pol, starts, interp=True))
+    st = [r["stats"] for r in runs[1:]]
+    kms = float(np.mean([x["kernel_ms"] for x in st]))
+    met = e2e.rollout_metrics(runs[-1])
+    return {"workload": f"C2 policy, {len(starts)} start cells x {p.nrzns} realizations, interpolated actions",
+            "trajectories": st[-1]["trajectories"], "moves": st[-1]["moves"], "kernel_ms": kms,
+            "trajectories_per_s": st[-1]["trajectories"] / (kms * 1e-3),
+            "moves_per_s": st[-1]["moves"] / (kms * 1e-3),
+            "success_rate": met["success_rate"], "time_mean": met["time_mean"],
+            "reqd_energy_mean": met["reqd_energy_mean"]}
+
+
+def _rollout_cpu_baseline(np, p, fields, pol, budget_s):
+    """The oracle's OpenMP rollout of the same trajectories on the host cores, over as
+    many start cells as fit the budget."""
+    import oracle
+    starts = _rollout_starts(p)
+    nth = min(16, oracle.max_threads())
+    t0 = time.perf_counter()
+    oracle.rollout(p, fields, pol, starts[:1], interp=True, nthreads=nth)
+    one = time.perf_counter() - t0
+    k = int(max(1, min(len(starts), budget_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    r = oracle.rollout(p, fields, pol, starts[:k], interp=True, nthreads=nth)
+    dt = time.perf_counter() - t0
+    n = len(r["success"])
+    return {"value": n / dt, "unit": "trajectories/s", "cores": nth, "kind": "port",
+            "sample": f"oracle rollout (OpenMP, {nth} threads) of {k} start cells x {p.nrzns} "
+                      f"realizations: {n} trajectories in {dt:.2f} s"}
+
+
 def _alpha_sweep(e2e, np, torch):
     """The reference's alpha sweep (src/my_script.sh:10-19: build + solve for alpha =
     0, 0.1, .., 1) on the C2 grid with reward method 4 and 2 speeds x 8 headings: one build with 11 alpha
@@ -273,11 +327,14 @@ def main(argv=None):
     dev = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in fields.items()}
     torch.cuda.synchronize()
 
+    policy = {}
+
     def step():
         m = e2e.build_model(p, dev)
         st = None
         if not args.skip_vi:
-            _, _, st = m.solve(thresh=args.thresh, max_sweeps=p.nt + 2)
+            pol, _, st = m.solve(thresh=args.thresh, max_sweeps=p.nt + 2)
+            policy["last"] = pol
         info = m.info
         m.close()
         return info, st
@@ -346,6 +403,11 @@ def main(argv=None):
         out["paths"] = _paths(e2e, np, torch)
         print("[bench] alpha sweep", file=sys.stderr, flush=True)
         out["alpha_sweep"] = _alpha_sweep(e2e, np, torch)
+    pol32 = None
+    if not args.skip_vi and args.workload == "C2":
+        print("[bench] rollouts", file=sys.stderr, flush=True)
+        pol32 = policy["last"].astype(np.int64).astype(np.int32)
+        out["rollout"] = _rollouts(e2e, np, p, dev, pol32)
     if not args.no_batched:
         print("[bench] batched goals", file=sys.stderr, flush=True)
         out["batched"] = _batched(e2e, np, max(args.steps, 3))
@@ -355,6 +417,8 @@ def main(argv=None):
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = _cpu_baseline(w, fields, args.cpu_budget)
+        if pol32 is not None:
+            out["rollout"]["cpu_baseline"] = _rollout_cpu_baseline(np, p, fields, pol32, 3.0)
     print(json.dumps(out))
     return 0
 
