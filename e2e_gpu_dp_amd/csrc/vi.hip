@@ -59,6 +59,7 @@ struct VIArgs {
     const uint32_t* cz;     // compressed sweep: count << 16 | (column - window base) per entry
     const int32_t* cbase;   // compressed sweep: base column per 256-entry window, -1: kept 8-byte
     double inv_nr;          // compressed sweep: 1 / nrzns
+    int ptab_n;             // compressed sweep: nrzns + 1 when the probability table sits in LDS, else 0
 };
 
 constexpr int kVIThreads = 512;   // 8 waves per block
@@ -391,6 +392,13 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
     const uint32_t s_begin = wp[w], s_end = wp[w + 1];
     double* wsd = stage[wv];
     float res = 0.0f;
+    // compressed entries: the probability of each count, float(count * (1 / nrzns)), once
+    // per block in LDS (the same expression as the inline decode, so the same bits)
+    extern __shared__ double ptab[];
+    if (COMP && A.ptab_n) {
+        for (int c = tid; c < A.ptab_n; c += kStreamThreads) ptab[c] = (double)(float)((double)c * A.inv_nr);
+        __syncthreads();
+    }
 
     if (s_begin < s_end) {
         const uint32_t ngroups = (s_end - s_begin + SPC - 1) / SPC;
@@ -492,8 +500,13 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
                 for (int u = 0; u < U; u++) e[u] = czp[kw * WIN + u * 64];
 #pragma unroll
                 for (int u = 0; u < U; u++) x[u] = ld_off(Vin, (uint32_t)base + (e[u] & 0xffffu));
+                if (A.ptab_n) {
 #pragma unroll
-                for (int u = 0; u < U; u++) pv[u] = (double)(float)((double)(e[u] >> 16) * A.inv_nr);
+                    for (int u = 0; u < U; u++) pv[u] = ptab[e[u] >> 16];
+                } else {
+#pragma unroll
+                    for (int u = 0; u < U; u++) pv[u] = (double)(float)((double)(e[u] >> 16) * A.inv_nr);
+                }
             } else {
                 int2 e[U];
 #pragma unroll
@@ -1266,7 +1279,9 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
             m->vi_part_cap = ntiles;
         }
     }
-    const size_t shmem = (size_t)vwin * sizeof(double);
+    // the compressed sweep keeps its probability table in LDS while it fits in 32 KB
+    const int ptab_n = (lean && comp && (int64_t)(m->p.nrzns + 1) * 8 <= 32768) ? m->p.nrzns + 1 : 0;
+    const size_t shmem = tiled ? (size_t)vwin * sizeof(double) : (size_t)ptab_n * sizeof(double);
     // persistent grid: as many blocks as can be resident at once (grid-stride over chunks,
     // or one contiguous share per wave), so that no block starts after the others finished
     const int64_t states_per_block = (threads / 64) * grp;
@@ -1364,6 +1379,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     A.cz = comp ? m->vi_cz : nullptr;
     A.cbase = comp ? m->vi_cbase : nullptr;
     A.inv_nr = comp ? 1.0 / (double)m->p.nrzns : 0.0;
+    A.ptab_n = ptab_n;
 
     hipLaunchKernelGGL(vi_init, dim3((unsigned)ceil_div(ns, 256)), dim3(256), 0, s, m->V[0], m->W[0], ns, m->st);
     E2E_HIP(hipGetLastError());
