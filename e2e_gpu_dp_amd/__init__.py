@@ -23,7 +23,7 @@ from ._abi import E2EError, check, lib
 from .configs import Params as PyParams
 
 __all__ = ["Model", "Server", "build_model", "import_coo", "c_params", "device_count", "E2EError",
-           "pack_params", "actions", "rollout", "rollout_metrics"]
+           "pack_params", "actions", "rollout", "rollout_metrics", "generate_environment"]
 
 _PFIELDS = [f[0] for f in _abi.Params._fields_]
 
@@ -302,6 +302,43 @@ def rollout_metrics(r: dict) -> dict:
         out[f"{name}_min"] = float(v.min()) if len(v) else float("nan")
         out[f"{name}_max"] = float(v.max()) if len(v) else float("nan")
         out[f"{name}_mean"] = float(v.mean()) if len(v) else float("nan")
+    return out
+
+
+def generate_environment(gsize: int = 25, nt: int = 10, n_wsamples: int = 1000, n_modes: int = 3,
+                         dt: float = None, dxy: float = None, A: float = 0.5, eps: float = 0.1,
+                         wx: float = np.pi, wy: float = 0.5 * np.pi, w_range=(np.pi / 10, 8 * np.pi / 10),
+                         A_sc: float = 3.0, obstacles=None, phi: bool = False):
+    """The double-gyre environment of
+    src/data_input/build_environment_DG_with_westward_clouds.py on the GPU, at refinement
+    degree 1 (defaults: the commented parameter block :424-437, dt = 10/nt, dxy =
+    2/gsize). obstacles: None, "static", or ("dynamic", upper_row, left_col, width,
+    (speed_x, speed_y)). Returns a dict of float32 / int32 arrays under the builder's
+    field names (u, v, ui, vi, yi, s, mask; with phi: phi, phi_i, phi_yi), the singular
+    values, and stats."""
+    dt = 10.0 / nt if dt is None else dt
+    dxy = 2.0 / gsize if dxy is None else dxy
+    mode, ru, cl, wd, sp = 0, 0, 0, 0, (0.0, 0.0)
+    if obstacles == "static":
+        mode = 1
+    elif obstacles is not None:
+        _, ru, cl, wd, sp = obstacles
+        mode = 2
+    ep = _abi.EnvParams(gsize, nt, n_wsamples, n_modes, dt, dxy, A, eps, wx, wy, w_range[0], w_range[1], A_sc,
+                        mode, ru, cl, wd, sp[0], sp[1])
+    g, k, nw = gsize, n_modes, n_wsamples
+    out = {"u": np.empty((nt, g, g), np.float32), "v": np.empty((nt, g, g), np.float32),
+           "ui": np.empty((nt, k, g, g), np.float32), "vi": np.empty((nt, k, g, g), np.float32),
+           "yi": np.empty((nt, nw, k), np.float32), "s": np.empty((nt, g, g), np.float32),
+           "mask": np.empty((nt, g, g), np.int32), "singular_values": np.empty((nt, k), np.float64)}
+    if phi:
+        out.update(phi=np.empty((nt, g, g), np.float32), phi_i=np.empty((nt, k, g, g), np.float32),
+                   phi_yi=np.empty((nt, nw, k), np.float32))
+    eo = _abi.EnvOut(**{name: (arr.ctypes.data if name in out else None)
+                        for name, arr in [(n, out.get(n)) for n, _ in _abi.EnvOut._fields_]})
+    st = _abi.EnvStats()
+    check(lib().e2e_env_generate(C.byref(ep), C.byref(eo), C.byref(st)))
+    out["stats"] = {kk: getattr(st, kk) for kk, _ in st._fields_}
     return out
 
 

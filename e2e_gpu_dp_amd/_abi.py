@@ -120,8 +120,25 @@ class RolloutStats(C.Structure):
         [("kernel_ms", C.c_double)]
 
 
+class EnvParams(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("gsize", "nt", "n_wsamples", "n_modes")] + \
+        [(n, C.c_double) for n in ("dt", "dxy", "A", "eps", "wx", "wy", "w_lo", "w_hi", "A_sc")] + \
+        [(n, C.c_int32) for n in ("obstacle_mode", "obs_upper_row", "obs_left_col", "obs_width")] + \
+        [("obs_speed_x", C.c_double), ("obs_speed_y", C.c_double), ("reserved", C.c_int32 * 4)]
+
+
+class EnvOut(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("u", "v", "ui", "vi", "yi", "s", "mask", "phi", "phi_i", "phi_yi",
+                                           "singular_values")]
+
+
+class EnvStats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("modes_ms", C.c_double), ("slices_per_pass", C.c_int32),
+                ("max_iterations", C.c_int32)]
+
+
 STRUCTS = [Params, Fields, BuildOpts, ModelInfo, SolveOpts, SolveStats, BatchOpts, BatchStats,
-           ServerOpts, RequestResult, ServerStats, RolloutOpts, RolloutStats]
+           ServerOpts, RequestResult, ServerStats, RolloutOpts, RolloutStats, EnvParams, EnvOut, EnvStats]
 
 VP = C.c_void_p
 P = C.POINTER
@@ -148,6 +165,7 @@ SIGNATURES = {
                                      P(BatchStats)]),
     "e2e_rollout": (C.c_int, [P(Params), P(Fields), VP, C.c_int32, VP, P(RolloutOpts), VP, VP, VP, VP,
                               P(RolloutStats)]),
+    "e2e_env_generate": (C.c_int, [P(EnvParams), P(EnvOut), P(EnvStats)]),
     "e2e_server_create": (C.c_int, [P(ServerOpts), P(VP)]),
     "e2e_server_add_env": (C.c_int, [VP, C.c_int64, P(Params), P(Fields)]),
     "e2e_server_submit": (C.c_int, [VP, C.c_int64, C.c_int32, C.c_int32, P(C.c_uint64)]),

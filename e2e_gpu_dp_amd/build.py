@@ -32,7 +32,8 @@ COMMON = [
     "-Wno-unused-command-line-argument", "-Wno-unknown-warning-option",
 ]
 
-LIB_SOURCES = ["abi.cpp", "build.hip", "scan.hip", "vi.hip", "spmm_vi.hip", "rollout.hip", "server.cpp"]
+LIB_SOURCES = ["abi.cpp", "build.hip", "scan.hip", "vi.hip", "spmm_vi.hip", "rollout.hip", "envgen.hip",
+               "server.cpp"]
 DRIVERS = {"build_model": "build_model_main.cpp", "spvi": "spvi_main.cpp"}
 
 
@@ -56,7 +57,9 @@ def build(jobs: int = 0, verbose: bool = True) -> Path:
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    # rocBLAS: the plain GEMMs of the input generator's Gram matrices and subspace iteration
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
+           "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -70,6 +70,7 @@ e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve
 e2e_status vi_solve_alphas(Model* m, const e2e_solve_opts* o, const e2e_batch_opts* bo, uint32_t* policy,
                            float* value, int32_t* col_sweeps, e2e_batch_stats* stats);
 e2e_status model_export_R_alpha(const Model* m, int32_t j, float* R);
+e2e_status env_generate(const e2e_env_params* ep, const e2e_env_out* out, e2e_env_stats* stats);
 e2e_status rollout(const e2e_params* p, const e2e_fields* f, const int32_t* policy, int32_t n_starts,
                    const int32_t* starts, const e2e_rollout_opts* o, int8_t* success, int32_t* travel,
                    double* energy, double* xy, e2e_rollout_stats* stats);
@@ -134,6 +135,9 @@ int32_t e2e_struct_size(int32_t which) {
         case 10: return (int32_t)sizeof(e2e_server_stats);
         case 11: return (int32_t)sizeof(e2e_rollout_opts);
         case 12: return (int32_t)sizeof(e2e_rollout_stats);
+        case 13: return (int32_t)sizeof(e2e_env_params);
+        case 14: return (int32_t)sizeof(e2e_env_out);
+        case 15: return (int32_t)sizeof(e2e_env_stats);
         default: return -1;
     }
 }
@@ -306,6 +310,10 @@ e2e_status e2e_rollout(const e2e_params* p, const e2e_fields* fields, const int3
                        const int32_t* starts, const e2e_rollout_opts* opts, int8_t* success, int32_t* travel,
                        double* energy, double* xy, e2e_rollout_stats* stats) {
     return guarded([&] { return rollout(p, fields, policy, n_starts, starts, opts, success, travel, energy, xy, stats); });
+}
+
+e2e_status e2e_env_generate(const e2e_env_params* p, const e2e_env_out* out, e2e_env_stats* stats) {
+    return guarded([&] { return env_generate(p, out, stats); });
 }
 
 }  // extern "C"

@@ -351,6 +351,55 @@ e2e_status e2e_rollout(const e2e_params* p, const e2e_fields* fields, const int3
                        int8_t* success, int32_t* travel, double* energy, double* xy,
                        e2e_rollout_stats* stats);
 
+/* Input generation: the double-gyre environment of
+ * src/data_input/build_environment_DG_with_westward_clouds.py (DG_velocity_field
+ * :105-240, DG_scalar_field :20-100, fill_obstacles :470-505) at refinement degree 1:
+ * realizations for n_wsamples forcing frequencies, the per-slice mean, the n_modes
+ * leading modes and coefficients of the centred realization matrix (the truncated
+ * np.linalg.svd of :227-236, computed as the leading eigenvectors of its Gram matrix),
+ * the cloud field and the obstacle mask, written in the layouts e2e_fields reads. */
+typedef struct e2e_env_params {
+    int32_t gsize;             /* cells per side (init_gsize, interpolate_degree 1) */
+    int32_t nt;                /* time slices */
+    int32_t n_wsamples;        /* realizations: forcing frequencies across [w_lo, w_hi] (>= 2) */
+    int32_t n_modes;           /* modes kept, 1..min(n_wsamples, gsize^2) */
+    double dt, dxy;            /* slice time step, cell size */
+    double A, eps, wx, wy;     /* double-gyre amplitude, perturbation, wavenumbers (the reference:
+                                  wx given, wy = pi/2) */
+    double w_lo, w_hi;         /* w_range */
+    double A_sc;               /* cloud amplitude */
+    int32_t obstacle_mode;     /* 0 none, 1 static box, 2 moving box */
+    int32_t obs_upper_row, obs_left_col, obs_width;  /* mode 2: the box at t = 0 */
+    double obs_speed_x, obs_speed_y;                  /* mode 2: cells per time step */
+    int32_t reserved[4];
+} e2e_env_params;
+
+typedef struct e2e_env_out {
+    float* u;                  /* [nt][g][g] all_u_mat (mean vx) */
+    float* v;                  /* [nt][g][g] all_v_mat */
+    float* ui;                 /* [nt][n_modes][g][g] all_ui_mat */
+    float* vi;                 /* [nt][n_modes][g][g] all_vi_mat */
+    float* yi;                 /* [nt][n_wsamples][n_modes] all_Yi */
+    float* s;                  /* [nt][g][g] all_s_mat (clouds), or NULL */
+    int32_t* mask;             /* [nt][g][g] obstacle_mask, or NULL */
+    float* phi;                /* [nt][g][g] all_phi_mat; phi, phi_i, phi_yi all or none */
+    float* phi_i;              /* [nt][n_modes][g][g] all_phi_i_mat */
+    float* phi_yi;             /* [nt][n_wsamples][n_modes] all_phi_Yi_mat */
+    double* singular_values;   /* [nt][n_modes] of the velocity matrix, or NULL */
+} e2e_env_out;
+
+typedef struct e2e_env_stats {
+    double total_ms;
+    double modes_ms;           /* Gram matrices, subspace iteration and mode assembly */
+    int32_t slices_per_pass;
+    int32_t max_iterations;    /* subspace iterations of the slowest pass */
+} e2e_env_stats;
+
+/* Host output arrays (caller-owned). Modes and coefficients carry the sign that makes
+ * each mode's first largest-magnitude entry positive (LAPACK's sign is arbitrary;
+ * every coefficient x mode product is unchanged by it). */
+e2e_status e2e_env_generate(const e2e_env_params* p, const e2e_env_out* out, e2e_env_stats* stats);
+
 /* Continuous-batching planning server (no counterpart in the reference, whose pipeline
  * is one build + one solve per goal: src/Makefile:7-19). Environments (fields + params,
  * keyed by env_id, e.g. the field seed) are registered once; requests (env, goal cell,
