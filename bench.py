@@ -20,7 +20,9 @@ Also reported on the same line:
   * "alpha_sweep": the 11-alpha sweep (src/my_script.sh:10-19) from one build and one
     batched solve vs 11 builds and 11 solves;
   * "rollout": the C2 policy's trajectories (100 start cells x 2500 realizations,
-    interpolated actions), with the oracle's CPU rate beside it.
+    interpolated actions), with the oracle's CPU rate beside it;
+  * "envgen": input generation at the C2 size (realizations, 5 leading modes per slice
+    from the Gram matrix, clouds, obstacles), with one numpy/LAPACK slice beside it.
 Per-kernel times come from HIP events recorded inside the library on the stream the
 kernels run on.
 """
@@ -240,6 +242,45 @@ def _rollout_cpu_baseline(np, p, fields, pol, budget_s):
                       f"realizations: {n} trajectories in {dt:.2f} s"}
 
 
+ENVGEN = dict(gsize=100, nt=120, n_wsamples=2500, n_modes=5)  # the C2 grid, horizon, sample, modes
+
+
+def _envgen_cfg(np):
+    return dict(**ENVGEN, dt=10 / ENVGEN["nt"], dxy=2 / ENVGEN["gsize"], A=0.5, eps=0.1, wx=np.pi,
+                wy=0.5 * np.pi, w_range=(np.pi / 10, 8 * np.pi / 10), A_sc=3.0)
+
+
+def _envgen(e2e, np):
+    """Input generation (src/data_input/build_environment_DG_with_westward_clouds.py) at the
+    C2 size: realizations, means, the 5 leading modes per slice, clouds, obstacles."""
+    cfg = _envgen_cfg(np)
+    runs = [e2e.generate_environment(**cfg, obstacles="static") for _ in range(2)]  # the first warms up
+    st = runs[-1]["stats"]
+    m, nw, nt = 2 * cfg["gsize"] ** 2, cfg["n_wsamples"], cfg["nt"]
+    gram_flops = 2.0 * min(nw, m) ** 2 * max(nw, m) * nt  # one Gram GEMM per slice
+    return {"workload": f"{cfg['gsize']}x{cfg['gsize']} grid, nt={nt}, {nw} forcing frequencies, "
+                        f"{cfg['n_modes']} modes, static obstacle",
+            "total_ms": st["total_ms"], "modes_ms": st["modes_ms"], "slices_per_s": nt / (st["total_ms"] * 1e-3),
+            "gram_tflops": gram_flops / (st["modes_ms"] * 1e-3) / 1e12, "subspace_iterations": st["max_iterations"],
+            "leading_singular_value_t60": float(runs[-1]["singular_values"][60, 0])}
+
+
+def _envgen_cpu_baseline(np):
+    """The oracle's numpy/LAPACK path (np.linalg.svd of the centred realizations, as the
+    reference) for one slice of the same workload."""
+    from oracle import envgen
+    cfg = _envgen_cfg(np)
+    t0 = time.perf_counter()
+    vx, vy, _ = envgen.realizations(cfg["gsize"], cfg["nt"], cfg["n_wsamples"], cfg["dt"], cfg["dxy"], cfg["A"],
+                                    cfg["eps"], cfg["wx"], cfg["wy"], *cfg["w_range"], t_slices=[60])
+    R = np.concatenate((vx[0], vy[0]), axis=1)
+    envgen.modes(R - R.mean(axis=0), cfg["n_modes"])
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "slices/s", "cores": min(16, os.cpu_count() or 1), "kind": "port",
+            "sample": f"one slice (t = 60) with numpy realizations and LAPACK SVD of the "
+                      f"{cfg['n_wsamples']} x {2 * cfg['gsize'] ** 2} matrix: {dt:.1f} s"}
+
+
 def _alpha_sweep(e2e, np, torch):
     """The reference's alpha sweep (src/my_script.sh:10-19: build + solve for alpha =
     0, 0.1, .., 1) on the C2 grid with reward method 4 and 2 speeds x 8 headings: one build with 11 alpha
@@ -403,6 +444,8 @@ def main(argv=None):
         out["paths"] = _paths(e2e, np, torch)
         print("[bench] alpha sweep", file=sys.stderr, flush=True)
         out["alpha_sweep"] = _alpha_sweep(e2e, np, torch)
+        print("[bench] input generation", file=sys.stderr, flush=True)
+        out["envgen"] = _envgen(e2e, np)
     pol32 = None
     if not args.skip_vi and args.workload == "C2":
         print("[bench] rollouts", file=sys.stderr, flush=True)
@@ -419,6 +462,8 @@ def main(argv=None):
         out["cpu_baseline"] = _cpu_baseline(w, fields, args.cpu_budget)
         if pol32 is not None:
             out["rollout"]["cpu_baseline"] = _rollout_cpu_baseline(np, p, fields, pol32, 3.0)
+        if "envgen" in out:
+            out["envgen"]["cpu_baseline"] = _envgen_cpu_baseline(np)
     print(json.dumps(out))
     return 0
 
