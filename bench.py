@@ -21,6 +21,8 @@ Also reported on the same line:
     batched solve vs 11 builds and 11 solves;
   * "rollout": the C2 policy's trajectories (100 start cells x 2500 realizations,
     interpolated actions), with the oracle's CPU rate beside it;
+  * "vi_backward": the same model solved by the exact backward pass (one launch per
+    slice), bit-identical to the sweeps once they have run nt times;
   * "envgen": input generation at the C2 size (realizations, 5 leading modes per slice
     from the Gram matrix, clouds, obstacles), with one numpy/LAPACK slice beside it.
 Per-kernel times come from HIP events recorded inside the library on the stream the
@@ -281,6 +283,24 @@ def _envgen_cpu_baseline(np):
                       f"{cfg['n_wsamples']} x {2 * cfg['gsize'] ** 2} matrix: {dt:.1f} s"}
 
 
+def _backward(e2e, np, p, dev, thresh):
+    """The exact backward pass (kernel_variant 7: one launch per slice, last to first) on
+    the C2 model, against the sweeps to the bench's residual; reports whether the two
+    return the same bits (they do once the sweeps have run nt times)."""
+    m = e2e.build_model(p, dev)
+    pj, vj, sj = m.solve(thresh=thresh, max_sweeps=p.nt + 2)
+    runs = [m.solve(kernel_variant=7) for _ in range(3)]  # the first warms up
+    pb, vb, sb = runs[-1]
+    m.close()
+    kms = float(np.mean([r[2]["sweep_kernel_ms"] for r in runs[1:]]))
+    sms = float(np.mean([r[2]["solve_ms"] for r in runs[1:]]))
+    return {"launches": sb["launches"], "kernel_ms": kms, "solve_ms": sms,
+            "sweeps_solve_ms": sj["solve_ms"], "sweeps": sj["sweeps"],
+            "speedup_vs_sweeps": sj["solve_ms"] / sms,
+            "bit_identical_to_sweeps": bool(np.array_equal(vj.view(np.uint32), vb.view(np.uint32))
+                                            and np.array_equal(pj, pb))}
+
+
 def _alpha_sweep(e2e, np, torch):
     """The reference's alpha sweep (src/my_script.sh:10-19: build + solve for alpha =
     0, 0.1, .., 1) on the C2 grid with reward method 4 and 2 speeds x 8 headings: one build with 11 alpha
@@ -446,6 +466,9 @@ def main(argv=None):
         out["alpha_sweep"] = _alpha_sweep(e2e, np, torch)
         print("[bench] input generation", file=sys.stderr, flush=True)
         out["envgen"] = _envgen(e2e, np)
+    if not args.skip_vi:
+        print("[bench] backward pass", file=sys.stderr, flush=True)
+        out["vi_backward"] = _backward(e2e, np, p, dev, args.thresh)
     pol32 = None
     if not args.skip_vi and args.workload == "C2":
         print("[bench] rollouts", file=sys.stderr, flush=True)

@@ -97,6 +97,9 @@ struct Model {
     uint32_t* vi_wpart = nullptr;  // [waves+1] first state of each wave's contiguous share
     int64_t vi_wpart_waves = 0;
     int vi_wpart_spc = 0;
+    uint32_t* vi_bw_wpart = nullptr;  // backward pass: [nt][waves+1] per-slice partitions
+    int64_t vi_bw_waves = 0;
+    int vi_bw_spc = 0;
     // jagged-diagonal copy of the CSR for the jagged sweep: per state, rows sorted by
     // length (longest first); entry k of the rows longer than k stored contiguously
     int2* vi_jnz = nullptr;       // [nnz + kNzPad]

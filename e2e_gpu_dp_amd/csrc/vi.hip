@@ -60,6 +60,7 @@ struct VIArgs {
     const int32_t* cbase;   // compressed sweep: base column per 256-entry window, -1: kept 8-byte
     double inv_nr;          // compressed sweep: 1 / nrzns
     int ptab_n;             // compressed sweep: nrzns + 1 when the probability table sits in LDS, else 0
+    int one_pass;           // lean sweep as one slice of the backward pass: no residual, no sweep state
 };
 
 constexpr int kVIThreads = 512;   // 8 waves per block
@@ -359,6 +360,32 @@ __global__ void vi_partition(const uint32_t* __restrict__ rp, int64_t ns, int na
     wpart[w] = (uint32_t)(s < ns ? s : ns);
 }
 
+// The backward pass's partition: for each slice t (blockIdx.y), wpart[t][w] = first state
+// of wave w within [t*nc, (t+1)*nc), balanced as vi_partition over that range.
+__global__ void vi_partition_slices(const uint32_t* __restrict__ rp, int64_t nc, int na, int64_t W, int spc,
+                                    uint32_t* __restrict__ wpart) {
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w > W) return;
+    const int64_t lo = (int64_t)blockIdx.y * nc, hi = lo + nc;
+    uint32_t* out = wpart + (int64_t)blockIdx.y * (W + 1);
+    if (w == 0 || w == W) {
+        out[w] = (uint32_t)(w == 0 ? lo : hi);
+        return;
+    }
+    auto weight = [&](int64_t s) { return (double)(rp[s * na] - rp[lo * na]) + 2.0 * (double)((s - lo) * na); };
+    const double target = weight(hi) * (double)w / (double)W;
+    int64_t a = lo, b = hi;
+    while (a < b) {
+        const int64_t mid = (a + b) / 2;
+        if (weight(mid) >= target)
+            b = mid;
+        else
+            a = mid + 1;
+    }
+    const int64_t s = lo + (a - lo + spc / 2) / spc * spc;
+    out[w] = (uint32_t)(s < hi ? s : hi);
+}
+
 // ---------------------------------------------------------------------------------
 // Lean streaming sweep (the default). Same work split as spmv_vi_stream (one contiguous
 // state range per wave, balanced by entries + rows, entries read as one stream in
@@ -376,9 +403,11 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
     __shared__ int is_last;
     __shared__ double stage[kStreamWaves][WIN + kRowReads];
     VIState* st = A.st;
-    if (st->conv >= 0) return;  // converged at an earlier sweep
-    const uint32_t k = st->k;
-    const double* __restrict__ Vin = (k & 1u) ? A.W0 : A.W1;  // float64 values between sweeps
+    if (!A.one_pass && st->conv >= 0) return;  // converged at an earlier sweep
+    // float64 values between sweeps. In the backward pass W0 == W1: the launch writes the
+    // values of one slice and reads only those of the next
+    const uint32_t k = A.one_pass ? 1u : st->k;
+    const double* __restrict__ Vin = (k & 1u) ? A.W0 : A.W1;
     double* __restrict__ Vout = (k & 1u) ? A.W1 : A.W0;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -529,6 +558,7 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
         }
         while (g < ngroups) finish_group();  // trailing groups whose rows are all empty
     }
+    if (A.one_pass) return;  // block-uniform
 
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) res = fmaxf(res, __shfl_xor(res, o, 64));
@@ -1141,7 +1171,10 @@ constexpr uint32_t kMaxBand = 4096;  // doubles: 32 KB
 //   4 jagged-diagonal, float64     (spmv_vi_jagged)
 //   5 lean streaming over the compressed 4-byte entries, float64 (spmv_vi_lean<.., true>;
 //     the default for built models whose probabilities are counts over nrzns < 2^16)
+//   6 backward pass (built models): one launch of the lean kernel per slice, from the last
+//     slice to the first, each reading the values its successor slice wrote
 constexpr int kDefaultVariant = 5;
+constexpr int kBackward = 6;
 constexpr int kDefaultImported = 2;  // imported models: their probabilities are arbitrary
 constexpr int kLeanBlocksPerCU = 5;  // fastest resident grid of the lean sweep (100x100x120 model)
 bool is_comp_variant(int variant) { return variant == 5; }
@@ -1199,8 +1232,14 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     const int nap = next_pow2((int)m->na);
     const int grp = 64 / nap;
     const bool comp_ok = m->built && m->p.nrzns > 0 && m->p.nrzns < 65536;
-    const int variant = opts.kernel_variant > 0 ? opts.kernel_variant - 1
-                        : (comp_ok ? kDefaultVariant : kDefaultImported);
+    const int variant_req = opts.kernel_variant > 0 ? opts.kernel_variant - 1
+                            : (comp_ok ? kDefaultVariant : kDefaultImported);
+    // the backward pass runs the default lean kernel once per slice; its transitions must
+    // go from each slice to the next, which holds for built models
+    const bool bw = variant_req == kBackward;
+    E2E_CHECK(!bw || (m->built && m->nc > 0 && m->nt > 0), E2E_ERR_INVALID,
+              "vi_solve: the backward pass (kernel_variant 7) needs a model built by e2e_model_build");
+    const int variant = bw ? (comp_ok ? kDefaultVariant : kDefaultImported) : variant_req;
     const bool comp = is_comp_variant(variant);
     E2E_CHECK(!comp || comp_ok, E2E_ERR_INVALID,
               "vi_solve: the compressed sweep needs a built model with nrzns below 65536");
@@ -1380,6 +1419,76 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     A.cbase = comp ? m->vi_cbase : nullptr;
     A.inv_nr = comp ? 1.0 / (double)m->p.nrzns : 0.0;
     A.ptab_n = ptab_n;
+    A.one_pass = 0;
+
+    if (bw) {
+        // one launch per slice, last to first: slice t's rows read only slice t+1, which the
+        // previous launch wrote; every slice takes its final values in this one pass
+        const int64_t nc = m->nc, nts = m->nt;
+        const int64_t groups = ceil_div(nc, (int64_t)grp);
+        const int64_t nbs = std::max<int64_t>(1, std::min<int64_t>(ceil_div(groups, (int64_t)kStreamWaves), nblocks));
+        const int64_t Ws = nbs * kStreamWaves;
+        if (m->vi_bw_waves != Ws || m->vi_bw_spc != grp) {
+            if (m->vi_bw_wpart) (void)hipFree(m->vi_bw_wpart);
+            m->vi_bw_wpart = nullptr;
+            m->vi_bw_waves = 0;
+            if (hipMalloc(&m->vi_bw_wpart, sizeof(uint32_t) * (Ws + 1) * nts) != hipSuccess) {
+                m->vi_bw_wpart = nullptr;
+                set_error("vi_solve: device allocation failed");
+                return E2E_ERR_NOMEM;
+            }
+            hipLaunchKernelGGL(vi_partition_slices, dim3((unsigned)ceil_div(Ws + 1, 256), (unsigned)nts), dim3(256), 0,
+                               s, m->row_ptr, nc, (int)m->na, Ws, grp, m->vi_bw_wpart);
+            E2E_HIP(hipGetLastError());
+            m->vi_bw_waves = Ws;
+            m->vi_bw_spc = grp;
+        }
+        VIArgs B = A;
+        B.one_pass = 1;
+        B.W0 = B.W1 = m->W[0];
+        hipEvent_t e0, e1;
+        E2E_HIP(hipEventCreate(&e0));
+        E2E_HIP(hipEventCreate(&e1));
+        E2E_HIP(hipEventRecord(e0, s));
+        for (int64_t t = nts - 1; t >= 0; t--) {
+            B.wpart = m->vi_bw_wpart + t * (Ws + 1);
+            hipLaunchKernelGGL(fn, dim3((unsigned)nbs), dim3(threads), shmem, s, B);
+        }
+        E2E_HIP(hipGetLastError());
+        E2E_HIP(hipEventRecord(e1, s));
+        hipLaunchKernelGGL(vi_to_f32, dim3((unsigned)ceil_div(ns, 256)), dim3(256), 0, s, m->W[0], m->V[0], ns);
+        E2E_HIP(hipGetLastError());
+        if (value) E2E_HIP(hipMemcpyAsync(value, m->V[0], sizeof(float) * ns, hipMemcpyDeviceToHost, s));
+        if (policy) E2E_HIP(hipMemcpyAsync(policy, m->pi, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
+        E2E_HIP(hipStreamSynchronize(s));
+        float ms = 0.f;
+        E2E_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (stats) {
+            *stats = e2e_solve_stats{};
+            stats->sweeps = 1;
+            stats->converged = 1;
+            stats->final_residual = 0.0f;
+            stats->loads_in_flight = lpr;
+            stats->kernel_variant = kBackward + 1;
+            stats->blocks = (int32_t)nbs;
+            stats->solve_ms = now_ms() - t_start;
+            stats->sweep_kernel_ms = ms;
+            stats->sweep_kernel_ms_per_sweep = ms;
+            const double meta = 4.0 * (m->rows + 1) + 4.0 * m->rows;
+            stats->bytes_per_sweep = 8.0 * m->nnz + meta + 12.0 * ns;
+            double entries = 8.0 * m->nnz;
+            if (comp) {
+                const double cw = (double)m->vi_cwin_comp, ww = (double)(m->vi_cwin - m->vi_cwin_comp);
+                entries = cw * (4.0 * kCompWin + 4.0) + ww * (8.0 * kCompWin + 4.0);
+            }
+            stats->moved_bytes_per_sweep = entries + meta + 20.0 * ns;
+            stats->compressed_windows = comp ? (int32_t)std::min<int64_t>(m->vi_cwin_comp, INT32_MAX) : 0;
+            stats->launches = (int32_t)nts;
+        }
+        return E2E_OK;
+    }
 
     hipLaunchKernelGGL(vi_init, dim3((unsigned)ceil_div(ns, 256)), dim3(256), 0, s, m->V[0], m->W[0], ns, m->st);
     E2E_HIP(hipGetLastError());

@@ -149,8 +149,10 @@ typedef struct e2e_solve_opts {
                                   1 chunked + compensated float, 2 chunked + fma float (faster; exceeds 1e-5
                                   over long horizons), 3 lean streaming + float64, 4 tiled with an LDS value
                                   band + float64, 5 jagged-diagonal + float64, 6 lean streaming over 4-byte
-                                  compressed entries + float64 (built models only); other values are an
-                                  error */
+                                  compressed entries + float64 (built models only), 7 backward pass (built
+                                  models only): one launch per slice from the last to the first, the fixed
+                                  point the sweeps reach after nt sweeps, bit for bit, in one pass (thresh
+                                  and max_sweeps unused; stats.sweeps = 1); other values are an error */
     int32_t blocks_per_cu;     /* resident sweep blocks per compute unit (0: chosen by the library) */
     int32_t comp_span;         /* variant 6: a 256-entry window whose columns span at least this many keeps
                                   its 8-byte entries (0: 65536, the most a 16-bit offset reaches; 1..65536) */

@@ -227,3 +227,47 @@ def test_vi_compressed_matches_lean_bitwise(gpu, case, span):
         # the default for a built model is the compressed sweep
         pc, vc, sc = m.solve(thresh=1e-6)
         assert sc["kernel_variant"] == 6 and np.array_equal(vc.view(np.uint32), va.view(np.uint32))
+
+
+@pytest.mark.parametrize("case", ["small", "terminal", "obstacles", "c1", "wide_flow"])
+def test_vi_backward_pass_equals_converged_sweeps(gpu, case):
+    """Transitions of a built model go from each slice to the next, so the sweeps reach
+    their fixed point after nt of them; the backward pass (kernel_variant 7: one launch
+    per slice, last to first) computes each slice once from its final successor, with the
+    same per-row float64 arithmetic, so it returns the same bits as sweeps run to a zero
+    residual."""
+    from e2e_gpu_dp_amd.configs import Workload
+    if case == "small":
+        w = small(g=11, nt=14, nr=400, nm=2, na=12, seed=53)
+    elif case == "terminal":
+        w = small(g=14, nt=20, nr=300, nm=3, seed=57, term_subgrid_size=3)
+    elif case == "obstacles":
+        b = small(g=16, nt=18, nr=300, nm=2, seed=59, term_subgrid_size=2)
+        w = Workload("obs", b.params, b.seed, b.sigmas, obstacles=[(0, 18, 5, 8, 5, 8)])
+    elif case == "c1":
+        w = workload("C1")
+    else:
+        w = small(g=30, nt=4, nr=400, nm=2, seed=13)
+    f = w.fields()
+    if case == "wide_flow":
+        from e2e_gpu_dp_amd.fields import make_fields
+        p = w.params
+        f = make_fields(p.gsize, p.nt, p.nmodes, p.nrzns, 13, F=p.F, mean_speed_frac=9.0, mode_rms_frac=6.0)
+    with e2e.build_model(w.params, f) as m:
+        for variant in (6, 3):  # compressed and 8-byte entries
+            pj, vj, sj = m.solve(thresh=1e-30, kernel_variant=variant, max_sweeps=w.params.nt + 4)
+            pb, vb, sb = m.solve(kernel_variant=7)
+            print(f"[{case}] sweeps {sj['sweeps']} ({sj['sweep_kernel_ms']:.3f} ms) vs backward "
+                  f"{sb['launches']} launches ({sb['sweep_kernel_ms']:.3f} ms)", flush=True)
+            assert sj["converged"] and sj["sweeps"] <= w.params.nt + 1
+            assert sb["kernel_variant"] == 7 and sb["sweeps"] == 1 and sb["launches"] == w.params.nt
+            assert np.array_equal(vj.view(np.uint32), vb.view(np.uint32))
+            assert np.array_equal(pj, pb)
+
+
+def test_vi_backward_pass_needs_a_built_model(gpu):
+    w = small(g=8, nt=6, nr=100, nm=1, seed=61)
+    ref = oracle_model(w.params, w.fields())
+    with e2e.import_coo(ref["ns"], ref["na"], ref["S1"], ref["S2"], ref["val"], ref["R"]) as m:
+        with pytest.raises(e2e.E2EError):
+            m.solve(kernel_variant=7)
