@@ -177,7 +177,7 @@ def test_vi_c2_accuracy(gpu):
 def test_vi_unknown_variant_is_an_error(gpu):
     w = small(g=6, nt=4, nr=50, nm=1, seed=71)
     with e2e.build_model(w.params, w.fields()) as m:
-        for bad in ({"kernel_variant": 7}, {"comp_span": 65537}):
+        for bad in ({"kernel_variant": 8}, {"comp_span": 65537}):
             with pytest.raises(e2e.E2EError) as ei:
                 m.solve(**bad)
             assert ei.value.code == 1
