@@ -332,7 +332,18 @@ def _alpha_sweep(e2e, np, torch):
         m.close()
         same += int(np.array_equal(vj.view(np.uint32), val[j].view(np.uint32)) and np.array_equal(pj, pol[j]))
     plain_ms = (time.perf_counter() - t0) * 1e3
+    # the same sweep with every column by the exact backward pass (variant -3)
+    t0 = time.perf_counter()
+    m = e2e.build_model(p, dev, alphas=alphas)
+    pol_b, val_b, _, st_b = m.solve_alphas(variant=-3)
+    m.close()
+    bw_ms = (time.perf_counter() - t0) * 1e3
+    same_b = sum(int(np.array_equal(val_b[j].view(np.uint32), val[j].view(np.uint32))
+                     and np.array_equal(pol_b[j], pol[j])) for j in range(len(alphas)))
     return {"workload": "C2-alpha: 100x100 x nt=120, 2 speeds x 8 headings, reward method 4, 11 alphas",
+            "backward_sweep_ms": bw_ms, "backward_solve_ms": st_b["solve_ms"],
+            "backward_speedup": plain_ms / bw_ms,
+            "backward_columns_equal_sweeps": f"{same_b}/{len(alphas)}",
             "sweep_ms": total_ms, "build_ms": build_ms, "solve_ms": st["solve_ms"],
             "per_alpha_builds_and_solves_ms": plain_ms, "speedup": plain_ms / total_ms,
             "columns_bit_identical": f"{same}/{len(alphas)}", "sweeps": st["sweeps"],

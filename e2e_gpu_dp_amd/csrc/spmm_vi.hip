@@ -1031,16 +1031,22 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
 // each column's rewards. Every column is the plain solve of the model built with that
 // alpha, bit for bit.
 static e2e_status vi_solve_alpha_columns(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* value,
-                                         int32_t* col_sweeps, e2e_batch_stats* stats) {
+                                         int32_t* col_sweeps, e2e_batch_stats* stats, bool backward) {
     const double t_start = now_ms();
     float* const R_own = m->R;
+    // backward: each column by the exact backward pass (kernel_variant 7)
+    e2e_solve_opts ob{};
+    ob.thresh = 0.1f;
+    ob.discount = 1.0f;
+    if (o) ob = *o;
+    if (backward) ob.kernel_variant = 7;
     e2e_status rc = E2E_OK;
     int conv = 0, maxsw = 0, launches = 0;
     double kms = 0.0, bytes = 0.0;
     for (int j = 0; j < m->n_alpha; j++) {
         m->R = m->Ralpha + (size_t)j * m->rows;
         e2e_solve_stats st{};
-        const e2e_status r = vi_solve(m, o, policy ? policy + (size_t)j * m->ns : nullptr,
+        const e2e_status r = vi_solve(m, &ob, policy ? policy + (size_t)j * m->ns : nullptr,
                                       value ? value + (size_t)j * m->ns : nullptr, &st);
         m->R = R_own;
         if (r != E2E_OK && r != E2E_ERR_NOT_CONVERGED && r != E2E_ERR_TIMEOUT) return r;
@@ -1056,7 +1062,7 @@ static e2e_status vi_solve_alpha_columns(Model* m, const e2e_solve_opts* o, uint
         *stats = e2e_batch_stats{};
         stats->sweeps = maxsw;
         stats->converged = conv;
-        stats->variant = -2;
+        stats->variant = backward ? -3 : -2;
         stats->cols_per_lane = 1;
         stats->states_per_wave = 0;
         stats->col_block = 1;
@@ -1085,7 +1091,8 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
     } else {
         E2E_CHECK(m->Ralpha && m->n_alpha >= 1 && k == m->n_alpha, E2E_ERR_INVALID,
                   "alpha columns need a model built with n_alpha > 0");
-        if (bo && bo->variant == -2) return vi_solve_alpha_columns(m, o, policy, value, col_sweeps, stats);
+        if (bo && (bo->variant == -2 || bo->variant == -3))
+            return vi_solve_alpha_columns(m, o, policy, value, col_sweeps, stats, bo->variant == -3);
     }
     e2e_solve_opts opts{};
     opts.thresh = 0.1f;

@@ -69,6 +69,21 @@ def test_alpha_solve_matches_plain_solves(gpu, variant):
         assert np.array_equal(pol[j], pj), j
 
 
+def test_alpha_backward_columns_match_converged_plain_solves(gpu):
+    """Variant -3: each alpha column by the exact backward pass, the fixed point of the
+    plain sweeps of a separate build with that alpha, bit for bit."""
+    p, f = _case("two_speeds")
+    with e2e.build_model(p, f, alphas=ALPHAS) as m:
+        pol, val, sw, st = m.solve_alphas(variant=-3)
+        assert st["variant"] == -3 and st["converged"] == len(ALPHAS) and (sw == 1).all()
+    for j, a in enumerate(ALPHAS):
+        with e2e.build_model(p.replace(alpha=float(a)), f) as mj:
+            pj, vj, sj = mj.solve(thresh=1e-30, max_sweeps=p.nt + 4)
+        assert sj["converged"]
+        assert np.array_equal(val[j].view(np.uint32), vj.view(np.uint32)), j
+        assert np.array_equal(pol[j], pj), j
+
+
 def test_alpha_columns_errors(gpu):
     p, f = _case("two_speeds")
     with pytest.raises(e2e.E2EError) as ei:  # only reward method 4 depends on alpha
