@@ -14,7 +14,8 @@ Also reported on the same line:
   * "serving": a Poisson(20/s) stream of planning requests over environments with
     grids {50^2, 100^2, 150^2}, nt in U[60,150], 8 or 16 headings, 1000-5000
     realizations: p50/p99 latency and solves/s of the continuous-batching server vs a
-    one-request-at-a-time queue;
+    one-request-at-a-time queue, and of the server in exact-pass mode (one backward pass
+    per scheduler step);
   * "paths": build cost of C2 with obstacle islands and with reward method 2 over a
     scalar field;
   * "alpha_sweep": the 11-alpha sweep (src/my_script.sh:10-19) from one build and one
@@ -157,6 +158,7 @@ def _serving(e2e, np, n_requests: int, seed: int = 5678):
 
     st, res = run(max_cols=64)
     st_q, res_q = run(max_cols=1, serial=True)
+    st_x, res_x = run(max_cols=64, exact_pass=True)
     same = sum(1 for a, b in zip(res, res_q) if a["value_at_start"] == b["value_at_start"])
     return {
         "workload": f"{n_requests} requests, Poisson 20/s, 6 environments "
@@ -167,6 +169,8 @@ def _serving(e2e, np, n_requests: int, seed: int = 5678):
         "queue_p50_ms": st_q["p50_ms"], "queue_p99_ms": st_q["p99_ms"],
         "queue_solves_per_s": st_q["solves_per_s"],
         "same_value_at_start_as_queue": f"{same}/{len(res)}",
+        "exact_pass": {"p50_ms": st_x["p50_ms"], "p99_ms": st_x["p99_ms"], "solves_per_s": st_x["solves_per_s"],
+                       "completed": st_x["completed"]},
     }
 
 

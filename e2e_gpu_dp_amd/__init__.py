@@ -349,9 +349,10 @@ class Server:
     """Continuous-batching planning server (C++ scheduler + builder threads in libe2edp)."""
 
     def __init__(self, hbm_budget_bytes: int = 0, max_cols: int = 64, sweeps_per_step: int = 4,
-                 keep_full_results: bool = False, serial: bool = False, thresh: float = 1e-6):
+                 keep_full_results: bool = False, serial: bool = False, thresh: float = 1e-6,
+                 exact_pass: bool = False):
         o = _abi.ServerOpts(hbm_budget_bytes, max_cols, sweeps_per_step, int(keep_full_results),
-                            int(serial), thresh, 0)
+                            int(serial), thresh, int(exact_pass))
         h = C.c_void_p()
         check(lib().e2e_server_create(C.byref(o), C.byref(h)))
         self._h = h

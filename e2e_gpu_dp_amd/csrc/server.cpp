@@ -41,6 +41,7 @@ e2e_status live_create(Model* m, int kp, float thresh, hipStream_t s, LiveBatch*
 void live_free(LiveBatch* L);
 e2e_status live_admit(LiveBatch* L, const int32_t* cols, const int32_t* goals, int n);
 e2e_status live_step(LiveBatch* L, int nsweeps, int ncols);
+e2e_status live_pass(LiveBatch* L, int ncols);
 e2e_status live_state(LiveBatch* L, BatchState* st, int32_t* conv, int32_t* start);
 e2e_status live_harvest(LiveBatch* L, int col, int64_t start_state, float* v_start, int32_t* a_start,
                         float* vcol, uint32_t* pcol);
@@ -272,7 +273,12 @@ struct Server {
                 int hi = 0;  // highest occupied slot + 1: the columns a sweep must cover
                 for (int c = 0; c < (int)e->slot.size(); c++)
                     if (e->slot[c]) hi = c + 1;
-                live_step(e->batch, spc, hi);
+                // sweeps (each column stops at its own first residual below thresh), or one
+                // exact backward pass that finishes every running column
+                if (o.exact_pass)
+                    live_pass(e->batch, hi);
+                else
+                    live_step(e->batch, spc, hi);
                 for (int c = 0; c < (int)e->slot.size(); c++)
                     if (e->slot[c]) e->steps_since_admit[c] += spc;
             }
@@ -303,7 +309,7 @@ struct Server {
                                      o.keep_full_results ? r->vcol.data() : nullptr,
                                      o.keep_full_results ? r->pcol.data() : nullptr);
                         std::lock_guard<std::mutex> lk(mu);
-                        r->sweeps = conv[c] - start[c] + 1;
+                        r->sweeps = o.exact_pass ? 1 : conv[c] - start[c] + 1;
                         e->slot[c] = 0;
                         e->live--;
                         e->last_used = now_ms();

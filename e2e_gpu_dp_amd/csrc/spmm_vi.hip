@@ -66,6 +66,8 @@ struct BatchArgs {
     int rcol_n;            // columns Rcol holds (padding columns read the last one)
     const uint32_t* wpart; // spmm_vi_stream: [waves+1] first state of each wave
     uint32_t* colmax;      // spmm_vi_stream: [KP] running column residual maxima (float bits)
+    int one_pass;          // spmm_vi_rows as one slice of a backward pass: states [s_lo, s_hi),
+    uint32_t s_lo, s_hi;   // V0 == V1, no residual and no sweep state
 };
 
 namespace {
@@ -297,8 +299,8 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_rows(const BatchArgs A) {
     __shared__ float fin[kBThreads];
     __shared__ int is_last;
     BatchState* st = A.st;
-    if (st->all_conv) return;
-    const uint32_t k = st->k;
+    if (!A.one_pass && st->all_conv) return;
+    const uint32_t k = A.one_pass ? 1u : st->k;
     const double* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
     double* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -319,13 +321,15 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_rows(const BatchArgs A) {
         live[c] = A.conv[j0 + c] < 0;
         rmax[c] = 0.0f;
     }
-    const uint32_t G = (ns + spw - 1) / spw;
+    // a sweep covers every state; one slice of a backward pass covers [s_lo, s_hi)
+    const uint32_t s_lo = A.one_pass ? A.s_lo : 0u, s_hi = A.one_pass ? A.s_hi : ns;
+    const uint32_t G = (s_hi - s_lo + spw - 1) / spw;
     const unsigned nb = gridDim.x, b = blockIdx.x;
     const unsigned pb = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;
     const uint32_t g_beg = (uint32_t)((uint64_t)G * pb / nb), g_end = (uint32_t)((uint64_t)G * (pb + 1) / nb);
     for (uint32_t g = g_beg + wv; g < g_end; g += kBWaves) {
-        const uint32_t s = g * spw + sl;
-        const bool sval = s < ns;
+        const uint32_t s = s_lo + g * spw + sl;
+        const bool sval = s < s_hi;
         const bool valid = sval && (uint32_t)a < na;
         const uint32_t sc = sval ? s : ns - 1u;
         const uint32_t t = sc / nc;
@@ -423,6 +427,7 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_rows(const BatchArgs A) {
             }
         }
     }
+    if (A.one_pass) return;  // block-uniform
     // per-column residual: the lanes of one column lane cl, then the waves
 #pragma unroll
     for (int c = 0; c < CPL; c++)
@@ -1312,6 +1317,12 @@ __global__ void live_column(const double* V0, const double* V1, const int32_t* p
     pout[s] = pi[s * kp + col];
 }
 
+// after a backward pass: every running column is done, its values in V0 (an even conv)
+__global__ void live_pass_done(int32_t* conv, int kp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kp && conv[i] < 0) conv[i] = 2;
+}
+
 __global__ void live_init(double* V0, double* V1, int64_t n, int32_t* goal, int32_t* conv, int32_t* start,
                           float* last_res, uint32_t* colmax, int kp, BatchState* st) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1422,6 +1433,33 @@ e2e_status live_step(LiveBatch* L, int nsweeps, int ncols) {
     const Plan& p = L->plans[width_class(ncols)];
     const dim3 grid = plan_grid(p, ncols);
     for (int n = 0; n < nsweeps; n++) hipLaunchKernelGGL(p.fn, grid, p.block, 0, L->s, L->A);
+    E2E_HIP(hipGetLastError());
+    return E2E_OK;
+}
+
+// One exact backward pass over the running columns 0 .. ncols-1 (kernel_variant 7's
+// schedule with the goal columns' arithmetic): spmm_vi_rows once per slice, from the last
+// to the first, each launch reading the values of the slice after it, which the previous
+// launch wrote. Every running column then holds its fixed point in V0 and is marked done.
+e2e_status live_pass(LiveBatch* L, int ncols) {
+    Model* m = L->m;
+    const int kp = L->b.kp;
+    const Variant& v = kVariants[(ncols > 4 && variant_fits(kVariants[20], (int)m->na, kp)) ? 20 : 17];
+    Plan p;
+    E2E_TRY(plan_of(m, L->b, v, L->s, &p));
+    const int spw = spw_of(v, (int)m->na);
+    const int64_t nc = m->nc, groups = ceil_div(nc, (int64_t)spw);
+    const dim3 grid((unsigned)std::min<int64_t>(ceil_div(groups, (int64_t)kBWaves), kBMaxBlocks),
+                    plan_grid(p, ncols).y);
+    BatchArgs B = L->A;
+    B.one_pass = 1;
+    B.V1 = B.V0;
+    for (int64_t t = m->nt - 1; t >= 0; t--) {
+        B.s_lo = (uint32_t)(t * nc);
+        B.s_hi = (uint32_t)((t + 1) * nc);
+        hipLaunchKernelGGL(p.fn, grid, p.block, 0, L->s, B);
+    }
+    hipLaunchKernelGGL(live_pass_done, dim3((unsigned)ceil_div(kp, 64)), dim3(64), 0, L->s, L->b.conv, kp);
     E2E_HIP(hipGetLastError());
     return E2E_OK;
 }

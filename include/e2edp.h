@@ -210,7 +210,9 @@ typedef struct e2e_server_opts {
     int32_t keep_full_results; /* 1: keep each request's full value/policy columns on the host */
     int32_t serial;            /* 1: one request on the GPU at a time (queue-only baseline) */
     float thresh;              /* VI stopping threshold (0: 1e-6) */
-    int32_t reserved;
+    int32_t exact_pass;        /* 1: each scheduler step runs one exact backward pass over a model's
+                                  running columns (one launch per slice, last to first), which ends
+                                  every one of them at its fixed point; 0: sweeps to thresh */
 } e2e_server_opts;
 
 typedef struct e2e_request_result {

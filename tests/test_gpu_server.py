@@ -78,3 +78,38 @@ def test_server_serial_queue_same_answers(gpu):
             pol, val = srv.columns(rid, 1)
             spol, sval, _ = _solo(p, f, g)
             assert np.array_equal(val.view(np.uint32), sval.view(np.uint32))
+
+
+def _fixed_point(params, fields, goal):
+    """The goal column's fixed point: batched sweeps run to a zero residual."""
+    with e2e.build_model(params.replace(term_subgrid_size=0), fields, keep_rsum=True) as m:
+        pol, val, sw, _ = m.solve_goals([goal], thresh=1e-30, max_sweeps=params.nt + 4)
+    return pol[0], val[0]
+
+
+def test_server_exact_pass_mix_equals_fixed_point(gpu):
+    """exact_pass: each scheduler step is one backward pass (one launch per slice, last to
+    first) over a model's running columns; every result, in the mix or alone, is its goal
+    column's fixed point bit for bit, after one pass."""
+    envs = _envs()
+    rng = np.random.default_rng(11)
+    with e2e.Server(keep_full_results=True, exact_pass=True) as srv:
+        for eid, (p, f) in envs.items():
+            srv.add_env(eid, p, f)
+        reqs = []
+        for i in range(40):
+            eid = 1 + (i % 2)
+            nc = envs[eid][0].nc
+            g, s0 = (int(x) for x in rng.integers(0, nc, 2))
+            reqs.append((srv.submit(eid, g, s0), eid, g, s0))
+        srv.drain(timeout_ms=120000)
+        st = srv.stats()
+        assert st["completed"] == 40 and st["models_built"] == 2
+        for rid, eid, g, s0 in reqs[:6] + reqs[-4:]:
+            r = srv.wait(rid)
+            assert r["status_name"] == "done" and r["sweeps"] == 1
+            pol, val = srv.columns(rid, eid)
+            fpol, fval = _fixed_point(*envs[eid], g)
+            assert np.array_equal(val.view(np.uint32), fval.view(np.uint32))
+            assert np.array_equal(pol, fpol)
+            assert r["value_at_start"] == val[s0] and r["action_at_start"] == int(np.int32(pol[s0]))
