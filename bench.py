@@ -10,7 +10,8 @@ transitions per second of the whole step.
 Also reported on the same line:
   * "batched": 64 goal columns on the 25x25 x nt=20, 8-heading model, solved as one
     SpMM-VI (autotuned, hipGraph sweeps) vs 64 sequential plain SpMV-VI solves of the
-    per-goal models, with the largest column difference (bar: 1e-5 relative);
+    per-goal models, with the largest column difference (bar: 1e-5 relative), and the
+    same columns by one exact backward pass (goal variant -3);
   * "serving": a Poisson(20/s) stream of planning requests over environments with
     grids {50^2, 100^2, 150^2}, nt in U[60,150], 8 or 16 headings, 1000-5000
     realizations: p50/p99 latency and solves/s of the continuous-batching server vs a
@@ -94,6 +95,12 @@ def _batched(e2e, np, steps: int):
     for _ in range(steps):
         pol_b, val_b, sw_b, st_b = shared.solve_goals(goals, thresh=1e-6, use_graph=True)
     t_batch = (time.perf_counter() - t0) / steps
+    # the same 64 columns by one exact backward pass (each column at its fixed point)
+    shared.solve_goals(goals, variant=-3)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pol_x, val_x, _, st_x = shared.solve_goals(goals, variant=-3)
+    t_back = (time.perf_counter() - t0) / steps
     plains = [e2e.build_model(p.replace(i_term=int(g) // p.gsize, j_term=int(g) % p.gsize,
                                         term_subgrid_size=1), f) for g in goals]
     for m in plains[:2]:
@@ -101,10 +108,11 @@ def _batched(e2e, np, steps: int):
     t0 = time.perf_counter()
     outs = [m.solve(thresh=1e-6, use_graph=True) for m in plains]
     t_plain = time.perf_counter() - t0
-    err = 0.0
+    err = err_x = 0.0
     for j, (pp, vp, sp) in enumerate(outs):
         v64 = vp.astype(np.float64)
         err = max(err, float((np.abs(val_b[j] - v64) / np.maximum(np.abs(v64), 1.0)).max()))
+        err_x = max(err_x, float((np.abs(val_x[j] - v64) / np.maximum(np.abs(v64), 1.0)).max()))
     for m in plains:
         m.close()
     shared.close()
@@ -116,6 +124,9 @@ def _batched(e2e, np, steps: int):
         "states_per_wave": st_b["states_per_wave"], "sweeps": st_b["sweeps"],
         "sweep_kernel_ms": st_b["sweep_kernel_ms"] / max(st_b["launches"], 1),
         "bytes_per_sweep": st_b["bytes_per_sweep"],
+        "backward_ms": t_back * 1e3, "backward_solves_per_s": 64 / t_back,
+        "backward_speedup_vs_plain": t_plain / t_back, "backward_max_rel_diff_vs_plain": err_x,
+        "backward_launches": st_x["launches"],
     }
 
 

@@ -1028,6 +1028,40 @@ static dim3 plan_grid(const Plan& p, int ncols) {
     return g;
 }
 
+// after a backward pass: every running column is done, its values in V0 (an even conv)
+__global__ void live_pass_done(int32_t* conv, int kp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kp && conv[i] < 0) conv[i] = 2;
+}
+
+
+// One exact backward pass over columns 0 .. ncols-1 of b (kernel_variant 7's schedule with
+// the batched columns' arithmetic): spmm_vi_rows once per slice, from the last to the
+// first, each launch reading the values of the slice after it, which the previous launch
+// wrote into V0. The running columns (conv < 0) then hold their fixed points in V0 and are
+// marked done with an even conv.
+static e2e_status backward_pass(const Model* m, BatchBufs& b, const BatchArgs& A, int ncols, hipStream_t s) {
+    const int kp = b.kp;
+    const Variant& v = kVariants[(ncols > 4 && variant_fits(kVariants[20], (int)m->na, kp)) ? 20 : 17];
+    Plan p;
+    E2E_TRY(plan_of(m, b, v, s, &p));
+    const int spw = spw_of(v, (int)m->na);
+    const int64_t nc = m->nc, groups = ceil_div(nc, (int64_t)spw);
+    const dim3 grid((unsigned)std::min<int64_t>(ceil_div(groups, (int64_t)kBWaves), kBMaxBlocks),
+                    plan_grid(p, ncols).y);
+    BatchArgs B = A;
+    B.one_pass = 1;
+    B.V1 = B.V0;
+    for (int64_t t = m->nt - 1; t >= 0; t--) {
+        B.s_lo = (uint32_t)(t * nc);
+        B.s_hi = (uint32_t)((t + 1) * nc);
+        hipLaunchKernelGGL(p.fn, grid, p.block, 0, s, B);
+    }
+    hipLaunchKernelGGL(live_pass_done, dim3((unsigned)ceil_div(kp, 64)), dim3(64), 0, s, b.conv, kp);
+    E2E_HIP(hipGetLastError());
+    return E2E_OK;
+}
+
 // Goal columns (goals != NULL) or alpha columns (goals == NULL: the model's n_alpha
 // reward vectors, the model's own terminal cells).
 e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* value, e2e_solve_stats* stats);
@@ -1127,6 +1161,38 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
     };
     const int max_sweeps = opts.max_sweeps > 0 ? opts.max_sweeps : (int)m->nt + 2;
     const int spc = opts.sweeps_per_check > 0 ? opts.sweeps_per_check : 8;
+
+    if (bopt.variant == -3) {  // every column by one exact backward pass
+        hipEvent_t e0, e1;
+        E2E_HIP(hipEventCreate(&e0));
+        E2E_HIP(hipEventCreate(&e1));
+        init();
+        E2E_HIP(hipEventRecord(e0, s));
+        E2E_TRY(backward_pass(m, b, A, k, s));
+        E2E_HIP(hipEventRecord(e1, s));
+        hipLaunchKernelGGL(batch_gather, dim3((unsigned)ceil_div(m->ns * k, 256)), dim3(256), 0, s, b.V[0], b.V[1],
+                           b.pi, b.conv, m->ns, kp, k, 0, b.vout, b.pout);
+        E2E_HIP(hipGetLastError());
+        if (value) E2E_HIP(hipMemcpyAsync(value, b.vout, sizeof(float) * m->ns * k, hipMemcpyDeviceToHost, s));
+        if (policy) E2E_HIP(hipMemcpyAsync(policy, b.pout, sizeof(int32_t) * m->ns * k, hipMemcpyDeviceToHost, s));
+        E2E_HIP(hipStreamSynchronize(s));
+        float ms = 0.f;
+        E2E_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        for (int j = 0; col_sweeps && j < k; j++) col_sweeps[j] = 1;
+        if (stats) {
+            *stats = e2e_batch_stats{};
+            stats->sweeps = 1;
+            stats->converged = k;
+            stats->variant = -3;
+            stats->solve_ms = now_ms() - t_start;
+            stats->sweep_kernel_ms = ms;
+            stats->bytes_per_sweep = 8.0 * m->nnz + 8.0 * m->rows + 4.0 + 20.0 * (double)k * m->ns;
+            stats->launches = (int32_t)m->nt;
+        }
+        return E2E_OK;
+    }
 
     // ---- autotune: time a few sweeps of every valid variant on this model
     double tune_ms = 0.0;
@@ -1317,12 +1383,6 @@ __global__ void live_column(const double* V0, const double* V1, const int32_t* p
     pout[s] = pi[s * kp + col];
 }
 
-// after a backward pass: every running column is done, its values in V0 (an even conv)
-__global__ void live_pass_done(int32_t* conv, int kp) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < kp && conv[i] < 0) conv[i] = 2;
-}
-
 __global__ void live_init(double* V0, double* V1, int64_t n, int32_t* goal, int32_t* conv, int32_t* start,
                           float* last_res, uint32_t* colmax, int kp, BatchState* st) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1441,28 +1501,7 @@ e2e_status live_step(LiveBatch* L, int nsweeps, int ncols) {
 // schedule with the goal columns' arithmetic): spmm_vi_rows once per slice, from the last
 // to the first, each launch reading the values of the slice after it, which the previous
 // launch wrote. Every running column then holds its fixed point in V0 and is marked done.
-e2e_status live_pass(LiveBatch* L, int ncols) {
-    Model* m = L->m;
-    const int kp = L->b.kp;
-    const Variant& v = kVariants[(ncols > 4 && variant_fits(kVariants[20], (int)m->na, kp)) ? 20 : 17];
-    Plan p;
-    E2E_TRY(plan_of(m, L->b, v, L->s, &p));
-    const int spw = spw_of(v, (int)m->na);
-    const int64_t nc = m->nc, groups = ceil_div(nc, (int64_t)spw);
-    const dim3 grid((unsigned)std::min<int64_t>(ceil_div(groups, (int64_t)kBWaves), kBMaxBlocks),
-                    plan_grid(p, ncols).y);
-    BatchArgs B = L->A;
-    B.one_pass = 1;
-    B.V1 = B.V0;
-    for (int64_t t = m->nt - 1; t >= 0; t--) {
-        B.s_lo = (uint32_t)(t * nc);
-        B.s_hi = (uint32_t)((t + 1) * nc);
-        hipLaunchKernelGGL(p.fn, grid, p.block, 0, L->s, B);
-    }
-    hipLaunchKernelGGL(live_pass_done, dim3((unsigned)ceil_div(kp, 64)), dim3(64), 0, L->s, L->b.conv, kp);
-    E2E_HIP(hipGetLastError());
-    return E2E_OK;
-}
+e2e_status live_pass(LiveBatch* L, int ncols) { return backward_pass(L->m, L->b, L->A, ncols, L->s); }
 
 // Waits for the stream, then reports each slot's convergence sweep (conv) and
 // admission sweep (start).

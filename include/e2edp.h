@@ -181,9 +181,10 @@ typedef struct e2e_solve_stats {
 typedef struct e2e_batch_opts {
     int32_t variant;           /* SpMM kernel variant (0..26); -1: chosen by the library (by the column
                                   count on models of 16M entries or more with up to 16 columns,
-                                  else timed on this model). Alpha columns also take -2: one plain
-                                  sweep chain per column over the shared model, and -3: the exact
-                                  backward pass (kernel_variant 7) per column */
+                                  else timed on this model). -3: every column by one exact backward
+                                  pass (kernel_variant 7's schedule), ending at its fixed point. Alpha
+                                  columns also take -2: one plain sweep chain per column over the
+                                  shared model (and run -3 as one backward pass per column) */
     int32_t use_graph;         /* 1: replay the sweep chain as a captured hipGraph */
     int32_t reserved[2];
 } e2e_batch_opts;

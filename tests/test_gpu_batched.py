@@ -127,6 +127,20 @@ def test_batched_row_kernels_bit_identical(gpu, na, k):
             assert ran & set(STATE_VARIANTS_KB)
 
 
+@pytest.mark.parametrize("k", [3, 40])
+def test_batched_backward_pass_equals_converged_columns(gpu, k):
+    """Variant -3: every goal column by one exact backward pass, bit for bit the columns of
+    the batched sweeps run to a zero residual."""
+    w = small(g=14, nt=12, nr=400, nm=3, seed=93)
+    goals = goal_cells(14, k, 29)
+    with _shared(w, w.fields()) as m:
+        pj, vj, sj, stj = m.solve_goals(goals, thresh=1e-30, max_sweeps=w.params.nt + 4)
+        pb, vb, sb, stb = m.solve_goals(goals, variant=-3)
+        assert stj["converged"] == k and stb["variant"] == -3 and (sb == 1).all()
+        assert np.array_equal(vb.view(np.uint32), vj.view(np.uint32))
+        assert np.array_equal(pb, pj)
+
+
 def test_batched_rejects_models_it_cannot_serve(gpu):
     w = small(g=8, nt=5, nr=64, nm=1, seed=75)
     f = w.fields()
