@@ -3,7 +3,10 @@
 
 Headline (`value`): the plain GPU path — Monte Carlo model build (move + LDS radix
 sort/count-by-key + CSR) followed by fused SpMV value iteration to residual 1e-6 — on
-the 100x100 x nt=120, 16-heading, 5-mode, 2500-realization workload. One step = one
+the 100x100 x nt=120, 16-heading, 5-mode, 2500-realization workload. Each sweep
+recomputes the slices whose values are not yet final (slice t is final from sweep nt - t
+on): the same sweeps, residuals and bits as sweeps over every state, which "vi.full_sweeps"
+times on a model built apart. One step = one
 end-to-end build + solve with the fields already resident in HBM; `value` = Monte Carlo
 transitions per second of the whole step.
 
@@ -485,6 +488,19 @@ def main(argv=None):
         out["roofline_vi"] = {"kernel": kname, "bound": "hbm", "achieved": gbps,
                               "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": gbps / PEAK_HBM_GBPS,
                               "traffic": _pmc_traffic(args.workload, kname)}
+        # the same solve with every slice recomputed on every sweep (what the step's
+        # solve skips: the slices already at their final values), on a model built apart
+        with e2e.build_model(p, dev) as mf:
+            _, vfull, sf = mf.solve(thresh=args.thresh, max_sweeps=p.nt + 2, full_sweeps=True)
+            _, vact, sa = mf.solve(thresh=args.thresh, max_sweeps=p.nt + 2)
+        pf = sf["sweep_kernel_ms_per_sweep"]
+        out["vi"]["full_sweeps"] = {
+            "sweeps": sf["sweeps"], "sweep_kernel_ms": pf, "solve_kernel_ms": sf["sweep_kernel_ms"],
+            "bytes_per_sweep": sf["bytes_per_sweep"], "gbps": sf["bytes_per_sweep"] / (pf * 1e-3) / 1e9,
+            "frac": sf["bytes_per_sweep"] / (pf * 1e-3) / 1e9 / PEAK_HBM_GBPS,
+            "active_solve_kernel_ms": sa["sweep_kernel_ms"],
+            "same_bits": bool(np.array_equal(vfull.view(np.uint32), vact.view(np.uint32)) and
+                              sf["sweeps"] == sa["sweeps"])}
     if not args.no_paths and args.workload == "C2":
         print("[bench] obstacle and scalar-reward builds", file=sys.stderr, flush=True)
         out["paths"] = _paths(e2e, np, torch)

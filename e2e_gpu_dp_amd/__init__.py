@@ -138,11 +138,14 @@ class Model:
     def solve(self, thresh: float = 0.1, discount: float = 1.0, max_sweeps: int = 0,
               sweeps_per_check: int = 0, use_graph: bool = False, loads_in_flight: int = 0,
               timeout_ms: float = 0.0, raise_on_nonconv: bool = True, kernel_variant: int = 0,
-              blocks_per_cu: int = 0, comp_span: int = 0):
-        """Sparse VI. Returns (policy uint32[Ns], value float32[Ns], stats dict)."""
+              blocks_per_cu: int = 0, comp_span: int = 0, full_sweeps: bool = False):
+        """Sparse VI. Returns (policy uint32[Ns], value float32[Ns], stats dict).
+        full_sweeps: recompute every slice on every sweep (by default a built model's
+        sweeps skip the slices that already hold their final values; same results)."""
         ns = self.info["ns"]
         o = _abi.SolveOpts(thresh, discount, max_sweeps, sweeps_per_check, int(use_graph),
-                           loads_in_flight, timeout_ms, kernel_variant, blocks_per_cu, comp_span)
+                           loads_in_flight, timeout_ms, kernel_variant, blocks_per_cu, comp_span,
+                           int(full_sweeps))
         st = _abi.SolveStats()
         pol = np.empty(ns, np.uint32)
         val = np.empty(ns, np.float32)
@@ -156,13 +159,16 @@ class Model:
 
     def solve_goals(self, goal_cells, thresh: float = 0.1, discount: float = 1.0,
                     max_sweeps: int = 0, sweeps_per_check: int = 0, variant: int = -1,
-                    use_graph: bool = False, timeout_ms: float = 0.0, raise_on_nonconv: bool = True):
+                    use_graph: bool = False, timeout_ms: float = 0.0, raise_on_nonconv: bool = True,
+                    full_sweeps: bool = False):
         """Batched multi-goal VI (CSR x dense-block SpMM). Returns (policy uint32[k, Ns],
-        value float32[k, Ns], per-column sweeps int32[k], stats dict)."""
+        value float32[k, Ns], per-column sweeps int32[k], stats dict). full_sweeps: every
+        sweep recomputes every slice (same results)."""
         goals = np.ascontiguousarray(goal_cells, np.int32).ravel()
         k = len(goals)
         ns = self.info["ns"]
-        o = _abi.SolveOpts(thresh, discount, max_sweeps, sweeps_per_check, 0, 0, timeout_ms)
+        o = _abi.SolveOpts(thresh, discount, max_sweeps, sweeps_per_check, 0, 0, timeout_ms,
+                           full_sweeps=int(full_sweeps))
         bo = _abi.BatchOpts(variant, int(use_graph))
         st = _abi.BatchStats()
         pol = np.empty((k, ns), np.uint32)
@@ -184,14 +190,16 @@ class Model:
 
     def solve_alphas(self, thresh: float = 0.1, discount: float = 1.0, max_sweeps: int = 0,
                      sweeps_per_check: int = 0, variant: int = -1, use_graph: bool = False,
-                     timeout_ms: float = 0.0, raise_on_nonconv: bool = True):
+                     timeout_ms: float = 0.0, raise_on_nonconv: bool = True, full_sweeps: bool = False):
         """Alpha-sweep VI over the model's alpha columns (one SpMM per sweep). Returns
-        (policy uint32[n_alpha, Ns], value float32[n_alpha, Ns], per-column sweeps, stats)."""
+        (policy uint32[n_alpha, Ns], value float32[n_alpha, Ns], per-column sweeps, stats).
+        full_sweeps: every sweep recomputes every slice (same results)."""
         k = self.n_alpha
         if k <= 0:
             raise ValueError("the model was built without alpha columns")
         ns = self.info["ns"]
-        o = _abi.SolveOpts(thresh, discount, max_sweeps, sweeps_per_check, 0, 0, timeout_ms)
+        o = _abi.SolveOpts(thresh, discount, max_sweeps, sweeps_per_check, 0, 0, timeout_ms,
+                           full_sweeps=int(full_sweeps))
         bo = _abi.BatchOpts(variant, int(use_graph))
         st = _abi.BatchStats()
         pol = np.empty((k, ns), np.uint32)

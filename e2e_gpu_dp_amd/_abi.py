@@ -65,7 +65,7 @@ class SolveOpts(C.Structure):
         ("thresh", C.c_float), ("discount", C.c_float), ("max_sweeps", C.c_int32),
         ("sweeps_per_check", C.c_int32), ("use_graph", C.c_int32), ("loads_in_flight", C.c_int32),
         ("timeout_ms", C.c_double), ("kernel_variant", C.c_int32), ("blocks_per_cu", C.c_int32),
-        ("comp_span", C.c_int32), ("reserved", C.c_int32),
+        ("comp_span", C.c_int32), ("full_sweeps", C.c_int32),
     ]
 
 

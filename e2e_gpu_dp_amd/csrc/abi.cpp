@@ -46,7 +46,7 @@ e2e_status free_model(Model* m) {
     batch_cache_free(m->batch);
     void* ptrs[] = {m->row_ptr, m->nz,      m->R,        m->Rsum,     m->V[0],   m->V[1],     m->W[0], m->W[1],
                     m->pi,      m->st,      m->vi_part,  m->vi_wpart, m->vi_tdesc, m->vi_jnz, m->vi_jmeta, m->vi_jR,
-                    m->vi_cz,   m->vi_cbase, m->Ralpha, m->vi_bw_wpart};
+                    m->vi_cz,   m->vi_cbase, m->Ralpha, m->vi_bw_wpart, m->vi_pre_wpart};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);

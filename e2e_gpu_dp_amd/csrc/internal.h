@@ -45,6 +45,16 @@ constexpr int kMaxModes = 16;
 // per-entry bounds checks.
 constexpr int64_t kNzPad = 4096;
 
+// Slices a sweep over a built model recomputes. Transitions go from slice t to t+1 only
+// and the last slice's rows have none, so the values of slice t are final from sweep
+// nt - t on; at sweep k (1-based) >= 3 the slices t >= nt - k + 2 hold their final
+// values in both value buffers (written at sweep k - 2 or before, unchanged since): the
+// sweep recomputes slices [0, active_slices(k, nt)) and gives every value, policy and
+// residual the full sweep gives.
+__host__ __device__ inline int active_slices(uint32_t k, int nt) {
+    return k <= 2u ? nt : (nt - (int)k + 2 > 1 ? nt - (int)k + 2 : 1);
+}
+
 // Device-side solver state of one solve (one per model).
 struct VIState {
     uint32_t k;          // 1-based index of the next sweep
@@ -67,6 +77,15 @@ void batch_cache_free(BatchCache* c);
 // wpart[0..W]: first state of each of W waves, balanced by entries + rows, group-aligned
 void launch_wave_partition(const uint32_t* rp, int64_t ns, int na, int64_t W, int spc, uint32_t* wpart,
                            hipStream_t s);
+struct Model;
+// m->slice_rp: the row pointer at every slice boundary of a built model (once per model)
+e2e_status fetch_slice_row_ptrs(Model* m, hipStream_t s);
+// algorithmic bytes of the rows of slices [0, T): entries, row pointers, R, plus
+// per_state bytes per state
+double prefix_sweep_bytes(const Model* m, int64_t T, double per_state);
+// wpart[t][0..W] for t < nt: the same partition of the states [0, (t+1) nc)
+void launch_prefix_partitions(const uint32_t* rp, int64_t nc, int na, int64_t nt, int64_t W, int spc,
+                              uint32_t* wpart, hipStream_t s);
 
 struct Model {
     e2e_params p{};
@@ -97,6 +116,10 @@ struct Model {
     uint32_t* vi_wpart = nullptr;  // [waves+1] first state of each wave's contiguous share
     int64_t vi_wpart_waves = 0;
     int vi_wpart_spc = 0;
+    uint32_t* vi_pre_wpart = nullptr;  // active-slice sweeps: [nt][waves+1] partitions of [0, (t+1) nc)
+    int64_t vi_pre_waves = 0;
+    int vi_pre_spc = 0;
+    std::vector<uint32_t> slice_rp;    // host copy of row_ptr at each slice boundary, [nt+1]
     uint32_t* vi_bw_wpart = nullptr;  // backward pass: [nt][waves+1] per-slice partitions
     int64_t vi_bw_waves = 0;
     int vi_bw_spc = 0;

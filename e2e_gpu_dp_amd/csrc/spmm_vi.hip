@@ -68,6 +68,10 @@ struct BatchArgs {
     uint32_t* colmax;      // spmm_vi_stream: [KP] running column residual maxima (float bits)
     int one_pass;          // spmm_vi_rows as one slice of a backward pass: states [s_lo, s_hi),
     uint32_t s_lo, s_hi;   // V0 == V1, no residual and no sweep state
+    uint32_t pre_nt;       // sweeps of a solve whose columns all start at sweep 1, on a built model:
+                           // nt, and each sweep recomputes the active slices only (0: every state)
+    uint32_t pre_stride;   // spmm_vi_stream over the active slices: waves + 1, the stride of the
+                           // per-prefix partitions in wpart
 };
 
 namespace {
@@ -203,7 +207,8 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_sweep(const BatchArgs A) {
         rmax[c] = 0.0f;
     }
     const int64_t stride = (int64_t)gridDim.x * kBWaves * SPW;
-    for (int64_t s = ((int64_t)blockIdx.x * kBWaves + wv) * SPW + grp; s < A.ns; s += stride) {
+    const int64_t s_end = A.pre_nt ? (int64_t)active_slices(k, (int)A.pre_nt) * A.nc : A.ns;
+    for (int64_t s = ((int64_t)blockIdx.x * kBWaves + wv) * SPW + grp; s < s_end; s += stride) {
         const int t = (int)(s / A.nc);
         const int cell = (int)(s - (int64_t)t * A.nc);
         const int colbase = (t + 1) * A.nc;
@@ -321,8 +326,10 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_rows(const BatchArgs A) {
         live[c] = A.conv[j0 + c] < 0;
         rmax[c] = 0.0f;
     }
-    // a sweep covers every state; one slice of a backward pass covers [s_lo, s_hi)
-    const uint32_t s_lo = A.one_pass ? A.s_lo : 0u, s_hi = A.one_pass ? A.s_hi : ns;
+    // a sweep covers every state (a built model's: the active slices); one slice of a
+    // backward pass covers [s_lo, s_hi)
+    const uint32_t s_lo = A.one_pass ? A.s_lo : 0u;
+    const uint32_t s_hi = A.one_pass ? A.s_hi : A.pre_nt ? (uint32_t)active_slices(k, (int)A.pre_nt) * nc : ns;
     const uint32_t G = (s_hi - s_lo + spw - 1) / spw;
     const unsigned nb = gridDim.x, b = blockIdx.x;
     const unsigned pb = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;
@@ -499,7 +506,9 @@ __global__ __launch_bounds__(kSThreads) void spmm_vi_stream(const BatchArgs A) {
     const unsigned nb = gridDim.x, b = blockIdx.x;
     const unsigned pb = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;
     const unsigned w = pb * kSWaves + wv;
-    const uint32_t s_begin = A.wpart[w], s_end = A.wpart[w + 1];
+    const uint32_t* wpart = A.wpart;  // a built model's sweep: the active slices' partition
+    if (A.pre_nt) wpart += (size_t)(active_slices(k, (int)A.pre_nt) - 1) * A.pre_stride;
+    const uint32_t s_begin = wpart[w], s_end = wpart[w + 1];
     double* wst = stage[wv];
     int2* whc = hitc[wv];
 
@@ -844,8 +853,11 @@ struct BatchBufs {
     uint32_t* wpart = nullptr;   // [wpart_W + 1] state partition of spmm_vi_stream
     int64_t wpart_W = 0;
     int wpart_spc = 0;
+    uint32_t* wpre = nullptr;    // [nt][wpart_W + 1] the same partition of each prefix of slices
+    int64_t wpre_W = 0;
+    int wpre_spc = 0;
     void release() {
-        void* ps[] = {V[0], V[1], pi, goal, conv, start, part, last_res, st, vout, pout, colmax, wpart};
+        void* ps[] = {V[0], V[1], pi, goal, conv, start, part, last_res, st, vout, pout, colmax, wpart, wpre};
         for (void* p : ps)
             if (p) (void)hipFree(p);
         *this = BatchBufs();
@@ -863,6 +875,7 @@ struct BatchCache {
     float graph_thresh = 0.f, graph_discount = 0.f;
     const float* graph_rcol = nullptr;  // goal columns (NULL) or the alpha rewards
     const uint32_t* graph_wpart = nullptr;
+    uint32_t graph_pre = 0;
 };
 
 void batch_cache_free(BatchCache* c) {
@@ -1237,11 +1250,36 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
     E2E_TRY(plan_of(m, b, V, s, &plan));
     A.wpart = b.wpart;
     const dim3 grid = plan_grid(plan, k);
+    // every column starts at sweep 1: on a built model each sweep recomputes the active
+    // slices only (active_slices); the streaming kernel takes each prefix's own partition
+    const bool pre = !opts.full_sweeps && m->built && m->nt > 0 && m->nc > 0 && m->ns == m->nt * m->nc;
+    if (pre) {
+        A.pre_nt = (uint32_t)m->nt;
+        if (V.kind == 2) {
+            if (b.wpre_W != b.wpart_W || b.wpre_spc != b.wpart_spc) {
+                if (b.wpre) (void)hipFree(b.wpre);
+                b.wpre = nullptr;
+                b.wpre_W = 0;
+                if (hipMalloc(&b.wpre, sizeof(uint32_t) * (b.wpart_W + 1) * m->nt) != hipSuccess) {
+                    b.wpre = nullptr;
+                    (void)hipGetLastError();
+                    set_error("batched sweep: device allocation failed");
+                    return E2E_ERR_NOMEM;
+                }
+                launch_prefix_partitions(m->row_ptr, m->nc, (int)m->na, m->nt, b.wpart_W, b.wpart_spc, b.wpre, s);
+                E2E_HIP(hipGetLastError());
+                b.wpre_W = b.wpart_W;
+                b.wpre_spc = b.wpart_spc;
+            }
+            A.wpart = b.wpre;
+            A.pre_stride = (uint32_t)(b.wpart_W + 1);
+        }
+    }
 
     if (bopt.use_graph &&
         (!c->graph || c->graph_variant != variant || c->graph_chunk != spc || c->graph_grid != (int)grid.x || c->graph_gy != (int)grid.y ||
          c->graph_thresh != opts.thresh || c->graph_discount != opts.discount || c->graph_rcol != A.Rcol ||
-         c->graph_wpart != A.wpart)) {
+         c->graph_wpart != A.wpart || c->graph_pre != A.pre_nt)) {
         if (c->graph) {
             (void)hipGraphExecDestroy(c->graph);
             c->graph = nullptr;
@@ -1265,6 +1303,7 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
         c->graph_discount = opts.discount;
         c->graph_rcol = A.Rcol;
         c->graph_wpart = A.wpart;
+        c->graph_pre = A.pre_nt;
     }
 
     init();
@@ -1319,8 +1358,16 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
         stats->solve_ms = now_ms() - t_start;
         stats->sweep_kernel_ms = kms;
         stats->tune_ms = tune_ms;
-        // entries, row pointers + R, and per column: V read, V written (8 B each), policy
+        // entries, row pointers + R, and per column: V read, V written (8 B each), policy;
+        // over the active slices, the mean over the sweeps of the slices each one ran
         stats->bytes_per_sweep = 8.0 * m->nnz + 8.0 * m->rows + 4.0 + 20.0 * (double)k * m->ns;
+        if (pre && last_k > 0) {
+            E2E_TRY(fetch_slice_row_ptrs(m, s));
+            double bsum = 0.0;
+            for (int kk = 1; kk <= last_k; kk++)
+                bsum += prefix_sweep_bytes(m, active_slices((uint32_t)kk, (int)m->nt), 20.0 * k);
+            stats->bytes_per_sweep = bsum / last_k;
+        }
         stats->launches = launched;
     }
     if (nconv < k) {

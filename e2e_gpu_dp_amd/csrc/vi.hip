@@ -61,6 +61,8 @@ struct VIArgs {
     double inv_nr;          // compressed sweep: 1 / nrzns
     int ptab_n;             // compressed sweep: nrzns + 1 when the probability table sits in LDS, else 0
     int one_pass;           // lean sweep as one slice of the backward pass: no residual, no sweep state
+    uint32_t pre_nt;        // lean sweep over a built model's active slices: nt (0: every state, every sweep)
+    uint32_t pre_stride;    // active slices: waves + 1, the stride of wpart's per-prefix partitions
 };
 
 constexpr int kVIThreads = 512;   // 8 waves per block
@@ -360,13 +362,19 @@ __global__ void vi_partition(const uint32_t* __restrict__ rp, int64_t ns, int na
     wpart[w] = (uint32_t)(s < ns ? s : ns);
 }
 
-// The backward pass's partition: for each slice t (blockIdx.y), wpart[t][w] = first state
-// of wave w within [t*nc, (t+1)*nc), balanced as vi_partition over that range.
+// Per-slice partitions, balanced as vi_partition over each range: for each slice t
+// (blockIdx.y), wpart[t][w] = first state of wave w within [t*nc, (t+1)*nc) (the backward
+// pass), or with prefix set within [0, (t+1)*nc) (the sweeps over the active slices);
+// bnd (if given) receives the row pointer at every slice boundary, bnd[t] = rp[t*nc*na].
 __global__ void vi_partition_slices(const uint32_t* __restrict__ rp, int64_t nc, int na, int64_t W, int spc,
-                                    uint32_t* __restrict__ wpart) {
+                                    uint32_t* __restrict__ wpart, int prefix, uint32_t* __restrict__ bnd) {
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (w > W) return;
-    const int64_t lo = (int64_t)blockIdx.y * nc, hi = lo + nc;
+    const int64_t hi = ((int64_t)blockIdx.y + 1) * nc, lo = prefix ? 0 : hi - nc;
+    if (bnd && w == W) {
+        bnd[blockIdx.y + 1] = rp[hi * na];
+        if (blockIdx.y == 0) bnd[0] = rp[0];
+    }
     uint32_t* out = wpart + (int64_t)blockIdx.y * (W + 1);
     if (w == 0 || w == W) {
         out[w] = (uint32_t)(w == 0 ? lo : hi);
@@ -416,7 +424,11 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
     const unsigned nb = gridDim.x, b = blockIdx.x;
     const unsigned pb = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;
     const unsigned w = pb * kStreamWaves + wv;
-    const CPtr<uint32_t> wp = (CPtr<uint32_t>)A.wpart;
+    // a built model's sweep recomputes the active slices only (active_slices), over
+    // that prefix's own partition
+    const uint32_t* wpart = A.wpart;
+    if (A.pre_nt) wpart += (size_t)(active_slices(k, (int)A.pre_nt) - 1) * A.pre_stride;
+    const CPtr<uint32_t> wp = (CPtr<uint32_t>)wpart;
     const CPtr<uint32_t> rps = (CPtr<uint32_t>)A.rp;  // uniform reads of the row pointers
     const uint32_t s_begin = wp[w], s_end = wp[w + 1];
     double* wsd = stage[wv];
@@ -1372,6 +1384,10 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
             m->vi_cwin_comp = (int64_t)h;
         }
     }
+    // sweeps over the active slices only (spmv_vi_lean): built models, whose transitions
+    // go from each slice to the next; not the backward pass, which has its own schedule
+    const bool active = lean && !bw && !opts.full_sweeps && m->built && m->nt > 0 && m->nc > 0 &&
+                        ns == m->nt * m->nc;
     if (lean) {  // (entries are addressed through 64-bit pointers)
         E2E_CHECK(ns * (int64_t)sizeof(double) < (1ll << 32) && (m->rows + 1) * 4 < (1ll << 32), E2E_ERR_CAPACITY,
                   "vi_solve: the lean sweep addresses states and rows with 32-bit byte offsets");
@@ -1391,6 +1407,26 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
             m->vi_wpart_waves = W;
             m->vi_wpart_spc = grp;
         }
+        if (active) {  // the partitions of every prefix of slices, once per model and grid
+            if (m->vi_pre_waves != W || m->vi_pre_spc != grp) {
+                if (m->vi_pre_wpart) (void)hipFree(m->vi_pre_wpart);
+                m->vi_pre_wpart = nullptr;
+                m->vi_pre_waves = 0;
+                // + the row pointer at each slice boundary (nt + 1), for the bytes of each sweep
+                if (hipMalloc(&m->vi_pre_wpart, sizeof(uint32_t) * ((W + 1) * m->nt + m->nt + 1)) != hipSuccess) {
+                    m->vi_pre_wpart = nullptr;
+                    set_error("vi_solve: device allocation failed");
+                    return E2E_ERR_NOMEM;
+                }
+                hipLaunchKernelGGL(vi_partition_slices, dim3((unsigned)ceil_div(W + 1, 256), (unsigned)m->nt),
+                                   dim3(256), 0, s, m->row_ptr, m->nc, (int)m->na, W, grp, m->vi_pre_wpart, 1,
+                                   m->vi_pre_wpart + (W + 1) * m->nt);
+                E2E_HIP(hipGetLastError());
+                m->vi_pre_waves = W;
+                m->vi_pre_spc = grp;
+                m->slice_rp.clear();
+            }
+        }
     }
     VIArgs A;
     std::memset(&A, 0, sizeof(A));  // padding too: the graph cache compares the bytes
@@ -1408,7 +1444,9 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     A.na = (int)m->na;
     A.thresh = opts.thresh;
     A.discount = opts.discount;
-    A.wpart = lean ? m->vi_wpart : nullptr;
+    A.wpart = lean ? (active ? m->vi_pre_wpart : m->vi_wpart) : nullptr;
+    A.pre_nt = active ? (uint32_t)m->nt : 0u;
+    A.pre_stride = active ? (uint32_t)(nblocks * kStreamWaves + 1) : 0u;
     A.tdesc = tiled ? m->vi_tdesc : nullptr;
     A.tile_states = tile_states;
     A.vwin = vwin;
@@ -1438,7 +1476,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
                 return E2E_ERR_NOMEM;
             }
             hipLaunchKernelGGL(vi_partition_slices, dim3((unsigned)ceil_div(Ws + 1, 256), (unsigned)nts), dim3(256), 0,
-                               s, m->row_ptr, nc, (int)m->na, Ws, grp, m->vi_bw_wpart);
+                               s, m->row_ptr, nc, (int)m->na, Ws, grp, m->vi_bw_wpart, 0, nullptr);
             E2E_HIP(hipGetLastError());
             m->vi_bw_waves = Ws;
             m->vi_bw_spc = grp;
@@ -1568,10 +1606,35 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     }
     if (value) E2E_HIP(hipMemcpyAsync(value, Vfin, sizeof(float) * ns, hipMemcpyDeviceToHost, s));
     if (policy) E2E_HIP(hipMemcpyAsync(policy, m->pi, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
+    if (stats && active && (int64_t)m->slice_rp.size() != m->nt + 1) {  // once per model
+        m->slice_rp.assign((size_t)m->nt + 1, 0u);
+        E2E_HIP(hipMemcpyAsync(m->slice_rp.data(), m->vi_pre_wpart + (m->vi_pre_waves + 1) * m->nt,
+                               sizeof(uint32_t) * (m->nt + 1), hipMemcpyDeviceToHost, s));
+    }
     E2E_HIP(hipStreamSynchronize(s));
     if (stats) {
         stats->sweeps = sweeps;
         stats->converged = hst.conv >= 0;
+        // entries, row pointers, R, V read + V write + policy write
+        const double meta = 4.0 * (m->rows + 1) + 4.0 * m->rows;
+        double bytes = 8.0 * m->nnz + meta + 12.0 * ns;
+        double entries = 8.0 * m->nnz;
+        if (comp) {  // whole aligned windows: 4 bytes per entry + base, 8 for wide windows
+            const double cw = (double)m->vi_cwin_comp, ww = (double)(m->vi_cwin - m->vi_cwin_comp);
+            entries = cw * (4.0 * kCompWin + 4.0) + ww * (8.0 * kCompWin + 4.0);
+        }
+        double moved = entries + meta + (f64 ? 20.0 : 12.0) * ns;
+        if (active && sweeps > 0) {  // the mean over the sweeps of the slices each one ran
+            double b = 0.0, mv = 0.0;
+            for (int k = 1; k <= sweeps; k++) {
+                const int64_t T = active_slices((uint32_t)k, (int)m->nt);
+                const double nnz_t = (double)m->slice_rp[(size_t)T];
+                b += prefix_sweep_bytes(m, T, 12.0);
+                mv += prefix_sweep_bytes(m, T, 20.0) - 8.0 * nnz_t + (m->nnz ? entries * nnz_t / (double)m->nnz : 0.0);
+            }
+            bytes = b / sweeps;
+            moved = mv / sweeps;
+        }
         stats->final_residual = hst.last_res;
         stats->loads_in_flight = lpr;
         stats->kernel_variant = variant + 1;
@@ -1579,15 +1642,8 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
         stats->solve_ms = now_ms() - t_start;
         stats->sweep_kernel_ms = kms;
         stats->sweep_kernel_ms_per_sweep = sweeps > 0 ? kms / sweeps : 0.0;
-        // entries, row pointers, R, V read + V write + policy write
-        const double meta = 4.0 * (m->rows + 1) + 4.0 * m->rows;
-        stats->bytes_per_sweep = 8.0 * m->nnz + meta + 12.0 * ns;
-        double entries = 8.0 * m->nnz;
-        if (comp) {  // whole aligned windows: 4 bytes per entry + base, 8 for wide windows
-            const double cw = (double)m->vi_cwin_comp, ww = (double)(m->vi_cwin - m->vi_cwin_comp);
-            entries = cw * (4.0 * kCompWin + 4.0) + ww * (8.0 * kCompWin + 4.0);
-        }
-        stats->moved_bytes_per_sweep = entries + meta + (f64 ? 20.0 : 12.0) * ns;
+        stats->bytes_per_sweep = bytes;
+        stats->moved_bytes_per_sweep = moved;
         stats->compressed_windows = comp ? (int32_t)std::min<int64_t>(m->vi_cwin_comp, INT32_MAX) : 0;
         stats->launches = launched;
     }
@@ -1596,6 +1652,28 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
 
 // wpart[0..W] of the lean state partition (vi_partition) for W waves and groups of spc
 // states, on stream s; shared with the batched streaming sweep
+e2e_status fetch_slice_row_ptrs(Model* m, hipStream_t s) {
+    if ((int64_t)m->slice_rp.size() == m->nt + 1) return E2E_OK;
+    std::vector<uint32_t> h((size_t)m->nt + 1);
+    const size_t pitch = sizeof(uint32_t) * (size_t)(m->nc * m->na);
+    E2E_HIP(hipMemcpy2DAsync(h.data(), sizeof(uint32_t), m->row_ptr, pitch, sizeof(uint32_t), (size_t)m->nt + 1,
+                             hipMemcpyDeviceToHost, s));
+    E2E_HIP(hipStreamSynchronize(s));
+    m->slice_rp = std::move(h);
+    return E2E_OK;
+}
+
+double prefix_sweep_bytes(const Model* m, int64_t T, double per_state) {
+    const double nnz_t = (double)m->slice_rp[(size_t)T], rows_t = (double)(T * m->nc * m->na);
+    return 8.0 * nnz_t + 4.0 * (rows_t + 1) + 4.0 * rows_t + per_state * (double)(T * m->nc);
+}
+
+void launch_prefix_partitions(const uint32_t* rp, int64_t nc, int na, int64_t nt, int64_t W, int spc,
+                              uint32_t* wpart, hipStream_t s) {
+    hipLaunchKernelGGL(vi_partition_slices, dim3((unsigned)ceil_div(W + 1, 256), (unsigned)nt), dim3(256), 0, s, rp,
+                       nc, na, W, spc, wpart, 1, nullptr);
+}
+
 void launch_wave_partition(const uint32_t* rp, int64_t ns, int na, int64_t W, int spc, uint32_t* wpart,
                            hipStream_t s) {
     hipLaunchKernelGGL(vi_partition, dim3((unsigned)ceil_div(W + 1, 256)), dim3(256), 0, s, rp, ns, na, W, spc,

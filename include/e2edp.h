@@ -156,7 +156,11 @@ typedef struct e2e_solve_opts {
     int32_t blocks_per_cu;     /* resident sweep blocks per compute unit (0: chosen by the library) */
     int32_t comp_span;         /* variant 6: a 256-entry window whose columns span at least this many keeps
                                   its 8-byte entries (0: 65536, the most a 16-bit offset reaches; 1..65536) */
-    int32_t reserved;
+    int32_t full_sweeps;       /* 1: every sweep recomputes every state. 0 (default): on a built model the
+                                  lean sweeps (variants 3 and 6) recompute, at sweep k >= 3, only the slices
+                                  t < nt - k + 2; the others already hold their final values (transitions go
+                                  from each slice to the next), so values, policies, residuals and the sweep
+                                  count are the same bits */
 } e2e_solve_opts;
 
 typedef struct e2e_solve_stats {
@@ -169,7 +173,8 @@ typedef struct e2e_solve_stats {
     double sweep_kernel_ms_per_sweep; /* sweep_kernel_ms / launches that did work */
     double bytes_per_sweep;    /* algorithmic bytes of one sweep in the reference's CSR terms:
                                   8 nnz + 4 (rows+1) + 4 rows + 12 ns (entries, row pointers, R, V read,
-                                  V write, policy write) */
+                                  V write, policy write); for sweeps over the active slices, the mean over
+                                  the sweeps of the same terms for the slices each one recomputed */
     double moved_bytes_per_sweep; /* the same traffic in the layout the variant reads: 4-byte compressed
                                   entries (8 for windows kept uncompressed) and float64 values */
     int32_t launches;

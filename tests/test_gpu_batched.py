@@ -141,6 +141,34 @@ def test_batched_backward_pass_equals_converged_columns(gpu, k):
         assert np.array_equal(pb, pj)
 
 
+@pytest.mark.parametrize("na,k", [(8, 16), (16, 40), (8, 3)])
+def test_batched_active_slices_equal_full_sweeps(gpu, na, k):
+    """Every sweep of a batched solve recomputes only the slices not yet at their final
+    values; every variant gives the bits, per-column sweep counts and residuals of sweeps
+    over every state, for columns stopping at different sweeps (thresh 1e-3) and at the
+    fixed point (1e-30)."""
+    g, nt = 12, 9
+    w = small(g=g, nt=nt, nr=300, nm=3, na=na, seed=97 + na)
+    f = w.fields()
+    goals = goal_cells(g, k, 5 + k)
+    with _shared(w, f) as m:
+        for variant in range(N_VARIANTS):
+            for thresh in (1e-3, 1e-30):
+                kw = dict(thresh=thresh, variant=variant, max_sweeps=nt + 4, raise_on_nonconv=False)
+                pf, vf, swf, sf = m.solve_goals(goals, full_sweeps=True, **kw)
+                pa, va, swa, sa = m.solve_goals(goals, **kw)
+                assert sa["variant"] == sf["variant"] and sa["status"] == sf["status"] == 0, variant
+                assert np.array_equal(swa, swf), (variant, thresh)
+                assert np.array_equal(va.view(np.uint32), vf.view(np.uint32)), (variant, thresh)
+                assert np.array_equal(pa, pf), (variant, thresh)
+                if sa["sweeps"] > 2:
+                    assert sa["bytes_per_sweep"] < sf["bytes_per_sweep"]
+        # graph-replayed sweeps over the active slices
+        pg, vg, swg, _ = m.solve_goals(goals, thresh=1e-3, use_graph=True, sweeps_per_check=4)
+        pf, vf, swf, _ = m.solve_goals(goals, thresh=1e-3, full_sweeps=True)
+        assert np.array_equal(vg.view(np.uint32), vf.view(np.uint32)) and np.array_equal(swg, swf)
+
+
 def test_batched_rejects_models_it_cannot_serve(gpu):
     w = small(g=8, nt=5, nr=64, nm=1, seed=75)
     f = w.fields()

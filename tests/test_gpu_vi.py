@@ -265,6 +265,50 @@ def test_vi_backward_pass_equals_converged_sweeps(gpu, case):
             assert np.array_equal(pj, pb)
 
 
+@pytest.mark.parametrize("case", ["small", "terminal", "obstacles", "c1"])
+def test_vi_active_slice_sweeps_equal_full_sweeps(gpu, case):
+    """A built model's lean sweeps skip the slices already at their final values (slice t
+    is final after sweep nt - t); values, policies, the sweep count and the residual of
+    the stopping sweep equal those of sweeps over every slice, bit for bit, for early
+    stops (thresh 0.1, 1e-3), the bench's 1e-6 and a zero residual."""
+    from e2e_gpu_dp_amd.configs import Workload
+    if case == "small":
+        w = small(g=11, nt=14, nr=400, nm=2, na=12, seed=53)
+    elif case == "terminal":
+        w = small(g=14, nt=20, nr=300, nm=3, seed=57, term_subgrid_size=3)
+    elif case == "obstacles":
+        b = small(g=16, nt=18, nr=300, nm=2, seed=59, term_subgrid_size=2)
+        w = Workload("obs", b.params, b.seed, b.sigmas, obstacles=[(0, 18, 5, 8, 5, 8)])
+    else:
+        w = workload("C1")
+    f = w.fields()
+    with e2e.build_model(w.params, f) as m:
+        rp = m.export_csr()[0].astype(np.float64)
+        nt, nc, na = w.params.nt, w.params.gsize ** 2, m.info["na"]
+
+        def sweep_bytes(T):  # entries, row pointers, R, V read, V write, policy of slices [0, T)
+            rows = T * nc * na
+            return 8 * rp[rows] + 4 * (rows + 1) + 4 * rows + 12 * T * nc
+
+        for variant in (6, 3):
+            for thresh, graph in ((0.1, False), (1e-3, True), (1e-6, False), (1e-30, True)):
+                kw = dict(thresh=thresh, kernel_variant=variant, max_sweeps=w.params.nt + 4, use_graph=graph,
+                          sweeps_per_check=3, raise_on_nonconv=False)
+                pf, vf, sf = m.solve(full_sweeps=True, **kw)
+                pa, va, sa = m.solve(**kw)
+                print(f"[{case} v{variant} {thresh:g}] sweeps {sa['sweeps']}: {sa['sweep_kernel_ms']:.3f} ms "
+                      f"vs full {sf['sweep_kernel_ms']:.3f} ms", flush=True)
+                assert sa["status"] == sf["status"] == 0
+                assert sa["sweeps"] == sf["sweeps"] and sa["final_residual"] == sf["final_residual"]
+                assert np.array_equal(va.view(np.uint32), vf.view(np.uint32))
+                assert np.array_equal(pa, pf)
+                # the bytes reported: every slice per full sweep, the recomputed ones per active sweep
+                n = sa["sweeps"]
+                assert sf["bytes_per_sweep"] == pytest.approx(sweep_bytes(nt), rel=1e-12)
+                acts = [nt if k <= 2 else max(nt - k + 2, 1) for k in range(1, n + 1)]
+                assert sa["bytes_per_sweep"] == pytest.approx(np.mean([sweep_bytes(T) for T in acts]), rel=1e-12)
+
+
 def test_vi_backward_pass_needs_a_built_model(gpu):
     w = small(g=8, nt=6, nr=100, nm=1, seed=61)
     ref = oracle_model(w.params, w.fields())
