@@ -1048,6 +1048,22 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                 oow = wave_sum(oow);
             }
             // ---- emit the nonzero bins in key order; lane l owns bins 4l .. 4l+3
+            // A key kk < 256 splits into (kk / Wj, kk % Wj), 1 <= Wj <= 256, without a
+            // division per bin: q = (kk * ceil(2^16 / Wj)) >> 16 is exact, since the product
+            // exceeds kk / Wj by less than 256 / 2^16 = 1/256, which for Wj < 256 stays below
+            // the distance 1/Wj from kk / Wj to the next integer (Wj = 256 gives kk >> 8 = 0).
+            const uint32_t wrec = (65536u + Wj - 1u) / Wj;
+            auto key_split = [&](uint32_t kk, int& di, int& dj) {
+                const uint32_t qq = __umul24(kk, wrec) >> 16;
+                di = (int)qq + di0;
+                dj = (int)(kk - __umul24(qq, Wj)) + dj0;
+            };
+            // fast rows whose landing box touches neither the grid's edge rows and columns
+            // nor a terminal cell: every lane's reward before its clamp penalties is tn0
+            const bool plain_box = fast && i1 + dimin > 0 && i1 + dimax < g - 1 && j1 + djmin > 0 &&
+                                   j1 + djmax < g - 1 &&
+                                   (i1 + dimax < A.i_term || i1 + dimin >= A.i_term + A.tsg ||
+                                    j1 + djmax < A.j_term || j1 + djmin >= A.j_term + A.tsg);
             uint32_t cnt4[4];
             int mine = 0;
             int rbin = 0;  // fast rows: sum over the lane's bins of count * (the k = edge reward)
@@ -1058,9 +1074,11 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                 for (int c = 0; c < kHistCopies; c++) cq += hist[(lane * 4 + q) * kHistCopies + c];
                 cnt4[q] = cq;
                 mine += cnt4[q] != 0u;
-                if (fast && cnt4[q] != 0u) {
+                if (fast && !plain_box && cnt4[q] != 0u) {
                     const uint32_t kk = (uint32_t)(lane * 4 + q);
-                    const int pi = i1 + (int)(kk / Wj) + di0, pj = j1 + (int)(kk % Wj) + dj0;
+                    int di, dj;
+                    key_split(kk, di, dj);
+                    const int pi = i1 + di, pj = j1 + dj;
                     const int n1 = g - 1;
                     const bool edge = (pi == 0) | (pi == n1) | (pj == 0) | (pj == n1);  // :339-341
                     const bool term = is_terminal(A, pi, pj);                             // :343-354
@@ -1071,7 +1089,8 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                 // every partial sum is a sum of lane rewards tk[t][edge] (each below
                 // int_lim, nr of them), so rbin's wave sum fits int32; the clamp part
                 // r_outbound * (clamp count) is summed in 64 bits
-                const int64_t tot = (int64_t)wave_sum(rbin) + (int64_t)A.r_out * (int64_t)wave_sum(kcl);
+                const int64_t tot = (plain_box ? (int64_t)tn0 * nr : (int64_t)wave_sum(rbin)) +
+                                    (int64_t)A.r_out * (int64_t)wave_sum(kcl);
                 rsum = (double)tot;
             }
             const int incl = wave_incl_scan(mine);
@@ -1087,7 +1106,8 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             for (int q = 0; q < 4; q++) {
                 if (cnt4[q] != 0u) {
                     const uint32_t kk = (uint32_t)(lane * 4 + q);
-                    const int di = (int)(kk / Wj) + di0, dj = (int)(kk % Wj) + dj0;
+                    int di, dj;
+                    key_split(kk, di, dj);
                     put_entry(A, row, u, nuq, arena_off, (uint32_t)((j1 + dj) + g * (i1 + di)), cnt4[q]);
                     u++;
 #pragma unroll
