@@ -72,11 +72,30 @@ struct BatchArgs {
                            // nt, and each sweep recomputes the active slices only (0: every state)
     uint32_t pre_stride;   // spmm_vi_stream over the active slices: waves + 1, the stride of the
                            // per-prefix partitions in wpart
+    int pre_live;          // live batches: a column's own sweep index is k - start + 1, and a block
+    const int32_t* start;  // recomputes the slices its youngest running column needs ([KP])
 };
 
 namespace {
 
 using BatchFn = void (*)(const BatchArgs);
+
+// Slices [0, act) that sweep k recomputes for the columns c0 .. c0+kb-1 (act = the
+// model's slices when every state is swept). A column admitted at sweep start[j] is at
+// its own sweep k - start[j] + 1, and active_slices of that index covers every slice
+// whose values are not yet final for it in both buffers; the youngest running column of
+// the block needs the most. Wave-uniform (scalar loads of conv and start).
+__device__ __forceinline__ int sweep_active(const BatchArgs& A, uint32_t k, int c0, int kb) {
+    if (!A.pre_nt) return A.nt;
+    uint32_t own = k;
+    if (A.pre_live) {
+        int young = 0;
+        for (int j = c0; j < c0 + kb; j++)
+            if (A.conv[j] < 0) young = max(young, A.start[j]);
+        own = young > 0 ? k - (uint32_t)young + 1u : 0xffffffffu;  // no running column: 1 slice
+    }
+    return active_slices(own, (int)A.pre_nt);
+}
 
 // float residual never below the double one r >= 0 (round to nearest, then one ulp up
 // if that fell below r), the same rounding as the plain sweeps
@@ -207,7 +226,7 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_sweep(const BatchArgs A) {
         rmax[c] = 0.0f;
     }
     const int64_t stride = (int64_t)gridDim.x * kBWaves * SPW;
-    const int64_t s_end = A.pre_nt ? (int64_t)active_slices(k, (int)A.pre_nt) * A.nc : A.ns;
+    const int64_t s_end = A.pre_nt ? (int64_t)sweep_active(A, k, blockIdx.y * KB, KB) * A.nc : A.ns;
     for (int64_t s = ((int64_t)blockIdx.x * kBWaves + wv) * SPW + grp; s < s_end; s += stride) {
         const int t = (int)(s / A.nc);
         const int cell = (int)(s - (int64_t)t * A.nc);
@@ -329,7 +348,9 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_rows(const BatchArgs A) {
     // a sweep covers every state (a built model's: the active slices); one slice of a
     // backward pass covers [s_lo, s_hi)
     const uint32_t s_lo = A.one_pass ? A.s_lo : 0u;
-    const uint32_t s_hi = A.one_pass ? A.s_hi : A.pre_nt ? (uint32_t)active_slices(k, (int)A.pre_nt) * nc : ns;
+    const uint32_t s_hi = A.one_pass ? A.s_hi
+                          : A.pre_nt ? (uint32_t)sweep_active(A, k, blockIdx.y * KB, KB) * nc
+                                     : ns;
     const uint32_t G = (s_hi - s_lo + spw - 1) / spw;
     const unsigned nb = gridDim.x, b = blockIdx.x;
     const unsigned pb = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;
@@ -507,7 +528,7 @@ __global__ __launch_bounds__(kSThreads) void spmm_vi_stream(const BatchArgs A) {
     const unsigned pb = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;
     const unsigned w = pb * kSWaves + wv;
     const uint32_t* wpart = A.wpart;  // a built model's sweep: the active slices' partition
-    if (A.pre_nt) wpart += (size_t)(active_slices(k, (int)A.pre_nt) - 1) * A.pre_stride;
+    if (A.pre_nt) wpart += (size_t)(sweep_active(A, k, blockIdx.y * KB, KB) - 1) * A.pre_stride;
     const uint32_t s_begin = wpart[w], s_end = wpart[w + 1];
     double* wst = stage[wv];
     int2* whc = hitc[wv];
@@ -945,6 +966,7 @@ static BatchArgs make_args(const Model* m, const BatchBufs& b, const e2e_solve_o
     A.rcol_n = 1;
     A.colmax = b.colmax;
     A.wpart = b.wpart;
+    A.start = b.start;
     return A;
 }
 
@@ -1082,6 +1104,35 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
 // Alpha columns one at a time: the plain solve (the default sweep kernel) with R set to
 // each column's rewards. Every column is the plain solve of the model built with that
 // alpha, bit for bit.
+// A built model's transitions go from each slice to the next (the sweeps over the
+// active slices need that); the state count is then nt * nc.
+static bool active_ok(const Model* m) {
+    return m->built && m->nt > 0 && m->nc > 0 && m->ns == m->nt * m->nc;
+}
+
+// The streaming kernel over the active slices: the partition of every prefix of slices
+// (b.wpre, once per model and grid), in place of the full partition.
+static e2e_status use_prefix_parts(const Model* m, BatchBufs& b, BatchArgs& A, hipStream_t s) {
+    if (b.wpre_W != b.wpart_W || b.wpre_spc != b.wpart_spc) {
+        if (b.wpre) (void)hipFree(b.wpre);
+        b.wpre = nullptr;
+        b.wpre_W = 0;
+        if (hipMalloc(&b.wpre, sizeof(uint32_t) * (b.wpart_W + 1) * m->nt) != hipSuccess) {
+            b.wpre = nullptr;
+            (void)hipGetLastError();
+            set_error("batched sweep: device allocation failed");
+            return E2E_ERR_NOMEM;
+        }
+        launch_prefix_partitions(m->row_ptr, m->nc, (int)m->na, m->nt, b.wpart_W, b.wpart_spc, b.wpre, s);
+        E2E_HIP(hipGetLastError());
+        b.wpre_W = b.wpart_W;
+        b.wpre_spc = b.wpart_spc;
+    }
+    A.wpart = b.wpre;
+    A.pre_stride = (uint32_t)(b.wpart_W + 1);
+    return E2E_OK;
+}
+
 static e2e_status vi_solve_alpha_columns(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* value,
                                          int32_t* col_sweeps, e2e_batch_stats* stats, bool backward) {
     const double t_start = now_ms();
@@ -1252,28 +1303,10 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
     const dim3 grid = plan_grid(plan, k);
     // every column starts at sweep 1: on a built model each sweep recomputes the active
     // slices only (active_slices); the streaming kernel takes each prefix's own partition
-    const bool pre = !opts.full_sweeps && m->built && m->nt > 0 && m->nc > 0 && m->ns == m->nt * m->nc;
+    const bool pre = !opts.full_sweeps && active_ok(m);
     if (pre) {
         A.pre_nt = (uint32_t)m->nt;
-        if (V.kind == 2) {
-            if (b.wpre_W != b.wpart_W || b.wpre_spc != b.wpart_spc) {
-                if (b.wpre) (void)hipFree(b.wpre);
-                b.wpre = nullptr;
-                b.wpre_W = 0;
-                if (hipMalloc(&b.wpre, sizeof(uint32_t) * (b.wpart_W + 1) * m->nt) != hipSuccess) {
-                    b.wpre = nullptr;
-                    (void)hipGetLastError();
-                    set_error("batched sweep: device allocation failed");
-                    return E2E_ERR_NOMEM;
-                }
-                launch_prefix_partitions(m->row_ptr, m->nc, (int)m->na, m->nt, b.wpart_W, b.wpart_spc, b.wpre, s);
-                E2E_HIP(hipGetLastError());
-                b.wpre_W = b.wpart_W;
-                b.wpre_spc = b.wpart_spc;
-            }
-            A.wpart = b.wpre;
-            A.pre_stride = (uint32_t)(b.wpart_W + 1);
-        }
+        if (V.kind == 2) E2E_TRY(use_prefix_parts(m, b, A, s));
     }
 
     if (bopt.use_graph &&
@@ -1494,6 +1527,18 @@ e2e_status live_create(Model* m, int kp, float thresh, hipStream_t s, LiveBatch*
         return st;
     }
     L->A = make_args(m, L->b, o);  // after plan_of: the streaming kernel's partition
+    // every block sweeps the slices its youngest running column needs (sweep_active)
+    if (active_ok(m)) {
+        L->A.pre_nt = (uint32_t)m->nt;
+        L->A.pre_live = 1;
+        bool stream = false;
+        for (int i = 0; i < kLiveN; i++) stream = stream || kVariants[width_variant(m, kp, i)].kind == 2;
+        if (stream) st = use_prefix_parts(m, L->b, L->A, s);
+        if (st != E2E_OK) {
+            live_free(L);
+            return st;
+        }
+    }
     const int64_t n = m->ns * kp;
     hipLaunchKernelGGL(live_init, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, L->b.V[0], L->b.V[1], n,
                        L->b.goal, L->b.conv, L->b.start, L->b.last_res, L->b.colmax, kp, L->b.st);
