@@ -956,7 +956,11 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                     djmin = min(djmin, djb);
                     djmax = max(djmax, djb);
                     const unsigned bin = (unsigned)(__mul24(di, kFixSide) + djb);
-                    atomicAdd(&hcopy[min(bin, (unsigned)kWaveBins - 1u) * kHistCopies], 1u);
+                    // copy (lane ^ (bin >> 3)) % 4 of the bin: any lane-to-copy map counts the
+                    // same, and this one gives bins 8 apart (the same LDS bank for one copy)
+                    // different copies (about 1% faster on the 100x100 x nt=120 build)
+                    const unsigned b2 = min(bin, (unsigned)kWaveBins - 1u);
+                    atomicAdd(&hist[b2 * kHistCopies + ((lane ^ (b2 >> 3)) & (kHistCopies - 1))], 1u);
                 }
             };
             const int nfull = nr / 64;
