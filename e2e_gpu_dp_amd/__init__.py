@@ -44,6 +44,16 @@ def device_count() -> int:
     return int(lib().e2e_device_count())
 
 
+def cached_device_bytes() -> int:
+    """Device memory kept mapped for reuse after models closed."""
+    return int(lib().e2e_cached_device_bytes())
+
+
+def release_cached_memory() -> None:
+    """Returns the device memory kept for reuse to the HIP runtime."""
+    lib().e2e_release_cached_memory()
+
+
 def pack_params(p) -> np.ndarray:
     """The reference's 32-float prob_params vector (src/build_model.cu:1032-1055)."""
     out = np.zeros(32, np.float32)

@@ -148,6 +148,8 @@ SIGNATURES = {
     "e2e_last_error": (C.c_char_p, []),
     "e2e_version": (C.c_char_p, []),
     "e2e_device_count": (C.c_int32, []),
+    "e2e_cached_device_bytes": (C.c_int64, []),
+    "e2e_release_cached_memory": (None, []),
     "e2e_struct_size": (C.c_int32, [C.c_int32]),
     "e2e_params_pack": (C.c_int, [P(Params), VP]),
     "e2e_actions": (C.c_int, [P(Params), VP, VP]),

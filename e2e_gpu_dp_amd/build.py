@@ -32,7 +32,7 @@ COMMON = [
     "-Wno-unused-command-line-argument", "-Wno-unknown-warning-option",
 ]
 
-LIB_SOURCES = ["abi.cpp", "build.hip", "scan.hip", "vi.hip", "spmm_vi.hip", "rollout.hip", "envgen.hip",
+LIB_SOURCES = ["abi.cpp", "devpool.cpp", "build.hip", "scan.hip", "vi.hip", "spmm_vi.hip", "rollout.hip", "envgen.hip",
                "server.cpp"]
 DRIVERS = {"build_model": "build_model_main.cpp", "spvi": "spvi_main.cpp"}
 

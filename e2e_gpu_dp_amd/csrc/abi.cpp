@@ -47,8 +47,7 @@ e2e_status free_model(Model* m) {
     void* ptrs[] = {m->row_ptr, m->nz,      m->R,        m->Rsum,     m->V[0],   m->V[1],     m->W[0], m->W[1],
                     m->pi,      m->st,      m->vi_part,  m->vi_wpart, m->vi_tdesc, m->vi_jnz, m->vi_jmeta, m->vi_jR,
                     m->vi_cz,   m->vi_cbase, m->Ralpha, m->vi_bw_wpart, m->vi_pre_wpart};
-    for (void* p : ptrs)
-        if (p) (void)hipFree(p);
+    for (void* p : ptrs) pool_free(p);
     if (m->own_stream && m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
     return E2E_OK;
@@ -119,6 +118,10 @@ int32_t e2e_device_count(void) {
     }
     return n;
 }
+
+int64_t e2e_cached_device_bytes(void) { return (int64_t)pool_cached_bytes(); }
+
+void e2e_release_cached_memory(void) { pool_release_all(); }
 
 int32_t e2e_struct_size(int32_t which) {
     switch (which) {

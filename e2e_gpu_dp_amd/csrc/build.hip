@@ -1504,11 +1504,11 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     DevBuf dNnz, dStage, dArena, dTop;
     if ((st = dmalloc(dNnz, sizeof(uint32_t) * rows))) return fail(st);
     if ((st = dmalloc(dStage, sizeof(uint2) * rows * K))) return fail(st);
-    if (hipMalloc(&m->R, sizeof(float) * rows) != hipSuccess) {
+    if (pool_malloc(&m->R, sizeof(float) * rows) != hipSuccess) {
         set_error("hipMalloc R failed");
         return fail(E2E_ERR_NOMEM);
     }
-    if (opts.keep_rsum && hipMalloc(&m->Rsum, sizeof(float) * rows) != hipSuccess) {
+    if (opts.keep_rsum && pool_malloc(&m->Rsum, sizeof(float) * rows) != hipSuccess) {
         set_error("hipMalloc Rsum failed");
         return fail(E2E_ERR_NOMEM);
     }
@@ -1708,7 +1708,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     DevBuf dTmp, dTot;
     if ((st = dmalloc(dTmp, sizeof(unsigned long long) * scan_tmp_words(rows)))) return fail(st);
     if ((st = dmalloc(dTot, sizeof(unsigned long long)))) return fail(st);
-    if (hipMalloc(&m->row_ptr, sizeof(uint32_t) * (rows + 1)) != hipSuccess) {
+    if (pool_malloc(&m->row_ptr, sizeof(uint32_t) * (rows + 1)) != hipSuccess) {
         set_error("hipMalloc row_ptr failed");
         return fail(E2E_ERR_NOMEM);
     }
@@ -1756,7 +1756,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
 }
 
 e2e_status alloc_nz(Model* m, int64_t nnz) {
-    if (hipMalloc(&m->nz, sizeof(int2) * (nnz + kNzPad)) != hipSuccess) {
+    if (pool_malloc(&m->nz, sizeof(int2) * (nnz + kNzPad)) != hipSuccess) {
         m->nz = nullptr;
         set_error("hipMalloc nz (%lld entries) failed", (long long)nnz);
         return E2E_ERR_NOMEM;

@@ -150,12 +150,24 @@ struct Model {
     e2e_model_info info{};
 };
 
+// Device allocations that outlive one model (devpool.cpp): blocks of 1 MiB or more freed
+// with pool_free stay mapped for later pool_malloc calls of a similar size. pool_free
+// takes any device pointer (others go to hipFree); a pooled block must not reach hipFree.
+hipError_t pool_malloc(void** p, size_t bytes);
+void pool_free(void* p);
+void pool_release_all();
+size_t pool_cached_bytes();
+template <typename T>
+inline hipError_t pool_malloc(T** p, size_t bytes) {
+    return pool_malloc(reinterpret_cast<void**>(p), bytes);
+}
+
 // Scoped device allocation for temporaries.
 struct DevBuf {
     void* p = nullptr;
     ~DevBuf() { reset(); }
     void reset() {
-        if (p) (void)hipFree(p);
+        if (p) pool_free(p);
         p = nullptr;
     }
     template <typename T>
@@ -163,7 +175,7 @@ struct DevBuf {
 };
 
 inline e2e_status dmalloc(DevBuf& b, size_t bytes) {
-    hipError_t e = hipMalloc(&b.p, bytes ? bytes : 16);
+    hipError_t e = pool_malloc(&b.p, bytes ? bytes : 16);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         b.p = nullptr;

@@ -25,6 +25,7 @@
 // (spvi_2.cu:669-698).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "internal.h"
@@ -1214,6 +1215,16 @@ SweepFn pick_sweep(int na, int variant, int u, int* u_used) {
 e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* value,
                     e2e_solve_stats* stats) {
     const double t_start = now_ms();
+    // E2E_TRACE set: host time of each phase to stderr (it waits for the stream)
+    const bool trace = std::getenv("E2E_TRACE") != nullptr;
+    double t_mark = t_start;
+    auto phase = [&](const char* what) {
+        if (!trace) return;
+        (void)hipStreamSynchronize(m->stream);
+        const double t = now_ms();
+        std::fprintf(stderr, "[e2e solve] %-28s %9.3f ms\n", what, t - t_mark);
+        t_mark = t;
+    };
     e2e_solve_opts opts{};
     opts.thresh = 0.1f;  // spvi_2.cu:757
     opts.discount = 1.0f;  // spvi_2.cu:61
@@ -1223,17 +1234,17 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     const int spc = opts.sweeps_per_check > 0 ? opts.sweeps_per_check : 16;
     hipStream_t s = m->stream;
     if (!m->V[0]) {
-        if (hipMalloc(&m->V[0], sizeof(float) * ns) != hipSuccess ||
-            hipMalloc(&m->W[0], sizeof(double) * ns) != hipSuccess ||
-            hipMalloc(&m->W[1], sizeof(double) * ns) != hipSuccess ||
-            hipMalloc(&m->V[1], sizeof(float) * ns) != hipSuccess ||
-            hipMalloc(&m->pi, sizeof(int32_t) * ns) != hipSuccess ||
-            hipMalloc(&m->st, sizeof(VIState)) != hipSuccess ||
-            hipMalloc(&m->vi_part, sizeof(float) * kVIMaxBlocks) != hipSuccess) {
+        if (pool_malloc(&m->V[0], sizeof(float) * ns) != hipSuccess ||
+            pool_malloc(&m->W[0], sizeof(double) * ns) != hipSuccess ||
+            pool_malloc(&m->W[1], sizeof(double) * ns) != hipSuccess ||
+            pool_malloc(&m->V[1], sizeof(float) * ns) != hipSuccess ||
+            pool_malloc(&m->pi, sizeof(int32_t) * ns) != hipSuccess ||
+            pool_malloc(&m->st, sizeof(VIState)) != hipSuccess ||
+            pool_malloc(&m->vi_part, sizeof(float) * kVIMaxBlocks) != hipSuccess) {
             void** bufs[] = {(void**)&m->V[0], (void**)&m->W[0], (void**)&m->W[1], (void**)&m->V[1],
                              (void**)&m->pi,   (void**)&m->st,   (void**)&m->vi_part};
             for (void** b : bufs) {  // all or nothing, so a later call allocates again
-                if (*b) (void)hipFree(*b);
+                if (*b) pool_free(*b);
                 *b = nullptr;
             }
             set_error("vi_solve: device allocation failed");
@@ -1274,9 +1285,9 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
         E2E_CHECK(m->na <= 64 && m->info.max_row_nnz < 65536, E2E_ERR_CAPACITY,
                   "vi_solve: the jagged sweep needs at most 64 actions and rows shorter than 65536");
         if (!m->vi_jnz) {
-            if (hipMalloc(&m->vi_jnz, sizeof(int2) * (m->nnz + kNzPad)) != hipSuccess ||
-                hipMalloc(&m->vi_jmeta, sizeof(uint32_t) * m->rows) != hipSuccess ||
-                hipMalloc(&m->vi_jR, sizeof(float) * m->rows) != hipSuccess) {
+            if (pool_malloc(&m->vi_jnz, sizeof(int2) * (m->nnz + kNzPad)) != hipSuccess ||
+                pool_malloc(&m->vi_jmeta, sizeof(uint32_t) * m->rows) != hipSuccess ||
+                pool_malloc(&m->vi_jR, sizeof(float) * m->rows) != hipSuccess) {
                 set_error("vi_solve: device allocation of the jagged layout failed");
                 return E2E_ERR_NOMEM;
             }
@@ -1296,10 +1307,10 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
         tile_states = (uint32_t)(16 * kStreamWaves * grp);
         ntiles = ceil_div(ns, (int64_t)tile_states);
         if (m->vi_tile_states != tile_states) {
-            if (m->vi_tdesc) (void)hipFree(m->vi_tdesc);
+            if (m->vi_tdesc) pool_free(m->vi_tdesc);
             m->vi_tdesc = nullptr;
             m->vi_tile_states = 0;
-            if (hipMalloc(&m->vi_tdesc, sizeof(uint2) * ntiles) != hipSuccess) {
+            if (pool_malloc(&m->vi_tdesc, sizeof(uint2) * ntiles) != hipSuccess) {
                 m->vi_tdesc = nullptr;
                 set_error("vi_solve: device allocation failed");
                 return E2E_ERR_NOMEM;
@@ -1319,10 +1330,10 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
         vwin = (uint32_t)std::min<uint64_t>((m->vi_maxspan + 63u) / 64u * 64u, kMaxBand);
         if (vwin == 0) vwin = 64;
         if (m->vi_part_cap < ntiles) {
-            (void)hipFree(m->vi_part);
+            pool_free(m->vi_part);
             m->vi_part = nullptr;
             m->vi_part_cap = 0;
-            if (hipMalloc(&m->vi_part, sizeof(float) * ntiles) != hipSuccess) {
+            if (pool_malloc(&m->vi_part, sizeof(float) * ntiles) != hipSuccess) {
                 m->vi_part = nullptr;
                 set_error("vi_solve: device allocation failed");
                 return E2E_ERR_NOMEM;
@@ -1351,16 +1362,16 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     const int span = opts.comp_span > 0 ? opts.comp_span : 65536;
     E2E_CHECK(span <= 65536, E2E_ERR_INVALID, "vi_solve: comp_span must be at most 65536 (got %d)", opts.comp_span);
     if (comp && m->vi_cz && m->vi_comp_span != span) {  // converted with another span
-        (void)hipFree(m->vi_cz);
-        (void)hipFree(m->vi_cbase);
+        pool_free(m->vi_cz);
+        pool_free(m->vi_cbase);
         m->vi_cz = nullptr;
         m->vi_cbase = nullptr;
     }
     if (comp && !m->vi_cz) {  // the compressed stream, once per model
         const int64_t nwin = ceil_div(m->nnz, (int64_t)kCompWin);
-        if (hipMalloc(&m->vi_cz, sizeof(uint32_t) * (size_t)std::max<int64_t>(nwin, 1) * kCompWin) != hipSuccess ||
-            hipMalloc(&m->vi_cbase, sizeof(int32_t) * (size_t)std::max<int64_t>(nwin, 1)) != hipSuccess) {
-            if (m->vi_cz) (void)hipFree(m->vi_cz);
+        if (pool_malloc(&m->vi_cz, sizeof(uint32_t) * (size_t)std::max<int64_t>(nwin, 1) * kCompWin) != hipSuccess ||
+            pool_malloc(&m->vi_cbase, sizeof(int32_t) * (size_t)std::max<int64_t>(nwin, 1)) != hipSuccess) {
+            if (m->vi_cz) pool_free(m->vi_cz);
             m->vi_cz = nullptr;
             m->vi_cbase = nullptr;
             set_error("vi_solve: device allocation of the compressed stream failed");
@@ -1393,10 +1404,10 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
                   "vi_solve: the lean sweep addresses states and rows with 32-bit byte offsets");
         const int64_t W = nblocks * kStreamWaves;
         if (m->vi_wpart_waves != W || m->vi_wpart_spc != grp) {
-            if (m->vi_wpart) (void)hipFree(m->vi_wpart);
+            if (m->vi_wpart) pool_free(m->vi_wpart);
             m->vi_wpart = nullptr;
             m->vi_wpart_waves = 0;
-            if (hipMalloc(&m->vi_wpart, sizeof(uint32_t) * (W + 1)) != hipSuccess) {
+            if (pool_malloc(&m->vi_wpart, sizeof(uint32_t) * (W + 1)) != hipSuccess) {
                 m->vi_wpart = nullptr;
                 set_error("vi_solve: device allocation failed");
                 return E2E_ERR_NOMEM;
@@ -1409,11 +1420,11 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
         }
         if (active) {  // the partitions of every prefix of slices, once per model and grid
             if (m->vi_pre_waves != W || m->vi_pre_spc != grp) {
-                if (m->vi_pre_wpart) (void)hipFree(m->vi_pre_wpart);
+                if (m->vi_pre_wpart) pool_free(m->vi_pre_wpart);
                 m->vi_pre_wpart = nullptr;
                 m->vi_pre_waves = 0;
                 // + the row pointer at each slice boundary (nt + 1), for the bytes of each sweep
-                if (hipMalloc(&m->vi_pre_wpart, sizeof(uint32_t) * ((W + 1) * m->nt + m->nt + 1)) != hipSuccess) {
+                if (pool_malloc(&m->vi_pre_wpart, sizeof(uint32_t) * ((W + 1) * m->nt + m->nt + 1)) != hipSuccess) {
                     m->vi_pre_wpart = nullptr;
                     set_error("vi_solve: device allocation failed");
                     return E2E_ERR_NOMEM;
@@ -1428,6 +1439,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
             }
         }
     }
+    phase("buffers, layouts, partitions");
     VIArgs A;
     std::memset(&A, 0, sizeof(A));  // padding too: the graph cache compares the bytes
     A.rp = m->row_ptr;
@@ -1467,10 +1479,10 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
         const int64_t nbs = std::max<int64_t>(1, std::min<int64_t>(ceil_div(groups, (int64_t)kStreamWaves), nblocks));
         const int64_t Ws = nbs * kStreamWaves;
         if (m->vi_bw_waves != Ws || m->vi_bw_spc != grp) {
-            if (m->vi_bw_wpart) (void)hipFree(m->vi_bw_wpart);
+            if (m->vi_bw_wpart) pool_free(m->vi_bw_wpart);
             m->vi_bw_wpart = nullptr;
             m->vi_bw_waves = 0;
-            if (hipMalloc(&m->vi_bw_wpart, sizeof(uint32_t) * (Ws + 1) * nts) != hipSuccess) {
+            if (pool_malloc(&m->vi_bw_wpart, sizeof(uint32_t) * (Ws + 1) * nts) != hipSuccess) {
                 m->vi_bw_wpart = nullptr;
                 set_error("vi_solve: device allocation failed");
                 return E2E_ERR_NOMEM;
@@ -1592,6 +1604,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    phase("sweeps");
     const int sweeps = hst.conv >= 0 ? hst.conv : (int)hst.k - 1;
     if (rc == E2E_OK && hst.conv < 0) rc = E2E_ERR_NOT_CONVERGED;
     if (rc == E2E_ERR_NOT_CONVERGED) set_error("not converged after %d sweeps (residual %g)", sweeps, hst.last_res);
@@ -1612,6 +1625,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
                                sizeof(uint32_t) * (m->nt + 1), hipMemcpyDeviceToHost, s));
     }
     E2E_HIP(hipStreamSynchronize(s));
+    phase("results to the host");
     if (stats) {
         stats->sweeps = sweeps;
         stats->converged = hst.conv >= 0;

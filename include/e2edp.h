@@ -246,6 +246,14 @@ const char* e2e_version(void);
 /* Number of HIP devices visible (0 without a GPU; never fails). */
 int32_t e2e_device_count(void);
 
+/* Device memory the library keeps mapped after models close (no reference counterpart:
+ * the reference allocates and frees per executable run). Freed blocks of 1 MiB or more
+ * stay mapped, up to E2E_POOL_BYTES bytes (environment; default 32 GiB, 0 = off), for
+ * later builds and solves of similar size. e2e_cached_device_bytes reports the idle
+ * bytes; e2e_release_cached_memory returns them to the HIP runtime. */
+int64_t e2e_cached_device_bytes(void);
+void e2e_release_cached_memory(void);
+
 /* sizeof of the ABI structs, for binding checks: 0 e2e_params, 1 e2e_fields,
  * 2 e2e_build_opts, 3 e2e_model_info, 4 e2e_solve_opts, 5 e2e_solve_stats,
  * 6 e2e_batch_opts, 7 e2e_batch_stats, 8 e2e_server_opts, 9 e2e_request_result,
