@@ -263,8 +263,16 @@ struct Server {
                 }
             }
             // ---- launch admissions and sweeps on every active model's stream (async)
+            // sweeps per model in this step: the largest running model takes spc, a model
+            // with fewer entries proportionally more (up to 16 spc), so that small models
+            // are not held to the sweep rate of the largest (extra sweeps of converged
+            // columns do nothing)
             const int spc = o.sweeps_per_step > 0 ? o.sweeps_per_step : 4;
+            int64_t nnz_max = 1;
+            for (Env* e : active) nnz_max = std::max<int64_t>(nnz_max, e->model->nnz);
             for (Env* e : active) {
+                const int64_t ratio = nnz_max / std::max<int64_t>(e->model->nnz, 1);
+                const int sw = o.exact_pass ? 1 : (int)std::min<int64_t>((int64_t)spc * 16, (int64_t)spc * std::max<int64_t>(ratio, 1));
                 if (!e->adm_cols.empty()) {
                     live_admit(e->batch, e->adm_cols.data(), e->adm_goals.data(), (int)e->adm_cols.size());
                     e->adm_cols.clear();
@@ -278,11 +286,11 @@ struct Server {
                 if (o.exact_pass)
                     live_pass(e->batch, hi);
                 else
-                    live_step(e->batch, spc, hi);
+                    live_step(e->batch, sw, hi);
                 for (int c = 0; c < (int)e->slot.size(); c++)
-                    if (e->slot[c]) e->steps_since_admit[c] += spc;
+                    if (e->slot[c]) e->steps_since_admit[c] += o.exact_pass ? spc : sw;
+                sweeps_launched += o.exact_pass ? spc : sw;
             }
-            sweeps_launched += (int64_t)spc * active.size();
             // ---- harvest converged columns
             for (Env* e : active) {
                 BatchState st{};

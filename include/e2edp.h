@@ -212,7 +212,9 @@ typedef struct e2e_batch_stats {
 typedef struct e2e_server_opts {
     int64_t hbm_budget_bytes;  /* device bytes for models + batches; 0: 80% of free memory */
     int32_t max_cols;          /* goal columns per model batch, 1..64 (0: 64) */
-    int32_t sweeps_per_step;   /* sweeps launched per model between scheduler checks (0: 4) */
+    int32_t sweeps_per_step;   /* sweeps launched per scheduler step on the running model with the most
+                                  entries (0: 4); a model with k times fewer entries gets k times as many,
+                                  up to 16 times */
     int32_t keep_full_results; /* 1: keep each request's full value/policy columns on the host */
     int32_t serial;            /* 1: one request on the GPU at a time (queue-only baseline) */
     float thresh;              /* VI stopping threshold (0: 1e-6) */
