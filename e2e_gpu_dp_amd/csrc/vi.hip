@@ -1580,11 +1580,15 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     int launched = 0;
     VIState hst{};
     e2e_status rc = E2E_OK;
+    // host checks after 16, 32, 64, 128, ... sweeps (chunks of 16, 16, 32, 64 and then 64)
+    // unless sweeps_per_check fixes the chunk: fewer waits, and launches after convergence
+    // return at once
+    int chunk = spc;
     while (launched < max_sweeps) {
-        int n = std::min(spc, max_sweeps - launched);
+        int n = std::min(chunk, max_sweeps - launched);
         E2E_HIP(hipEventRecord(e0, s));
-        if (opts.use_graph && n == spc) {
-            E2E_HIP(hipGraphLaunch(m->graph, s));
+        if (opts.use_graph && n % spc == 0) {
+            for (int r = 0; r < n / spc; r++) E2E_HIP(hipGraphLaunch(m->graph, s));
         } else {
             for (int i = 0; i < n; i++) hipLaunchKernelGGL(fn, dim3((unsigned)nblocks), dim3(threads), shmem, s, A);
             E2E_HIP(hipGetLastError());
@@ -1596,6 +1600,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
         E2E_HIP(hipEventElapsedTime(&ms, e0, e1));
         kms += ms;
         launched += n;
+        if (opts.sweeps_per_check <= 0 && launched >= 2 * spc) chunk = std::min(chunk * 2, 4 * spc);
         if (hst.conv >= 0) break;
         if (opts.timeout_ms > 0 && now_ms() - t_start > opts.timeout_ms) {
             rc = E2E_ERR_TIMEOUT;
