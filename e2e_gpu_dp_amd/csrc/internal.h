@@ -51,8 +51,11 @@ constexpr int64_t kNzPad = 4096;
 // values in both value buffers (written at sweep k - 2 or before, unchanged since): the
 // sweep recomputes slices [0, active_slices(k, nt)) and gives every value, policy and
 // residual the full sweep gives.
+// Always in [1, nt], for every k.
 __host__ __device__ inline int active_slices(uint32_t k, int nt) {
-    return k <= 2u ? nt : (nt - (int)k + 2 > 1 ? nt - (int)k + 2 : 1);
+    if (k <= 2u) return nt;
+    if (k >= (uint32_t)nt + 1u) return 1;
+    return nt - (int)k + 2;  // 3 <= k <= nt: [2, nt - 1]
 }
 
 // Device-side solver state of one solve (one per model).

@@ -92,7 +92,8 @@ __device__ __forceinline__ int sweep_active(const BatchArgs& A, uint32_t k, int 
         int young = 0;
         for (int j = c0; j < c0 + kb; j++)
             if (A.conv[j] < 0) young = max(young, A.start[j]);
-        own = young > 0 ? k - (uint32_t)young + 1u : 0xffffffffu;  // no running column: 1 slice
+        if (young <= 0) return 1;  // no running column in the block: nothing is written
+        own = k - (uint32_t)young + 1u;  // start <= k for a running column
     }
     return active_slices(own, (int)A.pre_nt);
 }

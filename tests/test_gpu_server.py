@@ -48,6 +48,29 @@ def test_server_mix_equals_solo(gpu):
             assert r["value_at_start"] == val[s0] and r["action_at_start"] == int(np.int32(pol[s0]))
 
 
+def test_server_long_steps_with_converged_column_blocks(gpu):
+    """Many sweeps per scheduler step: column blocks whose requests have all converged
+    keep being launched until the step's harvest, beside blocks still running; every
+    result still equals its solo solve."""
+    envs = _envs()
+    p, f = envs[2]
+    rng = np.random.default_rng(11)
+    goals = [int(x) for x in rng.integers(0, p.nc, 12)]
+    with e2e.Server(keep_full_results=True, thresh=1e-6, sweeps_per_step=64) as srv:
+        srv.add_env(2, p, f)
+        ids = [srv.submit(2, g, 0) for g in goals]
+        srv.drain(timeout_ms=120000)
+        assert srv.stats()["completed"] == len(goals)
+        for rid, g in zip(ids, goals):
+            r = srv.wait(rid)
+            assert r["status_name"] == "done"
+            pol, val = srv.columns(rid, 2)
+            spol, sval, ssw = _solo(p, f, g)
+            assert r["sweeps"] == ssw
+            assert np.array_equal(val.view(np.uint32), sval.view(np.uint32))
+            assert np.array_equal(pol, spol)
+
+
 def test_server_budget_evicts_and_rejects(gpu):
     envs = _envs()
     with e2e.Server(hbm_budget_bytes=120 << 20, keep_full_results=False) as srv:
