@@ -1345,11 +1345,14 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
     double kms = 0.0;
     int launched = 0;
     BatchState hst{};
+    // host checks after 8, 16, 32, 64 sweeps and then every 64 (chunks 8, 8, 16, 32, 64)
+    // unless sweeps_per_check fixes the chunk; launches after convergence return at once
+    int chunk = spc;
     while (launched < max_sweeps) {
-        const int n = std::min(spc, max_sweeps - launched);
+        const int n = std::min(chunk, max_sweeps - launched);
         E2E_HIP(hipEventRecord(e0, s));
-        if (bopt.use_graph && n == spc) {
-            E2E_HIP(hipGraphLaunch(c->graph, s));
+        if (bopt.use_graph && n % spc == 0) {
+            for (int r = 0; r < n / spc; r++) E2E_HIP(hipGraphLaunch(c->graph, s));
         } else {
             for (int i = 0; i < n; i++) hipLaunchKernelGGL(plan.fn, grid, plan.block, 0, s, A);
             E2E_HIP(hipGetLastError());
@@ -1361,6 +1364,7 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
         E2E_HIP(hipEventElapsedTime(&ms, e0, e1));
         kms += ms;
         launched += n;
+        if (opts.sweeps_per_check <= 0 && launched >= 2 * spc) chunk = std::min(chunk * 2, 8 * spc);
         if (hst.all_conv) break;
         if (opts.timeout_ms > 0 && now_ms() - t_start > opts.timeout_ms) break;
     }

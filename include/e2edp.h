@@ -142,7 +142,8 @@ typedef struct e2e_solve_opts {
     float discount;            /* Q = R + discount * P V (spvi_2.cu:61,189) */
     int32_t max_sweeps;        /* 0: nt + 2 for a built model, 100000 for an imported one */
     int32_t sweeps_per_check;  /* sweeps launched between host checks of the converged flag (0: checks after
-                                  16, 32, 64, 128 sweeps and then every 64) */
+                                  16, 32, 64, 128 sweeps and then every 64; batched solves after 8, 16,
+                                  32, 64 and then every 64) */
     int32_t use_graph;         /* 1: replay the sweep chain as a captured hipGraph */
     int32_t loads_in_flight;   /* nnz loads per lane per window: 2, 4 or 8 (0: chosen from nnz per state) */
     double timeout_ms;         /* 0: none */
