@@ -105,7 +105,7 @@ static int is_within_band(const lane_ctx* c, int i, int j, int i1, int j1, int i
     const float* ys = c->ys;
     int g = c->g;
     float cell_diag = fabsf(xs[1] - xs[0]) * 1.414213;
-    float x = xs[j], y = ys[g - 1 - i];
+    float x = xs[j], y = (g_mut == 11) ? ys[i] : ys[g - 1 - i];
     float x1 = xs[j1], y1 = ys[g - 1 - i1];
     float x2 = xs[j2], y2 = ys[g - 1 - i2];
     float A = (y2 - y1) / (x2 - x1);
@@ -125,9 +125,9 @@ static int goes_through_obstacle(const lane_ctx* c, long long sp1, long long sp2
     int jmin = p1j < p2j ? p1j : p2j, jmax = p1j > p2j ? p1j : p2j;
     for (int i = imin; i <= imax; i++)
         for (int j = jmin; j <= jmax; j++) {
-            if (!(i == p1i && j == p1j)) {
+            if (g_mut == 13 || !(i == p1i && j == p1j)) {
                 long long sp = j + (long long)g * i;
-                if (is_in_obstacle(c, sp, T) || is_in_obstacle(c, sp, T + 1))
+                if (is_in_obstacle(c, sp, T) || (g_mut != 12 && is_in_obstacle(c, sp, T + 1)))
                     if (is_within_band(c, i, j, p1i, p1j, p2i, p2j)) return 1;
             }
         }
@@ -145,11 +145,18 @@ static float step_reward(const e2e_params* p, float ac_speed, float rad1, float 
     float energy_reward, time_reward;
     if (method == 0) return -dt;
     if (method == 1) return -(Cf * ac_speed * ac_speed + Ct) * dt;
-    if (method == 2) return ((Cr * (rad2 + rad1) / 2) - (Cf * ac_speed * ac_speed) - Ct) * dt;
-    if (method == 3) return ((Cr * (rad2 + rad1) / 2) - Ct) * dt;
+    if (method == 2) {
+        if (g_mut == 9) return ((Cr * (rad2 + rad1)) - (Cf * ac_speed * ac_speed) - Ct) * dt;
+        return ((Cr * (rad2 + rad1) / 2) - (Cf * ac_speed * ac_speed) - Ct) * dt;
+    }
+    if (method == 3) {
+        if (g_mut == 15) return ((Cr * (rad1 + rad1) / 2) - Ct) * dt;
+        return ((Cr * (rad2 + rad1) / 2) - Ct) * dt;
+    }
     if (method == 4) {
         energy_reward = -(ac_speed * ac_speed) * dt;
         time_reward = -dt;
+        if (g_mut == 10) return (1 - alpha) * energy_reward + alpha * time_reward;
         return alpha * energy_reward + (1 - alpha) * time_reward;
     }
     return 0;
@@ -252,7 +259,7 @@ static int lane_eval(const lane_ctx* c, float speed, float fcos, float fsin, int
         if (!is_in_obstacle(c, sp, t)) {
             move_ref(c, fcos, fsin, vx, vy, t, posids, &r);
             long long sp2 = posids[1] + (long long)g * posids[0];
-            float rad2 = f->s ? f->s[(long long)(t + 1) * nc + sp2] : 0.0f;
+            float rad2 = f->s ? f->s[(long long)(g_mut == 14 ? t : t + 1) * nc + sp2] : 0.0f;
             float r_step = step_reward(p, speed, rad1, rad2);
             r += r_step;
             if (goes_through_obstacle(c, sp, sp2, t)) r = p->r_outbound;

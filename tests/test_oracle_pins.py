@@ -115,6 +115,166 @@ def pin_two_point_modes():
             assert (m["val"][rows] == f32(5) / f32(10)).all()
 
 
+# ------------------------------------------------------------------ reward methods 1-4
+
+def expected_step(p: Params, speed: float, rad1: float, rad2: float) -> float:
+    """One-step reward from what each method means (src/build_model.cu:231-265), in
+    float64 and in a different arrangement from the float32 source: method 1 charges
+    the energy F^2 plus a small time cost 0.01 per unit time; methods 2 and 3 credit the
+    mean of the scalar field at the start and landing cells; method 4 blends the energy
+    and time costs with weight alpha on energy."""
+    ct = 0.01
+    m = p.reward_type
+    if m == 0:
+        return -p.dt
+    if m == 1:
+        return -p.dt * (speed * speed + ct)
+    if m == 2:
+        return p.dt * (0.5 * rad1 + 0.5 * rad2 - speed * speed - ct)
+    if m == 3:
+        return p.dt * (0.5 * rad1 + 0.5 * rad2 - ct)
+    if m == 4:
+        return -p.dt * (p.alpha * speed * speed + (1.0 - p.alpha))
+    raise ValueError(m)
+
+
+def _scalar_field(nt, g):
+    """Strictly positive scalar field that differs between neighbouring cells and
+    between consecutive slices (so a read at the wrong cell or slice shows)."""
+    t, i, j = np.meshgrid(np.arange(nt), np.arange(g), np.arange(g), indexing="ij")
+    return (0.25 + 0.25 * ((3 * t + 5 * i + 7 * j) % 11)).astype(f32)
+
+
+def pin_reward_methods():
+    """Zero velocity field, 2 speeds (F/2, F) x 8 headings, dt = 2, a terminal cell and
+    late-penalty slices: every lane's reward is the move's penalties (expected_lane)
+    plus the method's step reward at rad1 = s[t, S1], rad2 = s[t+1, S2]."""
+    base = Params(gsize=9, nt=22, nrzns=2, nmodes=1, num_ac_speeds=2, num_ac_angles=8, i_term=4,
+                  j_term=5, dt=2.0, alpha=0.3)
+    f = zero_fields(base.gsize, base.nt, 1, base.nrzns)
+    f["s"] = _scalar_field(base.nt, base.gsize)
+    for method in (1, 2, 3, 4):
+        p = base.replace(reward_type=method)
+        speed, _, fc, fs = oracle.actions(p)
+        assert sorted(set(speed.tolist())) == [0.5, 1.0]
+        for t in (0, 2, 3):   # T = 2 is the last slice without the late penalty (T > nt - 20)
+            for a in range(p.na):
+                sp2, r = oracle.lanes(p, f, t, a)
+                for sp in range(p.nc):
+                    i1, j1 = divmod(sp, p.gsize)
+                    if (i1, j1) == (p.i_term, p.j_term):
+                        assert (sp2[sp] == sp).all() and (r[sp] == 0).all()
+                        continue
+                    e_sp2, e_r = expected_lane(p, fc[a], fs[a], 0.0, 0.0, t, i1, j1)
+                    i2, j2 = divmod(e_sp2, p.gsize)
+                    rad1 = float(f["s"][t, i1, j1])
+                    rad2 = float(f["s"][t + 1, i2, j2])
+                    e_r = e_r + p.dt + expected_step(p, float(speed[a]), rad1, rad2)
+                    assert (sp2[sp] == e_sp2).all(), (method, t, a, sp)
+                    assert np.allclose(r[sp], e_r, rtol=0, atol=2e-4), (method, t, a, sp, r[sp][0], e_r)
+        # the model's R is the mean of the (here identical) lane rewards
+        m = oracle.build(p, f, t_begin=2, t_end=4)
+        for t in (2, 3):
+            for a in (0, 5, 11):
+                sp2, r = oracle.lanes(p, f, t, a)
+                rows = a * p.ns + t * p.nc + np.arange(p.nc)
+                assert np.allclose(m["R"][rows], r[:, 0], rtol=0, atol=2e-4)
+
+
+def _band_hit(p, i, j, i1, j1, i2, j2) -> bool:
+    """Is cell (i, j) within the band of the S1-S2 centre line (exact rationals)?
+    Same row or column: always. Else the point-to-line distance of the cell centres
+    against the cell diagonal float32(dx * 1.414213)."""
+    if i1 == i2 or j1 == j2:
+        return True
+    g = p.gsize
+    X = lambda jj: Fraction(float(f32(p.x0))) + jj * Fraction(float(f32(p.dx)))
+    Y = lambda ii: Fraction(float(f32(p.y0))) + (g - 1 - ii) * Fraction(float(f32(p.dy)))
+    x, y, x1, y1, x2, y2 = X(j), Y(i), X(j1), Y(i1), X(j2), Y(i2)
+    num = ((y2 - y1) * (x - x1) - (x2 - x1) * (y - y1)) ** 2
+    den = (x2 - x1) ** 2 + (y2 - y1) ** 2
+    diag = Fraction(float(f32(float(f32(p.dx)) * 1.414213)))
+    lhs, rhs = num, diag * diag * den
+    if abs(lhs - rhs) < Fraction(1, 1000) * rhs:
+        return None   # within rounding of the boundary: not used as a case
+    return lhs < rhs
+
+
+def pin_obstacle_crossing():
+    """Brute force over single obstacle cells. A uniform flow moves every start cell by
+    a known (di, dj); an obstacle at slice t or t+1 anywhere in the S1-S2 bounding box
+    other than S1, and within the band of the S1-S2 line, turns the reward into
+    r_outbound (src/build_model.cu:444-477); an obstacle at S1 at slice t blocks the
+    move (S2 = S1, r = r_outbound, :526-545)."""
+    offsets = [(2, 3), (3, 1), (-2, 3), (1, -3), (0, 3), (3, 0), (2, 2), (-3, -2), (1, 2), (-1, -3)]
+    kinds = {"band": 0, "outside": 0, "line": 0, "start": 0}
+    for dxy in (1.0, 0.75):
+        p = Params(gsize=7, nt=3, nrzns=1, nmodes=0, num_ac_speeds=1, num_ac_angles=1,
+                   term_subgrid_size=0, dx=dxy, dy=dxy, x0=dxy / 2, y0=dxy / 2)
+        g = p.gsize
+        for di, dj in offsets:
+            # F = 1 along +x: (1 + u) dt = dj dx and v dt = -di dy (row i grows downwards)
+            f = zero_fields(g, p.nt, 0, 1, u=dj * dxy - 1.0, v=-di * dxy)
+            for i1, j1 in ((3, 3), (3 - (di > 0), 3 - (dj > 0))):
+                i2, j2 = i1 + di, j1 + dj
+                assert 0 <= i2 < g and 0 <= j2 < g
+                edge = i2 in (0, g - 1) or j2 in (0, g - 1)
+                free_r = p.r_outbound * edge - p.dt + 0.05 * p.r_outbound   # every T > nt - 20
+                for io in range(g):
+                    for jo in range(g):
+                        for at in (0, 1):   # obstacle at slice t = 0 only, or at t + 1 only
+                            f["mask"][:] = 0
+                            f["mask"][at, io, jo] = 1
+                            sp2, r = oracle.lane(p, f, 0, 0, i1 * g + j1, 0)
+                            if (io, jo) == (i1, j1) and at == 0:
+                                assert sp2 == i1 * g + j1 and r == p.r_outbound
+                                kinds["start"] += 1
+                                continue
+                            assert sp2 == i2 * g + j2, (di, dj, io, jo, sp2)
+                            inbox = min(i1, i2) <= io <= max(i1, i2) and min(j1, j2) <= jo <= max(j1, j2)
+                            if not inbox or (io, jo) == (i1, j1):
+                                assert abs(r - free_r) < 1e-4, (dxy, di, dj, io, jo, at, r)
+                                continue
+                            hit = _band_hit(p, io, jo, i1, j1, i2, j2)
+                            if hit is None:
+                                continue
+                            if di == 0 or dj == 0:
+                                kinds["line"] += 1
+                            else:
+                                kinds["band" if hit else "outside"] += 1
+                            e_r = p.r_outbound if hit else free_r
+                            assert abs(r - e_r) < 1e-4, (dxy, di, dj, i1, j1, io, jo, at, r, e_r)
+    assert min(kinds.values()) >= 8, kinds
+
+
+def pin_reward_sum():
+    """Non-integer lane rewards (method 2 over a random scalar field): each row's reward
+    sum is the exact sum of its lanes' float32 rewards rounded once to float32 (away
+    from rounding midpoints, any double-precision summation order gives that), and
+    R = Rsum / nr in float32 (compute_mean, src/build_model.cu:563-570)."""
+    w = small(g=7, nt=4, nr=200, nm=2, seed=11, reward_type=2, num_ac_angles=4)
+    p = w.params
+    f = w.fields()
+    f["s"] = make_fields(p.gsize, p.nt, 0, 1, 5, with_scalar=True)["s"] * f32(3.7)
+    m = oracle.build(p, f)
+    checked = 0
+    for t in range(p.nt - 1):
+        for a in range(p.na):
+            _, r = oracle.lanes(p, f, t, a)
+            for sp in range(p.nc):
+                row = a * p.ns + t * p.nc + sp
+                exact = sum((Fraction(float(x)) for x in r[sp]), Fraction(0))
+                near = f32(float(exact))
+                lo, hi = np.nextafter(near, f32(-np.inf)), np.nextafter(near, f32(np.inf))
+                mids = [(Fraction(float(near)) + Fraction(float(b))) / 2 for b in (lo, hi)]
+                if min(abs(exact - mm) for mm in mids) < abs(exact) * Fraction(1, 10 ** 12):
+                    continue
+                assert m["Rsum"][row] == near, (t, a, sp, m["Rsum"][row], float(exact))
+                assert m["R"][row] == f32(near / f32(p.nrzns))
+                checked += 1
+    assert checked > 0.99 * (p.nt - 1) * p.na * p.nc
+
+
 # ------------------------------------------------------------------ invariants
 
 def _rand_case(seed=3, **kw):
@@ -212,7 +372,8 @@ def pin_vi_brute_force():
         assert (V[-p.nc:] == 0).all() and (pi[-p.nc:] == 0).all()
 
 
-PINS = [pin_zero_field, pin_uniform_field, pin_two_point_modes, pin_model_invariants,
+PINS = [pin_zero_field, pin_uniform_field, pin_two_point_modes, pin_reward_methods,
+        pin_obstacle_crossing, pin_reward_sum, pin_model_invariants,
         pin_count_by_key_is_unique_count, pin_vi_brute_force]
 
 
@@ -231,6 +392,13 @@ MUTATIONS = {
     6: "miscounts the first distinct S2 of each row",
     7: "breaks argmax ties towards the last action",
     8: "drops R from the Bellman backup",
+    9: "drops the /2 of the scalar-field mean in reward method 2",
+    10: "swaps the energy and time weights of reward method 4",
+    11: "takes the band test's query y from ys[i] instead of ys[g-1-i]",
+    12: "tests obstacles at slice t only, not at t+1",
+    13: "lets the start cell itself count as a crossed obstacle",
+    14: "reads the landing cell's scalar field at slice t instead of t+1",
+    15: "uses the start cell's scalar field twice in reward method 3",
 }
 
 
