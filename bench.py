@@ -443,7 +443,10 @@ def main(argv=None):
     ms_per_step = elapsed * 1e3 / args.steps
     lanes = infos[0][0]["lanes"]
 
-    move_ms = float(np.mean([i["move_kernel_ms"] / max(i["move_launches"], 1) for i, _ in infos]))
+    # summed HIP-event time of every move/count launch of a build (one launch for a build
+    # with no deferred rows): all lanes over all of that time
+    move_ms = float(np.mean([i["move_kernel_ms"] for i, _ in infos]))
+    move_launches = infos[0][0]["move_launches"]
     # the per-row wavefront kernel, or the block-level kernel when the build used only that
     build_kernel = "mc_wave" if infos[0][0]["wave_blocks"] > 0 else "mc_move_count"
     flops_per_lane = 4 * p.nmodes + 18
@@ -461,7 +464,7 @@ def main(argv=None):
                    "gsize": p.gsize, "nt": p.nt, "na": p.na, "nrzns": p.nrzns, "nmodes": p.nmodes,
                    "states": p.ns, "nnz": infos[0][0]["nnz"], "lanes": lanes},
         "build": {"transitions_per_s": lanes / (build_ms * 1e-3), "build_ms": build_ms,
-                  "move_kernel_ms": move_ms, "scan_ms": infos[0][0]["scan_ms"],
+                  "move_kernel_ms": move_ms, "move_launches": move_launches, "scan_ms": infos[0][0]["scan_ms"],
                   "csr_ms": infos[0][0]["csr_ms"]},
         "roofline": {"kernel": build_kernel, "bound": "valu", "achieved": build_tflops,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
