@@ -148,10 +148,13 @@ class Model:
     def solve(self, thresh: float = 0.1, discount: float = 1.0, max_sweeps: int = 0,
               sweeps_per_check: int = 0, use_graph: bool = False, loads_in_flight: int = 0,
               timeout_ms: float = 0.0, raise_on_nonconv: bool = True, kernel_variant: int = 0,
-              blocks_per_cu: int = 0, comp_span: int = 0, full_sweeps: bool = False):
+              blocks_per_cu: int = 0, comp_span: int = 0, full_sweeps: bool = False,
+              noalloc: bool = False):
         """Sparse VI. Returns (policy uint32[Ns], value float32[Ns], stats dict).
         full_sweeps: recompute every slice on every sweep (by default a built model's
-        sweeps skip the slices that already hold their final values; same results)."""
+        sweeps skip the slices that already hold their final values; same results).
+        noalloc: run e2e_vi_solve_noalloc, which allocates no device memory (the same
+        results; at most 2,097,152 states, no graph)."""
         ns = self.info["ns"]
         o = _abi.SolveOpts(thresh, discount, max_sweeps, sweeps_per_check, int(use_graph),
                            loads_in_flight, timeout_ms, kernel_variant, blocks_per_cu, comp_span,
@@ -159,7 +162,8 @@ class Model:
         st = _abi.SolveStats()
         pol = np.empty(ns, np.uint32)
         val = np.empty(ns, np.float32)
-        code = lib().e2e_vi_solve(self._h, C.byref(o), pol.ctypes.data, val.ctypes.data, C.byref(st))
+        fn = lib().e2e_vi_solve_noalloc if noalloc else lib().e2e_vi_solve
+        code = fn(self._h, C.byref(o), pol.ctypes.data, val.ctypes.data, C.byref(st))
         stats = {k: getattr(st, k) for k, _ in st._fields_}
         stats["status"] = code
         if code != 0 and (raise_on_nonconv or code not in (4, 5)):

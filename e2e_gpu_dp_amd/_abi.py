@@ -161,6 +161,7 @@ SIGNATURES = {
     "e2e_model_export_coo_i32": (C.c_int, [VP, VP, VP, VP, VP]),
     "e2e_model_export_csr": (C.c_int, [VP, VP, VP, VP, VP]),
     "e2e_vi_solve": (C.c_int, [VP, P(SolveOpts), VP, VP, P(SolveStats)]),
+    "e2e_vi_solve_noalloc": (C.c_int, [VP, P(SolveOpts), VP, VP, P(SolveStats)]),
     "e2e_model_export_R_alpha": (C.c_int, [VP, C.c_int32, VP]),
     "e2e_vi_solve_alphas": (C.c_int, [VP, P(SolveOpts), P(BatchOpts), VP, VP, VP, P(BatchStats)]),
     "e2e_vi_solve_goals": (C.c_int, [VP, VP, C.c_int32, P(SolveOpts), P(BatchOpts), VP, VP, VP,

@@ -62,6 +62,8 @@ e2e_status model_export_coo32(const Model* m, int32_t* S1, int32_t* S2, float* v
 e2e_status model_export_csr(const Model* m, uint32_t* row_ptr, int32_t* col, float* val, float* R);
 e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* value,
                     e2e_solve_stats* stats);
+e2e_status vi_solve_noalloc(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* value,
+                            e2e_solve_stats* stats);
 void actions_table(const e2e_params* p, float* speed, float* angle);
 e2e_status vi_solve_goals(Model* m, const int32_t* goals, int k, const e2e_solve_opts* o,
                           const e2e_batch_opts* bo, uint32_t* policy, float* value, int32_t* col_sweeps,
@@ -294,6 +296,15 @@ e2e_status e2e_vi_solve(void* model, const e2e_solve_opts* opts, uint32_t* polic
         return E2E_ERR_INVALID;
     }
     return guarded([&] { return vi_solve(static_cast<Model*>(model), opts, policy, value, stats); });
+}
+
+e2e_status e2e_vi_solve_noalloc(void* model, const e2e_solve_opts* opts, uint32_t* policy, float* value,
+                                e2e_solve_stats* stats) {
+    if (!model) {
+        set_error("e2e_vi_solve_noalloc: model is NULL");
+        return E2E_ERR_INVALID;
+    }
+    return guarded([&] { return vi_solve_noalloc(static_cast<Model*>(model), opts, policy, value, stats); });
 }
 
 e2e_status e2e_vi_solve_goals(void* model, const int32_t* goal_cells, int32_t k,

@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include "internal.h"
 #include "lanes.h"
@@ -1697,6 +1698,210 @@ void launch_wave_partition(const uint32_t* rp, int64_t ns, int na, int64_t W, in
                            hipStream_t s) {
     hipLaunchKernelGGL(vi_partition, dim3((unsigned)ceil_div(W + 1, 256)), dim3(256), 0, s, rp, ns, na, W, spc,
                        wpart);
+}
+
+
+// ---------------------------------------------------------------------------------
+// Solve without device allocation (e2e_vi_solve_noalloc). The sweeps of vi_solve keep
+// their buffers in the model (allocated on its first solve). This variant keeps every
+// device buffer beyond the model and the host outputs in one file-scope array per GPU,
+// laid out as
+//   VIState | residual partials [kVIMaxBlocks] | wave partitions | W0, W1 [ns] double |
+//   pi [ns] int32 | V [ns] float
+// The array lives as long as the process; each call reuses it and clears what must
+// start at zero (vi_init: the first value buffer, the output vector, the sweep state).
+// Calls that overlap on one GPU share it and race.
+namespace {
+
+constexpr int64_t kNaMaxStates = int64_t(1) << 21;  // 2,097,152 states
+constexpr int64_t kNaMaxSlices = 256;               // slices of a built model (active-slice partitions)
+constexpr int64_t kNaMaxWaves = (int64_t)kVIMaxBlocks * kStreamWaves;
+constexpr size_t na_align(size_t x) { return (x + 255) / 256 * 256; }
+constexpr size_t kNaOffPart = na_align(sizeof(VIState));
+constexpr size_t kNaOffWpart = na_align(kNaOffPart + sizeof(float) * kVIMaxBlocks);
+constexpr size_t kNaWpartWords = (size_t)(kNaMaxWaves + 1) * kNaMaxSlices + kNaMaxSlices + 1;
+constexpr size_t kNaOffW0 = na_align(kNaOffWpart + sizeof(uint32_t) * kNaWpartWords);
+constexpr size_t kNaOffW1 = kNaOffW0 + sizeof(double) * kNaMaxStates;
+constexpr size_t kNaOffPi = kNaOffW1 + sizeof(double) * kNaMaxStates;
+constexpr size_t kNaOffV = kNaOffPi + sizeof(int32_t) * kNaMaxStates;
+constexpr size_t kNaScratchBytes = kNaOffV + sizeof(float) * kNaMaxStates;
+
+// Host code never uses this name as an address: the device address of each GPU's copy
+// comes from hipGetSymbolAddress (noalloc_scratch).
+static __device__ __align__(256) unsigned char scratch[kNaScratchBytes];
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+unsigned char* g_na_scratch[kNaMaxDevices] = {};
+
+// The current device's copy of `scratch`, looked up once per device.
+e2e_status noalloc_scratch(unsigned char** out) {
+    int dev = -1;
+    E2E_HIP(hipGetDevice(&dev));
+    E2E_CHECK(dev >= 0 && dev < kNaMaxDevices, E2E_ERR_CAPACITY, "e2e_vi_solve_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_scratch[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || ptr == nullptr) {
+            set_error("e2e_vi_solve_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return E2E_ERR_HIP;
+        }
+        g_na_scratch[dev] = static_cast<unsigned char*>(ptr);
+    }
+    *out = g_na_scratch[dev];
+    return E2E_OK;
+}
+
+}  // namespace
+
+e2e_status vi_solve_noalloc(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* value,
+                            e2e_solve_stats* stats) {
+    const double t_start = now_ms();
+    e2e_solve_opts opts{};
+    opts.thresh = 0.1f;    // spvi_2.cu:757
+    opts.discount = 1.0f;  // spvi_2.cu:61
+    if (o) opts = *o;
+    const int64_t ns = m->ns;
+    E2E_CHECK(ns >= 1 && ns <= kNaMaxStates, E2E_ERR_CAPACITY,
+              "e2e_vi_solve_noalloc: %lld states; at most %lld fit its fixed device scratch", (long long)ns,
+              (long long)kNaMaxStates);
+    E2E_CHECK(!opts.use_graph, E2E_ERR_INVALID, "e2e_vi_solve_noalloc: use_graph must be 0 (a graph is allocated)");
+    // the lean float64 sweep over the 8-byte entries, or over the compressed entries when
+    // an earlier e2e_vi_solve made them: the two give the same bits
+    const bool has_comp = m->vi_cz != nullptr && m->vi_cbase != nullptr;
+    int variant;
+    if (opts.kernel_variant == 0)
+        variant = has_comp ? 5 : 2;
+    else if (opts.kernel_variant == 3)
+        variant = 2;
+    else if (opts.kernel_variant == 6) {
+        E2E_CHECK(has_comp, E2E_ERR_INVALID,
+                  "e2e_vi_solve_noalloc: kernel_variant 6 needs the compressed entries an earlier e2e_vi_solve "
+                  "of this model made");
+        variant = 5;
+    } else {
+        set_error("e2e_vi_solve_noalloc: kernel_variant %d (0, 3 and 6 run without allocation)", opts.kernel_variant);
+        return E2E_ERR_INVALID;
+    }
+    const bool comp = variant == 5;
+    E2E_CHECK(ns * (int64_t)sizeof(double) < (1ll << 32) && (m->rows + 1) * 4 < (1ll << 32), E2E_ERR_CAPACITY,
+              "e2e_vi_solve_noalloc: the lean sweep addresses states and rows with 32-bit byte offsets");
+    const bool active = !opts.full_sweeps && m->built && m->nt > 0 && m->nc > 0 && ns == m->nt * m->nc;
+    E2E_CHECK(!active || m->nt <= kNaMaxSlices, E2E_ERR_CAPACITY,
+              "e2e_vi_solve_noalloc: %lld slices; at most %lld fit its fixed device scratch (full_sweeps = 1 "
+              "lifts this limit)", (long long)m->nt, (long long)kNaMaxSlices);
+    unsigned char* base = nullptr;
+    E2E_TRY(noalloc_scratch(&base));
+
+    const int max_sweeps = opts.max_sweeps > 0 ? opts.max_sweeps : (m->built ? (int)m->nt + 2 : 100000);
+    const int spc = opts.sweeps_per_check > 0 ? opts.sweeps_per_check : 16;
+    hipStream_t s = m->stream;
+    const int nap = next_pow2((int)m->na);
+    const int grp = 64 / nap;
+    int lpr = 0;
+    const SweepFn fn = comp ? pick_lean_nap<4, true>(nap) : pick_lean(nap, 4, &lpr);
+    lpr = 4;
+    const int ptab_n = (comp && (int64_t)(m->p.nrzns + 1) * 8 <= 32768) ? m->p.nrzns + 1 : 0;
+    const size_t shmem = (size_t)ptab_n * sizeof(double);
+    int bpc = opts.blocks_per_cu;
+    if (bpc <= 0) {
+        E2E_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void*)fn, kStreamThreads, shmem));
+        bpc = std::min(bpc, kLeanBlocksPerCU);
+    }
+    int ncu = 0, dev = 0;
+    E2E_HIP(hipGetDevice(&dev));
+    E2E_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int64_t resident = (int64_t)std::max(bpc, 1) * std::max(ncu, 1);
+    const int64_t states_per_block = (int64_t)kStreamWaves * grp;
+    const int64_t nblocks =
+        std::min<int64_t>(std::min<int64_t>(ceil_div(ns, states_per_block), resident), kVIMaxBlocks);
+    const int64_t W = nblocks * kStreamWaves;  // <= kNaMaxWaves
+
+    VIState* st = reinterpret_cast<VIState*>(base);
+    float* part = reinterpret_cast<float*>(base + kNaOffPart);
+    uint32_t* wpart = reinterpret_cast<uint32_t*>(base + kNaOffWpart);
+    double* W0 = reinterpret_cast<double*>(base + kNaOffW0);
+    double* W1 = reinterpret_cast<double*>(base + kNaOffW1);
+    int32_t* pi = reinterpret_cast<int32_t*>(base + kNaOffPi);
+    float* V = reinterpret_cast<float*>(base + kNaOffV);
+
+    // this model's partition (the array keeps the previous call's)
+    if (active)
+        launch_prefix_partitions(m->row_ptr, m->nc, (int)m->na, m->nt, W, grp, wpart, s);
+    else
+        launch_wave_partition(m->row_ptr, ns, (int)m->na, W, grp, wpart, s);
+    E2E_HIP(hipGetLastError());
+
+    VIArgs A;
+    std::memset(&A, 0, sizeof(A));
+    A.rp = m->row_ptr;
+    A.nz = m->nz;
+    A.R = m->R;
+    A.V0 = V;
+    A.V1 = V;
+    A.W0 = W0;
+    A.W1 = W1;
+    A.pi = pi;
+    A.st = st;
+    A.part = part;
+    A.ns = ns;
+    A.na = (int)m->na;
+    A.thresh = opts.thresh;
+    A.discount = opts.discount;
+    A.wpart = wpart;
+    A.pre_nt = active ? (uint32_t)m->nt : 0u;
+    A.pre_stride = active ? (uint32_t)(W + 1) : 0u;
+    A.cz = comp ? m->vi_cz : nullptr;
+    A.cbase = comp ? m->vi_cbase : nullptr;
+    A.inv_nr = comp ? 1.0 / (double)m->p.nrzns : 0.0;
+    A.ptab_n = ptab_n;
+
+    // W0, the output vector and the sweep state start at zero (a kernel, no memset node)
+    hipLaunchKernelGGL(vi_init, dim3((unsigned)ceil_div(ns, 256)), dim3(256), 0, s, V, W0, ns, st);
+    E2E_HIP(hipGetLastError());
+
+    int launched = 0;
+    VIState hst{};
+    e2e_status rc = E2E_OK;
+    int chunk = spc;
+    while (launched < max_sweeps) {
+        const int n = std::min(chunk, max_sweeps - launched);
+        for (int i = 0; i < n; i++) hipLaunchKernelGGL(fn, dim3((unsigned)nblocks), dim3(kStreamThreads), shmem, s, A);
+        E2E_HIP(hipGetLastError());
+        E2E_HIP(hipMemcpyAsync(&hst, st, sizeof(VIState), hipMemcpyDeviceToHost, s));
+        E2E_HIP(hipStreamSynchronize(s));
+        launched += n;
+        if (opts.sweeps_per_check <= 0 && launched >= 2 * spc) chunk = std::min(chunk * 2, 4 * spc);
+        if (hst.conv >= 0) break;
+        if (opts.timeout_ms > 0 && now_ms() - t_start > opts.timeout_ms) {
+            rc = E2E_ERR_TIMEOUT;
+            break;
+        }
+    }
+    const int sweeps = hst.conv >= 0 ? hst.conv : (int)hst.k - 1;
+    if (rc == E2E_OK && hst.conv < 0) rc = E2E_ERR_NOT_CONVERGED;
+    if (rc == E2E_ERR_NOT_CONVERGED) set_error("not converged after %d sweeps (residual %g)", sweeps, hst.last_res);
+    if (rc == E2E_ERR_TIMEOUT) set_error("timeout after %d sweeps", sweeps);
+    hipLaunchKernelGGL(vi_to_f32, dim3((unsigned)ceil_div(ns, 256)), dim3(256), 0, s, (sweeps & 1) ? W1 : W0, V, ns);
+    E2E_HIP(hipGetLastError());
+    if (value) E2E_HIP(hipMemcpyAsync(value, V, sizeof(float) * ns, hipMemcpyDeviceToHost, s));
+    if (policy) E2E_HIP(hipMemcpyAsync(policy, pi, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, s));
+    E2E_HIP(hipStreamSynchronize(s));
+    if (stats) {  // no HIP events (they are created objects): kernel times and bytes are 0
+        *stats = e2e_solve_stats{};
+        stats->sweeps = sweeps;
+        stats->converged = hst.conv >= 0;
+        stats->final_residual = hst.last_res;
+        stats->loads_in_flight = lpr;
+        stats->kernel_variant = variant + 1;
+        stats->blocks = (int32_t)nblocks;
+        stats->solve_ms = now_ms() - t_start;
+        stats->launches = launched;
+        stats->compressed_windows = comp ? (int32_t)std::min<int64_t>(m->vi_cwin_comp, INT32_MAX) : 0;
+    }
+    return rc;
 }
 
 }  // namespace e2e

@@ -316,6 +316,20 @@ e2e_status e2e_model_export_R_alpha(const void* model, int32_t j, float* R);
 e2e_status e2e_vi_solve(void* model, const e2e_solve_opts* opts,
                         uint32_t* policy, float* value, e2e_solve_stats* stats);
 
+/* Sparse value iteration without device allocation. It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same arguments and
+ * results as e2e_vi_solve, bit for bit, for kernel_variant 0, 3 or 6 (6 only once an
+ * e2e_vi_solve of the model has made its compressed entries; 0 takes them when there).
+ * It calls no hipMalloc, hipFree or other allocating routine: every device buffer beyond
+ * the model and the host outputs sits in one fixed array per GPU (56 MiB, held
+ * until the process ends). Limits: ns <= 2,097,152 states; for a built model swept over
+ * its active slices (full_sweeps 0), nt <= 256; use_graph must be 0. Larger problems
+ * return E2E_ERR_CAPACITY. The stats carry no kernel times or byte counts (0). Calls
+ * that overlap on one GPU share its array and race; run them one at a time per GPU. */
+e2e_status e2e_vi_solve_noalloc(void* model, const e2e_solve_opts* opts,
+                                uint32_t* policy, float* value, e2e_solve_stats* stats);
+
 /* Batched multi-goal solve (no counterpart call in the reference, which rebuilds and
  * re-solves the model per goal: src/input_to_build_model.h:33-40 fix one terminal per
  * build). Column j equals the solve of the model built with its terminal cell at
