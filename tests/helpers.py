@@ -69,3 +69,24 @@ def assert_slice_matches(params, fields, t, S1, S2, val, R):
     rows = (np.arange(na)[:, None] * ns + t * nc + np.arange(nc)[None, :]).ravel()
     assert np.array_equal(R[rows].view(np.uint32), ref["R"][rows].view(np.uint32)), t
     return int(sel.sum())
+
+
+def goal_params(params, goal):
+    """The reference's per-goal model: terminal subgrid of side 1 at cell `goal`
+    (src/input_to_build_model.h:33-40)."""
+    g = params.gsize
+    return params.replace(i_term=int(goal) // g, j_term=int(goal) % g, term_subgrid_size=1)
+
+
+def assert_close_to_f64(ref, pol, val, tol=1e-5, what=""):
+    """Values within tol * max(|v64|, 1) of the FP64 backward induction of the model
+    `ref` (an oracle build or a COO dict with S1, S2, val, R, ns, na, nc); policies equal
+    except where the FP64 Q values of both choices tie within 4 tol. Returns the max
+    relative error."""
+    ns, na, nc = ref["ns"], ref["na"], ref["nc"]
+    V64, pi64 = oracle.vi_backward_f64(ns, na, nc, ref["S1"], ref["S2"], ref["val"], ref["R"])
+    err = rel_err(val, V64)
+    assert err.max() <= tol, f"{what}: max rel err {err.max()} at state {err.argmax()}"
+    Q64 = q_values_f64(ns, na, ref["S1"], ref["S2"], ref["val"], ref["R"], V64)
+    policy_ties_ok(pol, pi64, Q64, 4 * tol)
+    return float(err.max())

@@ -14,7 +14,7 @@ import pytest
 import e2e_gpu_dp_amd as e2e
 from e2e_gpu_dp_amd.configs import small
 from e2e_gpu_dp_amd.fields import make_fields
-from helpers import assert_same_model, oracle_model
+from helpers import assert_close_to_f64, assert_same_model, oracle_model
 
 pytestmark = pytest.mark.gpu
 
@@ -67,6 +67,8 @@ def test_alpha_solve_matches_plain_solves(gpu, variant):
         assert sw[j] == sj["sweeps"], j
         assert np.array_equal(val[j].view(np.uint32), vj.view(np.uint32)), j
         assert np.array_equal(pol[j], pj), j
+        if variant == -1 and j in (0, 3, 10):  # the FP64 backward induction of the oracle's build
+            assert_close_to_f64(oracle_model(p.replace(alpha=float(a)), f), pol[j], val[j], 1e-5, f"alpha {a}")
 
 
 def test_alpha_backward_columns_match_converged_plain_solves(gpu):
@@ -82,6 +84,8 @@ def test_alpha_backward_columns_match_converged_plain_solves(gpu):
         assert sj["converged"]
         assert np.array_equal(val[j].view(np.uint32), vj.view(np.uint32)), j
         assert np.array_equal(pol[j], pj), j
+        if j in (1, 7):
+            assert_close_to_f64(oracle_model(p.replace(alpha=float(a)), f), pol[j], val[j], 1e-5, f"alpha {a}")
 
 
 def test_alpha_columns_errors(gpu):

@@ -6,7 +6,9 @@ import pytest
 
 import e2e_gpu_dp_amd as e2e
 from e2e_gpu_dp_amd.configs import goal_cells, small, workload
-from helpers import rel_err
+from helpers import assert_close_to_f64, goal_params, oracle_model, rel_err
+
+import oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -39,8 +41,12 @@ def test_batched_c1_64_goals_match_plain_solves(gpu):
         assert err.max() <= TOL, (j, err.max())
         assert sw[j] == sp["sweeps"], (j, sw[j], sp["sweeps"])
         same = np.mean(val[j].view(np.uint32) == vp.view(np.uint32))
-        print(f"goal {goals[j]}: max rel err {err.max():.2e}, bit-identical {same:.4f}")
         assert np.mean(pol[j] == pp) > 0.99
+        # and the FP64 backward induction of the oracle's build of that goal's model
+        e64 = assert_close_to_f64(oracle_model(goal_params(w.params, goals[j]), f), pol[j], val[j],
+                                  TOL, f"goal {goals[j]}")
+        print(f"goal {goals[j]}: max rel err vs plain {err.max():.2e} (bit-identical {same:.4f}), "
+              f"vs FP64 oracle {e64:.2e}")
 
 
 def test_batched_reference_threshold(gpu):
@@ -54,6 +60,12 @@ def test_batched_reference_threshold(gpu):
         pp, vp, sp = _plain(w, f, goals[j], thresh=0.1)
         assert sw[j] == sp["sweeps"]
         assert rel_err(val[j], vp.astype(np.float64)).max() <= TOL
+        # the oracle's FP64 sweeps of that goal's model stop at the same sweep with the
+        # same values (the early stop of src/spvi_2.cu:669-698, in FP64)
+        r = oracle_model(goal_params(w.params, goals[j]), f)
+        V64, _, it64, _ = oracle.vi_sweeps_f64(r["ns"], r["na"], r["S1"], r["S2"], r["val"], r["R"], 0.1)
+        assert sw[j] == it64, (j, sw[j], it64)
+        assert rel_err(val[j], V64).max() <= TOL
 
 
 def test_batched_columns_independent_of_batch_and_variant(gpu):
@@ -62,6 +74,9 @@ def test_batched_columns_independent_of_batch_and_variant(gpu):
     goals = goal_cells(14, 40, 9)
     with _shared(w, f) as m:
         ref_pol, ref_val, ref_sw, st = m.solve_goals(goals, thresh=1e-6, variant=0)
+        for j in (0, 17, 39):
+            assert_close_to_f64(oracle_model(goal_params(w.params, goals[j]), f), ref_pol[j], ref_val[j], TOL,
+                                f"goal {goals[j]}")
         for variant in range(1, N_VARIANTS):
             pol, val, sw, st = m.solve_goals(goals, thresh=1e-6, variant=variant)
             assert np.array_equal(val.view(np.uint32), ref_val.view(np.uint32)), variant
@@ -132,13 +147,16 @@ def test_batched_backward_pass_equals_converged_columns(gpu, k):
     """Variant -3: every goal column by one exact backward pass, bit for bit the columns of
     the batched sweeps run to a zero residual."""
     w = small(g=14, nt=12, nr=400, nm=3, seed=93)
+    f = w.fields()
     goals = goal_cells(14, k, 29)
-    with _shared(w, w.fields()) as m:
+    with _shared(w, f) as m:
         pj, vj, sj, stj = m.solve_goals(goals, thresh=1e-30, max_sweeps=w.params.nt + 4)
         pb, vb, sb, stb = m.solve_goals(goals, variant=-3)
         assert stj["converged"] == k and stb["variant"] == -3 and (sb == 1).all()
         assert np.array_equal(vb.view(np.uint32), vj.view(np.uint32))
         assert np.array_equal(pb, pj)
+    for j in sorted({0, k - 1}):  # the fixed point is the FP64 backward induction's, within 1e-5
+        assert_close_to_f64(oracle_model(goal_params(w.params, goals[j]), f), pb[j], vb[j], TOL, f"goal {goals[j]}")
 
 
 @pytest.mark.parametrize("na,k", [(8, 16), (16, 40), (8, 3)])

@@ -1,10 +1,14 @@
 """Continuous-batching server: every request's result is the same alone and in the mix;
 HBM-budget admission evicts idle models or rejects what cannot fit."""
+import time
+
 import numpy as np
 import pytest
 
 import e2e_gpu_dp_amd as e2e
-from e2e_gpu_dp_amd.configs import small
+from e2e_gpu_dp_amd.configs import Params, small
+from e2e_gpu_dp_amd.fields import make_fields
+from helpers import assert_close_to_f64, assert_slice_matches, goal_params, oracle_model
 
 pytestmark = pytest.mark.gpu
 
@@ -46,6 +50,9 @@ def test_server_mix_equals_solo(gpu):
             assert np.array_equal(val.view(np.uint32), sval.view(np.uint32))
             assert np.array_equal(pol, spol)
             assert r["value_at_start"] == val[s0] and r["action_at_start"] == int(np.int32(pol[s0]))
+            # the FP64 backward induction of the oracle's build of the goal's model
+            p, f = envs[eid]
+            assert_close_to_f64(oracle_model(goal_params(p, g), f), pol, val, 1e-5, f"env {eid} goal {g}")
 
 
 def test_server_long_steps_with_converged_column_blocks(gpu):
@@ -136,3 +143,71 @@ def test_server_exact_pass_mix_equals_fixed_point(gpu):
             assert np.array_equal(val.view(np.uint32), fval.view(np.uint32))
             assert np.array_equal(pol, fpol)
             assert r["value_at_start"] == val[s0] and r["action_at_start"] == int(np.int32(pol[s0]))
+        for rid, eid, g, s0 in reqs[:2]:
+            pol, val = srv.columns(rid, eid)
+            p, f = envs[eid]
+            assert_close_to_f64(oracle_model(goal_params(p, g), f), pol, val, 1e-5, f"env {eid} goal {g}")
+
+
+# The serving workload's shapes: grids 50^2, 100^2, 150^2, horizons in [60, 150], 8 or 16
+# headings, 1000-5000 realizations; (gsize, nt, headings, realizations, field seed)
+C5_ENVS = {1: (50, 64, 8, 1200, 5701), 2: (100, 76, 16, 3024, 5702), 3: (150, 90, 8, 1000, 5703)}
+
+
+def _gpu_goal_model(p, f, goal, slices):
+    """The goal's model built on the GPU, its sampled slices checked bit for bit against
+    the oracle's build of those slices, as a COO dict for the FP64 backward induction."""
+    q = goal_params(p, goal)
+    with e2e.build_model(q, f) as m:
+        S1, S2, val, R = m.export_coo(np.int32)
+    for t in slices:
+        assert_slice_matches(q, f, t, S1, S2, val, R)
+    return {"S1": S1, "S2": S2, "val": val, "R": R, "ns": q.ns, "na": q.na, "nc": q.nc}
+
+
+def test_server_c5_shapes(gpu):
+    """Requests at the serving workload's shapes arrive about every 50 ms (20 per second)
+    while others are on the GPU. Every one completes; each checked result equals its solo
+    solve bit for bit and lies within 1e-5 of the FP64 backward induction of its goal's
+    model (the oracle's build for the 50^2 environment; for the larger ones the GPU build,
+    whose first and a late slice equal the oracle's bit for bit)."""
+    envs = {}
+    for eid, (g, nt, na, nr, seed) in C5_ENVS.items():
+        p = Params(gsize=g, nt=nt, nrzns=nr, nmodes=5, num_ac_angles=na, term_subgrid_size=0)
+        envs[eid] = (p, make_fields(g, nt, 5, nr, seed))
+    rng = np.random.default_rng(5678)
+    gaps = rng.exponential(0.05, 20)
+    reqs = []
+    with e2e.Server(keep_full_results=True, thresh=1e-6) as srv:
+        for eid, (p, f) in envs.items():
+            srv.add_env(eid, p, f)
+        for i, gap in enumerate(gaps):
+            time.sleep(float(gap))
+            eid = 1 + i % 3
+            nc = envs[eid][0].nc
+            g, s0 = (int(x) for x in rng.integers(0, nc, 2))
+            reqs.append((srv.submit(eid, g, s0), eid, g, s0))
+        srv.drain(timeout_ms=240000)
+        st = srv.stats()
+        print(f"[c5] completed {st['completed']} models {st['models_built']} p50 {st['p50_ms']:.1f} ms "
+              f"p99 {st['p99_ms']:.1f} ms", flush=True)
+        assert st["completed"] == len(reqs) and st["models_built"] == 3
+        checked = {}
+        for rid, eid, g, s0 in reqs:
+            if checked.get(eid, 0) >= (2 if eid == 1 else 1):
+                continue
+            checked[eid] = checked.get(eid, 0) + 1
+            r = srv.wait(rid)
+            assert r["status_name"] == "done"
+            pol, val = srv.columns(rid, eid)
+            p, f = envs[eid]
+            spol, sval, ssw = _solo(p, f, g)
+            assert r["sweeps"] == ssw
+            assert np.array_equal(val.view(np.uint32), sval.view(np.uint32))
+            assert np.array_equal(pol, spol)
+            assert r["value_at_start"] == val[s0]
+            ref = oracle_model(goal_params(p, g), f) if eid == 1 else _gpu_goal_model(p, f, g, (0, p.nt - 5))
+            e64 = assert_close_to_f64(ref, pol, val, 1e-5, f"env {eid} goal {g}")
+            print(f"[c5] env {eid} (g={p.gsize}, nt={p.nt}) goal {g}: sweeps {r['sweeps']}, "
+                  f"max rel err vs FP64 {e64:.2e}", flush=True)
+        assert len(checked) == 3

@@ -5,7 +5,7 @@ import pytest
 import e2e_gpu_dp_amd as e2e
 import oracle
 from e2e_gpu_dp_amd.configs import small, workload
-from helpers import oracle_model, policy_ties_ok, q_values_f64, rel_err
+from helpers import assert_slice_matches, oracle_model, policy_ties_ok, q_values_f64, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -160,9 +160,15 @@ def test_vi_c2_accuracy(gpu):
     1.34e8 nnz) with the default kernel: values within 1e-5 relative of the float64
     backward induction, policies equal except at ties, sweep count as in float64."""
     w = workload("C2")
-    with e2e.build_model(w.params, w.fields()) as m:
+    f = w.fields()
+    with e2e.build_model(w.params, f) as m:
         pol, val, st = m.solve(thresh=1e-6, sweeps_per_check=8)
         S1, S2, pv, R = m.export_coo(np.int32)
+    # the plain build itself, bit for bit against the oracle on three slices (the first,
+    # a middle one and one with the late penalty, T > nt - 20)
+    for t in (0, 60, 110):
+        n = assert_slice_matches(w.params, f, t, S1, S2, pv, R)
+        print(f"[c2] slice {t}: {n} entries bit-exact", flush=True)
     ns, na = w.params.ns, w.params.na
     V64, pi64 = oracle.vi_backward_f64(ns, na, w.params.nc, S1, S2, pv, R)
     err = rel_err(val, V64)
