@@ -51,23 +51,75 @@ PEAK_FP32_TFLOPS = 157.3
 
 def _pmc_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` as measured with rocprofv3 PMC counters on this
-    workload (profiles/hbm_traffic.json, commands in profiles/c2_hbm_traffic.md); None
+    workload (profiles/hbm_traffic.json, commands in profiles/*_hbm_traffic.md); None
     when no measurement for this workload and kernel is on file."""
     f = ROOT / "profiles" / "hbm_traffic.json"
     try:
         d = json.loads(f.read_text())
     except (OSError, ValueError):
         return None
-    if d.get("workload") != workload or kernel not in d:
-        return None
-    return d[kernel]["traffic_bytes"]
+    k = d.get(workload, {}).get(kernel)
+    return k["traffic_bytes"] if k else None
+
+
+def _threads() -> int:
+    """Host threads of every CPU leg: OMP_NUM_THREADS when set (the CPU share a GPU box
+    gives one GPU), else every core this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def _host_info() -> dict:
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "threads_used": _threads()}
+
+
+def _cpu_vi(np, p, coo, thresh, budget_s: float, gpu_bytes_per_sweep: float):
+    """The oracle's OpenMP float32 sweeps (the reference's solve loop, src/spvi_2.cu:650-726,
+    on the host cores) over the same model: the whole solve to `thresh` when it fits the
+    budget, else a sample of sweeps extrapolated to the GPU's sweep count (labelled)."""
+    import oracle
+    th = _threads()
+    S1, S2, val, R = coo
+    ns, na = p.ns, p.na
+    t0 = time.perf_counter()
+    oracle.vi_sweeps_f32(ns, na, S1, S2, val, R, thresh, max_sweeps=1, nthreads=th)
+    t1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.vi_sweeps_f32(ns, na, S1, S2, val, R, thresh, max_sweeps=2, nthreads=th)
+    per = max(time.perf_counter() - t0 - t1, 1e-6)  # the row-pointer set-up cancels
+    full = per * (p.nt + 2) <= budget_s
+    if full:
+        t0 = time.perf_counter()
+        _, _, it, _ = oracle.vi_sweeps_f32(ns, na, S1, S2, val, R, thresh, max_sweeps=p.nt + 2, nthreads=th)
+        solve_s = time.perf_counter() - t0
+        sample = f"whole solve to {thresh:g}: {it} sweeps in {solve_s:.2f} s"
+    else:
+        it = p.nt
+        solve_s = t1 - per + per * it
+        sample = (f"runs of 1 and 2 sweeps timed ({per * 1e3:.0f} ms per sweep), extrapolated to {it} sweeps: "
+                  f"{solve_s:.1f} s (not run in full)")
+    return {"ms_per_sweep": per * 1e3, "sweeps": it, "solve_s": solve_s, "extrapolated": not full,
+            "gbps_equiv": gpu_bytes_per_sweep / per / 1e9, "cores": th, "kind": "port",
+            "sample": f"oracle float32 sweeps (OpenMP, {th} threads): {sample}"}
 
 
 def _cpu_baseline(w, fields, budget_s: float):
     """The oracle's OpenMP build on the host cores over the first slices of the same
     workload (bounded sample)."""
     import oracle
-    threads = min(16, os.cpu_count() or 1)
+    threads = _threads()
     p = w.params
     lanes_per_slice = p.nc * p.na * p.nrzns
     t0 = time.perf_counter()
@@ -85,7 +137,7 @@ def _cpu_baseline(w, fields, budget_s: float):
     }
 
 
-def _batched(e2e, np, steps: int):
+def _batched(e2e, np, steps: int, cpu: bool = True):
     """C1 model x 64 goals: batched SpMM-VI vs 64 sequential plain SpMV-VI solves."""
     from e2e_gpu_dp_amd.configs import goal_cells, workload
     w = workload("C1")
@@ -116,9 +168,21 @@ def _batched(e2e, np, steps: int):
         v64 = vp.astype(np.float64)
         err = max(err, float((np.abs(val_b[j] - v64) / np.maximum(np.abs(v64), 1.0)).max()))
         err_x = max(err_x, float((np.abs(val_x[j] - v64) / np.maximum(np.abs(v64), 1.0)).max()))
+    coos = [m.export_coo() for m in plains] if cpu else []
     for m in plains:
         m.close()
     shared.close()
+    cpu_leg = None
+    if cpu:  # the same 64 per-goal models, solved one after another on the host cores
+        import oracle
+        th = _threads()
+        t0 = time.perf_counter()
+        for (S1, S2, v, R) in coos:
+            oracle.vi_sweeps_f32(p.ns, p.na, S1, S2, v, R, 1e-6, max_sweeps=p.nt + 2, nthreads=th)
+        dt = time.perf_counter() - t0
+        cpu_leg = {"value": 64 / dt, "unit": "solves/s", "cores": th, "kind": "port",
+                   "sample": f"oracle float32 sweeps (OpenMP, {th} threads) of the 64 per-goal models one "
+                             f"after another, to 1e-6: {dt * 1e3:.1f} ms"}
     return {
         "workload": "25x25 x nt=20, 8 headings, 4 modes, 1000 realizations; 64 goal cells (seed 3456)",
         "solves_per_s": 64 / t_batch, "plain_solves_per_s": 64 / t_plain,
@@ -129,7 +193,7 @@ def _batched(e2e, np, steps: int):
         "bytes_per_sweep": st_b["bytes_per_sweep"],
         "backward_ms": t_back * 1e3, "backward_solves_per_s": 64 / t_back,
         "backward_speedup_vs_plain": t_plain / t_back, "backward_max_rel_diff_vs_plain": err_x,
-        "backward_launches": st_x["launches"],
+        "backward_launches": st_x["launches"], "cpu_baseline": cpu_leg,
     }
 
 
@@ -248,7 +312,7 @@ def _rollout_cpu_baseline(np, p, fields, pol, budget_s):
     many start cells as fit the budget."""
     import oracle
     starts = _rollout_starts(p)
-    nth = min(16, oracle.max_threads())
+    nth = _threads()
     t0 = time.perf_counter()
     oracle.rollout(p, fields, pol, starts[:1], interp=True, nthreads=nth)
     one = time.perf_counter() - t0
@@ -296,7 +360,7 @@ def _envgen_cpu_baseline(np):
     R = np.concatenate((vx[0], vy[0]), axis=1)
     envgen.modes(R - R.mean(axis=0), cfg["n_modes"])
     dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "slices/s", "cores": min(16, os.cpu_count() or 1), "kind": "port",
+    return {"value": 1.0 / dt, "unit": "slices/s", "cores": _threads(), "kind": "port",
             "sample": f"one slice (t = 60) with numpy realizations and LAPACK SVD of the "
                       f"{cfg['n_wsamples']} x {2 * cfg['gsize'] ** 2} matrix: {dt:.1f} s"}
 
@@ -368,6 +432,124 @@ def _alpha_sweep(e2e, np, torch):
             "variant": st["variant"], "cols_per_lane": st["cols_per_lane"]}
 
 
+def _c1(e2e, np, torch, thresh, cpu: bool):
+    """The small workload (25x25 x nt=20, 8 headings, 4 modes, 1000 realizations) end to
+    end: build + solve to `thresh` on the GPU (fields resident in HBM), and the same on
+    the host cores (the oracle's OpenMP build, then its float32 sweeps)."""
+    from e2e_gpu_dp_amd.configs import workload
+    w = workload("C1")
+    p = w.params
+    f = w.fields()
+    dev = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in f.items()}
+    torch.cuda.synchronize()
+
+    def step():
+        with e2e.build_model(p, dev) as m:
+            _, _, st = m.solve(thresh=thresh, max_sweeps=p.nt + 2)
+        return st
+
+    step()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        st = step()
+    gpu_ms = (time.perf_counter() - t0) * 1e3 / 5
+    out = {"workload": "C1: 25x25 x nt=20, 8 headings, 4 modes, 1000 realizations; build + solve to "
+                       f"{thresh:g}", "gpu_ms": gpu_ms, "sweeps": st["sweeps"],
+           "transitions_per_s": w.lanes / (gpu_ms * 1e-3)}
+    if cpu:
+        import oracle
+        th = _threads()
+        t0 = time.perf_counter()
+        r = oracle.build(p, f, nthreads=th)
+        tb = time.perf_counter() - t0
+        _, _, it, _ = oracle.vi_sweeps_f32(p.ns, p.na, r["S1"], r["S2"], r["val"], r["R"], thresh,
+                                           max_sweeps=p.nt + 2, nthreads=th)
+        tt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": w.lanes / tt, "unit": "transitions/s", "cores": th, "kind": "port",
+                               "build_s": tb, "solve_s": tt - tb, "sweeps": it,
+                               "sample": f"the whole workload: oracle build (OpenMP, {th} threads) {tb:.2f} s + "
+                                         f"float32 sweeps {tt - tb:.3f} s"}
+        out["gpu_vs_cpu"] = tt * 1e3 / gpu_ms
+    return out
+
+
+def _c4(e2e, np, torch, thresh, cpu: bool, cpu_budget: float):
+    """The large single MDP (200x200 x nt=200, 16 headings, 6 modes, 5000 realizations):
+    build + solve on the GPU (fields resident in HBM) with the build and sweep rooflines,
+    and the oracle on the host cores over bounded samples, extrapolated and labelled."""
+    from e2e_gpu_dp_amd.configs import workload
+    w = workload("C4")
+    p = w.params
+    print("[bench] C4 fields", file=sys.stderr, flush=True)
+    f = w.fields()
+    dev = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in f.items()}
+    torch.cuda.synchronize()
+
+    def step():
+        with e2e.build_model(p, dev) as m:
+            _, _, st = m.solve(thresh=thresh, max_sweeps=p.nt + 2)
+            return m.info, st
+
+    step()
+    runs = []
+    t0 = time.perf_counter()
+    for _ in range(2):
+        runs.append(step())
+    ms = (time.perf_counter() - t0) * 1e3 / 2
+    info, st = runs[-1]
+    move_ms = float(np.mean([i["move_kernel_ms"] for i, _ in runs]))
+    fl = 4 * p.nmodes + 18
+    tflops = w.lanes * fl / (move_ms * 1e-3) / 1e12
+    with e2e.build_model(p, dev) as m:
+        _, vf, sf = m.solve(thresh=thresh, max_sweeps=p.nt + 2, full_sweeps=True)
+        _, va, sa = m.solve(thresh=thresh, max_sweeps=p.nt + 2)
+        coo = m.export_coo() if cpu else None
+    del dev
+    pf = sf["sweep_kernel_ms_per_sweep"]
+    gbps = sf["bytes_per_sweep"] / (pf * 1e-3) / 1e9
+    out = {
+        "workload": f"C4: {p.gsize}x{p.gsize} grid x nt={p.nt}, {p.na} headings, {p.nmodes} DO modes, "
+                    f"{p.nrzns} realizations",
+        "states": p.ns, "nnz": info["nnz"], "lanes": w.lanes, "ms_per_step": ms,
+        "transitions_per_s": w.lanes / (ms * 1e-3), "build_ms": float(np.mean([i["build_ms"] for i, _ in runs])),
+        "move_kernel_ms": move_ms, "move_launches": info["move_launches"], "deferred_rows": info["n_deferred_rows"],
+        "solve_ms": float(np.mean([s["solve_ms"] for _, s in runs])), "sweeps": st["sweeps"],
+        "step_sweep_kernel_ms": st["sweep_kernel_ms_per_sweep"],
+        "step_sweeps_gbps": st["bytes_per_sweep"] / (st["sweep_kernel_ms_per_sweep"] * 1e-3) / 1e9,
+        "full_sweeps": {"sweep_kernel_ms": pf, "bytes_per_sweep": sf["bytes_per_sweep"], "gbps": gbps,
+                        "same_bits": bool(np.array_equal(vf.view(np.uint32), va.view(np.uint32))
+                                          and sf["sweeps"] == sa["sweeps"])},
+        "roofline": {"kernel": "mc_wave", "bound": "valu", "achieved": tflops, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": tflops / PEAK_FP32_TFLOPS, "traffic": _pmc_traffic("C4", "mc_wave"),
+                     "flops_per_lane": fl},
+        "roofline_vi": {"kernel": "spmv_vi_lean", "bound": "hbm", "achieved": gbps, "peak": PEAK_HBM_GBPS,
+                        "unit": "GB/s", "frac": gbps / PEAK_HBM_GBPS,
+                        "traffic": _pmc_traffic("C4", "spmv_vi_lean")},
+    }
+    if cpu:
+        import oracle
+        th = _threads()
+        lanes_slice = p.nc * p.na * p.nrzns
+        t0 = time.perf_counter()
+        oracle.build(p, f, 1, 2, th)
+        one = time.perf_counter() - t0
+        n = int(max(1, min(5, cpu_budget // max(one, 1e-3) + 1)))
+        if n > 1:
+            t0 = time.perf_counter()
+            oracle.build(p, f, 2, 2 + n - 1, th)
+            one = (one + time.perf_counter() - t0) / n
+        out["cpu_baseline"] = {
+            "value": lanes_slice / one, "unit": "transitions/s", "cores": th, "kind": "port",
+            "extrapolated_build_s": one * (p.nt - 1),
+            "sample": f"oracle build (OpenMP, {th} threads) of {n} time slice(s) from t = 1, "
+                      f"{lanes_slice:.3e} lanes each, {one:.1f} s per slice; the {p.nt - 1}-slice build "
+                      f"extrapolated from them: {one * (p.nt - 1):.0f} s (not run in full)"}
+        S1, S2, val, R = coo
+        out["cpu_baseline"]["vi"] = _cpu_vi(np, p, (S1, S2, val, R), thresh, 0.0, sf["bytes_per_sweep"])
+        del coo, S1, S2, val, R
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -383,6 +565,8 @@ def main(argv=None):
     ap.add_argument("--serving-requests", type=int, default=120)
     ap.add_argument("--no-paths", action="store_true",
                     help="skip the obstacle and scalar-reward build variants")
+    ap.add_argument("--no-c4", action="store_true", help="skip the large-MDP (200^2 x 200) section")
+    ap.add_argument("--no-c1", action="store_true", help="skip the small-workload end-to-end section")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 3 device-to-device copies of BYTES (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -458,7 +642,8 @@ def main(argv=None):
         "unit": "transitions/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f32 inputs and model (probabilities, rewards), f64 values",
+        "data": "synthetic",
         "config": {"workload": f"{w.name}: plain build+SpVI, {p.gsize}x{p.gsize} grid x nt={p.nt}, "
                                f"{p.na} headings, {p.nmodes} DO modes, {p.nrzns} realizations",
                    "gsize": p.gsize, "nt": p.nt, "na": p.na, "nrzns": p.nrzns, "nmodes": p.nmodes,
@@ -525,17 +710,32 @@ def main(argv=None):
         out["rollout"] = _rollouts(e2e, np, p, dev, pol32)
     if not args.no_batched:
         print("[bench] batched goals", file=sys.stderr, flush=True)
-        out["batched"] = _batched(e2e, np, max(args.steps, 3))
+        out["batched"] = _batched(e2e, np, max(args.steps, 3), cpu=not args.no_cpu)
     if not args.no_serving:
         print("[bench] serving", file=sys.stderr, flush=True)
         out["serving"] = _serving(e2e, np, args.serving_requests)
+    if not args.no_c1 and args.workload == "C2":
+        print("[bench] C1 end to end", file=sys.stderr, flush=True)
+        out["c1"] = _c1(e2e, np, torch, args.thresh, not args.no_cpu)
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        out["host"] = _host_info()
         out["cpu_baseline"] = _cpu_baseline(w, fields, args.cpu_budget)
+        if "vi" in out:
+            with e2e.build_model(p, dev) as mc:
+                coo = mc.export_coo()
+            out["vi"]["cpu_baseline"] = _cpu_vi(np, p, coo, args.thresh, 2 * args.cpu_budget,
+                                                out["vi"]["full_sweeps"]["bytes_per_sweep"])
+            del coo
         if pol32 is not None:
             out["rollout"]["cpu_baseline"] = _rollout_cpu_baseline(np, p, fields, pol32, 3.0)
         if "envgen" in out:
             out["envgen"]["cpu_baseline"] = _envgen_cpu_baseline(np)
+    if not args.no_c4 and args.workload == "C2":
+        del dev
+        torch.cuda.empty_cache()
+        print("[bench] C4", file=sys.stderr, flush=True)
+        out["c4"] = _c4(e2e, np, torch, args.thresh, not args.no_cpu, args.cpu_budget)
     print(json.dumps(out))
     return 0
 

@@ -39,6 +39,9 @@ def test_large_model_slices_invariants_and_solve(gpu):
         print(f"[large] built: nnz {info['nnz']} in {info['build_ms']:.0f} ms, "
               f"deferred rows {info['n_deferred_rows']}", flush=True)
         pol, vgpu, st = m.solve(thresh=1e-6, sweeps_per_check=16)
+        # the float32-value sweeps (1: compensated row sums, 2: plain fma), measured
+        # against float64 below
+        f32_runs = {kv: m.solve(thresh=1e-6, kernel_variant=kv, raise_on_nonconv=False) for kv in (1, 2)}
         S1, S2, val, R = m.export_coo(np.int32)  # the reference's action-major order
     del dev
     print(f"[large] solved in {st['sweeps']} sweeps, {st['sweep_kernel_ms_per_sweep']:.3f} ms/sweep", flush=True)
@@ -80,5 +83,11 @@ def test_large_model_slices_invariants_and_solve(gpu):
     err = rel_err(vgpu, V64)
     print(f"[large] max rel err vs float64 {err.max():.3e}", flush=True)
     assert err.max() <= TOL
+    for kv, (_, v32, s32) in f32_runs.items():
+        e32 = rel_err(v32, V64)
+        print(f"[large] float32 values, kernel_variant {kv}: max rel err {e32.max():.3e} "
+              f"(mean {e32.mean():.2e}), {s32['sweeps']} sweeps, {s32['sweep_kernel_ms_per_sweep']:.3f} ms/sweep",
+              flush=True)
+        assert np.isfinite(v32).all() and e32.max() < 1e-3
     Q64 = q_values_f64(ns, na, S1.astype(np.int64), S2.astype(np.int64), val, R, V64)
     policy_ties_ok(pol, pi64, Q64, 4 * TOL)

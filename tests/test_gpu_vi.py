@@ -163,6 +163,7 @@ def test_vi_c2_accuracy(gpu):
     f = w.fields()
     with e2e.build_model(w.params, f) as m:
         pol, val, st = m.solve(thresh=1e-6, sweeps_per_check=8)
+        f32_runs = {kv: m.solve(thresh=1e-6, kernel_variant=kv, raise_on_nonconv=False) for kv in (1, 2)}
         S1, S2, pv, R = m.export_coo(np.int32)
     # the plain build itself, bit for bit against the oracle on three slices (the first,
     # a middle one and one with the late penalty, T > nt - 20)
@@ -178,6 +179,12 @@ def test_vi_c2_accuracy(gpu):
     assert err.max() <= TOL
     Q64 = q_values_f64(ns, na, S1, S2, pv, R, V64)
     policy_ties_ok(pol, pi64, Q64, 4 * TOL)
+    # the float32-value sweeps (1: compensated row sums, 2: plain fma), measured
+    for kv, (_, v32, s32) in f32_runs.items():
+        e32 = rel_err(v32, V64)
+        print(f"[c2] float32 values, kernel_variant {kv}: max rel err {e32.max():.3e} (mean {e32.mean():.2e}), "
+              f"{s32['sweeps']} sweeps, {s32['sweep_kernel_ms_per_sweep']:.4f} ms/sweep", flush=True)
+        assert np.isfinite(v32).all() and e32.max() < 1e-3
 
 
 def test_vi_unknown_variant_is_an_error(gpu):
