@@ -58,12 +58,17 @@ __host__ __device__ inline int active_slices(uint32_t k, int nt) {
     return nt - (int)k + 2;  // 3 <= k <= nt: [2, nt - 1]
 }
 
+// Blocks per group of the lean sweep's two-level ticket, and the groups of its largest grid
+constexpr int kTicketGroup = 64;
+constexpr int kTicketGroups = 32;
+
 // Device-side solver state of one solve (one per model).
 struct VIState {
     uint32_t k;          // 1-based index of the next sweep
     int32_t conv;        // sweep at which the residual first fell below thresh, -1 if none
-    uint32_t done;       // blocks of the current sweep that finished (ticket)
+    uint32_t done;       // blocks (lean sweep: block groups) of the current sweep that finished (ticket)
     float last_res;      // residual of the last completed sweep
+    uint32_t gdone[kTicketGroups];  // lean sweep: blocks of each group of kTicketGroup that finished
 };
 
 // Per-column state of a batched (multi-goal) solve.

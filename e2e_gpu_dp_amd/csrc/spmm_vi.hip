@@ -74,6 +74,8 @@ struct BatchArgs {
                            // per-prefix partitions in wpart
     int pre_live;          // live batches: a column's own sweep index is k - start + 1, and a block
     const int32_t* start;  // recomputes the slices its youngest running column needs ([KP])
+    int g;                 // spmm_vi_halo: grid side (cells of slice t: i * g + j)
+    uint32_t gmagic;       // spmm_vi_halo: ceil(2^32 / g), so that umulhi(c, gmagic) = c / g for c < g^2
 };
 
 namespace {
@@ -150,9 +152,10 @@ constexpr int kBMaxBlocks = 512;
 // End of a sweep, after each wave has put its per-column residual maxima in red[wave]:
 // per-block column maxima to part[block x][KP]; the last block of the grid takes the
 // maxima over the x-blocks, freezes the columns below thresh and publishes the sweep.
-template <int KB>
+template <int KB, int NT = kBThreads>
 __device__ __forceinline__ void sweep_tail(const BatchArgs& A, float (*red)[KB], float* fin, int& is_last,
                                            uint32_t k) {
+    constexpr int NW = NT / 64;
     BatchState* st = A.st;
     const int tid = threadIdx.x;
     const int KP = A.kp;
@@ -160,7 +163,7 @@ __device__ __forceinline__ void sweep_tail(const BatchArgs& A, float (*red)[KB],
     if (tid < KB) {
         float m = 0.0f;
 #pragma unroll
-        for (int w = 0; w < kBWaves; w++) m = fmaxf(m, red[w][tid]);
+        for (int w = 0; w < NW; w++) m = fmaxf(m, red[w][tid]);
         A.part[(int64_t)blockIdx.x * KP + blockIdx.y * KB + tid] = m;
     }
     __threadfence();
@@ -170,8 +173,8 @@ __device__ __forceinline__ void sweep_tail(const BatchArgs& A, float (*red)[KB],
     if (is_last) {
         __threadfence();
         // column j's residual = max over the x-blocks of part[bx][j]; thread t takes
-        // column t % KP and every (kBThreads/KP)-th x-block, then LDS combines
-        const int groups = kBThreads / KP;
+        // column t % KP and every (NT/KP)-th x-block, then LDS combines
+        const int groups = NT / KP;
         {
             const int j = tid % KP, g0 = tid / KP;
             float r = 0.0f;
@@ -183,7 +186,7 @@ __device__ __forceinline__ void sweep_tail(const BatchArgs& A, float (*red)[KB],
         }
         __syncthreads();
         int running = 0;
-        for (int j = tid; j < KP; j += kBThreads) {
+        for (int j = tid; j < KP; j += NT) {
             float r = 0.0f;
             for (int g2 = 0; g2 < groups; g2++) r = fmaxf(r, fin[g2 * KP + j]);
             if (A.conv[j] < 0) {
@@ -466,6 +469,296 @@ __global__ __launch_bounds__(kBThreads) void spmm_vi_rows(const BatchArgs A) {
         for (int c = 0; c < CPL; c++) red[wv][cl * CPL + c] = rmax[c];
     }
     sweep_tail<KB>(A, red, fin, is_last, k);
+}
+
+// Halo sweep. Transitions of the built models go from a cell of slice t to cells near it
+// in slice t+1 (the named workloads reach at most 7 cells, 97 % of entries at most 3,
+// 99.5 % at most 4). A block takes a tile of kHTileH x kHTileW cells of one slice and
+// stages the values of slice t+1 over the tile widened by H cells on every side (its KB
+// columns, float64) in LDS with coalesced loads. Each wave then takes one tile row: a
+// run of consecutive cells, so one contiguous range of the state-major entries, which it
+// streams in windows of WIN entries (coalesced loads, staged in its own LDS window, the
+// next window's loads issued before the current one is consumed). Lane (sl, a, cl) =
+// state sl of the wave's current group of 64 / (LC * NAP) states, action a, column lane
+// cl (CPL adjacent columns) walks its row's entries of each window from LDS and reads
+// each entry's values from the halo (from global memory for an entry that lands outside
+// it). Per row and column the products are added in ascending entry order, as in every
+// other sweep, so the bits are theirs.
+constexpr int kHThreads = 256;
+constexpr int kHWaves = kHThreads / 64;
+constexpr int kHTileH = 4, kHTileW = 16;
+constexpr int kHMaxBlocks = 2048;
+
+template <int LC, int CPL, int H>
+__global__ __launch_bounds__(kHThreads) void spmm_vi_halo(const BatchArgs A) {
+    constexpr int KB = LC * CPL;
+    constexpr int HH = kHTileH + 2 * H, HW = kHTileW + 2 * H;
+    constexpr int U = 4, WIN = 64 * U;
+    constexpr int U2 = CPL <= 4 ? 4 : 2;  // entries per step of a lane's row walk
+    static_assert(CPL % 2 == 0, "columns are staged and read in pairs");
+    __shared__ double2 halo[HH * HW * KB / 2];
+    __shared__ int2 win[kHWaves][WIN];
+    __shared__ float red[kHWaves][KB];
+    __shared__ float fin[kHThreads];
+    __shared__ int is_last;
+    BatchState* st = A.st;
+    if (!A.one_pass && st->all_conv) return;
+    const uint32_t k = A.one_pass ? 1u : st->k;
+    const double* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
+    double* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nap = A.nap;
+    const int cl = lane % LC;
+    const int a = (lane / LC) & (nap - 1);
+    const int sl = lane / (LC * nap);
+    const int spw = 64 / (LC * nap);
+    const int jb0 = blockIdx.y * KB;        // the block's first column
+    const int j0 = jb0 + cl * CPL;          // this lane's first column
+    const int KP = A.kp;
+    const int g = A.g, nc = A.nc, na = A.na;
+    int goal[CPL];
+    bool live[CPL];
+    float rmax[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; c++) {
+        goal[c] = A.goal[j0 + c];
+        live[c] = A.conv[j0 + c] < 0;
+        rmax[c] = 0.0f;
+    }
+    const bool goals = A.Rcol == nullptr;
+    int2* wb = win[wv];
+    // tiles of the slices this launch covers: every active slice of the sweep, or the one
+    // slice of a backward-pass step
+    const int tr = (g + kHTileH - 1) / kHTileH, tcn = (g + kHTileW - 1) / kHTileW, tps = tr * tcn;
+    const int t_lo = A.one_pass ? (int)(A.s_lo / (uint32_t)nc) : 0;
+    const int t_hi = A.one_pass ? t_lo + 1 : A.pre_nt ? sweep_active(A, k, jb0, KB) : A.nt;
+    const int ntiles = (t_hi - t_lo) * tps;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int t = t_lo + tile / tps;
+        const int rem = tile - (t - t_lo) * tps;
+        const int ti = rem / tcn, tj = rem - ti * tcn;
+        const int i0 = ti * kHTileH, jc0 = tj * kHTileW;
+        const int th = min(kHTileH, g - i0), tw = min(kHTileW, g - jc0);
+        const int hi0 = i0 - H, hj0 = jc0 - H;  // grid cell of halo entry (0, 0)
+        const int colbase = (t + 1) * nc;
+        __syncthreads();  // the previous tile's reads of the halo are done
+        if (t + 1 < A.nt) {
+            // halo[(hi * HW + hj) * KB/2 + c2] = V(t+1)[cell (hi0 + hi, hj0 + hj)][columns
+            // jb0 + 2 c2, +1]; cells outside the grid are never read (every entry lands
+            // on a cell of the grid)
+            for (int q = tid; q < HH * HW * (KB / 2); q += kHThreads) {
+                const int c2 = q % (KB / 2), hc = q / (KB / 2);
+                const int hi = hc / HW, hj = hc - hi * HW;
+                const int i = hi0 + hi, j = hj0 + hj;
+                if (i >= 0 && i < g && j >= 0 && j < g)
+                    halo[q] = *reinterpret_cast<const double2*>(Vin + (int64_t)(colbase + i * g + j) * KP + jb0 +
+                                                                2 * c2);
+            }
+        }
+        __syncthreads();
+        if (wv >= th) continue;  // block-uniform per wave; the next tile's barrier waits for all
+        // does a goal of the block's columns lie in the halo (then entries landing in it
+        // are compared with the goals; else only those landing outside it)
+        bool goal_tile = false;
+        if (goals) {
+            for (int c = 0; c < KB; c++) {
+                const int gc = A.goal[jb0 + c];
+                if (gc >= 0) {
+                    const int gi2 = gc / g, gj2 = gc - gi2 * g;
+                    goal_tile = goal_tile || ((unsigned)(gi2 - hi0) < (unsigned)HH && (unsigned)(gj2 - hj0) < (unsigned)HW);
+                }
+            }
+        }
+        const float delta = (float)((double)A.r_term - (t > A.nt - 20 ? A.late_pen : 0.0));
+        // this wave's run: tile row wv, cells jc0 .. jc0+tw-1 (states s_run .. s_run+tw-1)
+        const int cell0 = (i0 + wv) * g + jc0;
+        const int64_t s_run = (int64_t)t * nc + cell0;
+        const int ngroups = (tw + spw - 1) / spw;
+        const uint32_t P0 = A.rp[s_run * na], Pend = A.rp[(s_run + tw) * na];
+        // row data of group gi (clamped to the run's last state, so always in range)
+        struct Meta {
+            uint32_t rb, re, ce;
+            float rr, rs;
+        };
+        auto load_meta = [&](int gi, Meta& M) {
+            const int q = min(gi * spw + sl, tw - 1);
+            const int64_t row = (s_run + q) * na + min(a, na - 1);
+            M.rb = A.rp[row];
+            M.re = A.rp[row + 1];
+            M.ce = A.rp[(s_run + min(gi * spw + spw, tw)) * na];
+            M.rr = A.R[row];
+            M.rs = A.Rsum ? A.Rsum[row] : 0.0f;
+        };
+        Meta cur, nxt;
+        int gi = 0;
+        load_meta(0, cur);
+        load_meta(min(1, ngroups - 1), nxt);
+        double sd[CPL];
+        int hit[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; c++) {
+            sd[c] = 0.0;
+            hit[c] = -1;
+        }
+        auto finish_group = [&]() {
+            const int q = gi * spw + sl;
+            const bool sval = q < tw;
+            const bool valid = sval && a < na;
+            const int64_t s = s_run + min(q, tw - 1);
+            const int64_t row = s * na + min(a, na - 1);
+            double qv[CPL];
+            int ba[CPL];
+#pragma unroll
+            for (int c = 0; c < CPL; c++) {
+                const float rj = !goals ? (valid ? A.Rcol[(int64_t)min(j0 + c, A.rcol_n - 1) * A.rows + row] : 0.0f)
+                                 : hit[c] >= 0 ? (cur.rs + (float)hit[c] * delta) / A.nr
+                                               : cur.rr;
+                double qq = valid ? (double)rj + (double)A.discount * sd[c] : -INFINITY;
+                if (qq != qq) qq = -INFINITY;
+                qv[c] = qq;
+                ba[c] = valid ? a : 0x7fffffff;
+            }
+            // max / first argmax over the NAP lanes of the state (strict '>' in action
+            // order, spvi_2.cu:112), as in spmm_vi_rows
+            for (int o = LC; o < LC * nap; o <<= 1) {
+#pragma unroll
+                for (int c = 0; c < CPL; c++) {
+                    const double q2 = __shfl_xor(qv[c], o, 64);
+                    const int a2 = __shfl_xor(ba[c], o, 64);
+                    if (q2 > qv[c] || (q2 == qv[c] && a2 < ba[c])) {
+                        qv[c] = q2;
+                        ba[c] = a2;
+                    }
+                }
+            }
+            if (a == 0 && sval) {
+                const int cell = cell0 + q;
+                const int64_t o = s * KP + j0;
+#pragma unroll
+                for (int c = 0; c < CPL; c++) {
+                    double vo;
+                    int po;
+                    if (cell == goal[c]) {
+                        vo = 0.0;
+                        po = 0;
+                    } else if (qv[c] > -1e6) {
+                        vo = qv[c];
+                        po = ba[c];
+                    } else {
+                        vo = -1e6;
+                        po = -1;
+                    }
+                    if (live[c]) {
+                        Vout[o + c] = vo;
+                        A.pi[o + c] = po;
+                        rmax[c] = fmaxf(rmax[c], res_up(fabs(vo - Vin[o + c])));
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CPL; c++) {
+                sd[c] = 0.0;
+                hit[c] = -1;
+            }
+            gi++;
+            cur = nxt;
+            load_meta(min(gi + 1, ngroups - 1), nxt);
+        };
+        // the run's entries, windows of WIN from P0; entries past Pend are loaded (the
+        // entry array has a zeroed tail of kNzPad entries) but belong to no row of the run.
+        // Each entry is decoded once, as it is staged: x = the halo index (double2 units)
+        // of its landing cell's record, or 0x80000000 | column for an entry that lands
+        // outside the halo; y = its probability bits
+        auto decode = [&](int2 e) {
+            const int cc = e.x - colbase;
+            const int i2 = (int)__umulhi((uint32_t)cc, A.gmagic);
+            const int hi = i2 - hi0, hj = cc - i2 * g - hj0;
+            const bool in = (unsigned)hi < (unsigned)HH && (unsigned)hj < (unsigned)HW;
+            return make_int2(in ? (hi * HW + hj) * (KB / 2) : (int)(0x80000000u | (uint32_t)e.x), e.y);
+        };
+        const uint32_t nwin = Pend > P0 ? (Pend - P0 + WIN - 1) / WIN : 0u;
+        int2 e[U];
+        if (nwin > 0) {
+#pragma unroll
+            for (int u = 0; u < U; u++) e[u] = A.nz[P0 + u * 64 + lane];
+        }
+        for (uint32_t kw = 0; kw < nwin; kw++) {
+            const uint32_t w0 = P0 + kw * WIN, wend = w0 + WIN;
+#pragma unroll
+            for (int u = 0; u < U; u++) wb[u * 64 + lane] = decode(e[u]);
+            if (kw + 1 < nwin) {  // the next window's loads, in flight while this one is used
+#pragma unroll
+                for (int u = 0; u < U; u++) e[u] = A.nz[wend + u * 64 + lane];
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (;;) {
+                const bool valid = gi * spw + sl < tw && a < na;
+                const uint32_t jlo = valid ? max(cur.rb, w0) : 0u, jhi = valid ? min(cur.re, wend) : 0u;
+                // U2 entries per step: their window reads, then their halo reads, are issued
+                // together (an entry outside the halo reads halo record 0 and is then taken
+                // from global memory); the products are added in entry order
+                for (uint32_t j = jlo; j < jhi; j += U2) {
+                    int2 en[U2];
+#pragma unroll
+                    for (int u = 0; u < U2; u++) en[u] = wb[min(j + (uint32_t)u, jhi - 1u) - w0];
+                    double2 hv[U2][CPL / 2];
+#pragma unroll
+                    for (int u = 0; u < U2; u++) {
+                        const double2* hp = halo + (en[u].x >= 0 ? en[u].x : 0) + cl * (CPL / 2);
+#pragma unroll
+                        for (int c2 = 0; c2 < CPL / 2; c2++) hv[u][c2] = hp[c2];
+                    }
+#pragma unroll
+                    for (int u = 0; u < U2; u++) {
+                        if (j + (uint32_t)u >= jhi) break;
+                        const float v = __int_as_float(en[u].y);
+                        const double pv = (double)v;
+                        if (en[u].x >= 0) {  // the landing cell's values from the halo
+#pragma unroll
+                            for (int c2 = 0; c2 < CPL / 2; c2++) {
+                                sd[2 * c2] = sd[2 * c2] + pv * hv[u][c2].x;
+                                sd[2 * c2 + 1] = sd[2 * c2 + 1] + pv * hv[u][c2].y;
+                            }
+                            if (goal_tile) {  // a goal of these columns lies in the halo
+                                const int hc = en[u].x / (KB / 2), hi = hc / HW;
+                                const int cc = (hi0 + hi) * g + hj0 + (hc - hi * HW);
+                                const int n = (int)rintf(v * (float)A.nr);
+#pragma unroll
+                                for (int c = 0; c < CPL; c++)
+                                    if (cc == goal[c]) hit[c] = n;
+                            }
+                        } else {  // outside the halo: from global memory
+                            const int col = en[u].x & 0x7fffffff;
+                            double x[CPL];
+                            VecLoad<CPL>::ld(Vin + (int64_t)col * KP + j0, x);
+                            const int cc = col - colbase;
+                            const int n = (int)rintf(v * (float)A.nr);
+#pragma unroll
+                            for (int c = 0; c < CPL; c++) {
+                                sd[c] = sd[c] + pv * x[c];
+                                if (goals && cc == goal[c]) hit[c] = n;
+                            }
+                        }
+                    }
+                }
+                if (cur.ce > wend) break;  // the group continues in the next window
+                finish_group();
+                if (gi >= ngroups) break;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        while (gi < ngroups) finish_group();  // trailing groups whose rows are all empty
+    }
+    if (A.one_pass) return;  // block-uniform
+#pragma unroll
+    for (int c = 0; c < CPL; c++)
+        for (int off = LC; off < 64; off <<= 1) rmax[c] = fmaxf(rmax[c], __shfl_xor(rmax[c], off, 64));
+    if (lane < LC) {
+#pragma unroll
+        for (int c = 0; c < CPL; c++) red[wv][cl * CPL + c] = rmax[c];
+    }
+    sweep_tail<KB, kHThreads>(A, red, fin, is_last, k);
 }
 
 // Streaming sweep with transposed gathers. The state partition and the entry stream
@@ -809,7 +1102,8 @@ __global__ void batch_gather(const double* V0, const double* V1, const int32_t* 
 
 // kind 0: spmm_vi_sweep (lc = 64/spw lanes per state); kind 1: spmm_vi_rows (lc lanes
 // per row); kind 2: spmm_vi_stream (cpl = KB columns per block, u = entries per lane
-// and window). Kinds 1 and 2 take a number of states per wave that depends on na.
+// and window); kind 3: spmm_vi_halo (lc lanes per row, built models only). Kinds 1-3
+// take a number of states per wave that depends on na.
 struct Variant {
     int cpl, lc;
     int kind;
@@ -832,6 +1126,9 @@ const Variant kVariants[] = {
     {4, 1, 2, 4, nullptr},                {8, 1, 2, 1, nullptr},
     {8, 1, 2, 2, nullptr},                {1, 1, 2, 4, nullptr},
     {2, 1, 2, 4, nullptr},
+    {4, 4, 3, 0, spmm_vi_halo<4, 4, 4>},  {4, 2, 3, 0, spmm_vi_halo<2, 4, 4>},
+    {2, 4, 3, 0, spmm_vi_halo<4, 2, 4>},  {4, 1, 3, 0, spmm_vi_halo<1, 4, 4>},
+    {8, 2, 3, 0, spmm_vi_halo<2, 8, 4>},  {2, 2, 3, 0, spmm_vi_halo<2, 2, 4>},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -853,10 +1150,24 @@ int spw_of(const Variant& v, int na) {
 
 bool variant_fits(const Variant& v, int na, int kp) { return spw_of(v, na) > 0 && kp % kb_of(v) == 0; }
 
+// the halo kernel needs a built model (cells of slice t move to cells of slice t+1, on a
+// square grid of at most 1024 cells a side, where umulhi by ceil(2^32 / g) divides exactly)
+bool halo_model(const Model* m) {
+    return m->built && m->nt > 0 && m->p.gsize >= 2 && m->p.gsize <= 1024 &&
+           m->nc == (int64_t)m->p.gsize * m->p.gsize && m->ns == m->nt * m->nc;
+}
+
+bool variant_ok(const Model* m, const Variant& v, int kp) {
+    return variant_fits(v, (int)m->na, kp) && (v.kind != 3 || halo_model(m));
+}
+
 // padded column count: 4 or 8 for small batches (the value block's stride per state is
 // then 32 or 64 bytes, so the gathers of a few columns touch fewer lines), else a
 // multiple of 16
 int padded_cols(int k) { return k <= 4 ? 4 : k <= 8 ? 8 : (int)ceil_div(k, 16) * 16; }
+
+// per-block residual rows of the partials buffer: the largest grid of any kernel
+constexpr int kPartRows = kHMaxBlocks > kBMaxBlocks ? kHMaxBlocks : kBMaxBlocks;
 
 struct BatchBufs {
     int kp = 0;
@@ -866,7 +1177,7 @@ struct BatchBufs {
     int32_t* goal = nullptr;
     int32_t* conv = nullptr;
     int32_t* start = nullptr;  // sweep index at which each column was admitted
-    float* part = nullptr;     // [kBMaxBlocks][kp]
+    float* part = nullptr;     // [kPartRows][kp]
     float* last_res = nullptr;
     BatchState* st = nullptr;
     float* vout = nullptr;
@@ -912,7 +1223,7 @@ static e2e_status alloc_bufs(const Model* m, BatchBufs& b, int kp) {
     if (hipMalloc(&b.V[0], n * 8) != hipSuccess || hipMalloc(&b.V[1], n * 8) != hipSuccess ||
         hipMalloc(&b.pi, n * 4) != hipSuccess || hipMalloc(&b.goal, kp * 4) != hipSuccess ||
         hipMalloc(&b.conv, kp * 4) != hipSuccess || hipMalloc(&b.start, kp * 4) != hipSuccess ||
-        hipMalloc(&b.part, (size_t)kBMaxBlocks * kp * 4) != hipSuccess ||
+        hipMalloc(&b.part, (size_t)kPartRows * kp * 4) != hipSuccess ||
         hipMalloc(&b.last_res, kp * 4) != hipSuccess || hipMalloc(&b.st, sizeof(BatchState)) != hipSuccess ||
         hipMalloc(&b.vout, n * 4) != hipSuccess || hipMalloc(&b.pout, n * 4) != hipSuccess ||
         hipMalloc(&b.colmax, kp * 4) != hipSuccess) {
@@ -968,6 +1279,8 @@ static BatchArgs make_args(const Model* m, const BatchBufs& b, const e2e_solve_o
     A.colmax = b.colmax;
     A.wpart = b.wpart;
     A.start = b.start;
+    A.g = m->p.gsize;
+    A.gmagic = m->p.gsize >= 2 ? (uint32_t)(((1ull << 32) + (uint64_t)m->p.gsize - 1) / (uint64_t)m->p.gsize) : 0u;
     return A;
 }
 
@@ -1018,6 +1331,22 @@ struct Plan {
 
 static e2e_status plan_of(const Model* m, BatchBufs& b, const Variant& v, hipStream_t s, Plan* p) {
     p->kb = kb_of(v);
+    if (v.kind == 3) {  // persistent blocks over the tiles of the swept slices
+        E2E_CHECK(halo_model(m), E2E_ERR_INVALID,
+                  "batched sweep: the halo kernel needs a model built by e2e_model_build (grid side <= 1024)");
+        int bpc = 0, ncu = 0, dev = 0;
+        E2E_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void*)v.fn, kHThreads, 0));
+        E2E_HIP(hipGetDevice(&dev));
+        E2E_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        const int64_t g = m->p.gsize;
+        const int64_t tiles = ceil_div(g, (int64_t)kHTileH) * ceil_div(g, (int64_t)kHTileW) * m->nt;
+        const int64_t nblocks = std::max<int64_t>(
+            1, std::min<int64_t>({(int64_t)std::max(bpc, 1) * std::max(ncu, 1), tiles, (int64_t)kHMaxBlocks}));
+        p->fn = v.fn;
+        p->block = dim3(kHThreads);
+        p->grid = dim3((unsigned)nblocks, (unsigned)(b.kp / p->kb));
+        return E2E_OK;
+    }
     if (v.kind != 2) {
         p->fn = v.fn;
         p->block = dim3(kBThreads);
@@ -1265,7 +1594,7 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
     hipEvent_t e0, e1;
     E2E_HIP(hipEventCreate(&e0));
     E2E_HIP(hipEventCreate(&e1));
-    if (variant < 0 || variant >= kNumVariants || !variant_fits(kVariants[variant], (int)m->na, kp)) {
+    if (variant < 0 || variant >= kNumVariants || !variant_ok(m, kVariants[variant], kp)) {
         if (c->tuned_kp == kp && c->tuned_variant >= 0) {
             variant = c->tuned_variant;
         } else if (m->nnz >= kTuneMaxNnz && kp <= 16) {
@@ -1276,7 +1605,7 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
             const int trial = std::min(3, max_sweeps);
             for (int vi = 0; vi < kNumVariants; vi++) {
                 const Variant& v = kVariants[vi];
-                if (!variant_fits(v, (int)m->na, kp)) continue;
+                if (!variant_ok(m, v, kp)) continue;
                 Plan pl;
                 E2E_TRY(plan_of(m, b, v, s, &pl));
                 A.wpart = b.wpart;

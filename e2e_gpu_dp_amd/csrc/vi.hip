@@ -70,6 +70,7 @@ struct VIArgs {
 constexpr int kVIThreads = 512;   // 8 waves per block
 constexpr int kVIWaves = kVIThreads / 64;
 constexpr int kVIMaxBlocks = 2048;  // 8 blocks of 512 threads per CU on 256 CUs
+static_assert(kVIMaxBlocks <= kTicketGroup * kTicketGroups, "the lean sweep's ticket groups cover its grid");
 
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
     s = a + b;
@@ -437,9 +438,10 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
     float res = 0.0f;
     // compressed entries: the probability of each count, float(count * (1 / nrzns)), once
     // per block in LDS (the same expression as the inline decode, so the same bits)
-    extern __shared__ double ptab[];
+    // (kept as float: each is a float rounding, widened to double exactly where it is used)
+    extern __shared__ float ptabf[];
     if (COMP && A.ptab_n) {
-        for (int c = tid; c < A.ptab_n; c += kStreamThreads) ptab[c] = (double)(float)((double)c * A.inv_nr);
+        for (int c = tid; c < A.ptab_n; c += kStreamThreads) ptabf[c] = (float)((double)c * A.inv_nr);
         __syncthreads();
     }
 
@@ -526,7 +528,6 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
         };
 
         const int2* __restrict__ nzp = A.nz + P0 + lane;
-        const uint32_t* __restrict__ czp = A.cz + P0 + lane;
         const CPtr<int32_t> cbs = (CPtr<int32_t>)A.cbase;
         for (uint32_t kw = 0; kw < nwin; kw++) {
             const uint32_t w0 = P0 + kw * WIN, wend = w0 + WIN;
@@ -538,18 +539,30 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
                 // product is within 2^-52 of count / nrzns, which for count < 2^16 lies
                 // at least 2^-41 away from any rounding midpoint of float); the window
                 // was kept compressed only if every entry decodes to its stored bits
+                // one 16-byte load per lane: entries w0 + 4 lane + u (the window is
+                // aligned to its 256 entries), staged at their offsets in the window
                 uint32_t e[U];
+                if constexpr (U == 4) {
+                    const uint4 e4 = *reinterpret_cast<const uint4*>(A.cz + w0 + 4u * (uint32_t)lane);
+                    e[0] = e4.x;
+                    e[1] = e4.y;
+                    e[2] = e4.z;
+                    e[3] = e4.w;
+                } else {  // (compressed windows are read with U = 4 only)
 #pragma unroll
-                for (int u = 0; u < U; u++) e[u] = czp[kw * WIN + u * 64];
+                    for (int u = 0; u < U; u++) e[u] = A.cz[w0 + (uint32_t)(U * lane + u)];
+                }
 #pragma unroll
                 for (int u = 0; u < U; u++) x[u] = ld_off(Vin, (uint32_t)base + (e[u] & 0xffffu));
                 if (A.ptab_n) {
 #pragma unroll
-                    for (int u = 0; u < U; u++) pv[u] = ptab[e[u] >> 16];
+                    for (int u = 0; u < U; u++) pv[u] = (double)ptabf[e[u] >> 16];
                 } else {
 #pragma unroll
                     for (int u = 0; u < U; u++) pv[u] = (double)(float)((double)(e[u] >> 16) * A.inv_nr);
                 }
+#pragma unroll
+                for (int u = 0; u < U; u++) wsd[U * lane + u] = pv[u] * x[u];
             } else {
                 int2 e[U];
 #pragma unroll
@@ -558,9 +571,9 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
                 for (int u = 0; u < U; u++) x[u] = ld_off(Vin, (uint32_t)e[u].x);
 #pragma unroll
                 for (int u = 0; u < U; u++) pv[u] = (double)__int_as_float(e[u].y);
-            }
 #pragma unroll
-            for (int u = 0; u < U; u++) wsd[u * 64 + lane] = pv[u] * x[u];
+                for (int u = 0; u < U; u++) wsd[u * 64 + lane] = pv[u] * x[u];
+            }
             __builtin_amdgcn_wave_barrier();
             for (;;) {
                 if (valid) row_part(wsd - w0, max(cur.rb, w0), min(cur.re, wend), sd);
@@ -584,7 +597,18 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
         for (int i = 0; i < kStreamWaves; i++) bm = fmaxf(bm, wmax[i]);
         A.part[blockIdx.x] = bm;
         __threadfence();
-        is_last = atomicAdd(&st->done, 1u) == gridDim.x - 1;
+        // two-level ticket: the last block of each group of kTicketGroup counts for its
+        // group (same-address atomics per sweep: kTicketGroup per group counter and one
+        // per group on the sweep's counter, instead of one per block)
+        const unsigned grp = blockIdx.x / kTicketGroup, ngrp = (gridDim.x + kTicketGroup - 1) / kTicketGroup;
+        const unsigned in_grp = min((unsigned)kTicketGroup, gridDim.x - grp * kTicketGroup);
+        int last = 0;
+        if (atomicAdd(&st->gdone[grp], 1u) == in_grp - 1u) {
+            st->gdone[grp] = 0u;  // every block of the group has counted: ready for the next sweep
+            __threadfence();
+            last = atomicAdd(&st->done, 1u) == ngrp - 1u;
+        }
+        is_last = last;
     }
     __syncthreads();
     if (is_last) {  // the last block of the sweep: reduce the partials, publish
@@ -1052,6 +1076,7 @@ __global__ void vi_init(float* V0, double* W0, int64_t ns, VIState* st) {
         V0[i] = 0.0f;
         W0[i] = 0.0;
     }
+    if (i < kTicketGroups) st->gdone[i] = 0u;
     if (i == 0) {
         st->k = 1u;
         st->conv = -1;
@@ -1191,6 +1216,33 @@ constexpr int kDefaultVariant = 5;
 constexpr int kBackward = 6;
 constexpr int kDefaultImported = 2;  // imported models: their probabilities are arbitrary
 constexpr int kLeanBlocksPerCU = 5;  // fastest resident grid of the lean sweep (100x100x120 model)
+// Resident blocks per CU of the lean sweep over compressed entries. Measured: full sweeps
+// of the 100x100x120 model (1.3e8 entries) 327 us at 5 blocks, 302 us at 8; of the
+// 200x200x200 model (1.0e9 entries) 2288 and 1832 us; its solve over the active slices
+// 237 and 196 ms; but the smaller model's solve over the active slices 22.3 ms at 5
+// against 23.4 ms at 8 (its late sweeps are a few slices each)
+constexpr int kCompBlocksPerCU = 8;
+constexpr int kCompBlocksActiveSmall = 5;   // active-slice solves of models below kCompBigNnz entries
+constexpr int64_t kCompBigNnz = int64_t(1) << 28;
+
+// Launch shape of the lean sweep over compressed entries: the probability table of the
+// counts sits in LDS (float, nrzns + 1 entries) when the blocks per CU wanted (want, or
+// kCompBlocksPerCU) still fit with it; else the probabilities are decoded inline (the
+// same bits) and the freed LDS keeps the occupancy.
+e2e_status comp_launch_shape(SweepFn fn, int nrzns, int want, int dflt, int* bpc, int* ptab_n) {
+    const int target = want > 0 ? want : dflt;
+    int n = (int64_t)(nrzns + 1) * 4 <= 32768 ? nrzns + 1 : 0;
+    int occ = 0;
+    if (n) {
+        E2E_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)fn, kStreamThreads,
+                                                             (size_t)n * sizeof(float)));
+        if (occ < target) n = 0;
+    }
+    if (!n) E2E_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)fn, kStreamThreads, 0));
+    *ptab_n = n;
+    *bpc = want > 0 ? want : std::max(1, std::min(occ, dflt));
+    return E2E_OK;
+}
 bool is_comp_variant(int variant) { return variant == 5; }
 bool is_lean_variant(int variant) { return variant == 2 || variant == 5; }
 bool is_tile_variant(int variant) { return variant == 3; }
@@ -1342,17 +1394,22 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
             m->vi_part_cap = ntiles;
         }
     }
-    // the compressed sweep keeps its probability table in LDS while it fits in 32 KB
-    const int ptab_n = (lean && comp && (int64_t)(m->p.nrzns + 1) * 8 <= 32768) ? m->p.nrzns + 1 : 0;
-    const size_t shmem = tiled ? (size_t)vwin * sizeof(double) : (size_t)ptab_n * sizeof(double);
     // persistent grid: as many blocks as can be resident at once (grid-stride over chunks,
-    // or one contiguous share per wave), so that no block starts after the others finished
-    const int64_t states_per_block = (threads / 64) * grp;
+    // or one contiguous share per wave), so that no block starts after the others finished;
+    // the compressed sweep keeps its probability table in LDS if that leaves its occupancy
+    int ptab_n = 0;
     int bpc = opts.blocks_per_cu;
-    if (bpc <= 0) {
-        E2E_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void*)fn, threads, shmem));
+    if (lean && comp) {
+        const bool active_small = !bw && !opts.full_sweeps && m->built && m->nnz < kCompBigNnz;
+        E2E_TRY(comp_launch_shape(fn, m->p.nrzns, opts.blocks_per_cu,
+                                  active_small ? kCompBlocksActiveSmall : kCompBlocksPerCU, &bpc, &ptab_n));
+    } else if (bpc <= 0) {
+        E2E_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void*)fn, threads,
+                                                             tiled ? (size_t)vwin * sizeof(double) : 0));
         if (lean) bpc = std::min(bpc, kLeanBlocksPerCU);
     }
+    const size_t shmem = tiled ? (size_t)vwin * sizeof(double) : (size_t)ptab_n * sizeof(float);
+    const int64_t states_per_block = (threads / 64) * grp;
     int ncu = 0, dev = 0;
     E2E_HIP(hipGetDevice(&dev));
     E2E_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1803,13 +1860,17 @@ e2e_status vi_solve_noalloc(Model* m, const e2e_solve_opts* o, uint32_t* policy,
     int lpr = 0;
     const SweepFn fn = comp ? pick_lean_nap<4, true>(nap) : pick_lean(nap, 4, &lpr);
     lpr = 4;
-    const int ptab_n = (comp && (int64_t)(m->p.nrzns + 1) * 8 <= 32768) ? m->p.nrzns + 1 : 0;
-    const size_t shmem = (size_t)ptab_n * sizeof(double);
+    int ptab_n = 0;
     int bpc = opts.blocks_per_cu;
-    if (bpc <= 0) {
-        E2E_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void*)fn, kStreamThreads, shmem));
+    if (comp) {
+        const bool active_small = !opts.full_sweeps && m->built && m->nnz < kCompBigNnz;
+        E2E_TRY(comp_launch_shape(fn, m->p.nrzns, opts.blocks_per_cu,
+                                  active_small ? kCompBlocksActiveSmall : kCompBlocksPerCU, &bpc, &ptab_n));
+    } else if (bpc <= 0) {
+        E2E_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void*)fn, kStreamThreads, 0));
         bpc = std::min(bpc, kLeanBlocksPerCU);
     }
+    const size_t shmem = (size_t)ptab_n * sizeof(float);
     int ncu = 0, dev = 0;
     E2E_HIP(hipGetDevice(&dev));
     E2E_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));

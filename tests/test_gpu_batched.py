@@ -6,6 +6,7 @@ import pytest
 
 import e2e_gpu_dp_amd as e2e
 from e2e_gpu_dp_amd.configs import goal_cells, small, workload
+from e2e_gpu_dp_amd.fields import make_fields
 from helpers import assert_close_to_f64, goal_params, oracle_model, rel_err
 
 import oracle
@@ -92,8 +93,10 @@ def test_batched_columns_independent_of_batch_and_variant(gpu):
 
 
 # SpMM variants 0-8: one lane group per state walks every row; 9-20: one lane group per
-# row (spmm_vi_rows), as (column lanes, columns per lane)
-N_VARIANTS = 27
+# row (spmm_vi_rows), as (column lanes, columns per lane); 27-32: the same over a tile's
+# LDS halo of the next slice's values (spmm_vi_halo)
+N_VARIANTS = 33
+HALO_VARIANTS = {27: (4, 4), 28: (2, 4), 29: (4, 2), 30: (1, 4), 31: (2, 8), 32: (2, 2)}
 ROWS_VARIANTS = {9: (4, 1), 10: (8, 1), 11: (4, 2), 12: (8, 2), 13: (4, 4), 14: (16, 1), 15: (8, 4),
                  16: (16, 2), 17: (1, 4), 18: (1, 8), 19: (2, 4), 20: (2, 8)}
 
@@ -104,25 +107,31 @@ STATE_VARIANTS_KB = {0: 64, 1: 64, 2: 64, 3: 32, 4: 32, 5: 32, 6: 16, 7: 16, 8: 
 STREAM_VARIANTS_KB = {21: 4, 22: 4, 23: 8, 24: 8, 25: 1, 26: 2}
 
 
-@pytest.mark.parametrize("na,k", [(8, 5), (8, 16), (16, 40), (12, 32), (4, 64), (16, 3)])
-def test_batched_row_kernels_bit_identical(gpu, na, k):
-    """Every variant that fits the model (row variants: column lanes x actions rounded up
-    to a power of two <= 64; all: column block dividing the padded width, 4 or 8 columns
-    for k <= 4 or 8, else a multiple of 16) runs as asked and gives the same bits; na = 12
-    leaves padding action lanes."""
-    g = 12
+@pytest.mark.parametrize("na,k,g,fast", [(8, 5, 12, 0), (8, 16, 12, 0), (16, 40, 12, 0), (12, 32, 12, 0),
+                                         (4, 64, 12, 0), (16, 3, 12, 0), (8, 16, 37, 0), (16, 16, 23, 1)])
+def test_batched_row_kernels_bit_identical(gpu, na, k, g, fast):
+    """Every variant that fits the model (row and halo variants: column lanes x actions
+    rounded up to a power of two <= 64; all: column block dividing the padded width, 4 or
+    8 columns for k <= 4 or 8, else a multiple of 16) runs as asked and gives the same
+    bits; na = 12 leaves padding action lanes. g = 37 and 23 leave partial halo tiles at
+    the grid's edges; the fast flow moves lanes up to 5+ cells, outside the halo."""
     w = small(g=g, nt=9, nr=300, nm=3, na=na, seed=91 + na)
-    f = w.fields()
+    p = w.params
+    f = make_fields(p.gsize, p.nt, p.nmodes, p.nrzns, 91 + na, mean_speed_frac=3.5) if fast else w.fields()
     goals = goal_cells(g, k, 17 + k)
     kp = 4 if k <= 4 else 8 if k <= 8 else -(-k // 16) * 16
     nap = 1 << (na - 1).bit_length()
     with _shared(w, f) as m:
+        if fast:  # some entries land more than 4 cells away: outside the halo
+            S1, S2, _, _ = m.export_coo()
+            sp1, sp2 = (S1 % p.ns) % p.nc, S2 % p.nc
+            assert (np.maximum(np.abs(sp2 // g - sp1 // g), np.abs(sp2 % g - sp1 % g)) > 4).any()
         ref_pol, ref_val, ref_sw, ref_st = m.solve_goals(goals, thresh=1e-6, variant=9)
         assert ref_st["variant"] == 9
         ran = set()
         for variant in range(N_VARIANTS):
-            if variant in ROWS_VARIANTS:
-                lc, cpl = ROWS_VARIANTS[variant]
+            if variant in ROWS_VARIANTS or variant in HALO_VARIANTS:
+                lc, cpl = ROWS_VARIANTS.get(variant) or HALO_VARIANTS[variant]
                 fits = lc * nap <= 64 and kp % (lc * cpl) == 0
             elif variant in STREAM_VARIANTS_KB:
                 fits = kp % STREAM_VARIANTS_KB[variant] == 0
@@ -138,6 +147,7 @@ def test_batched_row_kernels_bit_identical(gpu, na, k):
             assert np.array_equal(pol, ref_pol) and np.array_equal(sw, ref_sw), variant
         assert len(ran & set(ROWS_VARIANTS)) >= 2
         assert len(ran & set(STREAM_VARIANTS_KB)) >= 2
+        assert len(ran & set(HALO_VARIANTS)) >= 2
         if kp >= 16:
             assert ran & set(STATE_VARIANTS_KB)
 
