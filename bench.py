@@ -197,8 +197,9 @@ def _batched(e2e, np, steps: int, cpu: bool = True):
     }
 
 
-def _serving(e2e, np, n_requests: int, seed: int = 5678):
-    """Poisson(20/s) planning requests over mixed environments; server vs serial queue."""
+def _serving(e2e, np, n_requests: int, seed: int = 5678, rate: float = 20.0, only=None):
+    """Poisson(rate) planning requests over mixed environments; server vs serial queue.
+    only: run just the server with these options and return its figures."""
     from e2e_gpu_dp_amd.configs import Params
     from e2e_gpu_dp_amd.fields import make_fields
     rng = np.random.default_rng(seed)
@@ -211,7 +212,7 @@ def _serving(e2e, np, n_requests: int, seed: int = 5678):
         nm = 5
         p = Params(gsize=g, nt=nt, nrzns=nr, nmodes=nm, num_ac_angles=na, term_subgrid_size=0)
         envs[eid] = (p, make_fields(g, nt, nm, nr, seed + 1 + eid))
-    arrivals = np.cumsum(rng.exponential(1 / 20.0, n_requests))
+    arrivals = np.cumsum(rng.exponential(1 / rate, n_requests))
     reqs = []
     for t in arrivals:
         eid = int(rng.integers(0, len(envs)))
@@ -234,10 +235,24 @@ def _serving(e2e, np, n_requests: int, seed: int = 5678):
             res = [srv.poll(i) for i in ids]
         return st, res
 
+    if only is not None:
+        st, res = run(**only)
+        slow = sorted(range(len(res)), key=lambda i: -res[i]["latency_ms"])[:6]
+        return {"p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "solves_per_s": st["solves_per_s"],
+                "completed": st["completed"], "wall_ms": st["wall_ms"],
+                "slowest": [(i, reqs[i][1], round(reqs[i][0], 3), round(res[i]["latency_ms"], 1),
+                             round(res[i]["queue_ms"], 1), res[i]["cache_hit"]) for i in slow]}
     st, res = run(max_cols=64)
     st_q, res_q = run(max_cols=1, serial=True)
     st_x, res_x = run(max_cols=64, exact_pass=True)
     same = sum(1 for a, b in zip(res, res_q) if a["value_at_start"] == b["value_at_start"])
+    heavy = None
+    if rate == 20.0:  # a stream the one-at-a-time queue cannot keep up with
+        h = {name: _serving(e2e, np, 2 * n_requests, seed, rate=60.0, only=opts)
+             for name, opts in (("server", dict(max_cols=64)), ("queue", dict(max_cols=1, serial=True)),
+                                ("exact_pass", dict(max_cols=64, exact_pass=True)))}
+        heavy = {"workload": f"{2 * n_requests} requests, Poisson 60/s, the same 6 environments",
+                 **{k: {kk: v[kk] for kk in ("p50_ms", "p99_ms", "solves_per_s", "completed")} for k, v in h.items()}}
     return {
         "workload": f"{n_requests} requests, Poisson 20/s, 6 environments "
                     f"(grids {{50,100,150}}^2, nt U[60,150], 8/16 headings, 1000-5000 realizations)",
@@ -249,6 +264,7 @@ def _serving(e2e, np, n_requests: int, seed: int = 5678):
         "same_value_at_start_as_queue": f"{same}/{len(res)}",
         "exact_pass": {"p50_ms": st_x["p50_ms"], "p99_ms": st_x["p99_ms"], "solves_per_s": st_x["solves_per_s"],
                        "completed": st_x["completed"]},
+        "heavy": heavy,
     }
 
 

@@ -10,11 +10,12 @@
 //     while others are on the GPU join the running batch (continuous batching);
 //   * a column leaves the batch at the first sweep whose own residual is below thresh,
 //     which is where its solo solve stops; its slot is reused immediately;
-//   * a request whose model is absent triggers a build on a builder thread (its own
-//     stream), admitted only if the model's estimated footprint fits the HBM budget,
+//   * a request whose model is absent triggers a build on one of the builder threads (on
+//     its environment's stream), admitted only if the model's estimated footprint fits the HBM budget,
 //     after evicting idle models (no live columns) in least-recently-used order;
-//   * every model steps on its own stream, so builds and the sweeps of different
-//     models overlap on the device.
+//   * every model steps on its own stream, at its own pace (a model's next step is
+//     launched as soon as its previous one has finished and been harvested), so builds
+//     and the sweeps of different models overlap on the device.
 // Column arithmetic does not depend on the other columns or on when a column joined,
 // so a request's result is bit-identical alone and in the mix.
 #include <hip/hip_runtime.h>
@@ -50,6 +51,10 @@ int live_kp(const LiveBatch* L);
 namespace {
 
 enum ReqStatus { RQ_QUEUED = 0, RQ_RUNNING = 1, RQ_DONE = 2, RQ_REJECTED = 3, RQ_FAILED = 4 };
+
+// Models built at once (each on its environment's stream): a small model's first request
+// does not wait for the build of a large one that arrived just before it
+constexpr int kBuilders = 2;
 enum EnvState { ENV_ABSENT = 0, ENV_BUILDING = 1, ENV_READY = 2, ENV_FAILED = 3 };
 
 struct Req {
@@ -81,6 +86,8 @@ struct Env {
     int live = 0;                 // occupied slots
     std::vector<int32_t> adm_cols, adm_goals;  // admissions for the next boundary
     std::vector<int> steps_since_admit;        // per slot, sweeps launched since admission
+    hipEvent_t step_done = nullptr;            // recorded after the launches of a step
+    bool inflight = false;                     // a step is on the GPU (not yet harvested)
 };
 
 }  // namespace
@@ -93,7 +100,8 @@ struct Server {
     std::unordered_map<uint64_t, Req*> reqs;
     std::deque<uint64_t> pending;
     std::deque<int64_t> build_q;
-    std::thread sched, builder;
+    std::thread sched;
+    std::vector<std::thread> builders;  // kBuilders threads, each building one model at a time
     std::atomic<bool> stop{false};
     uint64_t next_id = 1;
     int64_t budget = 0, resident = 0, peak = 0;
@@ -262,73 +270,104 @@ struct Server {
                     continue;
                 }
             }
-            // ---- launch admissions and sweeps on every active model's stream (async)
-            // sweeps per model in this step: the largest running model takes spc, a model
-            // with fewer entries proportionally more (up to 16 spc), so that small models
-            // are not held to the sweep rate of the largest (extra sweeps of converged
-            // columns do nothing)
+            // ---- each model at its own pace: a model whose last step has finished is
+            // harvested and, if it still has running columns, admits and launches its
+            // next step at once; a model still on the GPU is left alone. Sweeps per step:
+            // the largest running model takes spc, a model with fewer entries
+            // proportionally more (up to 16 spc), so that small models are not held to
+            // the sweep rate of the largest (extra sweeps of converged columns do nothing)
             const int spc = o.sweeps_per_step > 0 ? o.sweeps_per_step : 4;
             int64_t nnz_max = 1;
             for (Env* e : active) nnz_max = std::max<int64_t>(nnz_max, e->model->nnz);
+            bool progressed = false;
             for (Env* e : active) {
+                if (e->inflight) {
+                    const hipError_t q = hipEventQuery(e->step_done);
+                    if (q == hipErrorNotReady) continue;
+                    e->inflight = false;
+                    progressed = true;
+                    harvest(e, conv, start);
+                }
+                bool running;
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    running = e->live > 0;
+                }
+                if (!running || !e->batch) continue;
                 const int64_t ratio = nnz_max / std::max<int64_t>(e->model->nnz, 1);
                 const int sw = o.exact_pass ? 1 : (int)std::min<int64_t>((int64_t)spc * 16, (int64_t)spc * std::max<int64_t>(ratio, 1));
-                if (!e->adm_cols.empty()) {
-                    live_admit(e->batch, e->adm_cols.data(), e->adm_goals.data(), (int)e->adm_cols.size());
-                    e->adm_cols.clear();
-                    e->adm_goals.clear();
-                }
+                std::vector<int32_t> cols, goals;
                 int hi = 0;  // highest occupied slot + 1: the columns a sweep must cover
-                for (int c = 0; c < (int)e->slot.size(); c++)
-                    if (e->slot[c]) hi = c + 1;
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    cols.swap(e->adm_cols);
+                    goals.swap(e->adm_goals);
+                    for (int c = 0; c < (int)e->slot.size(); c++) {
+                        if (e->slot[c]) {
+                            hi = c + 1;
+                            e->steps_since_admit[c] += o.exact_pass ? spc : sw;
+                        }
+                    }
+                }
+                if (!cols.empty()) live_admit(e->batch, cols.data(), goals.data(), (int)cols.size());
                 // sweeps (each column stops at its own first residual below thresh), or one
                 // exact backward pass that finishes every running column
                 if (o.exact_pass)
                     live_pass(e->batch, hi);
                 else
                     live_step(e->batch, sw, hi);
-                for (int c = 0; c < (int)e->slot.size(); c++)
-                    if (e->slot[c]) e->steps_since_admit[c] += o.exact_pass ? spc : sw;
+                (void)hipEventRecord(e->step_done, e->stream);
+                e->inflight = true;
+                progressed = true;
                 sweeps_launched += o.exact_pass ? spc : sw;
             }
-            // ---- harvest converged columns
-            for (Env* e : active) {
-                BatchState st{};
-                const int k = live_kp(e->batch);
-                conv.resize(k);
-                start.resize(k);
-                if (live_state(e->batch, &st, conv.data(), start.data()) != E2E_OK) continue;
-                const int max_sweeps = e->p.nt + 4;
-                for (int c = 0; c < k; c++) {
-                    const uint64_t rid = e->slot[c];
-                    if (!rid) continue;
-                    Req* r;
-                    {
-                        std::lock_guard<std::mutex> lk(mu);
-                        r = reqs[rid];
-                    }
-                    if (conv[c] >= 0) {
-                        const int64_t s0 = r->start;  // state (t = 0, start cell)
-                        if (o.keep_full_results) {
-                            r->vcol.resize(e->model->ns);
-                            r->pcol.resize(e->model->ns);
-                        }
-                        live_harvest(e->batch, c, s0, &r->v_start, &r->a_start,
-                                     o.keep_full_results ? r->vcol.data() : nullptr,
-                                     o.keep_full_results ? r->pcol.data() : nullptr);
-                        std::lock_guard<std::mutex> lk(mu);
-                        r->sweeps = o.exact_pass ? 1 : conv[c] - start[c] + 1;
-                        e->slot[c] = 0;
-                        e->live--;
-                        e->last_used = now_ms();
-                        complete(r, RQ_DONE);
-                    } else if (e->steps_since_admit[c] > max_sweeps + 8) {
-                        std::lock_guard<std::mutex> lk(mu);
-                        e->slot[c] = 0;  // the column keeps sweeping until the slot is reused
-                        e->live--;
-                        complete(r, RQ_FAILED);
-                    }
+            // nothing finished or started: poll again (a sleep would add a timer period,
+            // tens of microseconds or more, to every short step of a small model)
+            if (!progressed) std::this_thread::yield();
+        }
+    }
+
+    // the converged columns of a model whose step has finished: results to their requests
+    void harvest(Env* e, std::vector<int32_t>& conv, std::vector<int32_t>& start) {
+        BatchState st{};
+        const int k = live_kp(e->batch);
+        conv.resize(k);
+        start.resize(k);
+        if (live_state(e->batch, &st, conv.data(), start.data()) != E2E_OK) return;
+        const int max_sweeps = e->p.nt + 4;
+        for (int c = 0; c < k; c++) {
+            uint64_t rid;
+            Req* r;
+            int since;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                rid = e->slot[c];
+                if (!rid) continue;
+                // a column admitted for the next step has not run yet
+                if (std::find(e->adm_cols.begin(), e->adm_cols.end(), c) != e->adm_cols.end()) continue;
+                r = reqs[rid];
+                since = e->steps_since_admit[c];
+            }
+            if (conv[c] >= 0) {
+                const int64_t s0 = r->start;  // state (t = 0, start cell)
+                if (o.keep_full_results) {
+                    r->vcol.resize(e->model->ns);
+                    r->pcol.resize(e->model->ns);
                 }
+                live_harvest(e->batch, c, s0, &r->v_start, &r->a_start,
+                             o.keep_full_results ? r->vcol.data() : nullptr,
+                             o.keep_full_results ? r->pcol.data() : nullptr);
+                std::lock_guard<std::mutex> lk(mu);
+                r->sweeps = o.exact_pass ? 1 : conv[c] - start[c] + 1;
+                e->slot[c] = 0;
+                e->live--;
+                e->last_used = now_ms();
+                complete(r, RQ_DONE);
+            } else if (since > max_sweeps + 8) {
+                std::lock_guard<std::mutex> lk(mu);
+                e->slot[c] = 0;  // the column keeps sweeping until the slot is reused
+                e->live--;
+                complete(r, RQ_FAILED);
             }
         }
     }
@@ -371,7 +410,7 @@ e2e_status e2e_server_create(const e2e_server_opts* opts, void** out) {
         S->budget = (int64_t)(0.8 * (double)fr);
     }
     S->sched = std::thread([S] { S->sched_loop(); });
-    S->builder = std::thread([S] { S->builder_loop(); });
+    for (int i = 0; i < kBuilders; i++) S->builders.emplace_back([S] { S->builder_loop(); });
     *out = S;
     return E2E_OK;
 }
@@ -401,8 +440,15 @@ e2e_status e2e_server_add_env(void* srv, int64_t env_id, const e2e_params* p, co
             set_error("hipStreamCreate failed");
             return E2E_ERR_HIP;
         }
+        if (hipEventCreateWithFlags(&e->step_done, hipEventDisableTiming) != hipSuccess) {
+            (void)hipStreamDestroy(e->stream);
+            delete e;
+            set_error("hipEventCreate failed");
+            return E2E_ERR_HIP;
+        }
         std::lock_guard<std::mutex> lk(S->mu);
         if (S->envs.count(env_id)) {
+            (void)hipEventDestroy(e->step_done);
             (void)hipStreamDestroy(e->stream);
             delete e;
             set_error("environment %lld already registered", (long long)env_id);
@@ -582,11 +628,13 @@ e2e_status e2e_server_destroy(void* srv) {
     S->wake.notify_all();
     S->build_cv.notify_all();
     if (S->sched.joinable()) S->sched.join();
-    if (S->builder.joinable()) S->builder.join();
+    for (auto& t : S->builders)
+        if (t.joinable()) t.join();
     for (auto& kv : S->envs) {
         Env* e = kv.second;
         if (e->batch) live_free(e->batch);
         if (e->model) free_model(e->model);
+        if (e->step_done) (void)hipEventDestroy(e->step_done);
         if (e->stream) (void)hipStreamDestroy(e->stream);
         delete e;
     }

@@ -1294,15 +1294,20 @@ static dim3 grid_of(const Model* m, const Variant& v, int kp, int ncols) {
 }
 
 // Kernel by the width a sweep has to cover (live batches: highest live slot + 1; large
-// batched solves: the column count); all kernels share the layout and the sweep state, so the choice may change
-// from sweep to sweep. Measured on a 150x150 x 125 x 8-heading model (1.8e8 entries):
-// 2 columns 0.94 ms (spmm_vi_stream, 2 per block), 4 columns 1.57 ms (spmm_vi_rows<1,4,4>),
-// 16 columns 3.54 ms (spmm_vi_rows<2,8,2>), against 5.6 ms for spmm_vi_sweep<1,4> at any
-// width up to 16. A kernel that does not fit the model's action count is skipped.
-constexpr int kLiveWidths[] = {2, 4, 1 << 30};
-constexpr int kLiveVariants[] = {26, 17, 20};
+// batched solves: the column count); all kernels share the layout and the sweep state, so
+// the choice may change from sweep to sweep. Measured on a 150x150 x 125 x 8-heading model
+// (1.8e8 entries; plain sweep 0.37 ms), ms per sweep over every state:
+//   width   1-2: spmm_vi_stream<8, 2, 4> 0.95;
+//   width     4: spmm_vi_halo<1, 4> 1.38 (spmm_vi_rows<1, 4, 4> 1.59);
+//   width     8: spmm_vi_halo<2, 4> 1.97 (every row or state kernel 3.6 or more);
+//   width 16/32: spmm_vi_halo<2, 8> 3.53 / 6.90 (spmm_vi_rows<2, 8, 2> 3.59 / 7.24).
+// The halo kernels need a built model; where one does not fit, the second table's kernel
+// runs, and where that does not fit either, spmm_vi_sweep<1, 4> (fits every model).
+constexpr int kLiveWidths[] = {2, 4, 8, 1 << 30};
+constexpr int kLiveVariants[] = {26, 30, 28, 31};
+constexpr int kLiveAlt[] = {26, 17, 20, 20};
 constexpr int kLiveFallback = 6;  // spmm_vi_sweep<1, 4>: fits every model
-constexpr int kLiveN = 3;
+constexpr int kLiveN = 4;
 
 // index into kLiveWidths for a sweep over columns 0 .. ncols-1
 int width_class(int ncols) {
@@ -1311,9 +1316,11 @@ int width_class(int ncols) {
     return i;
 }
 
-// the variant of width class i on this model (the fallback where it does not fit)
+// the variant of width class i on this model
 int width_variant(const Model* m, int kp, int i) {
-    return variant_fits(kVariants[kLiveVariants[i]], (int)m->na, kp) ? kLiveVariants[i] : kLiveFallback;
+    if (variant_ok(m, kVariants[kLiveVariants[i]], kp)) return kLiveVariants[i];
+    if (variant_ok(m, kVariants[kLiveAlt[i]], kp)) return kLiveAlt[i];
+    return kLiveFallback;
 }
 
 // batched solves on models of at least this many entries and at most 16 columns take the
@@ -1407,13 +1414,21 @@ __global__ void live_pass_done(int32_t* conv, int kp) {
 // marked done with an even conv.
 static e2e_status backward_pass(const Model* m, BatchBufs& b, const BatchArgs& A, int ncols, hipStream_t s) {
     const int kp = b.kp;
-    const Variant& v = kVariants[(ncols > 4 && variant_fits(kVariants[20], (int)m->na, kp)) ? 20 : 17];
+    // the width table's kernel where it takes one slice per launch (the row and halo
+    // kernels), else the row kernels
+    int vi = width_variant(m, kp, width_class(ncols));
+    if (kVariants[vi].kind != 1 && kVariants[vi].kind != 3)
+        vi = (ncols > 4 && variant_fits(kVariants[20], (int)m->na, kp)) ? 20 : 17;
+    const Variant& v = kVariants[vi];
     Plan p;
     E2E_TRY(plan_of(m, b, v, s, &p));
     const int spw = spw_of(v, (int)m->na);
     const int64_t nc = m->nc, groups = ceil_div(nc, (int64_t)spw);
-    const dim3 grid((unsigned)std::min<int64_t>(ceil_div(groups, (int64_t)kBWaves), kBMaxBlocks),
-                    plan_grid(p, ncols).y);
+    const int64_t g = m->p.gsize;
+    const int64_t gx = v.kind == 3 ? std::min<int64_t>(ceil_div(g, (int64_t)kHTileH) * ceil_div(g, (int64_t)kHTileW),
+                                                       (int64_t)p.grid.x)
+                                   : std::min<int64_t>(ceil_div(groups, (int64_t)kBWaves), kBMaxBlocks);
+    const dim3 grid((unsigned)gx, plan_grid(p, ncols).y);
     BatchArgs B = A;
     B.one_pass = 1;
     B.V1 = B.V0;
@@ -1833,7 +1848,7 @@ struct LiveBatch {
 
 int64_t live_batch_bytes(const Model* m, int kp) {
     // V[2] float64, pi, and the float value / policy staging of a harvest
-    return (int64_t)m->ns * kp * (2 * 8 + 3 * 4) + (int64_t)kp * (44 + 4 * kBMaxBlocks) + 65536;
+    return (int64_t)m->ns * kp * (2 * 8 + 3 * 4) + (int64_t)kp * (44 + 4 * kPartRows) + 65536;
 }
 
 void live_free(LiveBatch* L);
