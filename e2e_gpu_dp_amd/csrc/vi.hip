@@ -529,23 +529,41 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
 
         const int2* __restrict__ nzp = A.nz + P0 + lane;
         const CPtr<int32_t> cbs = (CPtr<int32_t>)A.cbase;
+        // compressed windows: the next window's entries are requested as soon as this
+        // window's value gathers are (ecur holds them when has_next)
+        uint32_t ecur[U];
+        bool has_next = false;
+        auto load_comp = [&](uint32_t w, uint32_t* dst) {
+#pragma unroll
+            for (int u = 0; u < U; u++) dst[u] = A.cz[w + (uint32_t)(u * 64 + lane)];
+        };
+        if (COMP && nwin > 0 && cbs[P0 / WIN] >= 0) {
+            load_comp(P0, ecur);
+            has_next = true;
+        }
         for (uint32_t kw = 0; kw < nwin; kw++) {
             const uint32_t w0 = P0 + kw * WIN, wend = w0 + WIN;
             double x[U], pv[U];
             const int32_t base = COMP ? cbs[w0 / WIN] : -1;
+            const bool next_comp = COMP && kw + 1 < nwin && cbs[w0 / WIN + 1] >= 0;
             if (COMP && base >= 0) {
-                // 4-byte entries: the probability count / nrzns is rounded to float from
-                // the double count * (1 / nrzns) (the same float as the division: that
-                // product is within 2^-52 of count / nrzns, which for count < 2^16 lies
-                // at least 2^-41 away from any rounding midpoint of float); the window
-                // was kept compressed only if every entry decodes to its stored bits
-                // entries w0 + 64 u + lane (one 16-byte load per lane instead measured
-                // 301 against 296 us per sweep on the 100x100x120 model)
+                // 4-byte entries w0 + 64 u + lane: the probability count / nrzns is
+                // rounded to float from the double count * (1 / nrzns) (the same float as
+                // the division: that product is within 2^-52 of count / nrzns, which for
+                // count < 2^16 lies at least 2^-41 away from any rounding midpoint of
+                // float); the window was kept compressed only if every entry decodes to
+                // its stored bits
                 uint32_t e[U];
+                if (has_next) {
 #pragma unroll
-                for (int u = 0; u < U; u++) e[u] = A.cz[w0 + (uint32_t)(u * 64 + lane)];
+                    for (int u = 0; u < U; u++) e[u] = ecur[u];
+                } else {
+                    load_comp(w0, e);
+                }
 #pragma unroll
                 for (int u = 0; u < U; u++) x[u] = ld_off(Vin, (uint32_t)base + (e[u] & 0xffffu));
+                has_next = next_comp;
+                if (next_comp) load_comp(wend, ecur);
                 if (A.ptab_n) {
 #pragma unroll
                     for (int u = 0; u < U; u++) pv[u] = (double)ptabf[e[u] >> 16];
@@ -561,6 +579,8 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
                 for (int u = 0; u < U; u++) e[u] = nzp[kw * WIN + u * 64];
 #pragma unroll
                 for (int u = 0; u < U; u++) x[u] = ld_off(Vin, (uint32_t)e[u].x);
+                has_next = next_comp;
+                if (next_comp) load_comp(wend, ecur);
 #pragma unroll
                 for (int u = 0; u < U; u++) pv[u] = (double)__int_as_float(e[u].y);
 #pragma unroll
