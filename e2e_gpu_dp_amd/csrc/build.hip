@@ -1567,7 +1567,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
         E2E_CHECK(nw > 0 && opts.count_kernel == 0, E2E_ERR_CAPACITY,
                   "alpha columns need the per-row kernel (count_kernel 0, velocity samples within its LDS)");
         E2E_CHECK(nr <= 127 * 64, E2E_ERR_CAPACITY, "alpha columns count at most 8128 realizations per row");
-        if (hipMalloc(&m->Ralpha, sizeof(float) * (size_t)opts.n_alpha * rows) != hipSuccess) {
+        if (pool_malloc(&m->Ralpha, sizeof(float) * (size_t)opts.n_alpha * rows) != hipSuccess) {
             m->Ralpha = nullptr;
             set_error("hipMalloc of the alpha rewards failed");
             return fail(E2E_ERR_NOMEM);
@@ -1812,8 +1812,8 @@ e2e_status model_import(int64_t ns, int64_t na, int64_t nnz, const int32_t* S1, 
         (st = dmalloc(dTmp, sizeof(unsigned long long) * scan_tmp_words(rows))) ||
         (st = dmalloc(dTot, sizeof(unsigned long long))))
         return fail(st);
-    if (hipMalloc(&m->row_ptr, sizeof(uint32_t) * (rows + 1)) != hipSuccess ||
-        hipMalloc(&m->R, sizeof(float) * rows) != hipSuccess) {
+    if (pool_malloc(&m->row_ptr, sizeof(uint32_t) * (rows + 1)) != hipSuccess ||
+        pool_malloc(&m->R, sizeof(float) * rows) != hipSuccess) {
         set_error("import: device allocation failed");
         return fail(E2E_ERR_NOMEM);
     }

@@ -70,7 +70,20 @@ Pool& pool() {
 hipError_t pool_malloc(void** p, size_t bytes) {
     *p = nullptr;
     Pool& P = pool();
-    if (bytes < kMinPooled || P.limit() == 0) return hipMalloc(p, bytes);
+    if (bytes < kMinPooled || P.limit() == 0) {  // not pooled; on failure the idle blocks go back first
+        hipError_t e = hipMalloc(p, bytes);
+        if (e != hipSuccess && P.limit() != 0) {
+            (void)hipGetLastError();
+            int dev = 0;
+            if (hipGetDevice(&dev) == hipSuccess) {
+                std::lock_guard<std::mutex> lk(P.mu);
+                P.trim(dev, 0);
+            }
+            e = hipMalloc(p, bytes);
+        }
+        if (e != hipSuccess) (void)hipGetLastError();
+        return e;
+    }
     const size_t want = (bytes + kGranule - 1) / kGranule * kGranule;
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
@@ -102,7 +115,10 @@ hipError_t pool_malloc(void** p, size_t bytes) {
             P.trim(dev, 0);
         }
         e = hipMalloc(p, want);
-        if (e != hipSuccess) return e;
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return e;
+        }
     }
     std::lock_guard<std::mutex> lk(P.mu);
     P.out[*p] = Block{*p, want, dev};

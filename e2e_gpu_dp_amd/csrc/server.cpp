@@ -407,7 +407,8 @@ e2e_status e2e_server_create(const e2e_server_opts* opts, void** out) {
             set_error("hipMemGetInfo failed (no GPU?)");
             return E2E_ERR_HIP;
         }
-        S->budget = (int64_t)(0.8 * (double)fr);
+        // idle blocks the library keeps mapped (pool) are free memory for the server too
+        S->budget = (int64_t)(0.8 * ((double)fr + (double)pool_cached_bytes()));
     }
     S->sched = std::thread([S] { S->sched_loop(); });
     for (int i = 0; i < kBuilders; i++) S->builders.emplace_back([S] { S->builder_loop(); });

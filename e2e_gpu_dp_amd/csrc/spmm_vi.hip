@@ -1192,7 +1192,7 @@ struct BatchBufs {
     void release() {
         void* ps[] = {V[0], V[1], pi, goal, conv, start, part, last_res, st, vout, pout, colmax, wpart, wpre};
         for (void* p : ps)
-            if (p) (void)hipFree(p);
+            if (p) pool_free(p);
         *this = BatchBufs();
     }
 };
@@ -1220,14 +1220,13 @@ void batch_cache_free(BatchCache* c) {
 
 static e2e_status alloc_bufs(const Model* m, BatchBufs& b, int kp) {
     const size_t n = (size_t)m->ns * kp;
-    if (hipMalloc(&b.V[0], n * 8) != hipSuccess || hipMalloc(&b.V[1], n * 8) != hipSuccess ||
-        hipMalloc(&b.pi, n * 4) != hipSuccess || hipMalloc(&b.goal, kp * 4) != hipSuccess ||
-        hipMalloc(&b.conv, kp * 4) != hipSuccess || hipMalloc(&b.start, kp * 4) != hipSuccess ||
-        hipMalloc(&b.part, (size_t)kPartRows * kp * 4) != hipSuccess ||
-        hipMalloc(&b.last_res, kp * 4) != hipSuccess || hipMalloc(&b.st, sizeof(BatchState)) != hipSuccess ||
-        hipMalloc(&b.vout, n * 4) != hipSuccess || hipMalloc(&b.pout, n * 4) != hipSuccess ||
-        hipMalloc(&b.colmax, kp * 4) != hipSuccess) {
-        (void)hipGetLastError();
+    if (pool_malloc(&b.V[0], n * 8) != hipSuccess || pool_malloc(&b.V[1], n * 8) != hipSuccess ||
+        pool_malloc(&b.pi, n * 4) != hipSuccess || pool_malloc(&b.goal, kp * 4) != hipSuccess ||
+        pool_malloc(&b.conv, kp * 4) != hipSuccess || pool_malloc(&b.start, kp * 4) != hipSuccess ||
+        pool_malloc(&b.part, (size_t)kPartRows * kp * 4) != hipSuccess ||
+        pool_malloc(&b.last_res, kp * 4) != hipSuccess || pool_malloc(&b.st, sizeof(BatchState)) != hipSuccess ||
+        pool_malloc(&b.vout, n * 4) != hipSuccess || pool_malloc(&b.pout, n * 4) != hipSuccess ||
+        pool_malloc(&b.colmax, kp * 4) != hipSuccess) {
         b.release();
         set_error("batched solve: device allocation failed");
         return E2E_ERR_NOMEM;
@@ -1374,12 +1373,11 @@ static e2e_status plan_of(const Model* m, BatchBufs& b, const Variant& v, hipStr
     nblocks = std::max<int64_t>(1, std::min<int64_t>(nblocks, ceil_div(m->ns, (int64_t)kSWaves * spc)));
     const int64_t W = nblocks * kSWaves;
     if (b.wpart_W != W || b.wpart_spc != spc) {
-        if (b.wpart) (void)hipFree(b.wpart);
+        if (b.wpart) pool_free(b.wpart);
         b.wpart = nullptr;
         b.wpart_W = 0;
-        if (hipMalloc(&b.wpart, sizeof(uint32_t) * (W + 1)) != hipSuccess) {
+        if (pool_malloc(&b.wpart, sizeof(uint32_t) * (W + 1)) != hipSuccess) {
             b.wpart = nullptr;
-            (void)hipGetLastError();
             set_error("batched sweep: device allocation failed");
             return E2E_ERR_NOMEM;
         }
@@ -1459,12 +1457,11 @@ static bool active_ok(const Model* m) {
 // (b.wpre, once per model and grid), in place of the full partition.
 static e2e_status use_prefix_parts(const Model* m, BatchBufs& b, BatchArgs& A, hipStream_t s) {
     if (b.wpre_W != b.wpart_W || b.wpre_spc != b.wpart_spc) {
-        if (b.wpre) (void)hipFree(b.wpre);
+        if (b.wpre) pool_free(b.wpre);
         b.wpre = nullptr;
         b.wpre_W = 0;
-        if (hipMalloc(&b.wpre, sizeof(uint32_t) * (b.wpart_W + 1) * m->nt) != hipSuccess) {
+        if (pool_malloc(&b.wpre, sizeof(uint32_t) * (b.wpart_W + 1) * m->nt) != hipSuccess) {
             b.wpre = nullptr;
-            (void)hipGetLastError();
             set_error("batched sweep: device allocation failed");
             return E2E_ERR_NOMEM;
         }
@@ -1861,7 +1858,7 @@ e2e_status live_create(Model* m, int kp, float thresh, hipStream_t s, LiveBatch*
     L->m = m;
     L->s = s;
     e2e_status st = alloc_bufs(m, L->b, kp);
-    if (st != E2E_OK || hipMalloc(&L->d_list, 2 * kp * sizeof(int32_t)) != hipSuccess) {
+    if (st != E2E_OK || pool_malloc(&L->d_list, 2 * kp * sizeof(int32_t)) != hipSuccess) {
         L->b.release();
         delete L;
         if (st == E2E_OK) set_error("live batch: allocation failed");
@@ -1900,7 +1897,7 @@ e2e_status live_create(Model* m, int kp, float thresh, hipStream_t s, LiveBatch*
 void live_free(LiveBatch* L) {
     if (!L) return;
     if (L->s) (void)hipStreamSynchronize(L->s);
-    if (L->d_list) (void)hipFree(L->d_list);
+    if (L->d_list) pool_free(L->d_list);
     L->b.release();
     delete L;
 }
