@@ -14,7 +14,8 @@ Also reported on the same line:
   * "batched": 64 goal columns on the 25x25 x nt=20, 8-heading model, solved as one
     SpMM-VI (autotuned, hipGraph sweeps) vs 64 sequential plain SpMV-VI solves of the
     per-goal models, with the largest column difference (bar: 1e-5 relative), and the
-    same columns by one exact backward pass (goal variant -3);
+    same columns by one exact backward pass (goal variant -3); and 16 goal columns on the
+    100x100 x nt=120 model: the batched sweep's cost in plain sweeps and its roofline;
   * "serving": a Poisson(20/s) stream of planning requests over environments with
     grids {50^2, 100^2, 150^2}, nt in U[60,150], 8 or 16 headings, 1000-5000
     realizations: p50/p99 latency and solves/s of the continuous-batching server vs a
@@ -195,6 +196,36 @@ def _batched(e2e, np, steps: int, cpu: bool = True):
         "backward_speedup_vs_plain": t_plain / t_back, "backward_max_rel_diff_vs_plain": err_x,
         "backward_launches": st_x["launches"], "cpu_baseline": cpu_leg,
     }
+
+
+def _batched_c2(e2e, np, torch, k: int = 16):
+    """k goal columns on the shared 100x100 x nt=120 model (no terminal cell, reward sums
+    kept): the batched sweep over every state against the plain sweep over every state,
+    and its algorithmic bytes (entries and row data once, k value columns read and
+    written, k policies) over HBM peak."""
+    from e2e_gpu_dp_amd.configs import goal_cells, workload
+    w = workload("C2")
+    p = w.params.replace(term_subgrid_size=0)
+    f = w.fields()
+    dev = {kk: torch.from_numpy(np.ascontiguousarray(v)).cuda() for kk, v in f.items()}
+    goals = goal_cells(p.gsize, k, 3456)
+    with e2e.build_model(p, dev, keep_rsum=True) as m:
+        _, _, sp = m.solve(thresh=1e-6, full_sweeps=True)
+        m.solve_goals(goals, thresh=1e-6, full_sweeps=True)  # warm (kernel choice, buffers)
+        _, _, sw, st = m.solve_goals(goals, thresh=1e-6, full_sweeps=True)
+        nnz, rows, ns = m.info["nnz"], p.ns * p.na, p.ns
+    del dev
+    per = st["sweep_kernel_ms"] / max(st["launches"], 1)
+    plain = sp["sweep_kernel_ms_per_sweep"]
+    bytes_ = 8.0 * nnz + 8.0 * rows + 4.0 + 20.0 * k * ns
+    gbps = bytes_ / (per * 1e-3) / 1e9
+    return {"workload": f"C2 shared model (100x100 x nt=120, 16 headings, 2500 realizations), {k} goal columns, "
+                        f"sweeps over every state", "columns": k, "variant": st["variant"],
+            "sweep_kernel_ms": per, "plain_sweep_kernel_ms": plain, "plain_sweeps_per_batched_sweep": per / plain,
+            "per_column_vs_plain": per / plain / k,
+            "roofline": {"kernel": "spmm_vi (batched sweep)", "bound": "hbm", "achieved": gbps,
+                         "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": gbps / PEAK_HBM_GBPS,
+                         "bytes_per_sweep": bytes_}}
 
 
 def _serving(e2e, np, n_requests: int, seed: int = 5678, rate: float = 20.0, only=None):
@@ -727,6 +758,8 @@ def main(argv=None):
     if not args.no_batched:
         print("[bench] batched goals", file=sys.stderr, flush=True)
         out["batched"] = _batched(e2e, np, max(args.steps, 3), cpu=not args.no_cpu)
+        if args.workload == "C2":
+            out["batched"]["c2_16_goals"] = _batched_c2(e2e, np, torch)
     if not args.no_serving:
         print("[bench] serving", file=sys.stderr, flush=True)
         out["serving"] = _serving(e2e, np, args.serving_requests)

@@ -761,6 +761,369 @@ __global__ __launch_bounds__(kHThreads) void spmm_vi_halo(const BatchArgs A) {
     sweep_tail<KB, kHThreads>(A, red, fin, is_last, k);
 }
 
+// Tile sweep: the halo kernel's tiles and LDS halo, with one lane per row over all KB
+// columns of the block and a row group's entries staged whole.
+//   * The halo record of a cell holds its KB values of slice t+1 and, after them, the
+//     mask of the block's columns whose goal is that cell (a stride of KB + 2 doubles, so
+//     that lanes reading the same column pair of different cells fall on different banks).
+//     Cells outside the grid hold zeros.
+//   * A wave takes one tile row, in groups of 64 / NAP states: lane (state, action) walks
+//     its row over every entry of the group's window (windows are cut at group bounds,
+//     WIN = 64 * U entries; the next window's entries are loaded while this one is
+//     walked). Staging decodes each entry to the LDS offset of its landing cell's record;
+//     an entry landing outside the halo gets one of the wave's kTOvf overflow records,
+//     filled from global memory as the window is staged; a window whose entries need more
+//     ends at the first entry without one. The walk reads LDS only.
+//   * The walk reads U2 entries per step, all their records together, and the next
+//     step's entries ahead. A step slot past the row's end adds 0 * (a finite value),
+//     which leaves the sum's bits unchanged (a sum that starts at +0 is never -0).
+//   * Each row's Q (KB values) goes to LDS; lane (state, column) then takes the max and
+//     first argmax over the state's actions in action order and writes V, pi and the
+//     residual of one column, so the stores of a state's KB columns are contiguous.
+// The per-row, per-column arithmetic is the other sweeps': products added in ascending
+// entry order, the same Q, argmax and residual rounding, so the bits are theirs.
+constexpr int kTOvf = 8;  // overflow records per wave
+
+template <int KB, int U, int U2, int H, int QH>
+__global__ __launch_bounds__(kHThreads) void spmm_vi_tile(const BatchArgs A) {
+    constexpr int HH = kHTileH + 2 * H, HW = kHTileW + 2 * H;
+    constexpr int CS = KB + 2;  // doubles per record: KB values, goal mask, pad
+    constexpr int NREC = HH * HW;
+    constexpr int WIN = 64 * U;
+    constexpr int KQ = KB / QH;                          // columns per Q pass
+    constexpr int WB = WIN > 64 * KQ ? WIN : 64 * KQ;    // doubles per wave: entries, then Q
+    constexpr int QM = KQ / 2 - 1;                       // Q slots (double2) per row - 1
+    static_assert(KB % 2 == 0 && 64 % KB == 0 && KB <= 32 && (KB & (KB - 1)) == 0,
+                  "columns are staged in pairs, one column per lane");
+    static_assert(QH >= 1 && KB % QH == 0 && KQ % 2 == 0, "Q passes of whole column pairs");
+    static_assert(kHTileH == kHWaves, "one tile row per wave");
+    __shared__ __align__(16) double halo[(NREC + kHWaves * kTOvf) * CS];
+    __shared__ __align__(16) double wbuf[kHWaves][WB];
+    __shared__ float red[kHWaves][KB];
+    __shared__ float fin[kHThreads];
+    __shared__ int is_last;
+    BatchState* st = A.st;
+    if (!A.one_pass && st->all_conv) return;
+    const uint32_t k = A.one_pass ? 1u : st->k;
+    const double* __restrict__ Vin = (k & 1u) ? A.V0 : A.V1;
+    double* __restrict__ Vout = (k & 1u) ? A.V1 : A.V0;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nap = A.nap;
+    const int a = lane & (nap - 1);
+    const int sl = lane / nap;
+    const int spg = 64 / nap;  // states per group
+    const int jb0 = blockIdx.y * KB;
+    const int KP = A.kp;
+    const int g = A.g, nc = A.nc, na = A.na;
+    const bool goals = A.Rcol == nullptr;
+    // the columns of this lane in the argmax phase: cq + h * KQ in pass h
+    const int cq = lane % KQ;
+    int goal_q[QH];
+    bool live_q[QH];
+    float rmax[QH];
+#pragma unroll
+    for (int h = 0; h < QH; h++) {
+        goal_q[h] = A.goal[jb0 + h * KQ + cq];
+        live_q[h] = A.conv[jb0 + h * KQ + cq] < 0;
+        rmax[h] = 0.0f;
+    }
+    int2* const wb = reinterpret_cast<int2*>(wbuf[wv]);
+    double2* const qb = reinterpret_cast<double2*>(wbuf[wv]);
+    const int ovf0 = (NREC + wv * kTOvf) * CS;  // this wave's first overflow record
+    const uint64_t below = (1ull << lane) - 1ull;
+    // blocks in XCD-major order (consecutive blocks run on different XCDs): each XCD walks
+    // a contiguous band of tiles, so the halos it stages overlap in its own L2
+    const unsigned nb = gridDim.x;
+    const unsigned bx = (nb % 8u == 0u) ? (blockIdx.x % 8u) * (nb / 8u) + blockIdx.x / 8u : blockIdx.x;
+    const int tr = (g + kHTileH - 1) / kHTileH, tcn = (g + kHTileW - 1) / kHTileW, tps = tr * tcn;
+    const int t_lo = A.one_pass ? (int)(A.s_lo / (uint32_t)nc) : 0;
+    const int t_hi = A.one_pass ? t_lo + 1 : A.pre_nt ? sweep_active(A, k, jb0, KB) : A.nt;
+    const int ntiles = (t_hi - t_lo) * tps;
+    for (int tile = (int)bx; tile < ntiles; tile += (int)nb) {
+        const int t = t_lo + tile / tps;
+        const int rem = tile - (t - t_lo) * tps;
+        const int ti = rem / tcn, tj = rem - ti * tcn;
+        const int i0 = ti * kHTileH, jc0 = tj * kHTileW;
+        const int th = min(kHTileH, g - i0), tw = min(kHTileW, g - jc0);
+        const int hi0 = i0 - H, hj0 = jc0 - H;
+        const int colbase = (t + 1) * nc;
+        // this wave's run: tile row wv, cells jc0 .. jc0+tw-1 (none when wv >= th)
+        const bool act = wv < th;
+        const int cell0 = (i0 + min(wv, th - 1)) * g + jc0;
+        const int64_t s_run = (int64_t)t * nc + cell0;
+        const int ngroups = act ? (tw + spg - 1) / spg : 0;
+        // first entry of group gi (gi = ngroups: the run's end); lane i of gsv holds that
+        // of group i, for the groups after the first three
+        auto gstart = [&](int gi) { return A.rp[(s_run + min(gi * spg, tw)) * na]; };
+        const uint32_t gsv = (act && lane <= ngroups) ? A.rp[(s_run + min(lane * spg, tw)) * na] : 0u;
+        struct Meta {
+            uint32_t rb, re;
+            float rr, rs;
+        };
+        // the row data of this lane's row of group gi
+        auto load_meta = [&](int gi) {
+            Meta M;
+            const int q = gi * spg + sl;
+            const bool valid = q < tw && a < na;
+            const int64_t row = (s_run + min(q, tw - 1)) * na + min(a, na - 1);
+            M.rb = valid ? A.rp[row] : 0u;
+            M.re = valid ? A.rp[row + 1] : 0u;
+            M.rr = A.R[row];
+            M.rs = (goals && A.Rsum) ? A.Rsum[row] : 0.0f;
+            return M;
+        };
+        // group 0's row data and the run's first window, requested before the halo is staged
+        Meta cur{0u, 0u, 0.0f, 0.0f};
+        uint32_t G1 = 0u, G2 = 0u;   // ends of the current and the next group
+        uint32_t nws = 0u, nwe = 0u;  // the window held in e[]
+        int2 e[U];
+        if (act) {
+            cur = load_meta(0);
+            nws = gstart(0);
+            G1 = gstart(1);
+            G2 = ngroups >= 2 ? gstart(2) : G1;
+            nwe = min(nws + (uint32_t)WIN, G1);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint32_t idx = nws + (uint32_t)(u * 64 + lane);
+                e[u] = idx < nwe ? A.nz[idx] : make_int2(0, 0);
+            }
+        }
+        __syncthreads();  // the previous tile's reads of the halo are done
+        if (t + 1 < A.nt) {
+            for (int q = tid; q < NREC * (KB / 2); q += kHThreads) {
+                const int c2 = q % (KB / 2), hc = q / (KB / 2);
+                const int hi = hc / HW, hj = hc - hi * HW;
+                const int i = hi0 + hi, j = hj0 + hj;
+                double2 v = make_double2(0.0, 0.0);
+                if (i >= 0 && i < g && j >= 0 && j < g)
+                    v = *reinterpret_cast<const double2*>(Vin + (int64_t)(colbase + i * g + j) * KP + jb0 + 2 * c2);
+                *reinterpret_cast<double2*>(halo + hc * CS + 2 * c2) = v;
+            }
+            if (goals) {
+                for (int hc = tid; hc < NREC; hc += kHThreads) {
+                    const int hi = hc / HW, hj = hc - hi * HW;
+                    const int cell = (hi0 + hi) * g + hj0 + hj;
+                    uint32_t mask = 0u;
+#pragma unroll
+                    for (int c = 0; c < KB; c++) mask |= (A.goal[jb0 + c] == cell ? 1u : 0u) << c;
+                    reinterpret_cast<uint32_t*>(halo + hc * CS + KB)[0] = mask;
+                }
+            }
+        }
+        __syncthreads();
+        if (!act) continue;  // block-uniform per wave; the next tile's barrier waits for all
+        bool goal_tile = false;
+        if (goals) {
+            for (int c = 0; c < KB; c++) {
+                const int gc = A.goal[jb0 + c];
+                if (gc >= 0) {
+                    const int gi2 = gc / g, gj2 = gc - gi2 * g;
+                    goal_tile = goal_tile || ((unsigned)(gi2 - hi0) < (unsigned)HH && (unsigned)(gj2 - hj0) < (unsigned)HW);
+                }
+            }
+        }
+        const float delta = (float)((double)A.r_term - (t > A.nt - 20 ? A.late_pen : 0.0));
+        auto decode = [&](int2 e) {
+            const int cc = e.x - colbase;
+            const int i2 = (int)__umulhi((uint32_t)cc, A.gmagic);
+            const int hi = i2 - hi0, hj = cc - i2 * g - hj0;
+            const bool in = (unsigned)hi < (unsigned)HH && (unsigned)hj < (unsigned)HW;
+            return make_int2(in ? (hi * HW + hj) * CS : (int)(0x80000000u | (uint32_t)e.x), e.y);
+        };
+        for (int gi = 0; gi < ngroups; gi++) {
+            // requested now, used after this group's walk: the next group's row data, the
+            // end of the group after it, and the old values of this group's first (state,
+            // column) pass
+            const Meta nxt = gi + 1 < ngroups ? load_meta(gi + 1) : cur;
+            const uint32_t G3 = gi + 3 <= ngroups ? (uint32_t)__builtin_amdgcn_readlane((int)gsv, gi + 3) : G2;
+            const int qv = gi * spg + lane / KQ;
+            const double vold = (live_q[0] && lane / KQ < spg && qv < tw) ? Vin[(s_run + qv) * KP + jb0 + cq] : 0.0;
+            const int q = gi * spg + sl;
+            const bool valid = q < tw && a < na;
+            const int64_t row = (s_run + min(q, tw - 1)) * na + min(a, na - 1);
+            double sd[KB];
+            int hit[KB];
+#pragma unroll
+            for (int c = 0; c < KB; c++) {
+                sd[c] = 0.0;
+                hit[c] = -1;
+            }
+            for (;;) {
+                const uint32_t w0 = nws, wend = nwe;
+                // stage: decode; an entry landing outside the halo takes the next overflow
+                // record; when none is left the window ends at that entry (the first entry
+                // outside the halo always has one, so every window holds an entry)
+                int novf = 0;           // wave-uniform
+                uint32_t wcut = wend;   // wave-uniform: the window's end after staging
+                bool ovf_goal = false;  // an overflow record holds a goal of the block's columns
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (w0 + (uint32_t)(u * 64) >= wcut) break;  // wave-uniform: no entry left
+                    const uint32_t idx = w0 + (uint32_t)(u * 64 + lane);
+                    int2 d = decode(e[u]);
+                    const bool far = idx < wcut && d.x < 0;
+                    const uint64_t m = __ballot(far);
+                    if (m) {
+                        const int slot = novf + (int)__popcll(m & below);
+                        const uint64_t mo = __ballot(far && slot >= kTOvf);
+                        if (mo) wcut = w0 + (uint32_t)(u * 64 + __builtin_ctzll(mo));
+                        if (far && slot < kTOvf) {
+                            const int col = d.x & 0x7fffffff;
+                            const int r0 = ovf0 + slot * CS;
+                            const double2* xp = reinterpret_cast<const double2*>(Vin + (int64_t)col * KP + jb0);
+#pragma unroll
+                            for (int c2 = 0; c2 < KB / 2; c2++)
+                                *reinterpret_cast<double2*>(halo + r0 + 2 * c2) = xp[c2];
+                            uint32_t mask = 0u;
+                            if (goals) {
+#pragma unroll
+                                for (int c = 0; c < KB; c++) mask |= (A.goal[jb0 + c] == col - colbase ? 1u : 0u) << c;
+                            }
+                            reinterpret_cast<uint32_t*>(halo + r0 + KB)[0] = mask;
+                            ovf_goal = ovf_goal || mask != 0u;
+                            d.x = r0;
+                        }
+                        novf += (int)__popcll(m);
+                    }
+                    if (idx < wcut) wb[u * 64 + lane] = d;
+                }
+                const bool chk = goal_tile || __ballot(ovf_goal) != 0ull;
+                // the next window: the rest of this group, else the next group's first
+                const bool more = wcut < G1;
+                nws = wcut;
+                nwe = more ? min(wcut + (uint32_t)WIN, G1) : min(wcut + (uint32_t)WIN, G2);
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const uint32_t idx = nws + (uint32_t)(u * 64 + lane);
+                    if (idx < nwe) e[u] = A.nz[idx];
+                }
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t jlo = valid ? max(cur.rb, w0) : 0u, jhi = valid ? min(cur.re, wcut) : 0u;
+                if (jlo < jhi) {
+                    int2 en[U2];
+#pragma unroll
+                    for (int u = 0; u < U2; u++) en[u] = wb[min(jlo + (uint32_t)u, jhi - 1u) - w0];
+                    for (uint32_t j = jlo; j < jhi; j += U2) {
+                        double2 hv[U2][KB / 2];
+#pragma unroll
+                        for (int u = 0; u < U2; u++) {
+                            const double2* hp = reinterpret_cast<const double2*>(halo + en[u].x);
+#pragma unroll
+                            for (int c2 = 0; c2 < KB / 2; c2++) hv[u][c2] = hp[c2];
+                        }
+                        uint32_t mk[U2];
+#pragma unroll
+                        for (int u = 0; u < U2; u++)
+                            mk[u] = chk ? reinterpret_cast<const uint32_t*>(halo + en[u].x + KB)[0] : 0u;
+                        float vf[U2];
+                        double pv[U2];
+#pragma unroll
+                        for (int u = 0; u < U2; u++) {
+                            const bool inr = j + (uint32_t)u < jhi;
+                            vf[u] = __int_as_float(en[u].y);
+                            pv[u] = inr ? (double)vf[u] : 0.0;
+                            if (!inr) mk[u] = 0u;
+                        }
+#pragma unroll
+                        for (int u = 0; u < U2; u++) en[u] = wb[min(j + (uint32_t)(U2 + u), jhi - 1u) - w0];
+#pragma unroll
+                        for (int u = 0; u < U2; u++) {
+#pragma unroll
+                            for (int c2 = 0; c2 < KB / 2; c2++) {
+                                sd[2 * c2] = sd[2 * c2] + pv[u] * hv[u][c2].x;
+                                sd[2 * c2 + 1] = sd[2 * c2 + 1] + pv[u] * hv[u][c2].y;
+                            }
+                            if (mk[u]) {
+                                const int n = (int)rintf(vf[u] * (float)A.nr);
+#pragma unroll
+                                for (int c = 0; c < KB; c++)
+                                    if ((mk[u] >> c) & 1u) hit[c] = n;
+                            }
+                        }
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (!more) break;
+            }
+            // the rows' Q to LDS (over the entries, which are no longer read), KQ columns
+            // per pass: row r's column pair c2 at slot r * KQ/2 + (c2 ^ (r & QM)), so that
+            // the stores of one pair by consecutive rows, and the loads of one row by
+            // consecutive columns, fall on different banks. Then lane (state, column): max
+            // / first argmax over the actions in action order (strict '>', spvi_2.cu:112)
+            const double* const qd = reinterpret_cast<const double*>(qb);
+#pragma unroll
+            for (int h = 0; h < QH; h++) {
+                if (valid) {
+#pragma unroll
+                    for (int c2 = 0; c2 < KQ / 2; c2++) {
+                        double qq[2];
+#pragma unroll
+                        for (int e2 = 0; e2 < 2; e2++) {
+                            const int c = h * KQ + 2 * c2 + e2;
+                            const float rj = !goals ? A.Rcol[(int64_t)min(jb0 + c, A.rcol_n - 1) * A.rows + row]
+                                             : hit[c] >= 0 ? (cur.rs + (float)hit[c] * delta) / A.nr
+                                                           : cur.rr;
+                            double x = (double)rj + (double)A.discount * sd[c];
+                            if (x != x) x = -INFINITY;
+                            qq[e2] = x;
+                        }
+                        qb[lane * (KQ / 2) + (c2 ^ (lane & QM))] = make_double2(qq[0], qq[1]);
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                for (int p = lane; p < spg * KQ; p += 64) {
+                    const int s2 = p / KQ;
+                    const int q2 = gi * spg + s2;
+                    if (q2 >= tw) continue;
+                    double best = -INFINITY;
+                    int ba = 0x7fffffff;
+                    for (int a2 = 0; a2 < na; a2++) {
+                        const int r = s2 * nap + a2;
+                        const double x = qd[2 * (r * (KQ / 2) + ((cq >> 1) ^ (r & QM))) + (cq & 1)];
+                        if (x > best) {
+                            best = x;
+                            ba = a2;
+                        }
+                    }
+                    const int cell = cell0 + q2;
+                    const int64_t o = (s_run + q2) * KP + jb0 + h * KQ + cq;
+                    double vo;
+                    int po;
+                    if (cell == goal_q[h]) {
+                        vo = 0.0;
+                        po = 0;
+                    } else if (best > -1e6) {
+                        vo = best;
+                        po = ba;
+                    } else {
+                        vo = -1e6;
+                        po = -1;
+                    }
+                    if (live_q[h]) {
+                        const double vprev = (h == 0 && p == lane) ? vold : Vin[o];
+                        Vout[o] = vo;
+                        A.pi[o] = po;
+                        rmax[h] = fmaxf(rmax[h], res_up(fabs(vo - vprev)));
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+            cur = nxt;
+            G1 = G2;
+            G2 = G3;
+        }
+    }
+    if (A.one_pass) return;  // block-uniform
+#pragma unroll
+    for (int h = 0; h < QH; h++) {
+        for (int off = KQ; off < 64; off <<= 1) rmax[h] = fmaxf(rmax[h], __shfl_xor(rmax[h], off, 64));
+        if (lane < KQ) red[wv][h * KQ + lane] = rmax[h];
+    }
+    sweep_tail<KB, kHThreads>(A, red, fin, is_last, k);
+}
+
 // Streaming sweep with transposed gathers. The state partition and the entry stream
 // are those of the plain lean sweep: each wave takes a contiguous state range (wpart,
 // balanced by entries + rows), lanes (a, sl) = (action, state) of groups of SPC = 64/NAP
@@ -1102,13 +1465,14 @@ __global__ void batch_gather(const double* V0, const double* V1, const int32_t* 
 
 // kind 0: spmm_vi_sweep (lc = 64/spw lanes per state); kind 1: spmm_vi_rows (lc lanes
 // per row); kind 2: spmm_vi_stream (cpl = KB columns per block, u = entries per lane
-// and window); kind 3: spmm_vi_halo (lc lanes per row, built models only). Kinds 1-3
-// take a number of states per wave that depends on na.
+// and window); kind 3: spmm_vi_halo (lc lanes per row, built models only); kind 4:
+// spmm_vi_tile (cpl = KB columns per row lane, u = window entries per lane, built models
+// only). Kinds 1-4 take a number of states per wave that depends on na.
 struct Variant {
     int cpl, lc;
     int kind;
     int u;
-    BatchFn fn;  // kinds 0 and 1
+    BatchFn fn;  // kinds 0, 1, 3 and 4
 };
 
 const Variant kVariants[] = {
@@ -1129,6 +1493,9 @@ const Variant kVariants[] = {
     {4, 4, 3, 0, spmm_vi_halo<4, 4, 4>},  {4, 2, 3, 0, spmm_vi_halo<2, 4, 4>},
     {2, 4, 3, 0, spmm_vi_halo<4, 2, 4>},  {4, 1, 3, 0, spmm_vi_halo<1, 4, 4>},
     {8, 2, 3, 0, spmm_vi_halo<2, 8, 4>},  {2, 2, 3, 0, spmm_vi_halo<2, 2, 4>},
+    {16, 1, 4, 8, spmm_vi_tile<16, 8, 2, 4, 1>}, {16, 1, 4, 8, spmm_vi_tile<16, 8, 1, 5, 2>},
+    {8, 1, 4, 8, spmm_vi_tile<8, 8, 2, 5, 1>},   {16, 1, 4, 16, spmm_vi_tile<16, 16, 1, 4, 1>},
+    {16, 1, 4, 8, spmm_vi_tile<16, 8, 1, 4, 1>},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -1143,7 +1510,7 @@ int nap_of(int na) {
 // states per wavefront; 0 when the variant cannot take this many actions
 int spw_of(const Variant& v, int na) {
     if (v.kind == 0) return 64 / v.lc;
-    if (v.kind == 2) return 64 / nap_of(na);
+    if (v.kind == 2 || v.kind == 4) return 64 / nap_of(na);
     const int lanes = v.lc * nap_of(na);
     return lanes <= 64 ? 64 / lanes : 0;
 }
@@ -1158,7 +1525,7 @@ bool halo_model(const Model* m) {
 }
 
 bool variant_ok(const Model* m, const Variant& v, int kp) {
-    return variant_fits(v, (int)m->na, kp) && (v.kind != 3 || halo_model(m));
+    return variant_fits(v, (int)m->na, kp) && ((v.kind != 3 && v.kind != 4) || halo_model(m));
 }
 
 // padded column count: 4 or 8 for small batches (the value block's stride per state is
@@ -1295,15 +1662,15 @@ static dim3 grid_of(const Model* m, const Variant& v, int kp, int ncols) {
 // Kernel by the width a sweep has to cover (live batches: highest live slot + 1; large
 // batched solves: the column count); all kernels share the layout and the sweep state, so
 // the choice may change from sweep to sweep. Measured on a 150x150 x 125 x 8-heading model
-// (1.8e8 entries; plain sweep 0.37 ms), ms per sweep over every state:
+// (1.8e8 entries; plain sweep 0.39 ms), ms per sweep over every state:
 //   width   1-2: spmm_vi_stream<8, 2, 4> 0.95;
 //   width     4: spmm_vi_halo<1, 4> 1.38 (spmm_vi_rows<1, 4, 4> 1.59);
-//   width     8: spmm_vi_halo<2, 4> 1.97 (every row or state kernel 3.6 or more);
-//   width 16/32: spmm_vi_halo<2, 8> 3.53 / 6.90 (spmm_vi_rows<2, 8, 2> 3.59 / 7.24).
-// The halo kernels need a built model; where one does not fit, the second table's kernel
-// runs, and where that does not fit either, spmm_vi_sweep<1, 4> (fits every model).
+//   width     8: spmm_vi_tile<8, 8, 2, 5, 1> 1.49 (spmm_vi_halo<2, 4> 1.97);
+//   width 16/32: spmm_vi_tile<16, 16, 1, 4, 1> 2.33 / 4.72 (spmm_vi_halo<2, 8> 3.46 / 6.92).
+// The halo and tile kernels need a built model; where one does not fit, the second table's
+// kernel runs, and where that does not fit either, spmm_vi_sweep<1, 4> (fits every model).
 constexpr int kLiveWidths[] = {2, 4, 8, 1 << 30};
-constexpr int kLiveVariants[] = {26, 30, 28, 31};
+constexpr int kLiveVariants[] = {26, 30, 35, 36};
 constexpr int kLiveAlt[] = {26, 17, 20, 20};
 constexpr int kLiveFallback = 6;  // spmm_vi_sweep<1, 4>: fits every model
 constexpr int kLiveN = 4;
@@ -1337,7 +1704,7 @@ struct Plan {
 
 static e2e_status plan_of(const Model* m, BatchBufs& b, const Variant& v, hipStream_t s, Plan* p) {
     p->kb = kb_of(v);
-    if (v.kind == 3) {  // persistent blocks over the tiles of the swept slices
+    if (v.kind == 3 || v.kind == 4) {  // persistent blocks over the tiles of the swept slices
         E2E_CHECK(halo_model(m), E2E_ERR_INVALID,
                   "batched sweep: the halo kernel needs a model built by e2e_model_build (grid side <= 1024)");
         int bpc = 0, ncu = 0, dev = 0;
@@ -1415,7 +1782,7 @@ static e2e_status backward_pass(const Model* m, BatchBufs& b, const BatchArgs& A
     // the width table's kernel where it takes one slice per launch (the row and halo
     // kernels), else the row kernels
     int vi = width_variant(m, kp, width_class(ncols));
-    if (kVariants[vi].kind != 1 && kVariants[vi].kind != 3)
+    if (kVariants[vi].kind != 1 && kVariants[vi].kind != 3 && kVariants[vi].kind != 4)
         vi = (ncols > 4 && variant_fits(kVariants[20], (int)m->na, kp)) ? 20 : 17;
     const Variant& v = kVariants[vi];
     Plan p;
@@ -1423,7 +1790,7 @@ static e2e_status backward_pass(const Model* m, BatchBufs& b, const BatchArgs& A
     const int spw = spw_of(v, (int)m->na);
     const int64_t nc = m->nc, groups = ceil_div(nc, (int64_t)spw);
     const int64_t g = m->p.gsize;
-    const int64_t gx = v.kind == 3 ? std::min<int64_t>(ceil_div(g, (int64_t)kHTileH) * ceil_div(g, (int64_t)kHTileW),
+    const int64_t gx = (v.kind == 3 || v.kind == 4) ? std::min<int64_t>(ceil_div(g, (int64_t)kHTileH) * ceil_div(g, (int64_t)kHTileW),
                                                        (int64_t)p.grid.x)
                                    : std::min<int64_t>(ceil_div(groups, (int64_t)kBWaves), kBMaxBlocks);
     const dim3 grid((unsigned)gx, plan_grid(p, ncols).y);

@@ -186,8 +186,9 @@ typedef struct e2e_solve_stats {
 } e2e_solve_stats;
 
 typedef struct e2e_batch_opts {
-    int32_t variant;           /* SpMM kernel variant (0..32; 27..32 read the next slice's values from an
-                                  LDS halo of a tile of cells, built models only); -1: chosen by the library (by the column
+    int32_t variant;           /* SpMM kernel variant (0..37; 27..37 read the next slice's values from an
+                                  LDS halo of a tile of cells, built models only; 33..37 with one lane per
+                                  row over 8 or 16 columns); -1: chosen by the library (by the column
                                   count on models of 16M entries or more with up to 16 columns,
                                   else timed on this model). -3: every column by one exact backward
                                   pass (kernel_variant 7's schedule), ending at its fixed point. Alpha

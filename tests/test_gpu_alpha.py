@@ -54,13 +54,17 @@ def test_alpha_rewards_match_per_alpha_builds(gpu, name):
         assert np.array_equal(coo[3].view(np.uint32), m.export_R_alpha(0).view(np.uint32))
 
 
-@pytest.mark.parametrize("variant", [-1, -2])  # the batched SpMM; one plain sweep chain per column
+# the batched SpMM (autotuned, and the tile kernel over 16 columns); one plain sweep chain
+# per column
+@pytest.mark.parametrize("variant", [-1, 33, -2])
 def test_alpha_solve_matches_plain_solves(gpu, variant):
     p, f = _case("two_speeds")
     with e2e.build_model(p, f, alphas=ALPHAS) as m:
         pol, val, sw, st = m.solve_alphas(thresh=1e-6, variant=variant)
         assert st["converged"] == len(ALPHAS)
         assert (st["variant"] == -2) == (variant == -2)
+        if variant >= 0:
+            assert st["variant"] == variant
     for j, a in enumerate(ALPHAS):
         with e2e.build_model(p.replace(alpha=float(a)), f) as mj:
             pj, vj, sj = mj.solve(thresh=1e-6)

@@ -94,9 +94,12 @@ def test_batched_columns_independent_of_batch_and_variant(gpu):
 
 # SpMM variants 0-8: one lane group per state walks every row; 9-20: one lane group per
 # row (spmm_vi_rows), as (column lanes, columns per lane); 27-32: the same over a tile's
-# LDS halo of the next slice's values (spmm_vi_halo)
-N_VARIANTS = 33
+# LDS halo of the next slice's values (spmm_vi_halo); 33-37: one lane per row over all
+# the block's columns, with the halo and a row group's entries in LDS (spmm_vi_tile), by
+# column block
+N_VARIANTS = 38
 HALO_VARIANTS = {27: (4, 4), 28: (2, 4), 29: (4, 2), 30: (1, 4), 31: (2, 8), 32: (2, 2)}
+TILE_VARIANTS_KB = {33: 16, 34: 16, 35: 8, 36: 16, 37: 16}
 ROWS_VARIANTS = {9: (4, 1), 10: (8, 1), 11: (4, 2), 12: (8, 2), 13: (4, 4), 14: (16, 1), 15: (8, 4),
                  16: (16, 2), 17: (1, 4), 18: (1, 8), 19: (2, 4), 20: (2, 8)}
 
@@ -135,6 +138,8 @@ def test_batched_row_kernels_bit_identical(gpu, na, k, g, fast):
                 fits = lc * nap <= 64 and kp % (lc * cpl) == 0
             elif variant in STREAM_VARIANTS_KB:
                 fits = kp % STREAM_VARIANTS_KB[variant] == 0
+            elif variant in TILE_VARIANTS_KB:
+                fits = nap <= 64 and kp % TILE_VARIANTS_KB[variant] == 0
             else:
                 fits = kp % STATE_VARIANTS_KB[variant] == 0
             pol, val, sw, st = m.solve_goals(goals, thresh=1e-6, variant=variant)
@@ -148,6 +153,8 @@ def test_batched_row_kernels_bit_identical(gpu, na, k, g, fast):
         assert len(ran & set(ROWS_VARIANTS)) >= 2
         assert len(ran & set(STREAM_VARIANTS_KB)) >= 2
         assert len(ran & set(HALO_VARIANTS)) >= 2
+        if kp >= 8:
+            assert len(ran & set(TILE_VARIANTS_KB)) >= (2 if kp >= 16 else 1)
         if kp >= 16:
             assert ran & set(STATE_VARIANTS_KB)
 
