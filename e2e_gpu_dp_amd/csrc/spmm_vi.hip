@@ -802,6 +802,9 @@ __global__ __launch_bounds__(kHThreads) void spmm_vi_tile(const BatchArgs A) {
     __shared__ float red[kHWaves][KB];
     __shared__ float fin[kHThreads];
     __shared__ int is_last;
+    // the block's goal cells and their grid rows and columns (-1: none), read from LDS so
+    // that they take no scalar registers
+    __shared__ int goal_s[KB], goal_i[KB], goal_j[KB];
     BatchState* st = A.st;
     if (!A.one_pass && st->all_conv) return;
     const uint32_t k = A.one_pass ? 1u : st->k;
@@ -817,8 +820,19 @@ __global__ __launch_bounds__(kHThreads) void spmm_vi_tile(const BatchArgs A) {
     const int KP = A.kp;
     const int g = A.g, nc = A.nc, na = A.na;
     const bool goals = A.Rcol == nullptr;
-    // the columns of this lane in the argmax phase: cq + h * KQ in pass h
+    if (tid < KB) {  // read after the first tile's barrier
+        const int gc = A.goal[jb0 + tid];
+        goal_s[tid] = gc;
+        goal_i[tid] = gc >= 0 ? gc / g : -1;
+        goal_j[tid] = gc >= 0 ? gc - (gc / g) * g : -1;
+    }
+    // the columns of this lane in the argmax phase: cq + h * KQ in pass h; qoff[i]: the
+    // offset of column cq in row i of a run of QM + 1 rows whose first row is a multiple of
+    // QM + 1
     const int cq = lane % KQ;
+    int qoff[QM + 1];
+#pragma unroll
+    for (int i = 0; i <= QM; i++) qoff[i] = i * KQ + 2 * ((cq >> 1) ^ i) + (cq & 1);
     int goal_q[QH];
     bool live_q[QH];
     float rmax[QH];
@@ -907,23 +921,22 @@ __global__ __launch_bounds__(kHThreads) void spmm_vi_tile(const BatchArgs A) {
                     const int cell = (hi0 + hi) * g + hj0 + hj;
                     uint32_t mask = 0u;
 #pragma unroll
-                    for (int c = 0; c < KB; c++) mask |= (A.goal[jb0 + c] == cell ? 1u : 0u) << c;
+                    for (int c = 0; c < KB; c++) mask |= (goal_s[c] == cell ? 1u : 0u) << c;
                     reinterpret_cast<uint32_t*>(halo + hc * CS + KB)[0] = mask;
                 }
             }
         }
         __syncthreads();
         if (!act) continue;  // block-uniform per wave; the next tile's barrier waits for all
-        bool goal_tile = false;
+        // does a goal of the block's columns lie in the halo (then the walk reads the masks)
+        int gt = 0;
         if (goals) {
-            for (int c = 0; c < KB; c++) {
-                const int gc = A.goal[jb0 + c];
-                if (gc >= 0) {
-                    const int gi2 = gc / g, gj2 = gc - gi2 * g;
-                    goal_tile = goal_tile || ((unsigned)(gi2 - hi0) < (unsigned)HH && (unsigned)(gj2 - hj0) < (unsigned)HW);
-                }
-            }
+#pragma unroll
+            for (int c = 0; c < KB; c++)
+                gt |= (goal_i[c] >= 0 && (unsigned)(goal_i[c] - hi0) < (unsigned)HH &&
+                       (unsigned)(goal_j[c] - hj0) < (unsigned)HW) ? 1 : 0;
         }
+        const bool goal_tile = __builtin_amdgcn_readfirstlane(gt) != 0;
         const float delta = (float)((double)A.r_term - (t > A.nt - 20 ? A.late_pen : 0.0));
         auto decode = [&](int2 e) {
             const int cc = e.x - colbase;
@@ -979,7 +992,7 @@ __global__ __launch_bounds__(kHThreads) void spmm_vi_tile(const BatchArgs A) {
                             uint32_t mask = 0u;
                             if (goals) {
 #pragma unroll
-                                for (int c = 0; c < KB; c++) mask |= (A.goal[jb0 + c] == col - colbase ? 1u : 0u) << c;
+                                for (int c = 0; c < KB; c++) mask |= (goal_s[c] == col - colbase ? 1u : 0u) << c;
                             }
                             reinterpret_cast<uint32_t*>(halo + r0 + KB)[0] = mask;
                             ovf_goal = ovf_goal || mask != 0u;
@@ -1053,23 +1066,42 @@ __global__ __launch_bounds__(kHThreads) void spmm_vi_tile(const BatchArgs A) {
             // consecutive columns, fall on different banks. Then lane (state, column): max
             // / first argmax over the actions in action order (strict '>', spvi_2.cu:112)
             const double* const qd = reinterpret_cast<const double*>(qb);
+            // a goal column reached by a lane of a row of the wave (rare): only then the
+            // rewards of the goal columns are formed per column
+            bool anyhit = false;
+#pragma unroll
+            for (int c = 0; c < KB; c++) anyhit = anyhit || hit[c] >= 0;
+            const bool wave_hit = __ballot(anyhit) != 0ull;
 #pragma unroll
             for (int h = 0; h < QH; h++) {
                 if (valid) {
+                    double qq[KQ];
+                    if (!goals) {
+#pragma unroll
+                        for (int c = 0; c < KQ; c++) {
+                            const float rj = A.Rcol[(int64_t)min(jb0 + h * KQ + c, A.rcol_n - 1) * A.rows + row];
+                            qq[c] = (double)rj + (double)A.discount * sd[h * KQ + c];
+                        }
+                    } else if (wave_hit) {
+#pragma unroll
+                        for (int c = 0; c < KQ; c++) {
+                            const int hc = hit[h * KQ + c];
+                            const float rj = hc >= 0 ? (cur.rs + (float)hc * delta) / A.nr : cur.rr;
+                            qq[c] = (double)rj + (double)A.discount * sd[h * KQ + c];
+                        }
+                    } else if (A.discount == 1.0f) {  // 1 * sum is the sum, bit for bit
+#pragma unroll
+                        for (int c = 0; c < KQ; c++) qq[c] = (double)cur.rr + sd[h * KQ + c];
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < KQ; c++) qq[c] = (double)cur.rr + (double)A.discount * sd[h * KQ + c];
+                    }
 #pragma unroll
                     for (int c2 = 0; c2 < KQ / 2; c2++) {
-                        double qq[2];
-#pragma unroll
-                        for (int e2 = 0; e2 < 2; e2++) {
-                            const int c = h * KQ + 2 * c2 + e2;
-                            const float rj = !goals ? A.Rcol[(int64_t)min(jb0 + c, A.rcol_n - 1) * A.rows + row]
-                                             : hit[c] >= 0 ? (cur.rs + (float)hit[c] * delta) / A.nr
-                                                           : cur.rr;
-                            double x = (double)rj + (double)A.discount * sd[c];
-                            if (x != x) x = -INFINITY;
-                            qq[e2] = x;
-                        }
-                        qb[lane * (KQ / 2) + (c2 ^ (lane & QM))] = make_double2(qq[0], qq[1]);
+                        double x0 = qq[2 * c2], x1 = qq[2 * c2 + 1];
+                        if (x0 != x0) x0 = -INFINITY;
+                        if (x1 != x1) x1 = -INFINITY;
+                        qb[lane * (KQ / 2) + (c2 ^ (lane & QM))] = make_double2(x0, x1);
                     }
                 }
                 __builtin_amdgcn_wave_barrier();
@@ -1079,12 +1111,29 @@ __global__ __launch_bounds__(kHThreads) void spmm_vi_tile(const BatchArgs A) {
                     if (q2 >= tw) continue;
                     double best = -INFINITY;
                     int ba = 0x7fffffff;
-                    for (int a2 = 0; a2 < na; a2++) {
-                        const int r = s2 * nap + a2;
-                        const double x = qd[2 * (r * (KQ / 2) + ((cq >> 1) ^ (r & QM))) + (cq & 1)];
-                        if (x > best) {
-                            best = x;
-                            ba = a2;
+                    if (nap > QM) {  // r & QM = a2 & QM: the swizzle offsets repeat every QM + 1 rows
+                        for (int m = 0; m * (QM + 1) < na; m++) {
+                            const double* qm = qd + (s2 * nap + m * (QM + 1)) * KQ;
+#pragma unroll
+                            for (int i = 0; i <= QM; i++) {
+                                const int a2 = m * (QM + 1) + i;
+                                if (a2 < na) {
+                                    const double x = qm[qoff[i]];
+                                    if (x > best) {
+                                        best = x;
+                                        ba = a2;
+                                    }
+                                }
+                            }
+                        }
+                    } else {
+                        for (int a2 = 0; a2 < na; a2++) {
+                            const int r = s2 * nap + a2;
+                            const double x = qd[2 * (r * (KQ / 2) + ((cq >> 1) ^ (r & QM))) + (cq & 1)];
+                            if (x > best) {
+                                best = x;
+                                ba = a2;
+                            }
                         }
                     }
                     const int cell = cell0 + q2;
@@ -1493,7 +1542,7 @@ const Variant kVariants[] = {
     {4, 4, 3, 0, spmm_vi_halo<4, 4, 4>},  {4, 2, 3, 0, spmm_vi_halo<2, 4, 4>},
     {2, 4, 3, 0, spmm_vi_halo<4, 2, 4>},  {4, 1, 3, 0, spmm_vi_halo<1, 4, 4>},
     {8, 2, 3, 0, spmm_vi_halo<2, 8, 4>},  {2, 2, 3, 0, spmm_vi_halo<2, 2, 4>},
-    {16, 1, 4, 8, spmm_vi_tile<16, 8, 2, 4, 1>}, {16, 1, 4, 8, spmm_vi_tile<16, 8, 1, 5, 2>},
+    {8, 1, 4, 16, spmm_vi_tile<8, 16, 1, 4, 1>},  {16, 1, 4, 12, spmm_vi_tile<16, 12, 1, 4, 1>},
     {8, 1, 4, 8, spmm_vi_tile<8, 8, 2, 5, 1>},   {16, 1, 4, 16, spmm_vi_tile<16, 16, 1, 4, 1>},
     {16, 1, 4, 8, spmm_vi_tile<16, 8, 1, 4, 1>},
 };
@@ -1665,8 +1714,8 @@ static dim3 grid_of(const Model* m, const Variant& v, int kp, int ncols) {
 // (1.8e8 entries; plain sweep 0.39 ms), ms per sweep over every state:
 //   width   1-2: spmm_vi_stream<8, 2, 4> 0.95;
 //   width     4: spmm_vi_halo<1, 4> 1.38 (spmm_vi_rows<1, 4, 4> 1.59);
-//   width     8: spmm_vi_tile<8, 8, 2, 5, 1> 1.49 (spmm_vi_halo<2, 4> 1.97);
-//   width 16/32: spmm_vi_tile<16, 16, 1, 4, 1> 2.33 / 4.72 (spmm_vi_halo<2, 8> 3.46 / 6.92).
+//   width     8: spmm_vi_tile<8, 8, 2, 5, 1> 1.33 (spmm_vi_halo<2, 4> 1.98);
+//   width 16/32: spmm_vi_tile<16, 16, 1, 4, 1> 1.99 / 4.03 (spmm_vi_halo<2, 8> 3.45 / 6.90).
 // The halo and tile kernels need a built model; where one does not fit, the second table's
 // kernel runs, and where that does not fit either, spmm_vi_sweep<1, 4> (fits every model).
 constexpr int kLiveWidths[] = {2, 4, 8, 1 << 30};

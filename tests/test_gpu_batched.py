@@ -99,7 +99,7 @@ def test_batched_columns_independent_of_batch_and_variant(gpu):
 # column block
 N_VARIANTS = 38
 HALO_VARIANTS = {27: (4, 4), 28: (2, 4), 29: (4, 2), 30: (1, 4), 31: (2, 8), 32: (2, 2)}
-TILE_VARIANTS_KB = {33: 16, 34: 16, 35: 8, 36: 16, 37: 16}
+TILE_VARIANTS_KB = {33: 8, 34: 16, 35: 8, 36: 16, 37: 16}
 ROWS_VARIANTS = {9: (4, 1), 10: (8, 1), 11: (4, 2), 12: (8, 2), 13: (4, 4), 14: (16, 1), 15: (8, 4),
                  16: (16, 2), 17: (1, 4), 18: (1, 8), 19: (2, 4), 20: (2, 8)}
 
