@@ -46,7 +46,7 @@ class BuildOpts(C.Structure):
         ("fields_on_device", C.c_int32), ("keep_rsum", C.c_int32), ("slot_k", C.c_int32),
         ("radix_bits", C.c_int32), ("slices_per_launch", C.c_int32), ("block_threads", C.c_int32),
         ("stream", C.c_void_p), ("count_kernel", C.c_int32), ("paired_rounds", C.c_int32),
-        ("n_alpha", C.c_int32), ("reserved", C.c_int32), ("alphas", C.c_void_p),
+        ("n_alpha", C.c_int32), ("lane_moves", C.c_int32), ("alphas", C.c_void_p),
     ]
 
 
@@ -57,7 +57,7 @@ class ModelInfo(C.Structure):
         [(n, C.c_double) for n in ("build_ms", "move_kernel_ms", "scan_ms", "csr_ms")] + \
         [("has_rsum", C.c_int32), ("move_launches", C.c_int32), ("block_threads", C.c_int32),
          ("unit_cells", C.c_int32), ("wave_blocks", C.c_int32), ("n_deferred_rows", C.c_int32),
-         ("paired_rounds", C.c_int32), ("reserved", C.c_int32)]
+         ("paired_rounds", C.c_int32), ("counted_cells", C.c_int32)]
 
 
 class SolveOpts(C.Structure):

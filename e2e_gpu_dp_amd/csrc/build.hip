@@ -32,6 +32,8 @@ namespace e2e {
 namespace {
 
 
+constexpr int kStatSlots = 16;  // 64-bit build statistics (BuildArgs::stats)
+
 struct BuildArgs {
     const float* u;
     const float* v;
@@ -64,8 +66,13 @@ struct BuildArgs {
     unsigned long long* arena_top;
     unsigned long long arena_cap;
     unsigned long long* stats;  // 0 oow lanes, 1 overflow rows, 2 max nuq, 3 nonfinite,
-                                // 4 arena exhausted, 5 multipass rows
+                                // 4 arena exhausted, 5 multipass rows, 6 lane off the grid,
+                                // 7 alpha refusals, 8 counted cells (kStatSlots in all)
     int32_t pipe_rounds;  // per-row kernel: pair its rounds (few waves resident per SIMD)
+    const uint32_t* cell_list;  // per-row kernel in list mode: the cells (t * nc + cell) it builds, one per block
+    int tc_count;               // mc_count: cells per block
+    uint32_t* left_cells;       // mc_count: the cells it leaves to the per-row kernel
+    unsigned* left_count;
     int32_t n_alpha;      // alpha columns (reward method 4): rewards for each alpha too
     float* Ralpha;        // [n_alpha][rows_total]
     int64_t rows_total;
@@ -689,11 +696,490 @@ __device__ __forceinline__ void alpha_rewards(const BuildArgs& A, size_t row, fl
     }
 }
 
+// The reward values a lane of a row can get, when the row's reward sum is an exact int32
+// sum. Every reward is one of: rstep + r_k (+ r_terminal, or the late-slice penalty),
+// r_k = k * r_outbound summed one at a time, k = 0..3 (x clamp, y clamp, edge), or
+// r_outbound (obstacles), or 0. seg_int: all of them are integers below int_lim.
+// seg_lin: also linear in k with slope r_outbound, so a lane's reward is the k = edge
+// value (tt0 / tt1 landing on a terminal cell, tn0 / tn1 elsewhere, for k = 0 / 1) plus
+// r_outbound times its clamp count.
+struct RowRewards {
+    bool seg_int, seg_lin;
+    int tt0, tt1, tn0, tn1;
+};
+
+__device__ __forceinline__ RowRewards row_rewards(const BuildArgs& A, float rstep, bool late_slice, bool uses_rad) {
+    RowRewards w{!uses_rad, !uses_rad, 0, 0, 0, 0};
+    if (!w.seg_int) return w;
+    auto ok = [&](float v) { return v == rintf(v) && fabsf(v) < A.int_lim; };
+    float rk = 0.0f;
+    w.seg_int = ok(A.r_out) && ok(0.0f);
+    w.seg_lin = w.seg_int;
+    for (int kk = 0; kk < 4; kk++) {
+        const float other = late_slice ? (float)((double)rk + A.late_pen) : rk;
+        const float rt = (rk + A.r_term) + rstep, ro = other + rstep;
+        w.seg_int = w.seg_int && ok(rt) && ok(ro);
+        if (w.seg_int) {
+            if (kk == 0) {
+                w.tt0 = (int)rt;
+                w.tn0 = (int)ro;
+            } else if (kk == 1) {
+                w.tt1 = (int)rt;
+                w.tn1 = (int)ro;
+            }
+            const int64_t slope = (int64_t)kk * (int64_t)A.r_out;
+            w.seg_lin = w.seg_lin && (int64_t)(int)rt == (int64_t)w.tt0 + slope &&
+                        (int64_t)(int)ro == (int64_t)w.tn0 + slope;
+        }
+        rk = rk + A.r_out;
+    }
+    w.seg_lin = w.seg_lin && w.seg_int;
+    return w;
+}
+
+// ---------------------------------------------------------------------------------
+// Counted cells: the rows of a cell from one pass over its realizations.
+//
+// On unit cells a moving lane's landing column is a non-decreasing function of its
+// sample's vx alone, and its landing row (counted from the bottom) of vy alone:
+//   ord_x(v) = 0 (clamped low) | 1 + xind (inside) | XH + 2 (clamped high),
+//   xnew = xc + (fcos + v) * dt, xind = floor(ax) + (fract(ax) >= 0.5), ax = xnew - xs0,
+// every operation rounded as in lane_move (each one is monotone, and so is the clamp),
+// and ord_y the same with the row index. So for action a and each ord o the cell's
+// samples reach (ord_lo(a) < o <= ord_hi(a)), ord_x(v) >= o exactly when
+// key(v) >= tau(a, o), the smallest float key with ord_x >= o, found by bisection over
+// the keys between the cell's smallest and largest sample (key: the float's bits mapped
+// to an order-preserving uint32). A block then counts the cell once for all its actions:
+//   1. the boundaries tau of every action, one lane each, ranked per axis;
+//   2. every realization's position among them, I_x = #{tau_x <= key(vx)} and I_y, in a
+//      2-D histogram in LDS, then its inclusive prefix;
+//   3. per row, the landing cells of its box in ascending S2 order: a cell's count is a
+//      rectangle of the prefix (its I ranges cover its clamped and inside ords), and the
+//      reward is summed per landing cell and clamp as the per-lane fast pass sums it.
+// Same counts, order, rewards and statistics as moving every lane. One wavefront counts
+// one cell, with its own tables and grid in LDS (no block barrier), and recomputes the
+// samples when it needs them. A group of actions holds at most kCntMax boundaries per
+// axis; a cell that needs more is counted group by group.
+constexpr int kCntMax = 47;                    // boundaries per axis per group
+constexpr int kCntPitch = kCntMax + 2;         // grid row pitch in 16-bit bins
+constexpr int kCntBins = kCntPitch * kCntPitch;  // grid [I_y + 1][I_x + 1]
+constexpr int kCntWaves = 16;                  // waves of a counting block
+constexpr int kCntModes = 8;                   // DO modes a counting wave holds in registers
+
+// LDS of one counting wave
+struct CountWave {
+    uint32_t grid[(kCntBins + 1) / 2];  // 16-bit counts, two per word (a count is at most nrzns < 2^16)
+    uint32_t u[2][64];                  // the group's boundary keys per axis, ascending; padding 0xffffffff
+    uint8_t c[2][kCntMax + 1];          // boundary j of the group: #{u <= its key}
+    uint32_t d[kCntMax + 1];            // boundary j of the axis being ranked: action << 16 | ord
+};
+// LDS of a counting block: this table, the waves' tables, then Yi [nm][nr] when it fits
+struct CountShared {
+    float fcos[kMaxActions], fsin[kMaxActions], speed[kMaxActions];  // the action table, for per-lane reads
+    unsigned next_cell;                                              // the block's next cell for a free wave
+};
+
+__device__ __forceinline__ uint32_t fkey(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float kfloat(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+// the unit-cell index of lane_move<true> for a >= 0: floor(a) + (fract(a) >= 0.5)
+__device__ __forceinline__ int unit_index(float a) {
+    return cvt_flr_i32(a) + (__builtin_amdgcn_fractf(a) >= 0.5f ? 1 : 0);
+}
+
+// the scalars of the unit-cell move from one start cell
+struct CountGeo {
+    float xc, yc, xs0, xsn, ys0, ysn, dt;
+    int XH, YL;  // the column of a lane clamped high, the row of a lane clamped low
+};
+
+__device__ __forceinline__ int ord_x(const CountGeo& G, float fc, float v) {
+    const float xnew = G.xc + ((fc + v) * G.dt);
+    const int inside = 1 + unit_index(xnew - G.xs0);
+    return xnew < G.xs0 ? 0 : (xnew > G.xsn ? G.XH + 2 : inside);
+}
+__device__ __forceinline__ int ord_y(const CountGeo& G, float fs, float v) {
+    const float ynew = G.yc + ((fs + v) * G.dt);
+    const int inside = 1 + (G.YL - unit_index(G.ysn - ynew));
+    return ynew < G.ys0 ? 0 : (ynew > G.ysn ? G.YL + 2 : inside);
+}
+template <int AX>
+__device__ __forceinline__ int ord_of(const CountGeo& G, float f, float v) {
+    return AX ? ord_y(G, f, v) : ord_x(G, f, v);
+}
+
+// tau: the smallest key whose float has ord >= o, given ord(k0) < o <= ord(k1). The
+// bisection starts from a bracket around the real-number boundary (where the move's
+// unrounded position reaches the ord's edge), which the rounding moves by a few ulps of
+// the position; the bracket is used only if its ends verify, else [k0, k1].
+template <int AX>
+__device__ __forceinline__ uint32_t boundary_key(const CountGeo& G, float f, int o, uint32_t k0, uint32_t k1) {
+    float edge;
+    if (AX == 0)
+        edge = o == 1 ? G.xs0 : (o == G.XH + 2 ? G.xsn : G.xs0 + ((float)(o - 1) - 0.5f));
+    else
+        edge = o == 1 ? G.ys0 : (o == G.YL + 2 ? G.ysn : G.ysn - ((float)(G.YL + 1 - o) + 0.5f));
+    const float c = AX ? G.yc : G.xc;
+    const float vb = (edge - c) / G.dt - f;
+    const float d = 0x1p-20f * (fabsf(c) + fabsf(edge) + (fabsf(f) + fabsf(vb)) * G.dt + 1.0f) / G.dt;
+    uint32_t lo = max(fkey(vb - d), k0), hi = min(fkey(vb + d), k1);
+    if (!(lo < hi && ord_of<AX>(G, f, kfloat(lo)) < o && ord_of<AX>(G, f, kfloat(hi)) >= o)) {
+        lo = k0;
+        hi = k1;
+    }
+    // every key between two finite floats is a finite float
+    while (hi - lo > 1u) {
+        const uint32_t m = lo + ((hi - lo) >> 1);
+        if (ord_of<AX>(G, f, kfloat(m)) >= o)
+            hi = m;
+        else
+            lo = m;
+    }
+    return hi;
+}
+
+// Sums over 16-lane segments (SW 16: the sum of each row of 16 lanes, in all its lanes)
+// or over the wave (SW 64).
+template <int SW>
+__device__ __forceinline__ int seg_sum(int x) {
+    if (SW == 64) return wave_sum(x);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+    return __shfl(x, (int)(threadIdx.x & 63) | 15, 64);
+}
+
+// landing column / row of an ord
+__device__ __forceinline__ int col_of(const CountGeo& G, int o) { return o == 0 ? 0 : (o == G.XH + 2 ? G.XH : o - 1); }
+__device__ __forceinline__ int row_of(const CountGeo& G, int o) {
+    return o == 0 ? G.YL : (o == G.YL + 2 ? 0 : G.YL - (o - 1));
+}
+
+// Wave reductions of floats (DPP, as wave_reduce): every lane gets the result.
+__device__ __forceinline__ float wave_fmin(float x) {
+    auto op = [](int a, int b) { return __float_as_int(fminf(__int_as_float(a), __int_as_float(b))); };
+    return __int_as_float(wave_reduce(__float_as_int(x), __float_as_int(INFINITY), op));
+}
+__device__ __forceinline__ float wave_fmax(float x) {
+    auto op = [](int a, int b) { return __float_as_int(fmaxf(__int_as_float(a), __int_as_float(b))); };
+    return __int_as_float(wave_reduce(__float_as_int(x), __float_as_int(-INFINITY), op));
+}
+
+// One wavefront counts every row of a moving cell (the per-lane fast pass's conditions
+// hold for the cell: unit cells, no scalar-field reward, neither terminal nor obstacle)
+// and writes them as the per-lane pass would. Returns false, wave-uniformly and before
+// writing anything, when the cell is left to that pass: a non-finite sample, an
+// obstacle within the fixed window, a row without exact integer rewards, an action with
+// more than kCntMax boundaries on an axis. Y: the slice's Yi, [nm][nr] (YLDS, in LDS)
+// or [nr][nm] (global).
+template <bool OBS, bool YLDS>
+__device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __restrict__ Y, CountWave& Wv,
+                                           const CountShared& T, CountGeo& G, int t, int c, bool late_slice,
+                                           bool obs_slice, unsigned long long& oow_acc,
+                                           unsigned long long& overflow_acc, uint32_t& maxnuq_acc) {
+    const int lane = threadIdx.x & 63;
+    const int na = A.na, nr = A.nr, nm = A.nm, g = A.g, nc = A.nc;
+    const int i1 = c / g, j1 = c - i1 * g;
+    const size_t tslice = (size_t)t * nc, mid = tslice + c;
+    if (is_terminal(A, i1, j1) || (A.mask && A.mask[mid] == 1)) return false;
+    if (OBS && obs_slice) {  // an obstacle within the fixed window (at t or t + 1): the per-lane pass
+        bool near = false;
+        const int32_t* m0 = A.mask + tslice;
+        for (int k = lane; k < kFixSide * kFixSide; k += 64) {
+            const int i = i1 + k / kFixSide - kFixHalf, j = j1 + k % kFixSide - kFixHalf;
+            if (i >= 0 && i < g && j >= 0 && j < g) near = near || m0[j + g * i] == 1 || m0[nc + j + g * i] == 1;
+        }
+        if (__any(near)) return false;
+    }
+    G.xc = A.xs[j1];
+    G.yc = A.ys[g - 1 - i1];
+    const float u0 = A.u[mid], v0 = A.v[mid];
+    // the cell's mode coefficients, once (scalar registers)
+    float cu[kCntModes], cv[kCntModes];
+#pragma unroll
+    for (int k = 0; k < kCntModes; k++) {
+        const size_t o = ((size_t)t * nm + (k < nm ? k : 0)) * nc + c;
+        cu[k] = A.ui[o];
+        cv[k] = A.vi[o];
+    }
+    // realization w's velocity sample, extract_velocity :361-404 (the mode sum from 0 in
+    // k order, then added to the mean)
+    auto sample = [&](int w) __attribute__((always_inline)) {
+        float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kCntModes; k++) {
+            if (k < nm) {
+                const float y = YLDS ? Y[k * nr + w] : Y[w * nm + k];
+                sx = sx + cu[k] * y;
+                sy = sy + cv[k] * y;
+            }
+        }
+        return make_float2(u0 + sx, v0 + sy);
+    };
+    // the cell's sample range (-0 and +0 have the same ords, so the float order serves)
+    float x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+    bool nf = false;
+    for (int w = lane; w < nr; w += 64) {
+        const float2 v = sample(w);
+        nf |= !(isfinite(v.x) && isfinite(v.y));
+        x0 = fminf(x0, v.x);
+        x1 = fmaxf(x1, v.x);
+        y0 = fminf(y0, v.y);
+        y1 = fmaxf(y1, v.y);
+    }
+    if (__any(nf)) return false;
+    x0 = wave_fmin(x0);
+    x1 = wave_fmax(x1);
+    y0 = wave_fmin(y0);
+    y1 = wave_fmax(y1);
+    const uint32_t kx0 = fkey(x0), kx1 = fkey(x1), ky0 = fkey(y0), ky1 = fkey(y1);
+
+    // per action (lane a): the ords its samples reach, its landing box
+    const bool act = lane < na;
+    const int aa = act ? lane : 0;
+    const float fca = T.fcos[aa], fsa = T.fsin[aa];
+    const int olx = ord_x(G, fca, x0), ohx = ord_x(G, fca, x1);
+    const int oly = ord_y(G, fsa, y0), ohy = ord_y(G, fsa, y1);
+    const int nx = act ? ohx - olx : 0, ny = act ? ohy - oly : 0;
+    const int bxlo = col_of(G, olx), bylo = row_of(G, ohy);
+    const int bw = col_of(G, ohx) - bxlo + 1, ncell = bw * (row_of(G, oly) - bylo + 1);
+    const RowRewards rr = row_rewards(A, step_reward(A, T.speed[aa], 0.0f, 0.0f), late_slice, false);
+    bool ok = rr.seg_lin && nx >= 0 && ny >= 0 && nx <= kCntMax && ny <= kCntMax;
+    if (OBS && obs_slice)  // no obstacle within the fixed window: boxes must stay inside it
+        ok = ok && bxlo >= j1 - kFixHalf && col_of(G, ohx) <= j1 + kFixHalf && bylo >= i1 - kFixHalf &&
+             row_of(G, oly) <= i1 + kFixHalf;
+    if (!__all(!act || ok)) return false;
+    const int ix = wave_incl_scan(nx), iy = wave_incl_scan(ny);
+    const int fx = ix - nx, fy = iy - ny;  // first boundary of each action in the raw lists
+    const int del = (A.m - 1) / 2;
+    const int i_term = A.i_term, j_term = A.j_term, tsg = A.tsg, mwin = A.m, K = A.K;
+    // 16-bit bins: bin b is half b & 1 of word b >> 1
+    auto bin = [&](int b) __attribute__((always_inline)) {
+        return (int)reinterpret_cast<const uint16_t*>(Wv.grid)[b];
+    };
+
+    for (int start = 0; start < na;) {
+        // the group: the longest run of actions from start with at most kCntMax
+        // boundaries per axis (a prefix of the lanes from start: ix, iy grow with a;
+        // every action fits alone, so the clamps below change nothing)
+        const int bx = start ? __builtin_amdgcn_readlane(ix, start - 1) : 0;
+        const int by = start ? __builtin_amdgcn_readlane(iy, start - 1) : 0;
+        const bool fits = act && lane >= start && ix - bx <= kCntMax && iy - by <= kCntMax;
+        const int end = start + max(1, (int)__popcll(__ballot(fits)));
+        const int Tx = min(kCntMax, __builtin_amdgcn_readlane(ix, end - 1) - bx);
+        const int Ty = min(kCntMax, __builtin_amdgcn_readlane(iy, end - 1) - by);
+
+        // the boundaries, x then y: one lane each, bisection, rank
+#pragma unroll
+        for (int ax = 0; ax < 2; ax++) {
+            const int n_a = ax ? ny : nx, lo_a = ax ? oly : olx, first = ax ? fy - by : fx - bx;
+            if (lane >= start && lane < end)
+                for (int k = 0; k < n_a; k++) Wv.d[first + k] = ((uint32_t)lane << 16) | (uint32_t)(lo_a + 1 + k);
+            __builtin_amdgcn_wave_barrier();
+            const int nb = ax ? Ty : Tx;
+            uint32_t key = 0xffffffffu;
+            if (lane < nb) {
+                const uint32_t dsc = Wv.d[lane];
+                const int a = (int)(dsc >> 16), o = (int)(dsc & 0xffffu);
+                key = ax ? boundary_key<1>(G, T.fsin[a], o, ky0, ky1) : boundary_key<0>(G, T.fcos[a], o, kx0, kx1);
+            }
+            uint32_t cnt = 0u, pos = 0u;  // #{keys <= mine}, and a stable sort
+            for (int i = 0; i < nb; i++) {
+                const uint32_t ki = __builtin_amdgcn_readlane(key, i);
+                cnt += ki <= key ? 1u : 0u;
+                pos += (ki < key || (ki == key && i < lane)) ? 1u : 0u;
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (lane < nb) {
+                Wv.c[ax][lane] = (uint8_t)cnt;
+                Wv.u[ax][pos] = key;
+            } else {
+                Wv.u[ax][lane] = 0xffffffffu;
+            }
+        }
+        for (int i = lane; i < ((Ty + 2) * kCntPitch + 1) / 2; i += 64) Wv.grid[i] = 0u;
+        __builtin_amdgcn_wave_barrier();
+
+        // every realization's cell of the boundary grid, two rounds at a time
+        auto place = [&](float2 v) __attribute__((always_inline)) {
+            const uint32_t kx = fkey(v.x), ky = fkey(v.y);
+            int Ix = 0, Iy = 0;
+#pragma unroll
+            for (int h = 32; h >= 1; h >>= 1) {
+                Ix += Wv.u[0][Ix + h - 1] <= kx ? h : 0;
+                Iy += Wv.u[1][Iy + h - 1] <= ky ? h : 0;
+            }
+            return (Iy + 1) * kCntPitch + Ix + 1;
+        };
+        for (int w = lane; w < nr; w += 128) {
+            const bool two = w + 64 < nr;
+            const int b1 = place(sample(w)), b2 = place(sample(two ? w + 64 : w));
+            atomicAdd(&Wv.grid[b1 >> 1], 1u << ((b1 & 1) * 16));
+            if (two) atomicAdd(&Wv.grid[b2 >> 1], 1u << ((b2 & 1) * 16));
+        }
+        __builtin_amdgcn_wave_barrier();
+        // inclusive prefix: down the columns (lane x - 1 walks column x), then along the
+        // rows (a wave scan per row)
+        uint16_t* g16 = reinterpret_cast<uint16_t*>(Wv.grid);
+        if (lane <= Tx) {
+            int acc = 0;
+            for (int y = 1; y <= Ty + 1; y++) {
+                acc += g16[y * kCntPitch + lane + 1];
+                g16[y * kCntPitch + lane + 1] = (uint16_t)acc;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (int y = 1; y <= Ty + 1; y++) {
+            const int sc = wave_incl_scan(lane <= Tx ? (int)g16[y * kCntPitch + lane + 1] : 0);
+            if (lane <= Tx) g16[y * kCntPitch + lane + 1] = (uint16_t)sc;
+        }
+        __builtin_amdgcn_wave_barrier();
+
+        // The group's rows. Segment width SW: 16 lanes per row, four rows per pass, when
+        // their landing boxes have at most 16 cells; else 64, one row per pass (and
+        // chunks of 64 cells). Lane kk of row ar's segment takes landing cells kk, kk + SW,
+        // .. of the box in row-major order (ascending S2).
+        auto emit = [&](auto sw_tag, int ar, bool arow, int kk) __attribute__((always_inline)) {
+            constexpr int SW = decltype(sw_tag)::value;
+            const int src = arow ? ar : start;
+            const int rlx = __shfl(olx, src, 64), rhx = __shfl(ohx, src, 64);
+            const int rly = __shfl(oly, src, 64), rhy = __shfl(ohy, src, 64);
+            const int offx = __shfl(fx, src, 64) - bx, offy = __shfl(fy, src, 64) - by;
+            const int xlo = __shfl(bxlo, src, 64), ylo = __shfl(bylo, src, 64), W = __shfl(bw, src, 64);
+            const int nce = arow ? __shfl(ncell, src, 64) : 0;
+            const float rW = __builtin_amdgcn_rcpf((float)W);
+            // first grid column (row) of the realizations with ord >= o
+            auto gx = [&](int o) __attribute__((always_inline)) {
+                return o <= rlx ? 0 : (o > rhx ? Tx + 1 : (int)Wv.c[0][offx + o - rlx - 1]);
+            };
+            auto gy = [&](int o) __attribute__((always_inline)) {
+                return o <= rly ? 0 : (o > rhy ? Ty + 1 : (int)Wv.c[1][offy + o - rly - 1]);
+            };
+            auto rect = [&](int lx, int hx, int ly, int hy) __attribute__((always_inline)) {
+                return bin(hy * kCntPitch + hx) - bin(ly * kCntPitch + hx) - bin(hy * kCntPitch + lx) +
+                       bin(ly * kCntPitch + lx);
+            };
+            // landing cell k: its count, its lanes' clamps, its reward class (terminal * 2 +
+            // edge: the reward before the clamp penalties is tn0, tn1, tt0 or tt1), its lanes
+            // outside the neighb_gsize window, its S2 cell
+            auto cell = [&](int k, int& cnt, int& kc, int& cls, int& oo, uint32_t& sp2) __attribute__((always_inline)) {
+                // k / W from the reciprocal: (k + 1/2) / W is at least 1 / (2 W) from an integer
+                const int q = (int)(((float)k + 0.5f) * rW);
+                const int yi = ylo + q, xi = xlo + (k - q * W);
+                const int oax = max(rlx, xi == 0 ? 0 : xi + 1), obx = min(rhx, xi == G.XH ? G.XH + 2 : xi + 1);
+                const int oay = max(rly, yi == G.YL ? 0 : 1 + G.YL - yi), oby = min(rhy, yi == 0 ? G.YL + 2 : 1 + G.YL - yi);
+                const int lx = gx(oax), hx = gx(obx + 1), ly = gy(oay), hy = gy(oby + 1);
+                cnt = rect(lx, hx, ly, hy);
+                kc = 0;
+                if (oax == 0) kc += rect(lx, gx(1), ly, hy);                // x clamped low
+                if (obx == G.XH + 2) kc += rect(gx(G.XH + 2), hx, ly, hy);  // x clamped high
+                if (oay == 0) kc += rect(lx, hx, ly, gy(1));                // y clamped low
+                if (oby == G.YL + 2) kc += rect(lx, hx, gy(G.YL + 2), hy);  // y clamped high
+                const int n1 = g - 1;
+                const bool edge = (yi == 0) | (yi == n1) | (xi == 0) | (xi == n1);  // :339-341
+                const bool term = ((unsigned)(yi - i_term) < (unsigned)tsg) & ((unsigned)(xi - j_term) < (unsigned)tsg);  // :343-354
+                cls = (term ? 2 : 0) + (edge ? 1 : 0);
+                const int di = yi - i1, dj = xi - j1;
+                oo = (((unsigned)(di + del) >= (unsigned)mwin) | ((unsigned)(dj + del) >= (unsigned)mwin)) ? cnt : 0;
+                sp2 = (uint32_t)(xi + g * yi);
+            };
+            int nuq = 0, kcl = 0, oow = 0, cnt0 = 0, n0 = 0, n1c = 0, n2 = 0, n3 = 0;
+            uint32_t sp0 = 0u;
+            for (int k0 = 0; k0 < nce || (SW == 64 && k0 == 0); k0 += SW) {
+                int cnt = 0, kc = 0, cls = 0, oo = 0;
+                uint32_t sp2 = 0u;
+                if (k0 + kk < nce) cell(k0 + kk, cnt, kc, cls, oo, sp2);
+                if (k0 == 0) {
+                    cnt0 = cnt;
+                    sp0 = sp2;
+                }
+                nuq += cnt != 0 ? 1 : 0;
+                kcl += kc;
+                oow += oo;
+                n0 += cls == 0 ? cnt : 0;
+                n1c += cls == 1 ? cnt : 0;
+                n2 += cls == 2 ? cnt : 0;
+                n3 += cls == 3 ? cnt : 0;
+                if (SW == 16) break;  // one chunk: boxes of at most 16 cells
+            }
+            nuq = seg_sum<SW>(nuq);
+            kcl = seg_sum<SW>(kcl);
+            oow = seg_sum<SW>(oow);
+            n0 = seg_sum<SW>(n0);
+            n1c = seg_sum<SW>(n1c);
+            n2 = seg_sum<SW>(n2);
+            n3 = seg_sum<SW>(n3);
+            const size_t row = mid * na + ar;
+            const int seg0 = lane & ~(SW - 1);  // first lane of this segment
+            const bool lead = arow && lane == seg0 + SW - 1;
+            unsigned long long slot = 0;
+            if (lead && nuq > K) slot = atomicAdd(A.arena_top, (unsigned long long)nuq);
+            const unsigned long long arena_off = (unsigned long long)__shfl((long long)slot, seg0 + SW - 1, 64);
+            int u = 0;
+            for (int k0 = 0; k0 < nce || (SW == 64 && k0 == 0); k0 += SW) {
+                int cnt = cnt0;
+                uint32_t sp2 = sp0;
+                if (k0 > 0) {
+                    int kc, cls, oo;
+                    cnt = 0;
+                    if (k0 + kk < nce) cell(k0 + kk, cnt, kc, cls, oo, sp2);
+                }
+                const unsigned long long bal = __ballot(cnt != 0);
+                const unsigned long long segmask = SW == 64 ? ~0ull : (0xffffull << seg0);
+                const int below = (int)__popcll(bal & segmask & ((1ull << lane) - 1ull));
+                if (cnt != 0) put_entry(A, row, u + below, nuq, arena_off, sp2, (uint32_t)cnt);
+                u += (int)__popcll(bal & segmask);
+                if (SW == 16) break;
+            }
+            if (lead) {
+                // the fast pass's sum: per landing cell, plus r_outbound per clamp
+                const RowRewards rw = row_rewards(A, step_reward(A, T.speed[ar], 0.0f, 0.0f), late_slice, false);
+                const int64_t tot = (int64_t)n0 * rw.tn0 + (int64_t)n1c * rw.tn1 + (int64_t)n2 * rw.tt0 +
+                                    (int64_t)n3 * rw.tt1 + (int64_t)A.r_out * (int64_t)kcl;
+                const float fs_ = (float)(double)tot;  // compute_mean :563-570
+                A.row_nnz[row] = (uint32_t)nuq;
+                A.Rout[row] = fs_ / nr;
+                if (A.Rsum) A.Rsum[row] = fs_;
+                oow_acc += (unsigned long long)oow;
+                maxnuq_acc = max(maxnuq_acc, (uint32_t)nuq);
+                if (nuq > K) {
+                    overflow_acc++;
+                    if (arena_off + (unsigned long long)nuq > A.arena_cap) atomicOr(A.stats + 4, 1ull);
+                    else A.staging[row * K] = make_uint2((uint32_t)arena_off, (uint32_t)(arena_off >> 32));
+                }
+            }
+        };
+        for (int a0 = start; a0 < end; a0 += 4) {
+            int mx = 0;  // the largest landing box of rows a0 .. a0 + 3
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                if (a0 + r < end) mx = max(mx, __builtin_amdgcn_readlane(ncell, a0 + r));
+            if (mx <= 16) {
+                const int ar = a0 + (lane >> 4);
+                emit(std::integral_constant<int, 16>{}, ar, ar < end, lane & 15);
+            } else {
+                for (int r = 0; r < 4 && a0 + r < end; r++) emit(std::integral_constant<int, 64>{}, a0 + r, true, lane);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();  // the tables and grid are the next group's
+        start = end;
+    }
+    return true;
+}
+
 // LDS bytes of mc_wave<NW>.
 __host__ __device__ inline size_t wave_lds_bytes(int NW, int nr, int nm, bool stage_y) {
     return sizeof(float2) * (size_t)nr + NW * sizeof(uint32_t) * kHistWords +
            (stage_y ? sizeof(float) * (size_t)nr * nm : 0);
 }
+
+
 
 // Wave kernel: one wavefront per (cell, action) segment. A block of NW waves takes tc
 // consecutive cells of one slice; for each cell its threads form the velocity sample of
@@ -723,7 +1209,19 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
     uint32_t* hist = reinterpret_cast<uint32_t*>(vxy + nr) + (size_t)wv * kHistWords;  // [kWaveBins][kHistCopies] per wave
     uint32_t* hcopy = hist + (lane & (kHistCopies - 1));  // this lane's copy: hcopy[bin * kHistCopies]
     float* ylds = reinterpret_cast<float*>(reinterpret_cast<uint32_t*>(vxy + nr) + (size_t)NW * kHistWords);  // [nm][nr]
-    const int t = A.t0 + blockIdx.y;
+    // grid mode: blockIdx.y is the slice, blockIdx.x its tc cells; list mode: one cell of
+    // cell_list (the cells mc_count left to this kernel) per block
+    int t, c_begin, c_end;
+    if (A.cell_list) {
+        const uint32_t sc = A.cell_list[blockIdx.x];
+        t = (int)(sc / (uint32_t)nc);
+        c_begin = (int)(sc - (uint32_t)t * (uint32_t)nc);
+        c_end = c_begin + 1;
+    } else {
+        t = A.t0 + blockIdx.y;
+        c_begin = blockIdx.x * A.tc;
+        c_end = min(c_begin + A.tc, nc);
+    }
     const size_t tslice = (size_t)t * nc;
     for (int i = lane; i < kHistWords; i += 64) hist[i] = 0u;
     if (A.stage_y) {
@@ -743,7 +1241,6 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
     uint32_t maxnuq_acc = 0;
     bool bad = false, bad_index = false;
 
-    const int c_begin = blockIdx.x * A.tc, c_end = min(c_begin + A.tc, nc);
     for (int c = c_begin; c < c_end; c++) {
         const int i1 = c / g, j1 = c - i1 * g;
         const size_t mid = tslice + c;
@@ -851,38 +1348,9 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
             // late-slice penalty), r_k = k * r_outbound summed one at a time, k = 0..3
             // (x clamp, y clamp, edge), or r_outbound (obstacles), or 0. If all of them are
             // integers within int_lim the row sum is an exact int32 sum: no per-lane test.
-            bool seg_int = !uses_rad;
-            // the reward of a lane with k of those penalties that lands on a terminal cell
-            // or not: tt0 / tt1 and tn0 / tn1 for k = 0 / 1
-            int tt0 = 0, tt1 = 0, tn0 = 0, tn1 = 0;
-            bool seg_lin = seg_int;
-            if (seg_int) {
-                auto ok = [&](float v) { return v == rintf(v) && fabsf(v) < A.int_lim; };
-                float rk = 0.0f;
-                seg_int = ok(A.r_out) && ok(0.0f);
-                // linear in k with slope r_outbound: then a lane's reward is the k = edge
-                // value plus r_outbound times its clamp count
-                seg_lin = seg_int;
-                for (int kk = 0; kk < 4; kk++) {
-                    const float other = late_slice ? (float)((double)rk + A.late_pen) : rk;
-                    const float rt = (rk + A.r_term) + rstep, ro = other + rstep;
-                    seg_int = seg_int && ok(rt) && ok(ro);
-                    if (seg_int) {
-                        if (kk == 0) {
-                            tt0 = (int)rt;
-                            tn0 = (int)ro;
-                        } else if (kk == 1) {
-                            tt1 = (int)rt;
-                            tn1 = (int)ro;
-                        }
-                        const int64_t slope = (int64_t)kk * (int64_t)A.r_out;
-                        seg_lin = seg_lin && (int64_t)(int)rt == (int64_t)tt0 + slope &&
-                                  (int64_t)(int)ro == (int64_t)tn0 + slope;
-                    }
-                    rk = rk + A.r_out;
-                }
-                seg_lin = seg_lin && seg_int;
-            }
+            const RowRewards rwd = row_rewards(A, rstep, late_slice, uses_rad);
+            const bool seg_int = rwd.seg_int, seg_lin = rwd.seg_lin;
+            const int tt0 = rwd.tt0, tt1 = rwd.tt1, tn0 = rwd.tn0, tn1 = rwd.tn1;
             // Fast pass 1: unit cells, a start cell that moves (neither terminal nor an
             // obstacle), no scalar-field reward, integer rewards linear in the penalty
             // count, and no obstacle test to make: not an obstacle slice, or no obstacle
@@ -1141,6 +1609,83 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
         if (oow_acc) atomicAdd(A.stats + 0, oow_acc);
         if (overflow_acc) atomicAdd(A.stats + 1, overflow_acc);
         if (maxnuq_acc) atomicMax(A.stats + 2, (unsigned long long)maxnuq_acc);
+    }
+}
+
+// LDS bytes of mc_count: the shared table, the waves' tables, Yi [nm][nr] (YLDS).
+__host__ __device__ inline size_t count_lds_bytes(int nr, int nm, bool ylds) {
+    return sizeof(CountShared) + sizeof(CountWave) * kCntWaves + (ylds ? sizeof(float) * (size_t)nr * nm : 0);
+}
+
+// Counting kernel: the cells count_cell can take, all their rows at once. A block of
+// kCntWaves waves takes tc_count consecutive cells of one slice; each wave takes the
+// block's next cell until none is left and counts it alone. A cell that count_cell
+// declines goes to left_cells for the per-row kernel's list mode. Only for unit cells, no
+// scalar-field reward, no alpha columns and dt > 0 (the host checks).
+template <bool OBS, bool YLDS>
+__global__ __launch_bounds__(kCntWaves * 64) void mc_count(const BuildArgs A) {
+    constexpr int WG = kCntWaves * 64;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nr = A.nr, nm = A.nm, nc = A.nc, na = A.na;
+    CountShared& T = *reinterpret_cast<CountShared*>(smem_raw);
+    CountWave* waves = reinterpret_cast<CountWave*>(smem_raw + sizeof(CountShared));
+    float* ylds = reinterpret_cast<float*>(waves + kCntWaves);  // [nm][nr]
+    const int t = A.t0 + blockIdx.y;
+    // the action table, one entry per lane (kernel-argument arrays take uniform indices only)
+    for (int a = 0; a < na; a++)
+        if (tid == a) {
+            T.fcos[a] = A.act_fcos[a];
+            T.fsin[a] = A.act_fsin[a];
+            T.speed[a] = A.act_speed[a];
+        }
+    const int c_begin = blockIdx.x * A.tc_count, c_end = min(c_begin + A.tc_count, nc);
+    if (tid == 0) T.next_cell = (unsigned)c_begin;
+    const float* src = A.yi + (size_t)t * nr * nm;
+    if (YLDS)
+        for (int i = tid; i < nr * nm; i += WG) {
+            const int w = i / nm, k = i - w * nm;
+            ylds[k * nr + w] = src[i];
+        }
+    __syncthreads();
+    const bool obs_slice = A.slice_obs && (A.slice_obs[t] | A.slice_obs[t + 1]);
+    const bool late_slice = t > A.nt - 20;
+    unsigned long long oow_acc = 0, overflow_acc = 0, counted = 0;
+    uint32_t maxnuq_acc = 0;
+    CountGeo geo;
+    geo.xs0 = A.xs0;
+    geo.xsn = A.xsn;
+    geo.ys0 = A.ys0;
+    geo.ysn = A.ysn;
+    geo.dt = A.dt;
+    geo.XH = unit_index(A.xsn - A.xs0);
+    geo.YL = unit_index(A.ysn - A.ys0);
+    CountWave& Wv = waves[wv];
+    for (;;) {
+        unsigned cc = 0;
+        if (lane == 0) cc = atomicAdd(&T.next_cell, 1u);
+        const int c = __builtin_amdgcn_readfirstlane(__shfl((int)cc, 0, 64));  // wave-uniform
+        if (c >= c_end) break;
+        const bool done = count_cell<OBS, YLDS>(A, YLDS ? ylds : src, Wv, T, geo, t, c, late_slice, obs_slice,
+                                                oow_acc, overflow_acc, maxnuq_acc);
+        if (done)
+            counted++;
+        else if (lane == 0)
+            A.left_cells[atomicAdd(A.left_count, 1u)] = (uint32_t)((size_t)t * nc + c);
+    }
+    // rows were written by the last lane of their segment: the wave's totals
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        oow_acc += __shfl_xor(oow_acc, o, 64);
+        overflow_acc += __shfl_xor(overflow_acc, o, 64);
+        maxnuq_acc = max(maxnuq_acc, (uint32_t)__shfl_xor((int)maxnuq_acc, o, 64));
+    }
+    if (lane == 0) {
+        if (oow_acc) atomicAdd(A.stats + 0, oow_acc);
+        if (overflow_acc) atomicAdd(A.stats + 1, overflow_acc);
+        if (maxnuq_acc) atomicMax(A.stats + 2, (unsigned long long)maxnuq_acc);
+        if (counted) atomicAdd(A.stats + 8, counted);
     }
 }
 
@@ -1468,8 +2013,8 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     if ((st = dmalloc(dXY, sizeof(float) * 2 * g))) return fail(st);
     E2E_HIP(hipMemcpyAsync(dXY.p, xs.data(), sizeof(float) * g, hipMemcpyHostToDevice, s));
     E2E_HIP(hipMemcpyAsync(dXY.as<float>() + g, ys.data(), sizeof(float) * g, hipMemcpyHostToDevice, s));
-    if ((st = dmalloc(dStats, sizeof(unsigned long long) * 8))) return fail(st);
-    E2E_HIP(hipMemsetAsync(dStats.p, 0, sizeof(unsigned long long) * 8, s));
+    if ((st = dmalloc(dStats, sizeof(unsigned long long) * kStatSlots))) return fail(st);
+    E2E_HIP(hipMemsetAsync(dStats.p, 0, sizeof(unsigned long long) * kStatSlots, s));
     if (mask) {
         if ((st = dmalloc(dFlags, sizeof(int32_t) * nt))) return fail(st);
         E2E_HIP(hipMemsetAsync(dFlags.p, 0, sizeof(int32_t) * nt, s));
@@ -1555,6 +2100,8 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
                     : opts.paired_rounds == 1 ? 0
                     : (nw && (int)((160 * 1024) / wave_lds) * nw / 4 < 4 ? 1 : 0);
     m->info.paired_rounds = nw ? A.pipe_rounds : 0;
+    E2E_CHECK(opts.lane_moves == 0 || opts.lane_moves == 1, E2E_ERR_INVALID, "lane_moves must be 0 or 1 (got %d)",
+              opts.lane_moves);
     // alpha columns: rewards for n_alpha values of alpha from this one build
     E2E_CHECK(opts.n_alpha >= 0 && opts.n_alpha <= 64, E2E_ERR_INVALID, "n_alpha=%d not in 0..64", opts.n_alpha);
     A.n_alpha = 0;
@@ -1597,6 +2144,23 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
         A.defer_list = dDefer.as<uint32_t>();
         A.defer_count = dDcnt.as<unsigned>();
     }
+    // counted cells (mc_count) where the fast unit-cell pass applies to the whole model:
+    // unit cells, no scalar-field reward, no alpha columns, a forward time step
+    const bool uses_rad = sf && (p->reward_type == 2 || p->reward_type == 3);
+    // Yi of a slice in the counting block's LDS when it fits
+    const bool count_ylds = count_lds_bytes(nr, nm, true) <= 160 * 1024;
+    const size_t count_lds = count_lds_bytes(nr, nm, count_ylds);
+    const bool counting = nw > 0 && unit && !uses_rad && A.n_alpha == 0 && p->dt > 0.0f && !opts.lane_moves &&
+                          nr < 65536 && nm <= kCntModes;
+    A.tc_count = 160;
+    DevBuf dLeft, dLcnt;
+    if (counting) {
+        if ((st = dmalloc(dLeft, sizeof(uint32_t) * std::max<int64_t>((nt - 1) * nc, 1))) ||
+            (st = dmalloc(dLcnt, sizeof(unsigned))))
+            return fail(st);
+        A.left_cells = dLeft.as<uint32_t>();
+        A.left_count = dLcnt.as<unsigned>();
+    }
     phase("row buffers allocated");
     const int64_t nslices = nt - 1;
     const int64_t spl = opts.slices_per_launch > 0 ? opts.slices_per_launch : nslices;
@@ -1608,7 +2172,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     for (int attempt = 0; attempt < 4; attempt++) {
         if ((st = dmalloc(dArena, sizeof(uint2) * arena_cap))) break;
         E2E_HIP(hipMemsetAsync(dTop.p, 0, sizeof(unsigned long long), s));
-        E2E_HIP(hipMemsetAsync(dStats.p, 0, sizeof(unsigned long long) * 8, s));
+        E2E_HIP(hipMemsetAsync(dStats.p, 0, sizeof(unsigned long long) * kStatSlots, s));
         A.arena = dArena.as<uint2>();
         A.arena_top = dTop.as<unsigned long long>();
         A.arena_cap = arena_cap;
@@ -1616,20 +2180,15 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
         launches = 0;
         unsigned ndefer = 0;
         if (nw) E2E_HIP(hipMemsetAsync(dDcnt.p, 0, sizeof(unsigned), s));
-        for (int64_t t0 = 0; t0 < nslices; t0 += spl) {
-            const int64_t cnt = std::min(spl, nslices - t0);
-            A.t0 = (int)t0;
-            dim3 grid((unsigned)ceil_div(nc, A.tc), (unsigned)cnt);
+        if (counting) E2E_HIP(hipMemsetAsync(dLcnt.p, 0, sizeof(unsigned), s));
+        BuildFn wf = nw ? pick_wave(nw, any_obs, unit, A.n_alpha > 0) : nullptr;
+        if (nw && wave_lds > 64 * 1024)
+            E2E_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(wf), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)wave_lds));
+        // time one launch on the stream into move_ms
+        auto timed = [&](auto&& launch) -> e2e_status {
             E2E_HIP(hipEventRecord(e0, s));
-            if (nw) {
-                BuildFn wf = pick_wave(nw, any_obs, unit, A.n_alpha > 0);
-                if (wave_lds > 64 * 1024)
-                    E2E_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(wf),
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)wave_lds));
-                hipLaunchKernelGGL(wf, grid, dim3(nw * 64), wave_lds, s, A);
-            } else if ((st = dispatch_build(A, wg, any_obs, unit, grid, lds, s))) {
-                break;
-            }
+            E2E_TRY(launch());
             E2E_HIP(hipGetLastError());
             E2E_HIP(hipEventRecord(e1, s));
             E2E_HIP(hipEventSynchronize(e1));
@@ -1637,28 +2196,60 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
             E2E_HIP(hipEventElapsedTime(&ms, e0, e1));
             move_ms += ms;
             launches++;
+            return E2E_OK;
+        };
+        for (int64_t t0 = 0; t0 < nslices; t0 += spl) {
+            const int64_t cnt = std::min(spl, nslices - t0);
+            A.t0 = (int)t0;
+            dim3 grid((unsigned)ceil_div(nc, A.tc), (unsigned)cnt);
+            if (counting) {
+                BuildFn cf = any_obs ? (count_ylds ? mc_count<true, true> : mc_count<true, false>)
+                                     : (count_ylds ? mc_count<false, true> : mc_count<false, false>);
+                if (count_lds > 64 * 1024)
+                    E2E_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(cf),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)count_lds));
+                const dim3 cgrid((unsigned)ceil_div(nc, A.tc_count), (unsigned)cnt);
+                st = timed([&] {
+                    hipLaunchKernelGGL(cf, cgrid, dim3(kCntWaves * 64), count_lds, s, A);
+                    return E2E_OK;
+                });
+            } else if (nw) {
+                st = timed([&] {
+                    hipLaunchKernelGGL(wf, grid, dim3(nw * 64), wave_lds, s, A);
+                    return E2E_OK;
+                });
+            } else {
+                st = timed([&] { return dispatch_build(A, wg, any_obs, unit, grid, lds, s); });
+            }
+            if (st) break;
         }
         if (st) break;
+        if (counting) {  // the cells mc_count left: the per-row kernel in list mode
+            unsigned nleft = 0;
+            E2E_HIP(hipMemcpyAsync(&nleft, dLcnt.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+            E2E_HIP(hipStreamSynchronize(s));
+            if (nleft) {
+                A.cell_list = dLeft.as<uint32_t>();
+                st = timed([&] {
+                    hipLaunchKernelGGL(wf, dim3(nleft), dim3(nw * 64), wave_lds, s, A);
+                    return E2E_OK;
+                });
+                A.cell_list = nullptr;
+                if (st) break;
+            }
+        }
         if (nw) {  // rows the per-row kernel handed over: the block-level kernel in list mode
             E2E_HIP(hipMemcpyAsync(&ndefer, dDcnt.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
             E2E_HIP(hipStreamSynchronize(s));
             if (ndefer) {
                 A.seg_list = dDefer.as<uint32_t>();
-                E2E_HIP(hipEventRecord(e0, s));
-                st = dispatch_build(A, wg, any_obs, unit, dim3(ndefer), lds, s);
+                st = timed([&] { return dispatch_build(A, wg, any_obs, unit, dim3(ndefer), lds, s); });
                 A.seg_list = nullptr;
                 if (st) break;
-                E2E_HIP(hipGetLastError());
-                E2E_HIP(hipEventRecord(e1, s));
-                E2E_HIP(hipEventSynchronize(e1));
-                float ms = 0.f;
-                E2E_HIP(hipEventElapsedTime(&ms, e0, e1));
-                move_ms += ms;
-                launches++;
             }
         }
         m->info.n_deferred_rows = (int32_t)ndefer;
-        unsigned long long hs[8];
+        unsigned long long hs[kStatSlots];
         E2E_HIP(hipMemcpyAsync(hs, dStats.p, sizeof(hs), hipMemcpyDeviceToHost, s));
         E2E_HIP(hipStreamSynchronize(s));
         if (hs[3]) {
@@ -1686,6 +2277,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
         m->info.n_overflow_rows = (int64_t)hs[1];
         m->info.max_row_nnz = (int64_t)hs[2];
         m->info.n_multipass_rows = (int64_t)hs[5];
+        m->info.counted_cells = (int32_t)hs[8];
         if (!hs[4]) break;
         // arena exhausted: grow and rebuild
         unsigned long long used = 0;

@@ -106,7 +106,10 @@ typedef struct e2e_build_opts {
     int32_t n_alpha;           /* > 0: also build the rewards for these n_alpha values of alpha (at most 64;
                                   reward_type 4 only): the alpha sweep of src/my_script.sh:10-19 from one
                                   build, its transitions being the same for every alpha */
-    int32_t reserved;
+    int32_t lane_moves;        /* per-row kernel: 0 counts every cell the fast unit-cell pass can take at once
+                                  (each realization placed among all actions' landing-cell boundaries,
+                                  the same counts and rewards, bit for bit); 1 moves every (state, action,
+                                  realization) lane; other values are an error */
     const float* alphas;       /* [n_alpha] host array, each alpha as the reference forms it
                                   (stof(argv[1]) / 100, src/build_model.cu:937-939) */
 } e2e_build_opts;
@@ -134,7 +137,7 @@ typedef struct e2e_model_info {
     int32_t wave_blocks;       /* wavefronts per block of the per-row kernel (0: not used) */
     int32_t n_deferred_rows;   /* rows the per-row kernel handed to the block-level kernel */
     int32_t paired_rounds;     /* 1: the per-row kernel ran its rounds in pairs */
-    int32_t reserved;
+    int32_t counted_cells;     /* (slice, cell) pairs whose rows came from one count over the cell's samples */
 } e2e_model_info;
 
 typedef struct e2e_solve_opts {

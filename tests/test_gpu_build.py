@@ -70,20 +70,25 @@ CASES = ["tiny", "small", "obstacles_tsg2", "reward0", "reward1", "reward2", "re
          "cells_0p7", "cells_2_wg512", "wg1024"]
 
 
-# count_kernel 0: one wavefront per row, rows with wide S2 spreads handed to the
-# block-level kernel, its rounds one at a time (paired_rounds 1) or in pairs (2);
-# count_kernel 1: the block-level kernel (radix sort, forced block sizes) for all
-KERNELS = {"row": (0, 1), "row_paired": (0, 2), "block": (1, 0)}
+# count_kernel 0: counted cells (every cell the fast unit-cell pass can take, all its
+# rows from one count over its samples), the rest one wavefront per row, rows with wide
+# S2 spreads handed to the block-level kernel; lane_moves 1: no counted cells, every
+# lane moved, the per-row kernel's rounds one at a time (paired_rounds 1) or in pairs
+# (2); count_kernel 1: the block-level kernel (radix sort, forced block sizes) for all
+KERNELS = {"count": (0, 1, 0), "row": (0, 1, 1), "row_paired": (0, 2, 1), "block": (1, 0, 0)}
+# cases whose cells the counting kernel takes (unit cells, integer rewards)
+COUNTED = {"tiny", "small", "obstacles_tsg2", "reward0", "multipass", "radix2_overflow", "overflow_k1",
+           "fast_flow_oow", "many_rounds", "wg1024"}
 
 
 @pytest.mark.parametrize("kname", list(KERNELS))
 @pytest.mark.parametrize("name", CASES)
 def test_build_bit_exact(gpu, name, kname):
-    kernel, paired = KERNELS[kname]
+    kernel, paired, lanes = KERNELS[kname]
     p, f, opts = _case(name)
     print(f"[case {name} kernel {kname}] g={p.gsize} nt={p.nt} nr={p.nrzns} opts={opts}", flush=True)
     ref = oracle_model(p, f)
-    with e2e.build_model(p, f, count_kernel=kernel, paired_rounds=paired, **opts) as m:
+    with e2e.build_model(p, f, count_kernel=kernel, paired_rounds=paired, lane_moves=lanes, **opts) as m:
         assert_same_model(m.export_coo(), ref, name)
         info = m.info
         assert info["n_out_of_window"] == ref["oow"]
@@ -96,8 +101,9 @@ def test_build_bit_exact(gpu, name, kname):
             assert info["n_overflow_rows"] > 0
         if name == "fast_flow_oow":
             assert info["n_out_of_window"] > 0
-            if kernel == 0:
+            if kernel == 0 and lanes:
                 assert info["n_deferred_rows"] > 0  # spreads beyond the per-row kernel
+        assert (info["counted_cells"] > 0) == (kname == "count" and name in COUNTED)
         assert info["unit_cells"] == (p.dx == 1.0 and p.dy == 1.0)
         if "block_threads" in opts:
             if kernel == 1:
@@ -109,16 +115,18 @@ def test_build_bit_exact(gpu, name, kname):
         assert (S1 == ref["S1"]).all() and (S2 == ref["S2"]).all()
 
 
-@pytest.mark.parametrize("kernel", [0, 1])
-def test_build_c1_bit_exact(gpu, kernel):
+@pytest.mark.parametrize("kernel,lanes", [(0, 0), (0, 1), (1, 0)])
+def test_build_c1_bit_exact(gpu, kernel, lanes):
     w = workload("C1")
     f = w.fields()
     ref = oracle_model(w.params, f)
-    with e2e.build_model(w.params, f, count_kernel=kernel) as m:
+    with e2e.build_model(w.params, f, count_kernel=kernel, lane_moves=lanes) as m:
         assert_same_model(m.export_coo(), ref, "C1")
         assert m.info["lanes"] == w.lanes
         if kernel == 0:
             assert m.info["n_deferred_rows"] == 0
+        # every C1 cell but the terminal one is counted
+        assert m.info["counted_cells"] == (0 if kernel or lanes else (w.params.nt - 1) * (w.params.nc - 1))
 
 
 def test_build_deterministic_and_device_fields(gpu):
