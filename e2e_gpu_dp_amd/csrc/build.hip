@@ -877,14 +877,16 @@ __device__ __forceinline__ float wave_fmax(float x) {
 // writing anything, when the cell is left to that pass: a non-finite sample, an
 // obstacle within the fixed window, a row without exact integer rewards, an action with
 // more than kCntMax boundaries on an axis. Y: the slice's Yi, [nm][nr] (YLDS, in LDS)
-// or [nr][nm] (global).
-template <bool OBS, bool YLDS>
+// or [nr][nm] (global). NM: the number of DO modes when fixed at compile time (the mode
+// sums fully unrolled), 0 for any number up to kCntModes. OBS: the model has obstacles.
+template <bool YLDS, int NM>
 __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __restrict__ Y, CountWave& Wv,
                                            const CountShared& T, CountGeo& G, int t, int c, bool late_slice,
                                            bool obs_slice, unsigned long long& oow_acc,
                                            unsigned long long& overflow_acc, uint32_t& maxnuq_acc) {
     const int lane = threadIdx.x & 63;
-    const int na = A.na, nr = A.nr, nm = A.nm, g = A.g, nc = A.nc;
+    const int na = A.na, nr = A.nr, nm = NM ? NM : A.nm, g = A.g, nc = A.nc;
+    const bool OBS = A.slice_obs != nullptr;
     const int i1 = c / g, j1 = c - i1 * g;
     const size_t tslice = (size_t)t * nc, mid = tslice + c;
     if (is_terminal(A, i1, j1) || (A.mask && A.mask[mid] == 1)) return false;
@@ -903,7 +905,7 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
     // the cell's mode coefficients, once (scalar registers)
     float cu[kCntModes], cv[kCntModes];
 #pragma unroll
-    for (int k = 0; k < kCntModes; k++) {
+    for (int k = 0; k < (NM ? NM : kCntModes); k++) {
         const size_t o = ((size_t)t * nm + (k < nm ? k : 0)) * nc + c;
         cu[k] = A.ui[o];
         cv[k] = A.vi[o];
@@ -913,8 +915,8 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
     auto sample = [&](int w) __attribute__((always_inline)) {
         float sx = 0.0f, sy = 0.0f;
 #pragma unroll
-        for (int k = 0; k < kCntModes; k++) {
-            if (k < nm) {
+        for (int k = 0; k < (NM ? NM : kCntModes); k++) {
+            if (NM || k < nm) {
                 const float y = YLDS ? Y[k * nr + w] : Y[w * nm + k];
                 sx = sx + cu[k] * y;
                 sy = sy + cv[k] * y;
@@ -1622,7 +1624,7 @@ __host__ __device__ inline size_t count_lds_bytes(int nr, int nm, bool ylds) {
 // block's next cell until none is left and counts it alone. A cell that count_cell
 // declines goes to left_cells for the per-row kernel's list mode. Only for unit cells, no
 // scalar-field reward, no alpha columns and dt > 0 (the host checks).
-template <bool OBS, bool YLDS>
+template <bool YLDS, int NM>
 __global__ __launch_bounds__(kCntWaves * 64) void mc_count(const BuildArgs A) {
     constexpr int WG = kCntWaves * 64;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1667,7 +1669,7 @@ __global__ __launch_bounds__(kCntWaves * 64) void mc_count(const BuildArgs A) {
         if (lane == 0) cc = atomicAdd(&T.next_cell, 1u);
         const int c = __builtin_amdgcn_readfirstlane(__shfl((int)cc, 0, 64));  // wave-uniform
         if (c >= c_end) break;
-        const bool done = count_cell<OBS, YLDS>(A, YLDS ? ylds : src, Wv, T, geo, t, c, late_slice, obs_slice,
+        const bool done = count_cell<YLDS, NM>(A, YLDS ? ylds : src, Wv, T, geo, t, c, late_slice, obs_slice,
                                                 oow_acc, overflow_acc, maxnuq_acc);
         if (done)
             counted++;
@@ -1837,6 +1839,12 @@ BuildFn pick_wave(int nw, bool obs, bool unit, bool alpha) {
 // blocks, and 4-wave blocks measured fastest on both the 25x25 and the 100x100 workloads
 // (100x100 x nt=120: 108 ms against 121 ms with 8 waves and 254 ms with 16). 0 when even
 // that does not fit the LDS (velocity samples of nrzns realizations).
+// The counting kernel for Yi in LDS or not, its mode sums unrolled for 4, 5 or 6 modes.
+BuildFn pick_count(bool ylds, int nm) {
+    if (ylds) return nm == 4 ? mc_count<true, 4> : nm == 5 ? mc_count<true, 5> : nm == 6 ? mc_count<true, 6> : mc_count<true, 0>;
+    return nm == 4 ? mc_count<false, 4> : nm == 5 ? mc_count<false, 5> : nm == 6 ? mc_count<false, 6> : mc_count<false, 0>;
+}
+
 int choose_nw(int nr, int nm, bool stage_y) { return wave_lds_bytes(4, nr, nm, stage_y) <= 160 * 1024 ? 4 : 0; }
 
 e2e_status dispatch_build(const BuildArgs& A, int wg, bool obs, bool unit, dim3 grid, size_t lds,
@@ -2203,8 +2211,7 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
             A.t0 = (int)t0;
             dim3 grid((unsigned)ceil_div(nc, A.tc), (unsigned)cnt);
             if (counting) {
-                BuildFn cf = any_obs ? (count_ylds ? mc_count<true, true> : mc_count<true, false>)
-                                     : (count_ylds ? mc_count<false, true> : mc_count<false, false>);
+                BuildFn cf = pick_count(count_ylds, nm);
                 if (count_lds > 64 * 1024)
                     E2E_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(cf),
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)count_lds));
