@@ -63,6 +63,28 @@ def _pmc_traffic(workload: str, kernel: str):
     return k["traffic_bytes"] if k else None
 
 
+def _build_kernel(info: dict) -> str:
+    """The kernel that carries a build: the counting kernel when it took the cells, else
+    the per-row wavefront kernel, else the block-level kernel."""
+    if info["counted_cells"] > 0:
+        return "mc_count"
+    return "mc_wave" if info["wave_blocks"] > 0 else "mc_move_count"
+
+
+def _build_roofline(kernel: str, tflops: float, flops_per_lane: int, traffic):
+    """The build's rate in the reference's per-lane flops (the velocity sample's 2 nm
+    mul+add, 2 mean adds and the move's flops, for every (state, action, realization)
+    lane), over the move/count kernels' time. The counting kernel reaches the same counts
+    without executing those flops lane by lane (one sample per realization and cell,
+    placed among all actions' landing boundaries), so for it this is a lane-equivalent
+    rate; its executed VALU issue is in profiles/c2_count_kernel_counters.md."""
+    r = {"kernel": kernel, "bound": "valu", "achieved": tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+         "frac": tflops / PEAK_FP32_TFLOPS, "traffic": traffic, "flops_per_lane": flops_per_lane}
+    if kernel == "mc_count":
+        r["flops"] = "lane-equivalent: the per-lane flops of the moves the counting replaces"
+    return r
+
+
 def _threads() -> int:
     """Host threads of every CPU leg: OMP_NUM_THREADS when set (the CPU share a GPU box
     gives one GPU), else every core this process may run on."""
@@ -566,9 +588,8 @@ def _c4(e2e, np, torch, thresh, cpu: bool, cpu_budget: float):
         "full_sweeps": {"sweep_kernel_ms": pf, "bytes_per_sweep": sf["bytes_per_sweep"], "gbps": gbps,
                         "same_bits": bool(np.array_equal(vf.view(np.uint32), va.view(np.uint32))
                                           and sf["sweeps"] == sa["sweeps"])},
-        "roofline": {"kernel": "mc_wave", "bound": "valu", "achieved": tflops, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": tflops / PEAK_FP32_TFLOPS, "traffic": _pmc_traffic("C4", "mc_wave"),
-                     "flops_per_lane": fl},
+        "counted_cells": info["counted_cells"],
+        "roofline": _build_roofline(_build_kernel(info), tflops, fl, _pmc_traffic("C4", _build_kernel(info))),
         "roofline_vi": {"kernel": "spmv_vi_lean", "bound": "hbm", "achieved": gbps, "peak": PEAK_HBM_GBPS,
                         "unit": "GB/s", "frac": gbps / PEAK_HBM_GBPS,
                         "traffic": _pmc_traffic("C4", "spmv_vi_lean")},
@@ -674,12 +695,11 @@ def main(argv=None):
     ms_per_step = elapsed * 1e3 / args.steps
     lanes = infos[0][0]["lanes"]
 
-    # summed HIP-event time of every move/count launch of a build (one launch for a build
-    # with no deferred rows): all lanes over all of that time
+    # summed HIP-event time of every move/count launch of a build (the counting kernel,
+    # then the per-row kernel over the cells it left): all lanes over all of that time
     move_ms = float(np.mean([i["move_kernel_ms"] for i, _ in infos]))
     move_launches = infos[0][0]["move_launches"]
-    # the per-row wavefront kernel, or the block-level kernel when the build used only that
-    build_kernel = "mc_wave" if infos[0][0]["wave_blocks"] > 0 else "mc_move_count"
+    build_kernel = _build_kernel(infos[0][0])
     flops_per_lane = 4 * p.nmodes + 18
     build_tflops = lanes * flops_per_lane / (move_ms * 1e-3) / 1e12
     build_ms = float(np.mean([i["build_ms"] for i, _ in infos]))
@@ -697,12 +717,10 @@ def main(argv=None):
                    "states": p.ns, "nnz": infos[0][0]["nnz"], "lanes": lanes},
         "build": {"transitions_per_s": lanes / (build_ms * 1e-3), "build_ms": build_ms,
                   "move_kernel_ms": move_ms, "move_launches": move_launches, "scan_ms": infos[0][0]["scan_ms"],
-                  "csr_ms": infos[0][0]["csr_ms"]},
-        "roofline": {"kernel": build_kernel, "bound": "valu", "achieved": build_tflops,
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": build_tflops / PEAK_FP32_TFLOPS,
-                     "traffic": _pmc_traffic(args.workload, build_kernel),
-                     "flops_per_lane": flops_per_lane},
+                  "csr_ms": infos[0][0]["csr_ms"], "counted_cells": infos[0][0]["counted_cells"],
+                  "cells": (p.nt - 1) * p.nc},
+        "roofline": _build_roofline(build_kernel, build_tflops, flops_per_lane,
+                                    _pmc_traffic(args.workload, build_kernel)),
     }
     if not args.skip_vi:
         st = infos[-1][1]
