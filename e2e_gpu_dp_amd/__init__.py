@@ -374,9 +374,9 @@ class Server:
 
     def __init__(self, hbm_budget_bytes: int = 0, max_cols: int = 64, sweeps_per_step: int = 4,
                  keep_full_results: bool = False, serial: bool = False, thresh: float = 1e-6,
-                 exact_pass: bool = False):
+                 exact_pass: bool = False, max_models: int = 0):
         o = _abi.ServerOpts(hbm_budget_bytes, max_cols, sweeps_per_step, int(keep_full_results),
-                            int(serial), thresh, int(exact_pass))
+                            int(serial), thresh, int(exact_pass), max_models)
         h = C.c_void_p()
         check(lib().e2e_server_create(C.byref(o), C.byref(h)))
         self._h = h

@@ -94,7 +94,8 @@ class BatchStats(C.Structure):
 class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_cols", C.c_int32),
                 ("sweeps_per_step", C.c_int32), ("keep_full_results", C.c_int32),
-                ("serial", C.c_int32), ("thresh", C.c_float), ("exact_pass", C.c_int32)]
+                ("serial", C.c_int32), ("thresh", C.c_float), ("exact_pass", C.c_int32),
+                ("max_models", C.c_int32)]
 
 
 class RequestResult(C.Structure):

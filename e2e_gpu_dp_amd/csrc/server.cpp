@@ -15,7 +15,9 @@
 //     after evicting idle models (no live columns) in least-recently-used order;
 //   * every model steps on its own stream, at its own pace (a model's next step is
 //     launched as soon as its previous one has finished and been harvested), so builds
-//     and the sweeps of different models overlap on the device.
+//     overlap the sweeps on the device; the models whose steps run at once are the
+//     max_models (default 1) that hold the oldest running requests, so the oldest request
+//     is not slowed by the sweeps of models whose requests came later.
 // Column arithmetic does not depend on the other columns or on when a column joined,
 // so a request's result is bit-identical alone and in the mix.
 #include <hip/hip_runtime.h>
@@ -88,6 +90,7 @@ struct Env {
     std::vector<int> steps_since_admit;        // per slot, sweeps launched since admission
     hipEvent_t step_done = nullptr;            // recorded after the launches of a step
     bool inflight = false;                     // a step is on the GPU (not yet harvested)
+    bool may_launch = true;                    // among the models that run steps now (max_models)
 };
 
 }  // namespace
@@ -279,6 +282,31 @@ struct Server {
             const int spc = o.sweeps_per_step > 0 ? o.sweeps_per_step : 4;
             int64_t nnz_max = 1;
             for (Env* e : active) nnz_max = std::max<int64_t>(nnz_max, e->model->nnz);
+            // oldest running request first: only the max_models models holding the oldest
+            // running requests launch steps (the device has no stream priorities to give
+            // the oldest request its share, so the host limits what runs beside it)
+            const int max_models = o.max_models == 0 ? 1 : o.max_models;
+            if (max_models > 0 && (int)active.size() > max_models) {
+                std::vector<std::pair<double, Env*>> age;
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    for (Env* e : active) {
+                        double oldest = 1e300;
+                        for (uint64_t id : e->slot) {
+                            const auto it = id ? reqs.find(id) : reqs.end();
+                            if (it != reqs.end()) oldest = std::min(oldest, it->second->t_arrival);
+                        }
+                        age.emplace_back(oldest, e);
+                    }
+                }
+                std::stable_sort(age.begin(), age.end(),
+                                 [](const std::pair<double, Env*>& x, const std::pair<double, Env*>& y) {
+                                     return x.first < y.first;
+                                 });
+                for (size_t i = 0; i < age.size(); i++) age[i].second->may_launch = (int)i < max_models;
+            } else {
+                for (Env* e : active) e->may_launch = true;
+            }
             bool progressed = false;
             for (Env* e : active) {
                 if (e->inflight) {
@@ -293,7 +321,7 @@ struct Server {
                     std::lock_guard<std::mutex> lk(mu);
                     running = e->live > 0;
                 }
-                if (!running || !e->batch) continue;
+                if (!running || !e->batch || !e->may_launch) continue;
                 const int64_t ratio = nnz_max / std::max<int64_t>(e->model->nnz, 1);
                 const int sw = o.exact_pass ? 1 : (int)std::min<int64_t>((int64_t)spc * 16, (int64_t)spc * std::max<int64_t>(ratio, 1));
                 std::vector<int32_t> cols, goals;

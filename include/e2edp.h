@@ -228,6 +228,8 @@ typedef struct e2e_server_opts {
     int32_t exact_pass;        /* 1: each scheduler step runs one exact backward pass over a model's
                                   running columns (one launch per slice, last to first), which ends
                                   every one of them at its fixed point; 0: sweeps to thresh */
+    int32_t max_models;        /* models whose steps run at once: the ones holding the oldest running
+                                  requests (0: 1; -1: every model with running requests) */
 } e2e_server_opts;
 
 typedef struct e2e_request_result {

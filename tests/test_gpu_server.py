@@ -25,10 +25,11 @@ def _solo(params, fields, goal):
     return pol[0], val[0], sw[0]
 
 
-def test_server_mix_equals_solo(gpu):
+@pytest.mark.parametrize("max_models", [0, -1])  # the oldest requests' model first; every model at once
+def test_server_mix_equals_solo(gpu, max_models):
     envs = _envs()
     rng = np.random.default_rng(7)
-    with e2e.Server(keep_full_results=True, thresh=1e-6, sweeps_per_step=3) as srv:
+    with e2e.Server(keep_full_results=True, thresh=1e-6, sweeps_per_step=3, max_models=max_models) as srv:
         for eid, (p, f) in envs.items():
             srv.add_env(eid, p, f)
         reqs = []
