@@ -781,7 +781,8 @@ struct CountShared {
 
 __device__ __forceinline__ uint32_t fkey(float f) {
     const uint32_t b = __float_as_uint(f);
-    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    // negative: all bits flipped; else the sign bit set
+    return b ^ ((uint32_t)((int32_t)b >> 31) | 0x80000000u);
 }
 __device__ __forceinline__ float kfloat(uint32_t k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
@@ -1009,15 +1010,18 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
         __builtin_amdgcn_wave_barrier();
 
         // every realization's cell of the boundary grid, two rounds at a time
+        // rank #{u <= key} by bisection over the padded 64 keys, kept as a byte offset
+        const char* ux = reinterpret_cast<const char*>(Wv.u[0]);
+        const char* uy = reinterpret_cast<const char*>(Wv.u[1]);
         auto place = [&](float2 v) __attribute__((always_inline)) {
             const uint32_t kx = fkey(v.x), ky = fkey(v.y);
-            int Ix = 0, Iy = 0;
+            uint32_t ox = 0u, oy = 0u;
 #pragma unroll
             for (int h = 32; h >= 1; h >>= 1) {
-                Ix += Wv.u[0][Ix + h - 1] <= kx ? h : 0;
-                Iy += Wv.u[1][Iy + h - 1] <= ky ? h : 0;
+                ox |= *reinterpret_cast<const uint32_t*>(ux + ox + 4 * (h - 1)) <= kx ? 4u * h : 0u;
+                oy |= *reinterpret_cast<const uint32_t*>(uy + oy + 4 * (h - 1)) <= ky ? 4u * h : 0u;
             }
-            return (Iy + 1) * kCntPitch + Ix + 1;
+            return (int)((oy >> 2) + 1) * kCntPitch + (int)(ox >> 2) + 1;
         };
         for (int w = lane; w < nr; w += 128) {
             const bool two = w + 64 < nr;
