@@ -62,13 +62,20 @@ __host__ __device__ inline int active_slices(uint32_t k, int nt) {
 constexpr int kTicketGroup = 64;
 constexpr int kTicketGroups = 32;
 
+// Words between the lean sweep's counters: one 128-byte cache line each, since atomics on
+// one line serialize
+constexpr int kCounterStride = 32;
+
 // Device-side solver state of one solve (one per model).
 struct VIState {
     uint32_t k;          // 1-based index of the next sweep
     int32_t conv;        // sweep at which the residual first fell below thresh, -1 if none
     uint32_t done;       // blocks (lean sweep: block groups) of the current sweep that finished (ticket)
     float last_res;      // residual of the last completed sweep
-    uint32_t gdone[kTicketGroups];  // lean sweep: blocks of each group of kTicketGroup that finished
+    // lean sweep, at [i * kCounterStride]: blocks of group i of kTicketGroup that finished
+    uint32_t gdone[kTicketGroups * kCounterStride];
+    // lean sweep, at [x * kCounterStride]: next tail piece of XCD x's pool (0 between sweeps)
+    uint32_t pool[8 * kCounterStride];
 };
 
 // Per-column state of a batched (multi-goal) solve.

@@ -25,6 +25,7 @@
 // (spvi_2.cu:669-698).
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -65,8 +66,12 @@ struct VIArgs {
     int one_pass;           // lean sweep as one slice of the backward pass: no residual, no sweep state
     uint32_t pre_nt;        // lean sweep over a built model's active slices: nt (0: every state, every sweep)
     uint32_t pre_stride;    // active slices: waves + 1, the stride of wpart's per-prefix partitions
+    uint32_t pool_parts;    // lean sweep: tail pieces per wave taken from a per-XCD pool (0: none)
+    uint32_t head16;        // lean sweep with pool_parts: sixteenths of each range its wave sweeps itself
 };
 
+// bytes of VIState the host reads back: k, conv, done, last_res (not the counters)
+constexpr size_t kVIStateHead = offsetof(VIState, gdone);
 constexpr int kVIThreads = 512;   // 8 waves per block
 constexpr int kVIWaves = kVIThreads / 64;
 constexpr int kVIMaxBlocks = 2048;  // 8 blocks of 512 threads per CU on 256 CUs
@@ -398,21 +403,203 @@ __global__ void vi_partition_slices(const uint32_t* __restrict__ rp, int64_t nc,
 }
 
 // ---------------------------------------------------------------------------------
-// Lean streaming sweep (the default). Same work split as spmv_vi_stream (one contiguous
-// state range per wave, balanced by entries + rows, entries read as one stream in
-// windows of 64*U entries), built for occupancy: at most 64 VGPRs, so 8 waves per SIMD
-// keep enough loads in flight without software pipelining (a gather-bound stream runs
-// about 1.5x faster at 8 waves per SIMD than at 4). Wave-uniform values live in SGPRs
+// Lean streaming sweep (the default). One contiguous state range per wave, balanced by
+// entries + rows, its entries read as one stream in windows of 64*U entries; part of each
+// range goes to a per-XCD pool of pieces that the XCD's waves take as they finish (the
+// kernel below). Built for occupancy: at most 64 VGPRs, so 7 or 8 waves per SIMD keep
+// enough loads in flight without software pipelining (a gather-bound stream runs about
+// 1.5x faster at 8 waves per SIMD than at 4). Wave-uniform values live in SGPRs
 // (readfirstlane, scalar loads of the read-only partition and row pointers). The next
 // group's row data is requested when a group starts. Products and sums in float64.
+//
+// One wave's part of a lean sweep: the states [s_begin, s_end) (s_begin < s_end), their
+// entries read as one stream in windows of 64*U entries, NAP lanes per state and 64/NAP
+// states per group. Writes Vout, pi of those states and raises res to their residual.
+template <int NAP, int U, bool COMP>
+__device__ __forceinline__ void lean_range(const VIArgs& A, const double* __restrict__ Vin,
+                                           double* __restrict__ Vout, double* wsd, const float* ptabf,
+                                           uint32_t s_begin, uint32_t s_end, float& res) {
+    constexpr int SPC = 64 / NAP;
+    constexpr int WIN = 64 * U;
+    // lane is made opaque per call, so that what derives from it is recomputed per range
+    // rather than held in registers across the ranges of a wave (64 VGPRs: 8 waves per SIMD)
+    uint32_t tx = threadIdx.x;
+    asm volatile("" : "+v"(tx));
+    const int lane = (int)(tx & 63u);
+    const int a = lane % NAP, sl = lane / NAP;
+    const uint32_t na = (uint32_t)A.na;
+    const CPtr<uint32_t> rps = (CPtr<uint32_t>)A.rp;  // uniform reads of the row pointers
+    const uint32_t ngroups = (s_end - s_begin + SPC - 1) / SPC;
+    const uint32_t Pb = rps[s_begin * na], Pend = rps[s_end * na];
+    // COMP: windows are the stream's aligned 256-entry blocks (one base column each);
+    // entries before Pb or from Pend on are staged but belong to no row of this wave
+    const uint32_t P0 = COMP ? Pb / WIN * WIN : Pb;
+    const uint32_t nwin = Pend > Pb ? (Pend - P0 + WIN - 1) / WIN : 0u;
+    struct Meta {
+        uint32_t rb, re;
+        float rr;
+        double vold;
+    };
+    // row data of group g (clamped to the wave's last group, so always in range)
+    auto load_meta = [&](uint32_t g, Meta& M) {
+        const uint32_t s = min(s_begin + g * SPC + (uint32_t)sl, s_end - 1u);
+        const uint32_t row = s * na + min((uint32_t)a, na - 1u);
+        M.rb = ld_off(A.rp, row);
+        M.re = ld_off(A.rp, row + 1u);
+        M.rr = ld_off(A.R, row);
+        M.vold = ld_off(Vin, s);
+    };
+    Meta cur, nxt;
+    uint32_t g = 0;
+    load_meta(0, cur);
+    load_meta(min(1u, ngroups - 1u), nxt);
+    uint32_t ce = rps[min(s_begin + SPC, s_end) * na];
+    bool valid = (s_begin + (uint32_t)sl < s_end) && ((uint32_t)a < na);
+    double sd = 0.0;
+
+    auto finish_group = [&]() {
+        double q = valid ? (double)cur.rr + (double)A.discount * sd : -INFINITY;
+        if (q != q) q = -INFINITY;
+        int ba = valid ? a : 0x7fffffff;
+        // max / first argmax over the NAP lanes of each state; the combine is
+        // associative and commutative, so any reduction tree gives the same result
+        if (NAP <= 16) {
+            // DPP row shifts: after shifts 1, 2, .., NAP/2 the state's last lane
+            // (a = NAP-1) has combined exactly its own NAP lanes
+#pragma unroll
+            for (int o = 1; o < NAP; o <<= 1) {
+                const double q2 = dpp_row_shr(q, o, (double)-INFINITY);
+                const int a2 = dpp_row_shr(ba, o, 0x7fffffff);
+                const bool take = q2 > q || (q2 == q && a2 < ba);
+                q = take ? q2 : q;
+                ba = take ? a2 : ba;
+            }
+        } else {
+#pragma unroll
+            for (int o = 1; o < NAP; o <<= 1) {
+                const double q2 = __shfl_xor(q, o, 64);
+                const int a2 = __shfl_xor(ba, o, 64);
+                if (q2 > q || (q2 == q && a2 < ba)) {
+                    q = q2;
+                    ba = a2;
+                }
+            }
+        }
+        const bool writer = (NAP <= 16) ? (a == NAP - 1) : (a == 0);
+        if (writer && g < ngroups && s_begin + g * SPC + (uint32_t)sl < s_end) {
+            const uint32_t s = s_begin + g * SPC + (uint32_t)sl;
+            double vnew;
+            int32_t pol;
+            if (q > -1e6) {
+                vnew = q;
+                pol = ba;
+            } else {
+                vnew = -1e6;
+                pol = -1;
+            }
+            Vout[s] = vnew;
+            A.pi[s] = pol;
+            res = fmaxf(res, res_up(fabs(vnew - cur.vold)));
+        }
+        sd = 0.0;
+        g++;
+        cur = nxt;
+        load_meta(min(g + 1u, ngroups - 1u), nxt);
+        const uint32_t s0 = s_begin + g * SPC;
+        ce = rps[min(s0 + SPC, s_end) * na];
+        valid = (g < ngroups) && (s0 + (uint32_t)sl < s_end) && ((uint32_t)a < na);
+    };
+
+    const int2* __restrict__ nzp = A.nz + P0 + lane;
+    const CPtr<int32_t> cbs = (CPtr<int32_t>)A.cbase;
+    // compressed windows: the next window's entries are requested as soon as this
+    // window's value gathers are (ecur holds them when has_next)
+    uint32_t ecur[U];
+    bool has_next = false;
+    auto load_comp = [&](uint32_t w, uint32_t* dst) {
+#pragma unroll
+        for (int u = 0; u < U; u++) dst[u] = A.cz[w + (uint32_t)(u * 64 + lane)];
+    };
+    if (COMP && nwin > 0 && cbs[P0 / WIN] >= 0) {
+        load_comp(P0, ecur);
+        has_next = true;
+    }
+    for (uint32_t kw = 0; kw < nwin; kw++) {
+        const uint32_t w0 = P0 + kw * WIN, wend = w0 + WIN;
+        double x[U], pv[U];
+        const int32_t base = COMP ? cbs[w0 / WIN] : -1;
+        const bool next_comp = COMP && kw + 1 < nwin && cbs[w0 / WIN + 1] >= 0;
+        if (COMP && base >= 0) {
+            // 4-byte entries w0 + 64 u + lane: the probability count / nrzns is
+            // rounded to float from the double count * (1 / nrzns) (the same float as
+            // the division: that product is within 2^-52 of count / nrzns, which for
+            // count < 2^16 lies at least 2^-41 away from any rounding midpoint of
+            // float); the window was kept compressed only if every entry decodes to
+            // its stored bits
+            uint32_t e[U];
+            if (has_next) {
+#pragma unroll
+                for (int u = 0; u < U; u++) e[u] = ecur[u];
+            } else {
+                load_comp(w0, e);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) x[u] = ld_off(Vin, (uint32_t)base + (e[u] & 0xffffu));
+            has_next = next_comp;
+            if (next_comp) load_comp(wend, ecur);
+            if (A.ptab_n) {
+#pragma unroll
+                for (int u = 0; u < U; u++) pv[u] = (double)ptabf[e[u] >> 16];
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; u++) pv[u] = (double)(float)((double)(e[u] >> 16) * A.inv_nr);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) wsd[u * 64 + lane] = pv[u] * x[u];
+        } else {
+            int2 e[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) e[u] = nzp[kw * WIN + u * 64];
+#pragma unroll
+            for (int u = 0; u < U; u++) x[u] = ld_off(Vin, (uint32_t)e[u].x);
+            has_next = next_comp;
+            if (next_comp) load_comp(wend, ecur);
+#pragma unroll
+            for (int u = 0; u < U; u++) pv[u] = (double)__int_as_float(e[u].y);
+#pragma unroll
+            for (int u = 0; u < U; u++) wsd[u * 64 + lane] = pv[u] * x[u];
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (;;) {
+            if (valid) row_part(wsd - w0, max(cur.rb, w0), min(cur.re, wend), sd);
+            if (ce > wend) break;  // the group continues in the next window
+            finish_group();
+            if (g >= ngroups) break;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    while (g < ngroups) finish_group();  // trailing groups whose rows are all empty
+}
+
+// States [x, y) of one piece of a wave's range [sb, se) of the lean sweep, in whole groups
+// of spc states: j < 0 the head, the first head16/16 of the groups; j in [0, P) the j-th
+// of P near-equal pieces of the rest (the tail). The head and the P pieces tile [sb, se).
+__device__ __forceinline__ uint2 lean_piece(uint32_t sb, uint32_t se, uint32_t spc, uint32_t head16, uint32_t P,
+                                            int j) {
+    const uint32_t ng = se > sb ? (se - sb + spc - 1u) / spc : 0u;
+    const uint32_t hg = ng * head16 / 16u, tg = ng - hg;
+    const uint32_t g0 = j < 0 ? 0u : hg + tg * (uint32_t)j / P;
+    const uint32_t g1 = j < 0 ? hg : hg + tg * (uint32_t)(j + 1) / P;
+    return make_uint2(min(sb + g0 * spc, se), min(sb + g1 * spc, se));
+}
+
 template <int NAP, int U, bool COMP>
 __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
     constexpr int SPC = 64 / NAP;
-    constexpr int WIN = 64 * U;
-    static_assert(!COMP || WIN == kCompWin, "compressed windows are 256 entries");
+    static_assert(!COMP || 64 * U == kCompWin, "compressed windows are 256 entries");
     __shared__ float wmax[kStreamWaves];
     __shared__ int is_last;
-    __shared__ double stage[kStreamWaves][WIN + kRowReads];
+    __shared__ double stage[kStreamWaves][64 * U + kRowReads];
     VIState* st = A.st;
     if (!A.one_pass && st->conv >= 0) return;  // converged at an earlier sweep
     // float64 values between sweeps. In the backward pass W0 == W1: the launch writes the
@@ -422,8 +609,6 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
     double* __restrict__ Vout = (k & 1u) ? A.W1 : A.W0;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int a = lane % NAP, sl = lane / NAP;
-    const uint32_t na = (uint32_t)A.na;
     const unsigned nb = gridDim.x, b = blockIdx.x;
     const unsigned pb = (nb % 8u == 0u) ? (b % 8u) * (nb / 8u) + b / 8u : b;
     const unsigned w = pb * kStreamWaves + wv;
@@ -432,7 +617,6 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
     const uint32_t* wpart = A.wpart;
     if (A.pre_nt) wpart += (size_t)(active_slices(k, (int)A.pre_nt) - 1) * A.pre_stride;
     const CPtr<uint32_t> wp = (CPtr<uint32_t>)wpart;
-    const CPtr<uint32_t> rps = (CPtr<uint32_t>)A.rp;  // uniform reads of the row pointers
     const uint32_t s_begin = wp[w], s_end = wp[w + 1];
     double* wsd = stage[wv];
     float res = 0.0f;
@@ -445,157 +629,28 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
         __syncthreads();
     }
 
-    if (s_begin < s_end) {
-        const uint32_t ngroups = (s_end - s_begin + SPC - 1) / SPC;
-        const uint32_t Pb = rps[s_begin * na], Pend = rps[s_end * na];
-        // COMP: windows are the stream's aligned 256-entry blocks (one base column each);
-        // entries before Pb or from Pend on are staged but belong to no row of this wave
-        const uint32_t P0 = COMP ? Pb / WIN * WIN : Pb;
-        const uint32_t nwin = Pend > Pb ? (Pend - P0 + WIN - 1) / WIN : 0u;
-        struct Meta {
-            uint32_t rb, re;
-            float rr;
-            double vold;
-        };
-        // row data of group g (clamped to the wave's last group, so always in range)
-        auto load_meta = [&](uint32_t g, Meta& M) {
-            const uint32_t s = min(s_begin + g * SPC + (uint32_t)sl, s_end - 1u);
-            const uint32_t row = s * na + min((uint32_t)a, na - 1u);
-            M.rb = ld_off(A.rp, row);
-            M.re = ld_off(A.rp, row + 1u);
-            M.rr = ld_off(A.R, row);
-            M.vold = ld_off(Vin, s);
-        };
-        Meta cur, nxt;
-        uint32_t g = 0;
-        load_meta(0, cur);
-        load_meta(min(1u, ngroups - 1u), nxt);
-        uint32_t ce = rps[min(s_begin + SPC, s_end) * na];
-        bool valid = (s_begin + (uint32_t)sl < s_end) && ((uint32_t)a < na);
-        double sd = 0.0;
 
-        auto finish_group = [&]() {
-            double q = valid ? (double)cur.rr + (double)A.discount * sd : -INFINITY;
-            if (q != q) q = -INFINITY;
-            int ba = valid ? a : 0x7fffffff;
-            // max / first argmax over the NAP lanes of each state; the combine is
-            // associative and commutative, so any reduction tree gives the same result
-            if (NAP <= 16) {
-                // DPP row shifts: after shifts 1, 2, .., NAP/2 the state's last lane
-                // (a = NAP-1) has combined exactly its own NAP lanes
-#pragma unroll
-                for (int o = 1; o < NAP; o <<= 1) {
-                    const double q2 = dpp_row_shr(q, o, (double)-INFINITY);
-                    const int a2 = dpp_row_shr(ba, o, 0x7fffffff);
-                    const bool take = q2 > q || (q2 == q && a2 < ba);
-                    q = take ? q2 : q;
-                    ba = take ? a2 : ba;
-                }
-            } else {
-#pragma unroll
-                for (int o = 1; o < NAP; o <<= 1) {
-                    const double q2 = __shfl_xor(q, o, 64);
-                    const int a2 = __shfl_xor(ba, o, 64);
-                    if (q2 > q || (q2 == q && a2 < ba)) {
-                        q = q2;
-                        ba = a2;
-                    }
-                }
-            }
-            const bool writer = (NAP <= 16) ? (a == NAP - 1) : (a == 0);
-            if (writer && g < ngroups && s_begin + g * SPC + (uint32_t)sl < s_end) {
-                const uint32_t s = s_begin + g * SPC + (uint32_t)sl;
-                double vnew;
-                int32_t pol;
-                if (q > -1e6) {
-                    vnew = q;
-                    pol = ba;
-                } else {
-                    vnew = -1e6;
-                    pol = -1;
-                }
-                Vout[s] = vnew;
-                A.pi[s] = pol;
-                res = fmaxf(res, res_up(fabs(vnew - cur.vold)));
-            }
-            sd = 0.0;
-            g++;
-            cur = nxt;
-            load_meta(min(g + 1u, ngroups - 1u), nxt);
-            const uint32_t s0 = s_begin + g * SPC;
-            ce = rps[min(s0 + SPC, s_end) * na];
-            valid = (g < ngroups) && (s0 + (uint32_t)sl < s_end) && ((uint32_t)a < na);
-        };
-
-        const int2* __restrict__ nzp = A.nz + P0 + lane;
-        const CPtr<int32_t> cbs = (CPtr<int32_t>)A.cbase;
-        // compressed windows: the next window's entries are requested as soon as this
-        // window's value gathers are (ecur holds them when has_next)
-        uint32_t ecur[U];
-        bool has_next = false;
-        auto load_comp = [&](uint32_t w, uint32_t* dst) {
-#pragma unroll
-            for (int u = 0; u < U; u++) dst[u] = A.cz[w + (uint32_t)(u * 64 + lane)];
-        };
-        if (COMP && nwin > 0 && cbs[P0 / WIN] >= 0) {
-            load_comp(P0, ecur);
-            has_next = true;
-        }
-        for (uint32_t kw = 0; kw < nwin; kw++) {
-            const uint32_t w0 = P0 + kw * WIN, wend = w0 + WIN;
-            double x[U], pv[U];
-            const int32_t base = COMP ? cbs[w0 / WIN] : -1;
-            const bool next_comp = COMP && kw + 1 < nwin && cbs[w0 / WIN + 1] >= 0;
-            if (COMP && base >= 0) {
-                // 4-byte entries w0 + 64 u + lane: the probability count / nrzns is
-                // rounded to float from the double count * (1 / nrzns) (the same float as
-                // the division: that product is within 2^-52 of count / nrzns, which for
-                // count < 2^16 lies at least 2^-41 away from any rounding midpoint of
-                // float); the window was kept compressed only if every entry decodes to
-                // its stored bits
-                uint32_t e[U];
-                if (has_next) {
-#pragma unroll
-                    for (int u = 0; u < U; u++) e[u] = ecur[u];
-                } else {
-                    load_comp(w0, e);
-                }
-#pragma unroll
-                for (int u = 0; u < U; u++) x[u] = ld_off(Vin, (uint32_t)base + (e[u] & 0xffffu));
-                has_next = next_comp;
-                if (next_comp) load_comp(wend, ecur);
-                if (A.ptab_n) {
-#pragma unroll
-                    for (int u = 0; u < U; u++) pv[u] = (double)ptabf[e[u] >> 16];
-                } else {
-#pragma unroll
-                    for (int u = 0; u < U; u++) pv[u] = (double)(float)((double)(e[u] >> 16) * A.inv_nr);
-                }
-#pragma unroll
-                for (int u = 0; u < U; u++) wsd[u * 64 + lane] = pv[u] * x[u];
-            } else {
-                int2 e[U];
-#pragma unroll
-                for (int u = 0; u < U; u++) e[u] = nzp[kw * WIN + u * 64];
-#pragma unroll
-                for (int u = 0; u < U; u++) x[u] = ld_off(Vin, (uint32_t)e[u].x);
-                has_next = next_comp;
-                if (next_comp) load_comp(wend, ecur);
-#pragma unroll
-                for (int u = 0; u < U; u++) pv[u] = (double)__int_as_float(e[u].y);
-#pragma unroll
-                for (int u = 0; u < U; u++) wsd[u * 64 + lane] = pv[u] * x[u];
-            }
-            __builtin_amdgcn_wave_barrier();
-            for (;;) {
-                if (valid) row_part(wsd - w0, max(cur.rb, w0), min(cur.re, wend), sd);
-                if (ce > wend) break;  // the group continues in the next window
-                finish_group();
-                if (g >= ngroups) break;
-            }
-            __builtin_amdgcn_wave_barrier();
-        }
-        while (g < ngroups) finish_group();  // trailing groups whose rows are all empty
+    // Wave speeds differ by up to 1.4x on equal work, and a sweep lasts as long as its
+    // slowest wave. So (pool_parts > 0) each wave sweeps only the head of its range; the
+    // tails of the ranges of the waves of one XCD (blocks b = x mod 8; their ranges are
+    // one contiguous run of states, which keeps a piece's value gathers in that XCD's L2)
+    // form a pool of P pieces per wave, which the XCD's waves take in order from a counter
+    // as they finish. Every state is still swept once, by one wave, with its entries added
+    // in the same order: the same bits whichever wave takes a piece.
+    const uint32_t P = A.one_pass ? 0u : A.pool_parts;
+    const unsigned xcds = (nb % 8u == 0u) ? 8u : 1u;
+    const unsigned x = b % xcds;
+    const uint32_t wpx = nb / xcds * kStreamWaves, npool = wpx * P;
+    uint2 r = P ? lean_piece(s_begin, s_end, SPC, A.head16, P, -1) : make_uint2(s_begin, s_end);
+    for (;;) {
+        if (r.x < r.y) lean_range<NAP, U, COMP>(A, Vin, Vout, wsd, ptabf, r.x, r.y, res);
+        if (!P) break;
+        uint32_t q = 0u;
+        if (lane == 0) q = atomicAdd(&st->pool[x * kCounterStride], 1u);
+        q = __builtin_amdgcn_readfirstlane(q);
+        if (q >= npool) break;
+        const uint32_t wq = x * wpx + q / P;
+        r = lean_piece(wp[wq], wp[wq + 1u], SPC, A.head16, P, (int)(q % P));
     }
     if (A.one_pass) return;  // block-uniform
 
@@ -615,8 +670,8 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
         const unsigned grp = blockIdx.x / kTicketGroup, ngrp = (gridDim.x + kTicketGroup - 1) / kTicketGroup;
         const unsigned in_grp = min((unsigned)kTicketGroup, gridDim.x - grp * kTicketGroup);
         int last = 0;
-        if (atomicAdd(&st->gdone[grp], 1u) == in_grp - 1u) {
-            st->gdone[grp] = 0u;  // every block of the group has counted: ready for the next sweep
+        if (atomicAdd(&st->gdone[grp * kCounterStride], 1u) == in_grp - 1u) {
+            st->gdone[grp * kCounterStride] = 0u;  // every block of the group has counted: ready for the next sweep
             __threadfence();
             last = atomicAdd(&st->done, 1u) == ngrp - 1u;
         }
@@ -640,6 +695,8 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
             st->last_res = mx;
             if (mx < A.thresh) st->conv = (int32_t)k;
             st->done = 0u;
+#pragma unroll
+            for (int i = 0; i < 8; i++) st->pool[i * kCounterStride] = 0u;  // every wave has left its pool
             st->k = k + 1u;
         }
     }
@@ -1088,7 +1145,8 @@ __global__ void vi_init(float* V0, double* W0, int64_t ns, VIState* st) {
         V0[i] = 0.0f;
         W0[i] = 0.0;
     }
-    if (i < kTicketGroups) st->gdone[i] = 0u;
+    if (i < kTicketGroups) st->gdone[i * kCounterStride] = 0u;
+    if (i < 8) st->pool[i * kCounterStride] = 0u;
     if (i == 0) {
         st->k = 1u;
         st->conv = -1;
@@ -1227,14 +1285,22 @@ constexpr uint32_t kMaxBand = 4096;  // doubles: 32 KB
 constexpr int kDefaultVariant = 5;
 constexpr int kBackward = 6;
 constexpr int kDefaultImported = 2;  // imported models: their probabilities are arbitrary
+// Lean sweeps: each wave sweeps the first kLeanHead16/16 of its range and leaves the rest,
+// in kLeanPoolParts pieces, to its XCD's pool (spmv_vi_lean). Measured on full sweeps at 7
+// blocks per CU (100x100x120 model, 1.3e8 entries; 200x200x200, 1.0e9 entries): 289 and
+// 1855 us without the pool; with 4 pieces and a head of 6, 8, 10, 12 sixteenths 275 and
+// 1691, 268 and 1661, 263 and 1631, 259 and 1682 us; with 6 pieces and 10 sixteenths 268
+// and 1605 us, but the small model's sweeps over the active slices 7% slower than with 4
+constexpr uint32_t kLeanPoolParts = 4;
+constexpr uint32_t kLeanHead16 = 10;
 constexpr int kLeanBlocksPerCU = 5;  // fastest resident grid of the lean sweep (100x100x120 model)
-// Resident blocks per CU of the lean sweep over compressed entries. Measured: full sweeps
-// of the 100x100x120 model (1.3e8 entries) 327 us at 5 blocks, 302 us at 8; of the
-// 200x200x200 model (1.0e9 entries) 2288 and 1832 us; its solve over the active slices
-// 237 and 196 ms; but the smaller model's solve over the active slices 22.3 ms at 5
-// against 23.4 ms at 8 (its late sweeps are a few slices each)
-constexpr int kCompBlocksPerCU = 8;
-constexpr int kCompBlocksActiveSmall = 5;   // active-slice solves of models below kCompBigNnz entries
+// Resident blocks per CU of the lean sweep over compressed entries. Measured with the pool,
+// full sweeps of the 100x100x120 model: 290, 273, 268 us at 5, 6, 7 blocks (8 blocks: 335 us);
+// of the 200x200x200 model 1884, 1740, 1661 us (8 blocks: 1999 us); the sweeps over the
+// active slices of the smaller model (a few slices each near the end) 169, 168, 173 us
+// per sweep, of the larger 997, 921, 879 us
+constexpr int kCompBlocksPerCU = 7;
+constexpr int kCompBlocksActiveSmall = 6;   // active-slice solves of models below kCompBigNnz entries
 constexpr int64_t kCompBigNnz = int64_t(1) << 28;
 
 // Launch shape of the lean sweep over compressed entries: the probability table of the
@@ -1529,6 +1595,8 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     A.wpart = lean ? (active ? m->vi_pre_wpart : m->vi_wpart) : nullptr;
     A.pre_nt = active ? (uint32_t)m->nt : 0u;
     A.pre_stride = active ? (uint32_t)(nblocks * kStreamWaves + 1) : 0u;
+    A.pool_parts = lean ? kLeanPoolParts : 0u;
+    A.head16 = kLeanHead16;
     A.tdesc = tiled ? m->vi_tdesc : nullptr;
     A.tile_states = tile_states;
     A.vwin = vwin;
@@ -1664,7 +1732,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
             E2E_HIP(hipGetLastError());
         }
         E2E_HIP(hipEventRecord(e1, s));
-        E2E_HIP(hipMemcpyAsync(&hst, m->st, sizeof(VIState), hipMemcpyDeviceToHost, s));
+        E2E_HIP(hipMemcpyAsync(&hst, m->st, kVIStateHead, hipMemcpyDeviceToHost, s));
         E2E_HIP(hipStreamSynchronize(s));
         float ms = 0.f;
         E2E_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -1926,6 +1994,8 @@ e2e_status vi_solve_noalloc(Model* m, const e2e_solve_opts* o, uint32_t* policy,
     A.wpart = wpart;
     A.pre_nt = active ? (uint32_t)m->nt : 0u;
     A.pre_stride = active ? (uint32_t)(W + 1) : 0u;
+    A.pool_parts = kLeanPoolParts;
+    A.head16 = kLeanHead16;
     A.cz = comp ? m->vi_cz : nullptr;
     A.cbase = comp ? m->vi_cbase : nullptr;
     A.inv_nr = comp ? 1.0 / (double)m->p.nrzns : 0.0;
@@ -1943,7 +2013,7 @@ e2e_status vi_solve_noalloc(Model* m, const e2e_solve_opts* o, uint32_t* policy,
         const int n = std::min(chunk, max_sweeps - launched);
         for (int i = 0; i < n; i++) hipLaunchKernelGGL(fn, dim3((unsigned)nblocks), dim3(kStreamThreads), shmem, s, A);
         E2E_HIP(hipGetLastError());
-        E2E_HIP(hipMemcpyAsync(&hst, st, sizeof(VIState), hipMemcpyDeviceToHost, s));
+        E2E_HIP(hipMemcpyAsync(&hst, st, kVIStateHead, hipMemcpyDeviceToHost, s));
         E2E_HIP(hipStreamSynchronize(s));
         launched += n;
         if (opts.sweeps_per_check <= 0 && launched >= 2 * spc) chunk = std::min(chunk * 2, 4 * spc);
