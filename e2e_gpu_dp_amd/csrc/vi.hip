@@ -583,18 +583,22 @@ __device__ __forceinline__ void lean_range(const VIArgs& A, const double* __rest
 
 // States [x, y) of one piece of a wave's range [sb, se) of the lean sweep, in whole groups
 // of spc states: j < 0 the head, the first head16/16 of the groups; j in [0, P) the j-th
-// of P near-equal pieces of the rest (the tail). The head and the P pieces tile [sb, se).
+// of P pieces of the rest (the tail), each half the size of the one before (shares
+// 2^(P-1-j) / (2^P - 1) of the tail). The head and the P pieces tile [sb, se).
 __device__ __forceinline__ uint2 lean_piece(uint32_t sb, uint32_t se, uint32_t spc, uint32_t head16, uint32_t P,
                                             int j) {
     const uint32_t ng = se > sb ? (se - sb + spc - 1u) / spc : 0u;
     const uint32_t hg = ng * head16 / 16u, tg = ng - hg;
-    const uint32_t g0 = j < 0 ? 0u : hg + tg * (uint32_t)j / P;
-    const uint32_t g1 = j < 0 ? hg : hg + tg * (uint32_t)(j + 1) / P;
+    const uint32_t F = (1u << P) - 1u;  // the tail's pieces start at tg * (2^P - 2^(P-j)) / F
+    const uint32_t g0 = j < 0 ? 0u : hg + tg * (F + 1u - (1u << (P - (uint32_t)j))) / F;
+    const uint32_t g1 = j < 0 ? hg : hg + tg * (F + 1u - (1u << (P - (uint32_t)j - 1u))) / F;
     return make_uint2(min(sb + g0 * spc, se), min(sb + g1 * spc, se));
 }
 
 template <int NAP, int U, bool COMP>
-__global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
+// At most 80 SGPRs (the rest spill to VGPR lanes): with more, fewer than 8 waves per SIMD
+// are resident on the GPU, below what the occupancy calculation reports for the kernel
+__global__ __launch_bounds__(kStreamThreads) __attribute__((amdgpu_num_sgpr(80))) void spmv_vi_lean(const VIArgs A) {
     constexpr int SPC = 64 / NAP;
     static_assert(!COMP || 64 * U == kCompWin, "compressed windows are 256 entries");
     __shared__ float wmax[kStreamWaves];
@@ -634,9 +638,11 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
     // slowest wave. So (pool_parts > 0) each wave sweeps only the head of its range; the
     // tails of the ranges of the waves of one XCD (blocks b = x mod 8; their ranges are
     // one contiguous run of states, which keeps a piece's value gathers in that XCD's L2)
-    // form a pool of P pieces per wave, which the XCD's waves take in order from a counter
-    // as they finish. Every state is still swept once, by one wave, with its entries added
-    // in the same order: the same bits whichever wave takes a piece.
+    // form a pool of P pieces per wave, which the XCD's waves take from a counter as they
+    // finish: every wave's first (largest) piece, then every wave's second, and so on, so
+    // that the last pieces taken are the smallest. Every state is still swept once, by one
+    // wave, with its entries added in the same order: the same bits whichever wave takes
+    // a piece.
     const uint32_t P = A.one_pass ? 0u : A.pool_parts;
     const unsigned xcds = (nb % 8u == 0u) ? 8u : 1u;
     const unsigned x = b % xcds;
@@ -649,8 +655,8 @@ __global__ __launch_bounds__(kStreamThreads) void spmv_vi_lean(const VIArgs A) {
         if (lane == 0) q = atomicAdd(&st->pool[x * kCounterStride], 1u);
         q = __builtin_amdgcn_readfirstlane(q);
         if (q >= npool) break;
-        const uint32_t wq = x * wpx + q / P;
-        r = lean_piece(wp[wq], wp[wq + 1u], SPC, A.head16, P, (int)(q % P));
+        const uint32_t wq = x * wpx + q % wpx;
+        r = lean_piece(wp[wq], wp[wq + 1u], SPC, A.head16, P, (int)(q / wpx));
     }
     if (A.one_pass) return;  // block-uniform
 
@@ -1286,19 +1292,19 @@ constexpr int kDefaultVariant = 5;
 constexpr int kBackward = 6;
 constexpr int kDefaultImported = 2;  // imported models: their probabilities are arbitrary
 // Lean sweeps: each wave sweeps the first kLeanHead16/16 of its range and leaves the rest,
-// in kLeanPoolParts pieces, to its XCD's pool (spmv_vi_lean). Measured on full sweeps at 7
-// blocks per CU (100x100x120 model, 1.3e8 entries; 200x200x200, 1.0e9 entries): 289 and
-// 1855 us without the pool; with 4 pieces and a head of 6, 8, 10, 12 sixteenths 275 and
-// 1691, 268 and 1661, 263 and 1631, 259 and 1682 us; with 6 pieces and 10 sixteenths 268
-// and 1605 us, but the small model's sweeps over the active slices 7% slower than with 4
+// in kLeanPoolParts pieces of halving size, to its XCD's pool (spmv_vi_lean). Measured on
+// full sweeps at 7 blocks per CU (100x100x120 model, 1.3e8 entries; 200x200x200, 1.0e9
+// entries): 289 and 1855 us without the pool; with 4 equal pieces and a head of 6, 8, 10,
+// 12 sixteenths 275 and 1691, 268 and 1661, 263 and 1631, 259 and 1682 us; with 4 halving
+// pieces after a head of 10 (and at most 80 SGPRs) 255 and 1593 us
 constexpr uint32_t kLeanPoolParts = 4;
 constexpr uint32_t kLeanHead16 = 10;
 constexpr int kLeanBlocksPerCU = 5;  // fastest resident grid of the lean sweep (100x100x120 model)
 // Resident blocks per CU of the lean sweep over compressed entries. Measured with the pool,
-// full sweeps of the 100x100x120 model: 290, 273, 268 us at 5, 6, 7 blocks (8 blocks: 335 us);
-// of the 200x200x200 model 1884, 1740, 1661 us (8 blocks: 1999 us); the sweeps over the
-// active slices of the smaller model (a few slices each near the end) 169, 168, 173 us
-// per sweep, of the larger 997, 921, 879 us
+// full sweeps of the 100x100x120 model: 259, 255, 260 us at 6, 7, 8 blocks; of the
+// 200x200x200 model 1647, 1593, 1596 us; the sweeps over the active slices of the smaller
+// model (a few slices each near the end) 161, 168, 178 us per sweep, of the larger 872,
+// 848, 854 us
 constexpr int kCompBlocksPerCU = 7;
 constexpr int kCompBlocksActiveSmall = 6;   // active-slice solves of models below kCompBigNnz entries
 constexpr int64_t kCompBigNnz = int64_t(1) << 28;
