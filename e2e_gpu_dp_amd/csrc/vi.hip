@@ -68,6 +68,7 @@ struct VIArgs {
     uint32_t pre_stride;    // active slices: waves + 1, the stride of wpart's per-prefix partitions
     uint32_t pool_parts;    // lean sweep: tail pieces per wave taken from a per-XCD pool (0: none)
     uint32_t head16;        // lean sweep with pool_parts: sixteenths of each range its wave sweeps itself
+    uint32_t pool_min;      // lean sweep: states per wave from which a sweep uses the pool
 };
 
 // bytes of VIState the host reads back: k, conv, done, last_res (not the counters)
@@ -643,7 +644,10 @@ __global__ __launch_bounds__(kStreamThreads) __attribute__((amdgpu_num_sgpr(80))
     // that the last pieces taken are the smallest. Every state is still swept once, by one
     // wave, with its entries added in the same order: the same bits whichever wave takes
     // a piece.
-    const uint32_t P = A.one_pass ? 0u : A.pool_parts;
+    // (only for sweeps of at least pool_min states per wave: a short sweep would spend more
+    // on the pool's counters than it saves; every wave reads the same partition ends)
+    const uint32_t nwv = nb * kStreamWaves;
+    const uint32_t P = (A.one_pass || wp[nwv] - wp[0] < A.pool_min * nwv) ? 0u : A.pool_parts;
     const unsigned xcds = (nb % 8u == 0u) ? 8u : 1u;
     const unsigned x = b % xcds;
     const uint32_t wpx = nb / xcds * kStreamWaves, npool = wpx * P;
@@ -1299,6 +1303,11 @@ constexpr int kDefaultImported = 2;  // imported models: their probabilities are
 // pieces after a head of 10 (and at most 80 SGPRs) 255 and 1593 us
 constexpr uint32_t kLeanPoolParts = 4;
 constexpr uint32_t kLeanHead16 = 10;
+// A sweep uses the pool only with at least kLeanPoolMin states per wave: the sweeps over a
+// few active slices near the end of a solve otherwise spend about 45 us on the counters
+// (measured on the 100x100x120 model: 160 us per sweep of the whole solve with the pool
+// on every sweep, 153 us from 64 states per wave; 16, 32, 128: 158, 156, 159 us)
+constexpr uint32_t kLeanPoolMin = 64;
 constexpr int kLeanBlocksPerCU = 5;  // fastest resident grid of the lean sweep (100x100x120 model)
 // Resident blocks per CU of the lean sweep over compressed entries. Measured with the pool,
 // full sweeps of the 100x100x120 model: 259, 255, 260 us at 6, 7, 8 blocks; of the
@@ -1603,6 +1612,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     A.pre_stride = active ? (uint32_t)(nblocks * kStreamWaves + 1) : 0u;
     A.pool_parts = lean ? kLeanPoolParts : 0u;
     A.head16 = kLeanHead16;
+    A.pool_min = kLeanPoolMin;
     A.tdesc = tiled ? m->vi_tdesc : nullptr;
     A.tile_states = tile_states;
     A.vwin = vwin;
@@ -2002,6 +2012,7 @@ e2e_status vi_solve_noalloc(Model* m, const e2e_solve_opts* o, uint32_t* policy,
     A.pre_stride = active ? (uint32_t)(W + 1) : 0u;
     A.pool_parts = kLeanPoolParts;
     A.head16 = kLeanHead16;
+    A.pool_min = kLeanPoolMin;
     A.cz = comp ? m->vi_cz : nullptr;
     A.cbase = comp ? m->vi_cbase : nullptr;
     A.inv_nr = comp ? 1.0 / (double)m->p.nrzns : 0.0;
