@@ -766,13 +766,24 @@ constexpr int kCntBins = kCntPitch * kCntPitch;  // grid [I_y + 1][I_x + 1]
 constexpr int kCntWaves = 16;                  // waves of a counting block
 constexpr int kCntModes = 8;                   // DO modes a counting wave holds in registers
 
-// LDS of one counting wave
-struct CountWave {
-    uint32_t grid[(kCntBins + 1) / 2];  // 16-bit counts, two per word (a count is at most nrzns < 2^16)
+// LDS of one counting wave. The block's dynamic LDS starts with the waves' tables, each a
+// multiple of 256 bytes, so every u[ax] lies on a 256-byte boundary: a rank's bisection
+// then carries the LDS address of its position itself (the base OR the offset).
+struct alignas(256) CountWave {
     uint32_t u[2][64];                  // the group's boundary keys per axis, ascending; padding 0xffffffff
+    uint32_t grid[(kCntBins + 1) / 2];  // 16-bit counts, two per word (a count is at most nrzns < 2^16)
     uint8_t c[2][kCntMax + 1];          // boundary j of the group: #{u <= its key}
     uint32_t d[kCntMax + 1];            // boundary j of the axis being ranked: action << 16 | ord
 };
+static_assert(sizeof(CountWave) % 256 == 0, "counting-wave tables keep u on 256-byte boundaries");
+
+// 32-bit LDS address of a pointer into shared memory, and a load from such an address
+__device__ __forceinline__ uint32_t lds_addr(const void* p) {
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+__device__ __forceinline__ uint32_t lds_load(uint32_t a) {
+    return *(const __attribute__((address_space(3))) uint32_t*)(uintptr_t)a;
+}
 // LDS of a counting block: this table, the waves' tables, then Yi [nm][nr] when it fits
 struct CountShared {
     float fcos[kMaxActions], fsin[kMaxActions], speed[kMaxActions];  // the action table, for per-lane reads
@@ -1009,23 +1020,32 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
         for (int i = lane; i < ((Ty + 2) * kCntPitch + 1) / 2; i += 64) Wv.grid[i] = 0u;
         __builtin_amdgcn_wave_barrier();
 
-        // every realization's cell of the boundary grid, two rounds at a time
-        // rank #{u <= key} by bisection over the padded 64 keys, kept as a byte offset
-        const char* ux = reinterpret_cast<const char*>(Wv.u[0]);
-        const char* uy = reinterpret_cast<const char*>(Wv.u[1]);
-        auto place = [&](float2 v) __attribute__((always_inline)) {
-            const uint32_t kx = fkey(v.x), ky = fkey(v.y);
-            uint32_t ox = 0u, oy = 0u;
+        // every realization's cell of the boundary grid, two rounds at a time: the rank
+        // #{u <= key} of both samples on both axes by bisection over the padded 64 keys,
+        // the four lookups of a level issued together. A position is kept as the LDS
+        // address of u[ax][rank] (u[ax] is 256-byte aligned, so adding 4h is an OR)
+        const uint32_t ax0 = lds_addr(Wv.u[0]), ay0 = lds_addr(Wv.u[1]);
+        // bin of positions (ox, oy): (rank_y + 1) * kCntPitch + rank_x + 1
+        const int bin0 = kCntPitch + 1 - (int)(ay0 >> 2) * kCntPitch - (int)(ax0 >> 2);
+        auto place2 = [&](float2 v1, float2 v2, int& b1, int& b2) __attribute__((always_inline)) {
+            const uint32_t kx1 = fkey(v1.x), ky1 = fkey(v1.y), kx2 = fkey(v2.x), ky2 = fkey(v2.y);
+            uint32_t ox1 = ax0, oy1 = ay0, ox2 = ax0, oy2 = ay0;
 #pragma unroll
             for (int h = 32; h >= 1; h >>= 1) {
-                ox |= *reinterpret_cast<const uint32_t*>(ux + ox + 4 * (h - 1)) <= kx ? 4u * h : 0u;
-                oy |= *reinterpret_cast<const uint32_t*>(uy + oy + 4 * (h - 1)) <= ky ? 4u * h : 0u;
+                const uint32_t ex1 = lds_load(ox1 + 4u * (h - 1)), ey1 = lds_load(oy1 + 4u * (h - 1));
+                const uint32_t ex2 = lds_load(ox2 + 4u * (h - 1)), ey2 = lds_load(oy2 + 4u * (h - 1));
+                ox1 |= ex1 <= kx1 ? 4u * h : 0u;
+                oy1 |= ey1 <= ky1 ? 4u * h : 0u;
+                ox2 |= ex2 <= kx2 ? 4u * h : 0u;
+                oy2 |= ey2 <= ky2 ? 4u * h : 0u;
             }
-            return (int)((oy >> 2) + 1) * kCntPitch + (int)(ox >> 2) + 1;
+            b1 = (int)(oy1 >> 2) * kCntPitch + (int)(ox1 >> 2) + bin0;
+            b2 = (int)(oy2 >> 2) * kCntPitch + (int)(ox2 >> 2) + bin0;
         };
         for (int w = lane; w < nr; w += 128) {
             const bool two = w + 64 < nr;
-            const int b1 = place(sample(w)), b2 = place(sample(two ? w + 64 : w));
+            int b1, b2;
+            place2(sample(w), sample(two ? w + 64 : w), b1, b2);
             atomicAdd(&Wv.grid[b1 >> 1], 1u << ((b1 & 1) * 16));
             if (two) atomicAdd(&Wv.grid[b2 >> 1], 1u << ((b2 & 1) * 16));
         }
@@ -1620,7 +1640,7 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
 
 // LDS bytes of mc_count: the shared table, the waves' tables, Yi [nm][nr] (YLDS).
 __host__ __device__ inline size_t count_lds_bytes(int nr, int nm, bool ylds) {
-    return sizeof(CountShared) + sizeof(CountWave) * kCntWaves + (ylds ? sizeof(float) * (size_t)nr * nm : 0);
+    return sizeof(CountWave) * kCntWaves + sizeof(CountShared) + (ylds ? sizeof(float) * (size_t)nr * nm : 0);
 }
 
 // Counting kernel: the cells count_cell can take, all their rows at once. A block of
@@ -1631,13 +1651,13 @@ __host__ __device__ inline size_t count_lds_bytes(int nr, int nm, bool ylds) {
 template <bool YLDS, int NM>
 __global__ __launch_bounds__(kCntWaves * 64) void mc_count(const BuildArgs A) {
     constexpr int WG = kCntWaves * 64;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
+    extern __shared__ __align__(256) unsigned char smem_raw[];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nr = A.nr, nm = A.nm, nc = A.nc, na = A.na;
-    CountShared& T = *reinterpret_cast<CountShared*>(smem_raw);
-    CountWave* waves = reinterpret_cast<CountWave*>(smem_raw + sizeof(CountShared));
-    float* ylds = reinterpret_cast<float*>(waves + kCntWaves);  // [nm][nr]
+    CountWave* waves = reinterpret_cast<CountWave*>(smem_raw);
+    CountShared& T = *reinterpret_cast<CountShared*>(smem_raw + sizeof(CountWave) * kCntWaves);
+    float* ylds = reinterpret_cast<float*>(smem_raw + sizeof(CountWave) * kCntWaves + sizeof(CountShared));  // [nm][nr]
     const int t = A.t0 + blockIdx.y;
     // the action table, one entry per lane (kernel-argument arrays take uniform indices only)
     for (int a = 0; a < na; a++)
@@ -1668,13 +1688,16 @@ __global__ __launch_bounds__(kCntWaves * 64) void mc_count(const BuildArgs A) {
     geo.XH = unit_index(A.xsn - A.xs0);
     geo.YL = unit_index(A.ysn - A.ys0);
     CountWave& Wv = waves[wv];
+    // the bisection's address arithmetic needs the tables on 256-byte boundaries; were the
+    // dynamic LDS placed otherwise, every cell would go to the per-row kernel instead
+    const bool aligned = (lds_addr(smem_raw) & 255u) == 0u;
     for (;;) {
         unsigned cc = 0;
         if (lane == 0) cc = atomicAdd(&T.next_cell, 1u);
         const int c = __builtin_amdgcn_readfirstlane(__shfl((int)cc, 0, 64));  // wave-uniform
         if (c >= c_end) break;
-        const bool done = count_cell<YLDS, NM>(A, YLDS ? ylds : src, Wv, T, geo, t, c, late_slice, obs_slice,
-                                                oow_acc, overflow_acc, maxnuq_acc);
+        const bool done = aligned && count_cell<YLDS, NM>(A, YLDS ? ylds : src, Wv, T, geo, t, c, late_slice,
+                                                           obs_slice, oow_acc, overflow_acc, maxnuq_acc);
         if (done)
             counted++;
         else if (lane == 0)
