@@ -759,10 +759,11 @@ __device__ __forceinline__ RowRewards row_rewards(const BuildArgs& A, float rste
 // Same counts, order, rewards and statistics as moving every lane. One wavefront counts
 // one cell, with its own tables and grid in LDS (no block barrier), and recomputes the
 // samples when it needs them. A group of actions holds at most kCntMax boundaries per
-// axis; a cell that needs more is counted group by group.
-constexpr int kCntMax = 47;                    // boundaries per axis per group
-constexpr int kCntPitch = kCntMax + 2;         // grid row pitch in 16-bit bins
-constexpr int kCntBins = kCntPitch * kCntPitch;  // grid [I_y + 1][I_x + 1]
+// axis and a grid of at most kCntArea bins; a cell that needs more is counted group by
+// group (on the 100x100x120 fields 1.28 groups per cell, against 1.54 with at most 47
+// boundaries per axis in a square grid of the same area).
+constexpr int kCntMax = 63;                    // boundaries per axis per group (ranks fit 64 keys)
+constexpr int kCntArea = 49 * 49;              // grid [I_y + 1][I_x + 1] bins, row pitch I_x + 2
 constexpr int kCntWaves = 16;                  // waves of a counting block
 constexpr int kCntModes = 8;                   // DO modes a counting wave holds in registers
 
@@ -771,7 +772,7 @@ constexpr int kCntModes = 8;                   // DO modes a counting wave holds
 // then carries the LDS address of its position itself (the base OR the offset).
 struct alignas(256) CountWave {
     uint32_t u[2][64];                  // the group's boundary keys per axis, ascending; padding 0xffffffff
-    uint32_t grid[(kCntBins + 1) / 2];  // 16-bit counts, two per word (a count is at most nrzns < 2^16)
+    uint32_t grid[(kCntArea + 1) / 2];  // 16-bit counts, two per word (a count is at most nrzns < 2^16)
     uint8_t c[2][kCntMax + 1];          // boundary j of the group: #{u <= its key}
     uint32_t d[kCntMax + 1];            // boundary j of the axis being ranked: action << 16 | ord
 };
@@ -964,7 +965,7 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
     const int bxlo = col_of(G, olx), bylo = row_of(G, ohy);
     const int bw = col_of(G, ohx) - bxlo + 1, ncell = bw * (row_of(G, oly) - bylo + 1);
     const RowRewards rr = row_rewards(A, step_reward(A, T.speed[aa], 0.0f, 0.0f), late_slice, false);
-    bool ok = rr.seg_lin && nx >= 0 && ny >= 0 && nx <= kCntMax && ny <= kCntMax;
+    bool ok = rr.seg_lin && nx >= 0 && ny >= 0 && nx <= kCntMax && ny <= kCntMax && (nx + 2) * (ny + 2) <= kCntArea;
     if (OBS && obs_slice)  // no obstacle within the fixed window: boxes must stay inside it
         ok = ok && bxlo >= j1 - kFixHalf && col_of(G, ohx) <= j1 + kFixHalf && bylo >= i1 - kFixHalf &&
              row_of(G, oly) <= i1 + kFixHalf;
@@ -980,14 +981,17 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
 
     for (int start = 0; start < na;) {
         // the group: the longest run of actions from start with at most kCntMax
-        // boundaries per axis (a prefix of the lanes from start: ix, iy grow with a;
-        // every action fits alone, so the clamps below change nothing)
+        // boundaries per axis and at most kCntArea grid bins (a prefix of the lanes from
+        // start: ix, iy grow with a; every action fits alone, so the clamps below change
+        // nothing)
         const int bx = start ? __builtin_amdgcn_readlane(ix, start - 1) : 0;
         const int by = start ? __builtin_amdgcn_readlane(iy, start - 1) : 0;
-        const bool fits = act && lane >= start && ix - bx <= kCntMax && iy - by <= kCntMax;
+        const bool fits = act && lane >= start && ix - bx <= kCntMax && iy - by <= kCntMax &&
+                          (ix - bx + 2) * (iy - by + 2) <= kCntArea;
         const int end = start + max(1, (int)__popcll(__ballot(fits)));
         const int Tx = min(kCntMax, __builtin_amdgcn_readlane(ix, end - 1) - bx);
         const int Ty = min(kCntMax, __builtin_amdgcn_readlane(iy, end - 1) - by);
+        const int P = Tx + 2;  // the grid's row pitch
 
         // the boundaries, x then y: one lane each, bisection, rank
 #pragma unroll
@@ -1017,7 +1021,7 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
                 Wv.u[ax][lane] = 0xffffffffu;
             }
         }
-        for (int i = lane; i < ((Ty + 2) * kCntPitch + 1) / 2; i += 64) Wv.grid[i] = 0u;
+        for (int i = lane; i < ((Ty + 2) * P + 1) / 2; i += 64) Wv.grid[i] = 0u;
         __builtin_amdgcn_wave_barrier();
 
         // every realization's cell of the boundary grid, two rounds at a time: the rank
@@ -1025,8 +1029,8 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
         // the four lookups of a level issued together. A position is kept as the LDS
         // address of u[ax][rank] (u[ax] is 256-byte aligned, so adding 4h is an OR)
         const uint32_t ax0 = lds_addr(Wv.u[0]), ay0 = lds_addr(Wv.u[1]);
-        // bin of positions (ox, oy): (rank_y + 1) * kCntPitch + rank_x + 1
-        const int bin0 = kCntPitch + 1 - (int)(ay0 >> 2) * kCntPitch - (int)(ax0 >> 2);
+        // bin of positions (ox, oy): (rank_y + 1) * P + rank_x + 1
+        const int bin0 = P + 1 - (int)(ay0 >> 2) * P - (int)(ax0 >> 2);
         auto place2 = [&](float2 v1, float2 v2, int& b1, int& b2) __attribute__((always_inline)) {
             const uint32_t kx1 = fkey(v1.x), ky1 = fkey(v1.y), kx2 = fkey(v2.x), ky2 = fkey(v2.y);
             uint32_t ox1 = ax0, oy1 = ay0, ox2 = ax0, oy2 = ay0;
@@ -1039,8 +1043,8 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
                 ox2 |= ex2 <= kx2 ? 4u * h : 0u;
                 oy2 |= ey2 <= ky2 ? 4u * h : 0u;
             }
-            b1 = (int)(oy1 >> 2) * kCntPitch + (int)(ox1 >> 2) + bin0;
-            b2 = (int)(oy2 >> 2) * kCntPitch + (int)(ox2 >> 2) + bin0;
+            b1 = (int)__umul24(oy1 >> 2, (uint32_t)P) + (int)(ox1 >> 2) + bin0;  // LDS addresses < 2^16
+            b2 = (int)__umul24(oy2 >> 2, (uint32_t)P) + (int)(ox2 >> 2) + bin0;
         };
         for (int w = lane; w < nr; w += 128) {
             const bool two = w + 64 < nr;
@@ -1056,14 +1060,14 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
         if (lane <= Tx) {
             int acc = 0;
             for (int y = 1; y <= Ty + 1; y++) {
-                acc += g16[y * kCntPitch + lane + 1];
-                g16[y * kCntPitch + lane + 1] = (uint16_t)acc;
+                acc += g16[y * P + lane + 1];
+                g16[y * P + lane + 1] = (uint16_t)acc;
             }
         }
         __builtin_amdgcn_wave_barrier();
         for (int y = 1; y <= Ty + 1; y++) {
-            const int sc = wave_incl_scan(lane <= Tx ? (int)g16[y * kCntPitch + lane + 1] : 0);
-            if (lane <= Tx) g16[y * kCntPitch + lane + 1] = (uint16_t)sc;
+            const int sc = wave_incl_scan(lane <= Tx ? (int)g16[y * P + lane + 1] : 0);
+            if (lane <= Tx) g16[y * P + lane + 1] = (uint16_t)sc;
         }
         __builtin_amdgcn_wave_barrier();
 
@@ -1088,8 +1092,7 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
                 return o <= rly ? 0 : (o > rhy ? Ty + 1 : (int)Wv.c[1][offy + o - rly - 1]);
             };
             auto rect = [&](int lx, int hx, int ly, int hy) __attribute__((always_inline)) {
-                return bin(hy * kCntPitch + hx) - bin(ly * kCntPitch + hx) - bin(hy * kCntPitch + lx) +
-                       bin(ly * kCntPitch + lx);
+                return bin(hy * P + hx) - bin(ly * P + hx) - bin(hy * P + lx) + bin(ly * P + lx);
             };
             // landing cell k: its count, its lanes' clamps, its reward class (terminal * 2 +
             // edge: the reward before the clamp penalties is tn0, tn1, tt0 or tt1), its lanes
