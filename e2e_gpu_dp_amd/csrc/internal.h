@@ -78,12 +78,18 @@ struct VIState {
     uint32_t pool[8 * kCounterStride];
 };
 
+// Column blocks (grid.y) of a batched tile sweep that take their tiles from counters
+constexpr int kTileRows = 8;
+
 // Per-column state of a batched (multi-goal) solve.
 struct BatchState {
     uint32_t k;          // 1-based index of the next sweep
     uint32_t done;       // blocks finished in the current sweep
     int32_t all_conv;    // 1 once every column converged
     uint32_t pad;
+    // tile sweeps, at [(y * 8 + x) * kCounterStride]: tiles of XCD x's band that column
+    // block y's blocks have taken beyond their first (0 between sweeps)
+    uint32_t tnext[kTileRows * 8 * kCounterStride];
 };
 
 struct BatchCache;

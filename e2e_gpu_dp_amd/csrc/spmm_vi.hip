@@ -36,12 +36,16 @@
 // other columns share the batch.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <vector>
 
 #include "internal.h"
 #include "lanes.h"
 
 namespace e2e {
+
+// bytes of BatchState the host reads back: k, done, all_conv (not the tile counters)
+constexpr size_t kBatchStateHead = offsetof(BatchState, tnext);
 
 struct BatchArgs {
     const uint32_t* rp;
@@ -196,6 +200,8 @@ __device__ __forceinline__ void sweep_tail(const BatchArgs& A, float (*red)[KB],
             }
         }
         const int n_running = __syncthreads_count(running > 0);
+        // every block has counted, so every tile sweep's block has left its counters
+        for (int i = tid; i < kTileRows * 8; i += NT) st->tnext[i * kCounterStride] = 0u;
         if (tid == 0) {
             st->all_conv = n_running == 0;
             st->done = 0u;
@@ -534,7 +540,24 @@ __global__ __launch_bounds__(kHThreads) void spmm_vi_halo(const BatchArgs A) {
     const int t_lo = A.one_pass ? (int)(A.s_lo / (uint32_t)nc) : 0;
     const int t_hi = A.one_pass ? t_lo + 1 : A.pre_nt ? sweep_active(A, k, jb0, KB) : A.nt;
     const int ntiles = (t_hi - t_lo) * tps;
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // XCD x (blocks b = x mod 8) walks the band [tlo, thi) of the tiles; a block's first
+    // tile is its own and it takes the band's next from the XCD's counter as it finishes,
+    // as in spmm_vi_tile (the backward pass and column blocks beyond kTileRows stride)
+    const unsigned nb = gridDim.x;
+    const unsigned xcds = (nb % 8u == 0u) ? 8u : 1u;
+    const unsigned xcd = blockIdx.x % xcds, nbx = nb / xcds, bi = blockIdx.x / xcds;
+    const int tlo = (int)((int64_t)ntiles * xcd / xcds), thi = (int)((int64_t)ntiles * (xcd + 1) / xcds);
+    const bool dyn = !A.one_pass && blockIdx.y < (unsigned)kTileRows;
+    uint32_t* const tctr = dyn ? st->tnext + (blockIdx.y * 8u + xcd) * (unsigned)kCounterStride : nullptr;
+    __shared__ int next_tile;
+    auto advance = [&](int tile) {
+        if (!dyn) return tile + (int)nbx;
+        __syncthreads();  // every wave has read next_tile
+        if (tid == 0) next_tile = tlo + (int)nbx + (int)atomicAdd(tctr, 1u);
+        __syncthreads();
+        return next_tile;
+    };
+    for (int tile = tlo + (int)bi; tile < thi; tile = advance(tile)) {
         const int t = t_lo + tile / tps;
         const int rem = tile - (t - t_lo) * tps;
         const int ti = rem / tcn, tj = rem - ti * tcn;
@@ -846,15 +869,33 @@ __global__ __launch_bounds__(kHThreads) void spmm_vi_tile(const BatchArgs A) {
     double2* const qb = reinterpret_cast<double2*>(wbuf[wv]);
     const int ovf0 = (NREC + wv * kTOvf) * CS;  // this wave's first overflow record
     const uint64_t below = (1ull << lane) - 1ull;
-    // blocks in XCD-major order (consecutive blocks run on different XCDs): each XCD walks
-    // a contiguous band of tiles, so the halos it stages overlap in its own L2
+    // XCD x (blocks b = x mod 8; consecutive blocks run on different XCDs) walks a
+    // contiguous band [tlo, thi) of the sweep's tiles, so the halos it stages overlap in
+    // its own L2. A block's first tile is its own; as it finishes one it takes the band's
+    // next from the XCD's counter (tiles of equal work end at different times on different
+    // CUs, and edge tiles are smaller), except in the backward pass (no sweep state to
+    // reset) and for column blocks beyond kTileRows, which stride the band. Each tile is
+    // still swept once, by one block: the same bits whichever block takes it.
     const unsigned nb = gridDim.x;
-    const unsigned bx = (nb % 8u == 0u) ? (blockIdx.x % 8u) * (nb / 8u) + blockIdx.x / 8u : blockIdx.x;
+    const unsigned xcds = (nb % 8u == 0u) ? 8u : 1u;
+    const unsigned xcd = blockIdx.x % xcds, nbx = nb / xcds, bi = blockIdx.x / xcds;
     const int tr = (g + kHTileH - 1) / kHTileH, tcn = (g + kHTileW - 1) / kHTileW, tps = tr * tcn;
     const int t_lo = A.one_pass ? (int)(A.s_lo / (uint32_t)nc) : 0;
     const int t_hi = A.one_pass ? t_lo + 1 : A.pre_nt ? sweep_active(A, k, jb0, KB) : A.nt;
     const int ntiles = (t_hi - t_lo) * tps;
-    for (int tile = (int)bx; tile < ntiles; tile += (int)nb) {
+    const int tlo = (int)((int64_t)ntiles * xcd / xcds), thi = (int)((int64_t)ntiles * (xcd + 1) / xcds);
+    const bool dyn = !A.one_pass && blockIdx.y < (unsigned)kTileRows;
+    uint32_t* const tctr = dyn ? st->tnext + (blockIdx.y * 8u + xcd) * (unsigned)kCounterStride : nullptr;
+    __shared__ int next_tile;
+    // the block's next tile (a loop increment, so the waves that skip a tile reach it too)
+    auto advance = [&](int tile) {
+        if (!dyn) return tile + (int)nbx;
+        __syncthreads();  // every wave has read next_tile
+        if (tid == 0) next_tile = tlo + (int)nbx + (int)atomicAdd(tctr, 1u);
+        __syncthreads();
+        return next_tile;
+    };
+    for (int tile = tlo + (int)bi; tile < thi; tile = advance(tile)) {
         const int t = t_lo + tile / tps;
         const int rem = tile - (t - t_lo) * tps;
         const int ti = rem / tcn, tj = rem - ti * tcn;
@@ -1491,6 +1532,7 @@ __global__ void batch_init(double* V0, int64_t n, const int32_t* goal, int32_t* 
         start[i] = 1;
         last_res[i] = 0.0f;
     }
+    if (i < kTileRows * 8) st->tnext[i * kCounterStride] = 0u;
     if (i == 0) {
         st->k = 1u;
         st->done = 0u;
@@ -2115,7 +2157,7 @@ static e2e_status vi_solve_batch(Model* m, const int32_t* goals, int k, const e2
             E2E_HIP(hipGetLastError());
         }
         E2E_HIP(hipEventRecord(e1, s));
-        E2E_HIP(hipMemcpyAsync(&hst, b.st, sizeof(hst), hipMemcpyDeviceToHost, s));
+        E2E_HIP(hipMemcpyAsync(&hst, b.st, kBatchStateHead, hipMemcpyDeviceToHost, s));
         E2E_HIP(hipStreamSynchronize(s));
         float ms = 0.f;
         E2E_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -2239,6 +2281,7 @@ __global__ void live_init(double* V0, double* V1, int64_t n, int32_t* goal, int3
         start[i] = 1;
         last_res[i] = 0.0f;
     }
+    if (i < kTileRows * 8) st->tnext[i * kCounterStride] = 0u;
     if (i == 0) {
         st->k = 1u;
         st->done = 0u;
@@ -2361,7 +2404,7 @@ e2e_status live_pass(LiveBatch* L, int ncols) { return backward_pass(L->m, L->b,
 // admission sweep (start).
 e2e_status live_state(LiveBatch* L, BatchState* st, int32_t* conv, int32_t* start) {
     const int kp = L->b.kp;
-    E2E_HIP(hipMemcpyAsync(st, L->b.st, sizeof(BatchState), hipMemcpyDeviceToHost, L->s));
+    E2E_HIP(hipMemcpyAsync(st, L->b.st, kBatchStateHead, hipMemcpyDeviceToHost, L->s));
     E2E_HIP(hipMemcpyAsync(conv, L->b.conv, kp * 4, hipMemcpyDeviceToHost, L->s));
     E2E_HIP(hipMemcpyAsync(start, L->b.start, kp * 4, hipMemcpyDeviceToHost, L->s));
     E2E_HIP(hipStreamSynchronize(L->s));
