@@ -164,18 +164,25 @@ __device__ __forceinline__ void sweep_tail(const BatchArgs& A, float (*red)[KB],
     const int tid = threadIdx.x;
     const int KP = A.kp;
     __syncthreads();
+    // The block's partials go out by agent-scope atomic exchanges (wave 0: KB <= 64), and
+    // the last block takes them by exchanges too: the same atomics as the ticket, whose
+    // counts every XCD sees. The ticket's increment depends on the exchanges' returned
+    // values, so it counts only once the partials are in place. No __threadfence: each
+    // writes back the XCD's whole L2.
+    uint32_t one = 1u;
     if (tid < KB) {
         float m = 0.0f;
 #pragma unroll
         for (int w = 0; w < NW; w++) m = fmaxf(m, red[w][tid]);
-        A.part[(int64_t)blockIdx.x * KP + blockIdx.y * KB + tid] = m;
+        const uint32_t prev =
+            __hip_atomic_exchange(reinterpret_cast<uint32_t*>(A.part) + (int64_t)blockIdx.x * KP + blockIdx.y * KB + tid,
+                                  __float_as_uint(m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        asm volatile("" : "+v"(one) : "v"(prev));
     }
-    __threadfence();
     __syncthreads();
-    if (tid == 0) is_last = atomicAdd(&st->done, 1u) == gridDim.x * gridDim.y - 1;
+    if (tid == 0) is_last = atomicAdd(&st->done, one) == gridDim.x * gridDim.y - 1;
     __syncthreads();
     if (is_last) {
-        __threadfence();
         // column j's residual = max over the x-blocks of part[bx][j]; thread t takes
         // column t % KP and every (NT/KP)-th x-block, then LDS combines
         const int groups = NT / KP;
@@ -184,8 +191,8 @@ __device__ __forceinline__ void sweep_tail(const BatchArgs& A, float (*red)[KB],
             float r = 0.0f;
 #pragma unroll 8
             for (unsigned bx = g0; bx < gridDim.x; bx += groups)
-                r = fmaxf(r, __hip_atomic_load(A.part + (int64_t)bx * KP + j, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT));
+                r = fmaxf(r, __uint_as_float(__hip_atomic_exchange(reinterpret_cast<uint32_t*>(A.part) + (int64_t)bx * KP + j,
+                                                                   0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)));
             fin[tid] = r;
         }
         __syncthreads();
@@ -1467,18 +1474,20 @@ __global__ __launch_bounds__(kSThreads) void spmm_vi_stream(const BatchArgs A) {
         for (int c = 0; c < KB; c++) red[wv][c] = rmax[c];
     }
     __syncthreads();
+    // per-column atomic maxima, then the ticket, ordered by the maxima's returned values
+    // (as in sweep_tail; no __threadfence)
+    uint32_t one = 1u;
     if (tid < KB) {
         float m = 0.0f;
 #pragma unroll
         for (int i = 0; i < kSWaves; i++) m = fmaxf(m, red[i][tid]);
-        atomicMax(A.colmax + jb0 + tid, __float_as_uint(m));
+        const uint32_t prev = atomicMax(A.colmax + jb0 + tid, __float_as_uint(m));
+        asm volatile("" : "+v"(one) : "v"(prev));
     }
-    __threadfence();
     __syncthreads();
-    if (tid == 0) is_last = atomicAdd(&st->done, 1u) == gridDim.x * gridDim.y - 1;
+    if (tid == 0) is_last = atomicAdd(&st->done, one) == gridDim.x * gridDim.y - 1;
     __syncthreads();
     if (is_last) {
-        __threadfence();
         int running = 0;
         for (int j = tid; j < KP; j += kSThreads) {
             const float r = __uint_as_float(atomicExch(A.colmax + j, 0u));
