@@ -159,6 +159,28 @@ def test_batched_row_kernels_bit_identical(gpu, na, k, g, fast):
             assert ran & set(STATE_VARIANTS_KB)
 
 
+def test_batched_tiles_from_counters_and_strides(gpu):
+    """64 goal columns on a model with more tiles than resident blocks: the tile and halo
+    sweeps' blocks of the first 8 column blocks take their tiles from per-XCD counters,
+    those of later column blocks (the 4-column halo variants have 16) stride their XCD's
+    band of tiles; every variant gives the bits, sweep counts and policies of the row
+    kernel, over every state and over the active slices."""
+    g, nt = 64, 12
+    w = small(g=g, nt=nt, nr=300, nm=3, na=8, seed=101)
+    f = w.fields()
+    goals = goal_cells(g, 64, 41)
+    with _shared(w, f) as m:
+        ref_pol, ref_val, ref_sw, ref_st = m.solve_goals(goals, thresh=1e-6, variant=9)
+        assert ref_st["variant"] == 9
+        for variant in sorted(set(HALO_VARIANTS) | set(TILE_VARIANTS_KB)):
+            for full in (False, True):
+                pol, val, sw, st = m.solve_goals(goals, thresh=1e-6, variant=variant, full_sweeps=full)
+                print(f"variant {variant} full={full}", flush=True)
+                assert st["variant"] == variant
+                assert np.array_equal(val.view(np.uint32), ref_val.view(np.uint32)), (variant, full)
+                assert np.array_equal(pol, ref_pol) and np.array_equal(sw, ref_sw), (variant, full)
+
+
 @pytest.mark.parametrize("k", [3, 40])
 def test_batched_backward_pass_equals_converged_columns(gpu, k):
     """Variant -3: every goal column by one exact backward pass, bit for bit the columns of
