@@ -328,3 +328,31 @@ def test_vi_backward_pass_needs_a_built_model(gpu):
     with e2e.import_coo(ref["ns"], ref["na"], ref["S1"], ref["S2"], ref["val"], ref["R"]) as m:
         with pytest.raises(e2e.E2EError):
             m.solve(kernel_variant=7)
+
+
+def test_vi_work_pool_layouts_agree_bitwise(gpu):
+    """The lean sweep's work pool: at 1 and 2 blocks per CU the 60x60 x 40 model's waves
+    hold 140 and 70 states, so every long sweep hands the tails of their ranges to the
+    per-XCD pools (pieces of different sizes taken by different waves); at 7 blocks per CU
+    (20 states per wave) no sweep uses the pool. Each state is still swept once with its
+    entries in the same order: values, policies, sweep counts and the stopping residual
+    agree bit for bit, over every slice and over the active slices, and stay within 1e-5
+    of FP64."""
+    w = small(g=60, nt=40, nr=300, nm=3, seed=61)
+    f = w.fields()
+    with e2e.build_model(w.params, f) as m:
+        for full in (True, False):
+            runs = {}
+            for bpc in (1, 2, 7):
+                pol, val, st = m.solve(thresh=1e-6, max_sweeps=w.params.nt + 4, blocks_per_cu=bpc, full_sweeps=full)
+                print(f"full={full} bpc={bpc}: blocks {st['blocks']}, sweeps {st['sweeps']}", flush=True)
+                assert st["blocks"] == bpc * 256 or st["blocks"] < bpc * 256
+                runs[bpc] = (pol, val, st)
+            p0, v0, s0 = runs[1]
+            for bpc in (2, 7):
+                p1, v1, s1 = runs[bpc]
+                assert s1["sweeps"] == s0["sweeps"] and s1["final_residual"] == s0["final_residual"], bpc
+                assert np.array_equal(v1.view(np.uint32), v0.view(np.uint32)), bpc
+                assert np.array_equal(p1, p0), bpc
+        ref = oracle_model(w.params, f)
+        _check_solve(ref, p0, v0, s0)
