@@ -1314,18 +1314,18 @@ constexpr int kDefaultImported = 2;  // imported models: their probabilities are
 constexpr uint32_t kLeanPoolParts = 4;
 constexpr uint32_t kLeanHead16 = 10;
 // A sweep uses the pool only with at least kLeanPoolMin states per wave: the sweeps over a
-// few active slices near the end of a solve otherwise spend about 45 us on the counters
-// (measured on the 100x100x120 model: 160 us per sweep of the whole solve with the pool
-// on every sweep, 153 us from 64 states per wave; 16, 32, 128: 158, 156, 159 us)
-constexpr uint32_t kLeanPoolMin = 64;
+// few active slices near the end of a solve otherwise spend more on the counters than
+// the pool saves (measured on the 100x100x120 model at 7 blocks per CU: 132, 126 and
+// 123.5 us per sweep of the whole solve from 32, 64 and 128 states per wave)
+constexpr uint32_t kLeanPoolMin = 128;
 constexpr int kLeanBlocksPerCU = 5;  // fastest resident grid of the lean sweep (100x100x120 model)
-// Resident blocks per CU of the lean sweep over compressed entries. Measured with the pool,
-// full sweeps of the 100x100x120 model: 259, 255, 260 us at 6, 7, 8 blocks; of the
-// 200x200x200 model 1647, 1593, 1596 us; the sweeps over the active slices of the smaller
-// model (a few slices each near the end) 161, 168, 178 us per sweep, of the larger 872,
-// 848, 854 us
-constexpr int kCompBlocksPerCU = 7;
-constexpr int kCompBlocksActiveSmall = 6;   // active-slice solves of models below kCompBigNnz entries
+// Resident blocks per CU of the lean sweep over compressed entries. Measured with the pool
+// and the fence-free ticket, full sweeps of the 100x100x120 model: 233.5, 222.6, 223.2 us
+// at 6, 7, 8 blocks; of the 200x200x200 model 1629, 1557, 1538 us; the sweeps over the
+// active slices of the smaller model 128.3, 126.1, 125.9 us per sweep, of the larger 850,
+// 812, 806 us
+constexpr int kCompBlocksPerCU = 8;
+constexpr int kCompBlocksActiveSmall = 8;   // active-slice solves of models below kCompBigNnz entries
 constexpr int64_t kCompBigNnz = int64_t(1) << 28;
 
 // Launch shape of the lean sweep over compressed entries: the probability table of the
