@@ -372,7 +372,7 @@ REQUEST_STATUS = {0: "queued", 1: "running", 2: "done", 3: "rejected", 4: "faile
 class Server:
     """Continuous-batching planning server (C++ scheduler + builder threads in libe2edp)."""
 
-    def __init__(self, hbm_budget_bytes: int = 0, max_cols: int = 64, sweeps_per_step: int = 4,
+    def __init__(self, hbm_budget_bytes: int = 0, max_cols: int = 16, sweeps_per_step: int = 4,
                  keep_full_results: bool = False, serial: bool = False, thresh: float = 1e-6,
                  exact_pass: bool = False, max_models: int = 0):
         o = _abi.ServerOpts(hbm_budget_bytes, max_cols, sweeps_per_step, int(keep_full_results),

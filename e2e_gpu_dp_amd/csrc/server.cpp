@@ -420,7 +420,11 @@ e2e_status e2e_server_create(const e2e_server_opts* opts, void** out) {
     *out = nullptr;
     Server* S = new Server();
     if (opts) S->o = *opts;
-    if (S->o.max_cols <= 0) S->o.max_cols = 64;
+    // 16 columns per model batch by default: narrower batches keep each sweep short.
+    // Measured (six environments of 50^2 to 150^2 cells, Poisson arrivals), p50 / p99 ms
+    // at 64, 16 and 8 columns: 20/s 19 / 130, 19 / 126, 19 / 126; 60/s 79 / 208, 62 / 193,
+    // 62 / 198; 100/s 224 / 372, 163 / 323, 166 / 258; the same throughput
+    if (S->o.max_cols <= 0) S->o.max_cols = 16;
     if (S->o.max_cols > 64) {
         delete S;
         set_error("max_cols must be <= 64");

@@ -218,7 +218,7 @@ typedef struct e2e_batch_stats {
 
 typedef struct e2e_server_opts {
     int64_t hbm_budget_bytes;  /* device bytes for models + batches; 0: 80% of free memory */
-    int32_t max_cols;          /* goal columns per model batch, 1..64 (0: 64) */
+    int32_t max_cols;          /* goal columns per model batch, 1..64 (0: 16) */
     int32_t sweeps_per_step;   /* sweeps launched per scheduler step on the running model with the most
                                   entries (0: 4); a model with k times fewer entries gets k times as many,
                                   up to 16 times */
