@@ -207,11 +207,13 @@ __device__ __forceinline__ void sweep_tail(const BatchArgs& A, float (*red)[KB],
             }
         }
         const int n_running = __syncthreads_count(running > 0);
-        // every block has counted, so every tile sweep's block has left its counters
-        for (int i = tid; i < kTileRows * 8; i += NT) st->tnext[i * kCounterStride] = 0u;
+        // every block has counted, so every tile sweep's block has left its counters; the
+        // counters the next sweep's atomics count on are reset by atomics too
+        for (int i = tid; i < kTileRows * 8; i += NT)
+            __hip_atomic_exchange(&st->tnext[i * kCounterStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (tid == 0) {
             st->all_conv = n_running == 0;
-            st->done = 0u;
+            __hip_atomic_exchange(&st->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             st->k = k + 1u;
         }
     }
@@ -1500,7 +1502,7 @@ __global__ __launch_bounds__(kSThreads) void spmm_vi_stream(const BatchArgs A) {
         const int n_running = __syncthreads_count(running > 0);
         if (tid == 0) {
             st->all_conv = n_running == 0;
-            st->done = 0u;
+            __hip_atomic_exchange(&st->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             st->k = k + 1u;
         }
     }

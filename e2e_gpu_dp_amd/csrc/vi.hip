@@ -634,7 +634,6 @@ __global__ __launch_bounds__(kStreamThreads) __attribute__((amdgpu_num_sgpr(80))
         __syncthreads();
     }
 
-
     // Wave speeds differ by up to 1.4x on equal work, and a sweep lasts as long as its
     // slowest wave. So (pool_parts > 0) each wave sweeps only the head of its range; the
     // tails of the ranges of the waves of one XCD (blocks b = x mod 8; their ranges are
@@ -714,9 +713,12 @@ __global__ __launch_bounds__(kStreamThreads) __attribute__((amdgpu_num_sgpr(80))
             for (int i = 0; i < kStreamWaves; i++) mx = fmaxf(mx, wmax[i]);
             st->last_res = mx;
             if (mx < A.thresh) st->conv = (int32_t)k;
-            st->done = 0u;
+            // the counters the next sweep's atomics count on, reset by atomics too (every
+            // wave has left its pool)
+            __hip_atomic_exchange(&st->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
-            for (int i = 0; i < 8; i++) st->pool[i * kCounterStride] = 0u;  // every wave has left its pool
+            for (int i = 0; i < 8; i++)
+                __hip_atomic_exchange(&st->pool[i * kCounterStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             st->k = k + 1u;
         }
     }
