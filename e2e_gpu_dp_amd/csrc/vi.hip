@@ -1327,8 +1327,6 @@ constexpr int kLeanBlocksPerCU = 5;  // fastest resident grid of the lean sweep 
 // active slices of the smaller model 128.3, 126.1, 125.9 us per sweep, of the larger 850,
 // 812, 806 us
 constexpr int kCompBlocksPerCU = 8;
-constexpr int kCompBlocksActiveSmall = 8;   // active-slice solves of models below kCompBigNnz entries
-constexpr int64_t kCompBigNnz = int64_t(1) << 28;
 
 // Launch shape of the lean sweep over compressed entries: the probability table of the
 // counts sits in LDS (float, nrzns + 1 entries) when the blocks per CU wanted (want, or
@@ -1505,9 +1503,7 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     int ptab_n = 0;
     int bpc = opts.blocks_per_cu;
     if (lean && comp) {
-        const bool active_small = !bw && !opts.full_sweeps && m->built && m->nnz < kCompBigNnz;
-        E2E_TRY(comp_launch_shape(fn, m->p.nrzns, opts.blocks_per_cu,
-                                  active_small ? kCompBlocksActiveSmall : kCompBlocksPerCU, &bpc, &ptab_n));
+        E2E_TRY(comp_launch_shape(fn, m->p.nrzns, opts.blocks_per_cu, kCompBlocksPerCU, &bpc, &ptab_n));
     } else if (bpc <= 0) {
         E2E_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void*)fn, threads,
                                                              tiled ? (size_t)vwin * sizeof(double) : 0));
@@ -1971,9 +1967,7 @@ e2e_status vi_solve_noalloc(Model* m, const e2e_solve_opts* o, uint32_t* policy,
     int ptab_n = 0;
     int bpc = opts.blocks_per_cu;
     if (comp) {
-        const bool active_small = !opts.full_sweeps && m->built && m->nnz < kCompBigNnz;
-        E2E_TRY(comp_launch_shape(fn, m->p.nrzns, opts.blocks_per_cu,
-                                  active_small ? kCompBlocksActiveSmall : kCompBlocksPerCU, &bpc, &ptab_n));
+        E2E_TRY(comp_launch_shape(fn, m->p.nrzns, opts.blocks_per_cu, kCompBlocksPerCU, &bpc, &ptab_n));
     } else if (bpc <= 0) {
         E2E_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void*)fn, kStreamThreads, 0));
         bpc = std::min(bpc, kLeanBlocksPerCU);
