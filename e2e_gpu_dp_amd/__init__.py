@@ -231,12 +231,13 @@ class Model:
 def build_model(params, fields: dict, keep_rsum: bool = False, slot_k: int = 0,
                 radix_bits: int = 0, slices_per_launch: int = 0, block_threads: int = 0,
                 stream: int = 0, count_kernel: int = 0, paired_rounds: int = 0, alphas=None,
-                lane_moves: int = 0) -> Model:
+                lane_moves: int = 0, comp_span: int = 0) -> Model:
     """Monte Carlo build on the GPU. `fields` holds u, v, ui, vi, yi and optionally s,
     mask (numpy arrays, or contiguous torch CUDA tensors). `alphas` (reward method 4):
     also build the reward vector of each of these alpha values (float32, at most 64),
     for export_R_alpha and solve_alphas. `lane_moves` 1: move every lane even where a
-    cell can be counted at once (the same model either way)."""
+    cell can be counted at once (the same model either way). `comp_span`: column span
+    limit of the compressed entry stream the build writes (0: 65536)."""
     keep = []
     ptrs = {}
     on_dev = None
@@ -253,7 +254,8 @@ def build_model(params, fields: dict, keep_rsum: bool = False, slot_k: int = 0,
     al = None if alphas is None else np.ascontiguousarray(alphas, np.float32).ravel()
     o = _abi.BuildOpts(int(bool(on_dev)), int(keep_rsum), slot_k, radix_bits, slices_per_launch,
                        block_threads, stream or None, count_kernel, paired_rounds,
-                       0 if al is None else len(al), lane_moves, None if al is None else al.ctypes.data)
+                       0 if al is None else len(al), lane_moves, None if al is None else al.ctypes.data,
+                       comp_span)
     h = C.c_void_p()
     check(lib().e2e_model_build(C.byref(c_params(params)), C.byref(f), C.byref(o), C.byref(h)))
     m = Model(h.value, params)

@@ -46,7 +46,7 @@ class BuildOpts(C.Structure):
         ("fields_on_device", C.c_int32), ("keep_rsum", C.c_int32), ("slot_k", C.c_int32),
         ("radix_bits", C.c_int32), ("slices_per_launch", C.c_int32), ("block_threads", C.c_int32),
         ("stream", C.c_void_p), ("count_kernel", C.c_int32), ("paired_rounds", C.c_int32),
-        ("n_alpha", C.c_int32), ("lane_moves", C.c_int32), ("alphas", C.c_void_p),
+        ("n_alpha", C.c_int32), ("lane_moves", C.c_int32), ("alphas", C.c_void_p), ("comp_span", C.c_int32),
     ]
 
 

@@ -73,6 +73,8 @@ struct BuildArgs {
     int tc_count;               // mc_count: cells per block
     uint32_t* left_cells;       // mc_count: the cells it leaves to the per-row kernel
     unsigned* left_count;
+    int packed;                 // staging and arena entries as one u32 each, count << 16 | sp2 (counts and
+                                // cells below 2^16), and an overflow row's arena offset as one u32
     int32_t n_alpha;      // alpha columns (reward method 4): rewards for each alpha too
     float* Ralpha;        // [n_alpha][rows_total]
     int64_t rows_total;
@@ -256,10 +258,24 @@ __device__ __forceinline__ void put_entry(const BuildArgs& A, size_t row, int u,
                                           unsigned long long arena_off, uint32_t sp2,
                                           uint32_t count) {
     if (nuq <= A.K) {
-        A.staging[row * A.K + u] = make_uint2(sp2, count);
+        if (A.packed)
+            reinterpret_cast<uint32_t*>(A.staging)[row * A.K + u] = count << 16 | sp2;
+        else
+            A.staging[row * A.K + u] = make_uint2(sp2, count);
     } else if (arena_off + (unsigned long long)nuq <= A.arena_cap) {
-        A.arena[arena_off + u] = make_uint2(sp2, count);
+        if (A.packed)
+            reinterpret_cast<uint32_t*>(A.arena)[arena_off + u] = count << 16 | sp2;
+        else
+            A.arena[arena_off + u] = make_uint2(sp2, count);
     }
+}
+
+// An overflow row's arena offset, in its first staging slot.
+__device__ __forceinline__ void put_arena_offset(const BuildArgs& A, size_t row, unsigned long long arena_off) {
+    if (A.packed)
+        reinterpret_cast<uint32_t*>(A.staging)[row * A.K] = (uint32_t)arena_off;
+    else
+        A.staging[row * A.K] = make_uint2((uint32_t)arena_off, (uint32_t)(arena_off >> 32));
 }
 
 constexpr int kMiscInts = 256;  // wave range data [0,128), scan scratch [128,144), u64 at [160]
@@ -597,7 +613,7 @@ __global__ __launch_bounds__(WG) void mc_move_count(const BuildArgs A) {
                 if (nuq > A.K) {
                     overflow_acc++;
                     if (arena_off + (unsigned long long)nuq > A.arena_cap) atomicOr(A.stats + 4, 1ull);
-                    else A.staging[row * A.K] = make_uint2((uint32_t)arena_off, (uint32_t)(arena_off >> 32));
+                    else put_arena_offset(A, row, arena_off);
                 }
             }
             __syncthreads();
@@ -1182,7 +1198,7 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
                 if (nuq > K) {
                     overflow_acc++;
                     if (arena_off + (unsigned long long)nuq > A.arena_cap) atomicOr(A.stats + 4, 1ull);
-                    else A.staging[row * K] = make_uint2((uint32_t)arena_off, (uint32_t)(arena_off >> 32));
+                    else put_arena_offset(A, row, arena_off);
                 }
             }
         };
@@ -1629,7 +1645,7 @@ __global__ __launch_bounds__(NW * 64) void mc_wave(const BuildArgs A) {
                 if (nuq > A.K) {
                     overflow_acc++;
                     if (arena_off + (unsigned long long)nuq > A.arena_cap) atomicOr(A.stats + 4, 1ull);
-                    else A.staging[row * A.K] = make_uint2((uint32_t)arena_off, (uint32_t)(arena_off >> 32));
+                    else put_arena_offset(A, row, arena_off);
                 }
             }
         }
@@ -1723,35 +1739,117 @@ __global__ __launch_bounds__(kCntWaves * 64) void mc_count(const BuildArgs A) {
     }
 }
 
-// (c) CSR scatter: one thread per (row, slot). col = (t+1)*nc + sp2, val = count/nr
-// (reduce_kernel :635-637: S2 = state1D_from_spid(t+1, sp_id2), prob = count/nrzns).
-__global__ __launch_bounds__(256) void csr_scatter(const uint2* __restrict__ staging,
-                                                   const uint2* __restrict__ arena,
-                                                   const uint32_t* __restrict__ row_nnz,
-                                                   const uint32_t* __restrict__ row_ptr,
-                                                   int2* __restrict__ nz, int64_t rows, int K,
-                                                   int64_t rows_per_slice, int64_t nc, int nr) {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t row = idx / K;
-    const int u0 = (int)(idx - row * K);
-    if (row >= rows) return;
-    const uint32_t n = row_nnz[row];
-    const int64_t t = row / rows_per_slice;
-    const int64_t colbase = (t + 1) * nc;
-    const uint32_t dst = row_ptr[row];
-    if ((int)n <= K) {
-        if (u0 < (int)n) {
-            uint2 e = staging[row * K + u0];
-            nz[dst + u0] = make_int2((int)(colbase + e.x), __float_as_int((float)e.y / nr));
+// (c) The row holding the first entry of each 256-entry window (row_ptr[r] <= 256 w <
+// row_ptr[r + 1]), and in wrow[nwin] the row holding the last entry; one thread per row.
+__global__ __launch_bounds__(256) void window_rows(const uint32_t* __restrict__ row_ptr, int64_t rows, int64_t nnz,
+                                                   int64_t nwin, uint32_t* __restrict__ wrow) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const uint32_t b = row_ptr[r], e = row_ptr[r + 1];
+    for (uint32_t w = (b + kCompWin - 1) / kCompWin; w * (uint32_t)kCompWin < e; w++) wrow[w] = (uint32_t)r;
+    if ((int64_t)b < nnz && (int64_t)e >= nnz) wrow[nwin] = (uint32_t)r;
+}
+
+// (c) CSR entries and the compressed stream, one wavefront per window of kCompWin
+// entries. Each lane of the window's rows marks that row's entries in LDS; each entry
+// then reads its staging slot (or its overflow-arena slot; PACKED: 4-byte slots, count
+// << 16 | sp2) and writes
+//   nz[j] = (col, val): col = (t+1)*nc + sp2, val = count/nr (reduce_kernel :635-637:
+//   S2 = state1D_from_spid(t+1, sp_id2), prob = count/nrzns);
+//   cz[j] (when cz is given): count << 16 | col - base, when the window keeps 4-byte
+//   entries (the rule at kCompWin, with span for kCompSpan), else 0 and cbase[w] = -1.
+// Entries past nnz in the last window get cz 0. ncomp counts the windows kept.
+template <bool PACKED>
+__global__ __launch_bounds__(256) void csr_emit(const uint2* __restrict__ staging, const uint2* __restrict__ arena,
+                                                const uint32_t* __restrict__ row_ptr,
+                                                const uint32_t* __restrict__ wrow, int2* __restrict__ nz,
+                                                uint32_t* __restrict__ cz, int32_t* __restrict__ cbase,
+                                                int64_t nnz, int64_t nwin, int K, uint32_t rows_per_slice,
+                                                int64_t nc, int nr, double inv_nr, int span,
+                                                unsigned long long* __restrict__ ncomp) {
+    // per entry of the window: its row, and its slot (staging index, or arena index with
+    // the top bit set)
+    __shared__ uint32_t rowof[4][kCompWin];
+    __shared__ unsigned long long slotof[4][kCompWin];
+    constexpr unsigned long long kArenaBit = 1ull << 63;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t wstride = (int64_t)gridDim.x * 4;
+    unsigned long long kept = 0;
+    int64_t w = (int64_t)blockIdx.x * 4 + wv;
+    // rows rlo .. rhi hold every entry of window w (rhi: the next window's first row, or
+    // the last entry's row; the empty rows of the last slice lie past it)
+    int64_t rlo = w < nwin ? (int64_t)wrow[w] : 0, rhi = w < nwin ? (int64_t)wrow[w + 1] : 0;
+    for (; w < nwin; w += wstride) {
+        const int64_t j0 = w * kCompWin, j1 = min(j0 + (int64_t)kCompWin, nnz);
+        const int64_t wn = w + wstride;  // the next window's rows, loaded ahead
+        const int64_t nlo = wn < nwin ? (int64_t)wrow[wn] : 0, nhi = wn < nwin ? (int64_t)wrow[wn + 1] : 0;
+        for (int64_t r = rlo + lane; r <= rhi; r += 64) {
+            const uint32_t rb = row_ptr[r], re = row_ptr[r + 1];
+            const int64_t b = max((int64_t)rb, j0), e = min((int64_t)re, j1);
+            if (b >= e) continue;
+            // entry j's slot is slot + j (modulo 2^64), plus the arena bit
+            unsigned long long slot = (unsigned long long)r * (unsigned)K - rb, arena_bit = 0ull;
+            if (re - rb > (uint32_t)K) {  // an overflow row: its arena offset is in its first slot
+                if (PACKED) {
+                    slot = (unsigned long long)reinterpret_cast<const uint32_t*>(staging)[(int64_t)r * K] - rb;
+                } else {
+                    const uint2 h = staging[(int64_t)r * K];
+                    slot = ((unsigned long long)h.x | ((unsigned long long)h.y << 32)) - rb;
+                }
+                arena_bit = kArenaBit;
+            }
+            for (int64_t j = b; j < e; j++) {
+                rowof[wv][j - j0] = (uint32_t)r;
+                slotof[wv][j - j0] = (slot + (unsigned long long)j) | arena_bit;
+            }
         }
-    } else {
-        const uint2 h = staging[row * K];
-        const unsigned long long off = (unsigned long long)h.x | ((unsigned long long)h.y << 32);
-        for (int u = u0; u < (int)n; u += K) {
-            uint2 e = arena[off + u];
-            nz[dst + u] = make_int2((int)(colbase + e.x), __float_as_int((float)e.y / nr));
+        __builtin_amdgcn_wave_barrier();
+        int32_t col[4];
+        uint32_t cnt[4];
+        bool ok = true;
+        int32_t cmin = 0x7fffffff, cmax = -1;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t j = j0 + u * 64 + lane;
+            col[u] = -1;
+            cnt[u] = 0u;
+            if (j < j1) {
+                const uint32_t r = rowof[wv][u * 64 + lane];
+                const unsigned long long sl = slotof[wv][u * 64 + lane];
+                uint2 e;
+                if (PACKED) {
+                    const uint32_t v = (sl & kArenaBit) ? reinterpret_cast<const uint32_t*>(arena)[sl & ~kArenaBit]
+                                                        : reinterpret_cast<const uint32_t*>(staging)[sl];
+                    e = make_uint2(v & 0xffffu, v >> 16);
+                } else {
+                    e = (sl & kArenaBit) ? arena[sl & ~kArenaBit] : staging[sl];
+                }
+                const int32_t c = (int32_t)((int64_t)(r / rows_per_slice + 1) * nc + e.x);
+                const float p = (float)e.y / nr;
+                nz[j] = make_int2(c, __float_as_int(p));
+                col[u] = c;
+                cnt[u] = e.y;
+                ok = ok && e.y < 65536u && __float_as_int((float)((double)e.y * inv_nr)) == __float_as_int(p);
+                cmin = min(cmin, c);
+                cmax = max(cmax, c);
+            }
         }
+        __builtin_amdgcn_wave_barrier();  // rowof and slotof are the next window's
+        rlo = nlo;
+        rhi = nhi;
+        if (!cz) continue;
+        for (int o = 32; o >= 1; o >>= 1) {
+            cmin = min(cmin, __shfl_xor(cmin, o, 64));
+            cmax = max(cmax, __shfl_xor(cmax, o, 64));
+        }
+        ok = __all(ok) && cmax >= 0 && (int64_t)cmax - cmin < span;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            cz[j0 + u * 64 + lane] = ok && col[u] >= 0 ? (cnt[u] << 16) | (uint32_t)(col[u] - cmin) : 0u;
+        if (lane == 0) cbase[w] = ok ? cmin : -1;
+        kept += ok ? 1ull : 0ull;
     }
+    if (lane == 0 && kept) atomicAdd(ncomp, kept);
 }
 
 __global__ void fill_tail(uint32_t* row_nnz, float* R, float* Rsum, int64_t from, int64_t to) {
@@ -2088,7 +2186,10 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     const int64_t rows = m->rows;
     DevBuf dNnz, dStage, dArena, dTop;
     if ((st = dmalloc(dNnz, sizeof(uint32_t) * rows))) return fail(st);
-    if ((st = dmalloc(dStage, sizeof(uint2) * rows * K))) return fail(st);
+    // 4-byte staging entries when every count and landing cell fits 16 bits
+    A.packed = nr < 65536 && nc <= 65536 ? 1 : 0;
+    const size_t slot_bytes = A.packed ? sizeof(uint32_t) : sizeof(uint2);
+    if ((st = dmalloc(dStage, slot_bytes * rows * K))) return fail(st);
     if (pool_malloc(&m->R, sizeof(float) * rows) != hipSuccess) {
         set_error("hipMalloc R failed");
         return fail(E2E_ERR_NOMEM);
@@ -2142,6 +2243,9 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     m->info.paired_rounds = nw ? A.pipe_rounds : 0;
     E2E_CHECK(opts.lane_moves == 0 || opts.lane_moves == 1, E2E_ERR_INVALID, "lane_moves must be 0 or 1 (got %d)",
               opts.lane_moves);
+    E2E_CHECK(opts.comp_span >= 0 && opts.comp_span <= kCompSpan, E2E_ERR_INVALID,
+              "comp_span must be 0..65536 (got %d)", opts.comp_span);
+    const int comp_span = opts.comp_span > 0 ? opts.comp_span : kCompSpan;
     // alpha columns: rewards for n_alpha values of alpha from this one build
     E2E_CHECK(opts.n_alpha >= 0 && opts.n_alpha <= 64, E2E_ERR_INVALID, "n_alpha=%d not in 0..64", opts.n_alpha);
     A.n_alpha = 0;
@@ -2210,7 +2314,12 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     double move_ms = 0.0;
     int launches = 0;
     for (int attempt = 0; attempt < 4; attempt++) {
-        if ((st = dmalloc(dArena, sizeof(uint2) * arena_cap))) break;
+        if (A.packed && arena_cap >= (1ull << 32)) {  // packed arena offsets are 32-bit
+            set_error("overflow arena of %llu entries exceeds 32-bit offsets", arena_cap);
+            st = E2E_ERR_CAPACITY;
+            break;
+        }
+        if ((st = dmalloc(dArena, slot_bytes * arena_cap))) break;
         E2E_HIP(hipMemsetAsync(dTop.p, 0, sizeof(unsigned long long), s));
         E2E_HIP(hipMemsetAsync(dStats.p, 0, sizeof(unsigned long long) * kStatSlots, s));
         A.arena = dArena.as<uint2>();
@@ -2360,12 +2469,44 @@ e2e_status model_build(const e2e_params* p, const e2e_fields* f, const e2e_build
     m->nnz = (int64_t)total;
     phase("row-count scan");
     if ((st = alloc_nz(m, (int64_t)total))) return fail(st);
-    hipLaunchKernelGGL(csr_scatter, dim3((unsigned)ceil_div(rows * K, 256)), dim3(256), 0, s,
-                       dStage.as<uint2>(), dArena.as<uint2>(), A.row_nnz, m->row_ptr, m->nz, rows, K,
-                       nc * na, nc, nr);
-    E2E_HIP(hipGetLastError());
+    // the CSR entries and, when every count fits 16 bits, the lean sweep's compressed
+    // stream, in one pass over the staging slots
+    E2E_CHECK(rows < (1ll << 32), E2E_ERR_CAPACITY, "build: %lld rows exceed 32-bit row indices", (long long)rows);
+    const int64_t nwin = ceil_div(m->nnz, (int64_t)kCompWin);
+    const bool comp = nr < 65536;
+    DevBuf dWrow, dKept;
+    if ((st = dmalloc(dWrow, sizeof(uint32_t) * (size_t)(nwin + 1)))) return fail(st);
+    if ((st = dmalloc(dKept, sizeof(unsigned long long)))) return fail(st);
+    if (comp && (pool_malloc(&m->vi_cz, sizeof(uint32_t) * (size_t)std::max<int64_t>(nwin, 1) * kCompWin) != hipSuccess ||
+                 pool_malloc(&m->vi_cbase, sizeof(int32_t) * (size_t)std::max<int64_t>(nwin, 1)) != hipSuccess)) {
+        if (m->vi_cz) pool_free(m->vi_cz);
+        m->vi_cz = nullptr;
+        m->vi_cbase = nullptr;
+        set_error("hipMalloc of the compressed entry stream failed");
+        return fail(E2E_ERR_NOMEM);
+    }
+    E2E_HIP(hipMemsetAsync(dKept.p, 0, sizeof(unsigned long long), s));
+    unsigned long long kept = 0;
+    if (nwin > 0) {
+        hipLaunchKernelGGL(window_rows, dim3((unsigned)ceil_div(rows, 256)), dim3(256), 0, s, m->row_ptr, rows,
+                           m->nnz, nwin, dWrow.as<uint32_t>());
+        E2E_HIP(hipGetLastError());
+        hipLaunchKernelGGL(A.packed ? csr_emit<true> : csr_emit<false>,
+                           dim3((unsigned)std::min<int64_t>(ceil_div(nwin, 4), 16384)), dim3(256), 0, s,
+                           dStage.as<uint2>(), dArena.as<uint2>(), m->row_ptr, dWrow.as<uint32_t>(), m->nz, m->vi_cz,
+                           m->vi_cbase, m->nnz, nwin, K, (uint32_t)(nc * na), nc, nr, 1.0 / (double)nr,
+                           comp_span, dKept.as<unsigned long long>());
+        E2E_HIP(hipGetLastError());
+    }
     E2E_HIP(hipEventRecord(c2, s));
+    E2E_HIP(hipMemcpyAsync(&kept, dKept.p, sizeof(kept), hipMemcpyDeviceToHost, s));
     E2E_HIP(hipEventSynchronize(c2));
+    E2E_HIP(hipStreamSynchronize(s));
+    if (comp) {
+        m->vi_cwin = nwin;
+        m->vi_cwin_comp = (int64_t)kept;
+        m->vi_comp_span = comp_span;
+    }
     float ms1 = 0, ms2 = 0;
     E2E_HIP(hipEventElapsedTime(&ms1, c0, c1));
     E2E_HIP(hipEventElapsedTime(&ms2, c1, c2));

@@ -44,6 +44,13 @@ constexpr int kMaxModes = 16;
 // nnz, so that streaming kernels may load whole windows of up to kNzPad entries without
 // per-entry bounds checks.
 constexpr int64_t kNzPad = 4096;
+// Compressed entry stream of the lean sweep: windows of kCompWin consecutive CSR entries.
+// A window whose columns span less than kCompSpan, and whose probabilities are all
+// count / nr for a count < 2^16 that (float)(count * (1.0 / nr)) reproduces bit for bit,
+// is kept at 4 bytes per entry (count << 16 | column - base, with base = the window's
+// smallest column); any other window has base -1 and is read from the 8-byte entries.
+constexpr int kCompWin = 256;
+constexpr int kCompSpan = 65536;
 
 // Slices a sweep over a built model recomputes. Transitions go from slice t to t+1 only
 // and the last slice's rows have none, so the values of slice t are final from sweep

@@ -119,7 +119,7 @@ struct Server {
         const e2e_params& p = e->p;
         const int64_t nc = (int64_t)p.gsize * p.gsize, rows = nc * p.nt * p.num_ac_speeds * p.num_ac_angles;
         const int64_t fields = 4 * nc * p.nt * (4 + 2 * (int64_t)p.nmodes) + 4 * (int64_t)p.nt * p.nrzns * p.nmodes;
-        const int64_t model = rows * 12 + rows * 12 * 8;           // row_ptr, R, Rsum + nnz at ~12/row
+        const int64_t model = rows * 12 + rows * 12 * 12;          // row_ptr, R, Rsum + nnz at ~12/row, 8 + 4 B each
         const int64_t temps = rows * (16 * 8 + 4) + std::max<int64_t>(1 << 20, rows * 2) * 8;
         return fields + model + temps;
     }

@@ -343,7 +343,6 @@ __device__ __forceinline__ void row_part(const double* q, uint32_t jb, uint32_t 
 }
 
 constexpr int kStreamThreads = 256;
-constexpr int kCompWin = 256;  // entries per window of the compressed stream
 constexpr int kStreamWaves = kStreamThreads / 64;
 
 // wpart[w] = first state of wave w: the smallest multiple of spc whose weight
@@ -1518,8 +1517,8 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
     const int64_t nblocks =
         tiled ? ntiles
               : std::min<int64_t>(std::min<int64_t>(ceil_div(ns, states_per_block), resident), kVIMaxBlocks);
-    const int span = opts.comp_span > 0 ? opts.comp_span : 65536;
-    E2E_CHECK(span <= 65536, E2E_ERR_INVALID, "vi_solve: comp_span must be at most 65536 (got %d)", opts.comp_span);
+    const int span = opts.comp_span > 0 ? opts.comp_span : kCompSpan;
+    E2E_CHECK(span <= kCompSpan, E2E_ERR_INVALID, "vi_solve: comp_span must be at most 65536 (got %d)", opts.comp_span);
     if (comp && m->vi_cz && m->vi_comp_span != span) {  // converted with another span
         pool_free(m->vi_cz);
         pool_free(m->vi_cbase);
@@ -1929,8 +1928,8 @@ e2e_status vi_solve_noalloc(Model* m, const e2e_solve_opts* o, uint32_t* policy,
               "e2e_vi_solve_noalloc: %lld states; at most %lld fit its fixed device scratch", (long long)ns,
               (long long)kNaMaxStates);
     E2E_CHECK(!opts.use_graph, E2E_ERR_INVALID, "e2e_vi_solve_noalloc: use_graph must be 0 (a graph is allocated)");
-    // the lean float64 sweep over the 8-byte entries, or over the compressed entries when
-    // an earlier e2e_vi_solve made them: the two give the same bits
+    // the lean float64 sweep over the compressed entries when the model has them (a build
+    // with nrzns < 65536 makes them), else over the 8-byte entries: the same bits
     const bool has_comp = m->vi_cz != nullptr && m->vi_cbase != nullptr;
     int variant;
     if (opts.kernel_variant == 0)
@@ -1939,8 +1938,8 @@ e2e_status vi_solve_noalloc(Model* m, const e2e_solve_opts* o, uint32_t* policy,
         variant = 2;
     else if (opts.kernel_variant == 6) {
         E2E_CHECK(has_comp, E2E_ERR_INVALID,
-                  "e2e_vi_solve_noalloc: kernel_variant 6 needs the compressed entries an earlier e2e_vi_solve "
-                  "of this model made");
+                  "e2e_vi_solve_noalloc: kernel_variant 6 needs the compressed entries of a built model with "
+                  "nrzns below 65536");
         variant = 5;
     } else {
         set_error("e2e_vi_solve_noalloc: kernel_variant %d (0, 3 and 6 run without allocation)", opts.kernel_variant);

@@ -112,6 +112,10 @@ typedef struct e2e_build_opts {
                                   realization) lane; other values are an error */
     const float* alphas;       /* [n_alpha] host array, each alpha as the reference forms it
                                   (stof(argv[1]) / 100, src/build_model.cu:937-939) */
+    int32_t comp_span;         /* the compressed entry stream the build writes beside the CSR entries (when
+                                  nrzns < 65536) keeps a 256-entry window at 4 bytes per entry when its
+                                  columns span less than this (0: 65536, at most 65536); lower values only
+                                  serve to test windows kept at 8 bytes */
 } e2e_build_opts;
 
 typedef struct e2e_model_info {
@@ -129,7 +133,7 @@ typedef struct e2e_model_info {
     double build_ms;           /* wall time of e2e_model_build                             */
     double move_kernel_ms;     /* summed HIP-event time of the move/sort/count kernel      */
     double scan_ms;            /* HIP-event time of the row-count scan                     */
-    double csr_ms;             /* HIP-event time of the CSR scatter                        */
+    double csr_ms;             /* HIP-event time of writing the CSR and compressed entries */
     int32_t has_rsum;
     int32_t move_launches;
     int32_t block_threads;     /* workgroup size the build kernel ran with */
@@ -326,8 +330,8 @@ e2e_status e2e_vi_solve(void* model, const e2e_solve_opts* opts,
 /* Sparse value iteration without device allocation. It has no counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc). Same arguments and
- * results as e2e_vi_solve, bit for bit, for kernel_variant 0, 3 or 6 (6 only once an
- * e2e_vi_solve of the model has made its compressed entries; 0 takes them when there).
+ * results as e2e_vi_solve, bit for bit, for kernel_variant 0, 3 or 6 (6 needs the
+ * compressed entries a build makes when nrzns < 65536; 0 takes them when there).
  * It calls no hipMalloc, hipFree or other allocating routine: every device buffer beyond
  * the model and the host outputs sits in one fixed array per GPU (56 MiB, held
  * until the process ends). Limits: ns <= 2,097,152 states; for a built model swept over

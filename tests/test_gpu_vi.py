@@ -242,6 +242,34 @@ def test_vi_compressed_matches_lean_bitwise(gpu, case, span):
         assert sc["kernel_variant"] == 6 and np.array_equal(vc.view(np.uint32), va.view(np.uint32))
 
 
+@pytest.mark.parametrize("case,span", [("small", 0), ("small", 60), ("c1", 0), ("c1", 150)])
+def test_vi_compressed_stream_from_the_build(gpu, case, span):
+    """The build writes the compressed entries in the pass that writes the CSR entries.
+    They give the same solve, bit for bit, as the 8-byte entries, and keep the same
+    windows as the solve's own conversion from the CSR entries (forced by a column-span
+    change and back). A small span limit leaves part of the windows at 8 bytes (on
+    these models the windows' column spans have medians of 58 and 146)."""
+    w = small(g=13, nt=10, nr=300, nm=2, na=12, seed=71) if case == "small" else workload("C1")
+    with e2e.build_model(w.params, w.fields(), comp_span=span) as m:
+        nwin = -(-m.info["nnz"] // 256)
+        pa, va, sa = m.solve(thresh=1e-6, kernel_variant=3)
+        pb, vb, sb = m.solve(thresh=1e-6, kernel_variant=6, comp_span=span)   # the build's stream
+        print(f"[{case} span {span}] windows {nwin}, kept by the build {sb['compressed_windows']}", flush=True)
+        assert sb["kernel_variant"] == 6 and sa["sweeps"] == sb["sweeps"]
+        assert np.array_equal(va.view(np.uint32), vb.view(np.uint32)) and np.array_equal(pa, pb)
+        if span == 0:
+            assert sb["compressed_windows"] == nwin
+        else:
+            assert 0 < sb["compressed_windows"] < nwin
+        m.solve(thresh=1e-6, kernel_variant=6, comp_span=(span or 65536) - 1)
+        pc, vc, sc = m.solve(thresh=1e-6, kernel_variant=6, comp_span=span)  # converted again
+        assert sc["compressed_windows"] == sb["compressed_windows"]
+        assert np.array_equal(vc.view(np.uint32), va.view(np.uint32)) and np.array_equal(pc, pa)
+    with pytest.raises(e2e.E2EError) as ei:
+        e2e.build_model(w.params, w.fields(), comp_span=65537)
+    assert ei.value.code == 1
+
+
 @pytest.mark.parametrize("case", ["small", "terminal", "obstacles", "c1", "wide_flow"])
 def test_vi_backward_pass_equals_converged_sweeps(gpu, case):
     """Transitions of a built model go from each slice to the next, so the sweeps reach

@@ -42,15 +42,15 @@ def test_noalloc_equals_solve_on_consecutive_models(gpu, thresh):
             # no buffer of the model's own solve was made, and the pool was not touched
             assert m.info["device_bytes"] == bytes0
             assert e2e.cached_device_bytes() == cached0
-            assert na[2]["kernel_variant"] == 3   # 8-byte entries: no compressed stream yet
+            assert na[2]["kernel_variant"] == 6   # the build made the compressed entries
             ref = m.solve(thresh=thresh)           # variant 6, allocates the model's buffers
             assert ref[2]["kernel_variant"] == 6
             _same(na, ref)
-            # now that the compressed entries exist, the default takes them; same bits
-            na6 = m.solve(thresh=thresh, noalloc=True)
-            assert na6[2]["kernel_variant"] == 6
-            _same(na6, ref)
-            _same(m.solve(thresh=thresh, noalloc=True, kernel_variant=3), ref)
+            # over the 8-byte entries: same bits
+            na3 = m.solve(thresh=thresh, noalloc=True, kernel_variant=3)
+            assert na3[2]["kernel_variant"] == 3
+            _same(na3, ref)
+            _same(m.solve(thresh=thresh, noalloc=True), ref)
             # every state and slice on every sweep: same bits again
             _same(m.solve(thresh=thresh, noalloc=True, full_sweeps=True), ref)
         if thresh == 1e-6:
@@ -69,6 +69,8 @@ def test_noalloc_imported_model(gpu):
         na = m.solve(thresh=1e-6, noalloc=True)
         assert ref[2]["kernel_variant"] == 3 and na[2]["kernel_variant"] == 3
         _same(na, ref)
+        # no compressed entries: an imported model has no slice structure
+        assert _code(lambda: m.solve(thresh=1e-6, noalloc=True, kernel_variant=6)) == E2E_ERR_INVALID
 
 
 def _code(fn, msg=None):
@@ -84,10 +86,9 @@ def test_noalloc_refusals(gpu):
     with e2e.build_model(w.params, w.fields()) as m:
         assert _code(lambda: m.solve(thresh=1e-6, noalloc=True, use_graph=True)) == E2E_ERR_INVALID
         assert _code(lambda: m.solve(thresh=1e-6, noalloc=True, kernel_variant=1)) == E2E_ERR_INVALID
-        # variant 6 before any e2e_vi_solve made the compressed entries
-        assert _code(lambda: m.solve(thresh=1e-6, noalloc=True, kernel_variant=6)) == E2E_ERR_INVALID
         ok = m.solve(thresh=1e-6, noalloc=True)
         _same(ok, m.solve(thresh=1e-6))
+        _same(m.solve(thresh=1e-6, noalloc=True, kernel_variant=6), ok)
     # one state more than the array holds
     ns = MAX_STATES + 1
     R = np.zeros(ns, np.float32)
