@@ -902,6 +902,50 @@ __device__ __forceinline__ float wave_fmax(float x) {
     return __int_as_float(wave_reduce(__float_as_int(x), __float_as_int(-INFINITY), op));
 }
 
+// Every realization's cell of a group's boundary grid, two rounds at a time: the rank
+// #{u <= key} of both samples on both axes by bisection, the four lookups of a level
+// issued together, over the first 2^LX - 1 keys of the x axis and 2^LY - 1 of the y axis
+// (the group has at most that many; the padding keys 0xffffffff lie above every finite
+// sample's key). A position is kept as the LDS address of u[ax][rank] (u[ax] is 256-byte
+// aligned, so adding 4h is an OR). The count of bin (rank_y + 1) * P + rank_x + 1 is
+// raised by one in the 16-bit grid.
+template <int LX, int LY, typename SampleFn>
+__device__ __forceinline__ void place_group(const SampleFn& sample, int lane, int nr, uint32_t ax0, uint32_t ay0,
+                                            int P, uint32_t* grid) {
+    const int bin0 = P + 1 - (int)(ay0 >> 2) * P - (int)(ax0 >> 2);
+    for (int w = lane; w < nr; w += 128) {
+        const bool two = w + 64 < nr;
+        const float2 v1 = sample(w), v2 = sample(two ? w + 64 : w);
+        const uint32_t kx1 = fkey(v1.x), ky1 = fkey(v1.y), kx2 = fkey(v2.x), ky2 = fkey(v2.y);
+        uint32_t ox1 = ax0, oy1 = ay0, ox2 = ax0, oy2 = ay0;
+#pragma unroll
+        for (int h = 32; h >= 1; h >>= 1) {
+            const bool dx = h < (1 << LX), dy = h < (1 << LY);
+            uint32_t ex1 = 0u, ex2 = 0u, ey1 = 0u, ey2 = 0u;
+            if (dx) {
+                ex1 = lds_load(ox1 + 4u * (h - 1));
+                ex2 = lds_load(ox2 + 4u * (h - 1));
+            }
+            if (dy) {
+                ey1 = lds_load(oy1 + 4u * (h - 1));
+                ey2 = lds_load(oy2 + 4u * (h - 1));
+            }
+            if (dx) {
+                ox1 |= ex1 <= kx1 ? 4u * h : 0u;
+                ox2 |= ex2 <= kx2 ? 4u * h : 0u;
+            }
+            if (dy) {
+                oy1 |= ey1 <= ky1 ? 4u * h : 0u;
+                oy2 |= ey2 <= ky2 ? 4u * h : 0u;
+            }
+        }
+        const int b1 = (int)__umul24(oy1 >> 2, (uint32_t)P) + (int)(ox1 >> 2) + bin0;  // LDS addresses < 2^16
+        const int b2 = (int)__umul24(oy2 >> 2, (uint32_t)P) + (int)(ox2 >> 2) + bin0;
+        atomicAdd(&grid[b1 >> 1], 1u << ((b1 & 1) * 16));
+        if (two) atomicAdd(&grid[b2 >> 1], 1u << ((b2 & 1) * 16));
+    }
+}
+
 // One wavefront counts every row of a moving cell (the per-lane fast pass's conditions
 // hold for the cell: unit cells, no scalar-field reward, neither terminal nor obstacle)
 // and writes them as the per-lane pass would. Returns false, wave-uniformly and before
@@ -1042,34 +1086,20 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
         for (int i = lane; i < ((Ty + 2) * P + 1) / 2; i += 64) Wv.grid[i] = 0u;
         __builtin_amdgcn_wave_barrier();
 
-        // every realization's cell of the boundary grid, two rounds at a time: the rank
-        // #{u <= key} of both samples on both axes by bisection over the padded 64 keys,
-        // the four lookups of a level issued together. A position is kept as the LDS
-        // address of u[ax][rank] (u[ax] is 256-byte aligned, so adding 4h is an OR)
+        // every realization's cell of the boundary grid (place_group), with as many
+        // bisection levels per axis as the group's boundaries need (at least 4)
         const uint32_t ax0 = lds_addr(Wv.u[0]), ay0 = lds_addr(Wv.u[1]);
-        // bin of positions (ox, oy): (rank_y + 1) * P + rank_x + 1
-        const int bin0 = P + 1 - (int)(ay0 >> 2) * P - (int)(ax0 >> 2);
-        auto place2 = [&](float2 v1, float2 v2, int& b1, int& b2) __attribute__((always_inline)) {
-            const uint32_t kx1 = fkey(v1.x), ky1 = fkey(v1.y), kx2 = fkey(v2.x), ky2 = fkey(v2.y);
-            uint32_t ox1 = ax0, oy1 = ay0, ox2 = ax0, oy2 = ay0;
-#pragma unroll
-            for (int h = 32; h >= 1; h >>= 1) {
-                const uint32_t ex1 = lds_load(ox1 + 4u * (h - 1)), ey1 = lds_load(oy1 + 4u * (h - 1));
-                const uint32_t ex2 = lds_load(ox2 + 4u * (h - 1)), ey2 = lds_load(oy2 + 4u * (h - 1));
-                ox1 |= ex1 <= kx1 ? 4u * h : 0u;
-                oy1 |= ey1 <= ky1 ? 4u * h : 0u;
-                ox2 |= ex2 <= kx2 ? 4u * h : 0u;
-                oy2 |= ey2 <= ky2 ? 4u * h : 0u;
-            }
-            b1 = (int)__umul24(oy1 >> 2, (uint32_t)P) + (int)(ox1 >> 2) + bin0;  // LDS addresses < 2^16
-            b2 = (int)__umul24(oy2 >> 2, (uint32_t)P) + (int)(ox2 >> 2) + bin0;
-        };
-        for (int w = lane; w < nr; w += 128) {
-            const bool two = w + 64 < nr;
-            int b1, b2;
-            place2(sample(w), sample(two ? w + 64 : w), b1, b2);
-            atomicAdd(&Wv.grid[b1 >> 1], 1u << ((b1 & 1) * 16));
-            if (two) atomicAdd(&Wv.grid[b2 >> 1], 1u << ((b2 & 1) * 16));
+        const int lx = max(4, 32 - __clz(Tx)), ly = max(4, 32 - __clz(Ty));
+        switch (lx * 8 + ly) {
+            case 4 * 8 + 4: place_group<4, 4>(sample, lane, nr, ax0, ay0, P, Wv.grid); break;
+            case 4 * 8 + 5: place_group<4, 5>(sample, lane, nr, ax0, ay0, P, Wv.grid); break;
+            case 4 * 8 + 6: place_group<4, 6>(sample, lane, nr, ax0, ay0, P, Wv.grid); break;
+            case 5 * 8 + 4: place_group<5, 4>(sample, lane, nr, ax0, ay0, P, Wv.grid); break;
+            case 5 * 8 + 5: place_group<5, 5>(sample, lane, nr, ax0, ay0, P, Wv.grid); break;
+            case 5 * 8 + 6: place_group<5, 6>(sample, lane, nr, ax0, ay0, P, Wv.grid); break;
+            case 6 * 8 + 4: place_group<6, 4>(sample, lane, nr, ax0, ay0, P, Wv.grid); break;
+            case 6 * 8 + 5: place_group<6, 5>(sample, lane, nr, ax0, ay0, P, Wv.grid); break;
+            default: place_group<6, 6>(sample, lane, nr, ax0, ay0, P, Wv.grid); break;
         }
         __builtin_amdgcn_wave_barrier();
         // inclusive prefix: down the columns (lane x - 1 walks column x), then along the
