@@ -115,6 +115,26 @@ def test_build_bit_exact(gpu, name, kname):
         assert (S1 == ref["S1"]).all() and (S2 == ref["S2"]).all()
 
 
+@pytest.mark.parametrize("lanes", [0, 1])
+def test_build_wide_grid_8byte_staging(gpu, lanes):
+    """A 260 x 260 grid has more cells than 16 bits hold, so the staging slots and the
+    overflow arena keep 8-byte (cell, count) entries: the same model as the oracle's, bit
+    for bit, through both the counting and the per-row kernels (2 slots a row, so most
+    rows overflow), and the build's compressed entries give the 8-byte entries' solve."""
+    w = small(g=260, nt=3, nr=40, nm=1, na=4, seed=37)
+    p, f = w.params, w.fields()
+    print(f"[wide grid lane_moves {lanes}] g={p.gsize} nt={p.nt} nr={p.nrzns}", flush=True)
+    ref = oracle_model(p, f)
+    with e2e.build_model(p, f, lane_moves=lanes, slot_k=2) as m:
+        assert_same_model(m.export_coo(), ref, "wide grid")
+        assert m.info["n_overflow_rows"] > 0
+        assert (m.info["counted_cells"] > 0) == (lanes == 0)
+        pa, va, sa = m.solve(thresh=1e-6, kernel_variant=3)
+        pb, vb, sb = m.solve(thresh=1e-6, kernel_variant=6)
+        assert sb["kernel_variant"] == 6 and sb["compressed_windows"] == -(-m.info["nnz"] // 256)
+        assert np.array_equal(va.view(np.uint32), vb.view(np.uint32)) and np.array_equal(pa, pb)
+
+
 @pytest.mark.parametrize("kernel,lanes", [(0, 0), (0, 1), (1, 0)])
 def test_build_c1_bit_exact(gpu, kernel, lanes):
     w = workload("C1")
