@@ -184,7 +184,9 @@ struct Server {
                 e->state = ENV_READY;
                 n_built++;
                 // replace the estimate by what the model actually holds
-                const int64_t actual = (int64_t)(m->rows * 12 + m->nnz * 8) + live_batch_bytes_est(e);
+                const int64_t actual = (int64_t)(m->rows * 12 + m->nnz * 8) +
+                                       (m->vi_cz ? (int64_t)(4 * kCompWin + 4) * m->vi_cwin : 0) +
+                                       live_batch_bytes_est(e);
                 resident += actual - e->reserved;
                 e->reserved = actual;
             } else {
