@@ -1519,13 +1519,13 @@ e2e_status vi_solve(Model* m, const e2e_solve_opts* o, uint32_t* policy, float* 
               : std::min<int64_t>(std::min<int64_t>(ceil_div(ns, states_per_block), resident), kVIMaxBlocks);
     const int span = opts.comp_span > 0 ? opts.comp_span : kCompSpan;
     E2E_CHECK(span <= kCompSpan, E2E_ERR_INVALID, "vi_solve: comp_span must be at most 65536 (got %d)", opts.comp_span);
-    if (comp && m->vi_cz && m->vi_comp_span != span) {  // converted with another span
+    if (comp && m->vi_cz && m->vi_comp_span != span) {  // written or converted with another span
         pool_free(m->vi_cz);
         pool_free(m->vi_cbase);
         m->vi_cz = nullptr;
         m->vi_cbase = nullptr;
     }
-    if (comp && !m->vi_cz) {  // the compressed stream, once per model
+    if (comp && !m->vi_cz) {  // the build wrote none for this span: convert the CSR entries
         const int64_t nwin = ceil_div(m->nnz, (int64_t)kCompWin);
         if (pool_malloc(&m->vi_cz, sizeof(uint32_t) * (size_t)std::max<int64_t>(nwin, 1) * kCompWin) != hipSuccess ||
             pool_malloc(&m->vi_cbase, sizeof(int32_t) * (size_t)std::max<int64_t>(nwin, 1)) != hipSuccess) {
