@@ -999,16 +999,31 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
         }
         return make_float2(u0 + sx, v0 + sy);
     };
-    // the cell's sample range (-0 and +0 have the same ords, so the float order serves)
+    // the cell's sample range (-0 and +0 have the same ords, so the float order serves).
+    // With Yi in LDS two rounds at a time: a sample with a non-finite component has a
+    // non-finite x + y; so has a finite one whose sum overflows, which only hands the
+    // cell to the per-lane pass (the same rows). With Yi read from L2, one round at a
+    // time (two were slower on the 200x200x200 fields).
     float x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
     bool nf = false;
-    for (int w = lane; w < nr; w += 64) {
-        const float2 v = sample(w);
-        nf |= !(isfinite(v.x) && isfinite(v.y));
-        x0 = fminf(x0, v.x);
-        x1 = fmaxf(x1, v.x);
-        y0 = fminf(y0, v.y);
-        y1 = fmaxf(y1, v.y);
+    if constexpr (YLDS) {
+        for (int w = lane; w < nr; w += 128) {
+            const float2 v1 = sample(w), v2 = sample(w + 64 < nr ? w + 64 : w);
+            nf |= !(isfinite(v1.x + v1.y) && isfinite(v2.x + v2.y));
+            x0 = fminf(fminf(x0, v1.x), v2.x);
+            x1 = fmaxf(fmaxf(x1, v1.x), v2.x);
+            y0 = fminf(fminf(y0, v1.y), v2.y);
+            y1 = fmaxf(fmaxf(y1, v1.y), v2.y);
+        }
+    } else {
+        for (int w = lane; w < nr; w += 64) {
+            const float2 v = sample(w);
+            nf |= !(isfinite(v.x) && isfinite(v.y));
+            x0 = fminf(x0, v.x);
+            x1 = fmaxf(x1, v.x);
+            y0 = fminf(y0, v.y);
+            y1 = fmaxf(y1, v.y);
+        }
     }
     if (__any(nf)) return false;
     x0 = wave_fmin(x0);
