@@ -985,16 +985,18 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
         cu[k] = A.ui[o];
         cv[k] = A.vi[o];
     }
-    // realization w's velocity sample, extract_velocity :361-404 (the mode sum from 0 in
-    // k order, then added to the mean)
+    // realization w's velocity sample, extract_velocity :361-404 (the mode sum in k
+    // order, then added to the mean). The reference's sum starts from 0; starting from
+    // the first product instead changes only the sign of a zero sum, and -0 and +0 have
+    // the same ord, so no boundary key lies between them and every rank is the same
     auto sample = [&](int w) __attribute__((always_inline)) {
         float sx = 0.0f, sy = 0.0f;
 #pragma unroll
         for (int k = 0; k < (NM ? NM : kCntModes); k++) {
             if (NM || k < nm) {
                 const float y = YLDS ? Y[k * nr + w] : Y[w * nm + k];
-                sx = sx + cu[k] * y;
-                sy = sy + cv[k] * y;
+                sx = k ? sx + cu[k] * y : cu[k] * y;
+                sy = k ? sy + cv[k] * y : cv[k] * y;
             }
         }
         return make_float2(u0 + sx, v0 + sy);
