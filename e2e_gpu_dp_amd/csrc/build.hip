@@ -801,6 +801,10 @@ __device__ __forceinline__ uint32_t lds_addr(const void* p) {
 __device__ __forceinline__ uint32_t lds_load(uint32_t a) {
     return *(const __attribute__((address_space(3))) uint32_t*)(uintptr_t)a;
 }
+// a generic pointer to the shared-memory word at 32-bit LDS address a
+__device__ __forceinline__ void* lds_ptr(uint32_t a) {
+    return (void*)(__attribute__((address_space(3))) void*)(uintptr_t)a;
+}
 // LDS of a counting block: this table, the waves' tables, then Yi [nm][nr] when it fits
 struct CountShared {
     float fcos[kMaxActions], fsin[kMaxActions], speed[kMaxActions];  // the action table, for per-lane reads
@@ -912,7 +916,9 @@ __device__ __forceinline__ float wave_fmax(float x) {
 template <int LX, int LY, typename SampleFn>
 __device__ __forceinline__ void place_group(const SampleFn& sample, int lane, int nr, uint32_t ax0, uint32_t ay0,
                                             int P, uint32_t* grid) {
-    const int bin0 = P + 1 - (int)(ay0 >> 2) * P - (int)(ax0 >> 2);
+    // byte address of the 16-bit count of bin b = (rank_y + 1) * P + rank_x + 1: the grid's
+    // address + 2 b = (oy * P + ox) / 2 + base2 (oy * P and ox are multiples of 4)
+    const uint32_t base2 = lds_addr(grid) + 2u * (uint32_t)(P + 1) - ((ay0 * (uint32_t)P + ax0) >> 1);
     for (int w = lane; w < nr; w += 128) {
         const bool two = w + 64 < nr;
         const float2 v1 = sample(w), v2 = sample(two ? w + 64 : w);
@@ -939,10 +945,12 @@ __device__ __forceinline__ void place_group(const SampleFn& sample, int lane, in
                 oy2 |= ey2 <= ky2 ? 4u * h : 0u;
             }
         }
-        const int b1 = (int)__umul24(oy1 >> 2, (uint32_t)P) + (int)(ox1 >> 2) + bin0;  // LDS addresses < 2^16
-        const int b2 = (int)__umul24(oy2 >> 2, (uint32_t)P) + (int)(ox2 >> 2) + bin0;
-        atomicAdd(&grid[b1 >> 1], 1u << ((b1 & 1) * 16));
-        if (two) atomicAdd(&grid[b2 >> 1], 1u << ((b2 & 1) * 16));
+        // (LDS addresses < 2^16, so the products fit 24 bits by 24 bits)
+        const uint32_t a1 = ((__umul24(oy1, (uint32_t)P) + ox1) >> 1) + base2;
+        const uint32_t a2 = ((__umul24(oy2, (uint32_t)P) + ox2) >> 1) + base2;
+        // the word holding the count, and its half
+        atomicAdd(reinterpret_cast<uint32_t*>(lds_ptr(a1 & ~3u)), 1u << ((a1 & 2u) << 3));
+        if (two) atomicAdd(reinterpret_cast<uint32_t*>(lds_ptr(a2 & ~3u)), 1u << ((a2 & 2u) << 3));
     }
 }
 
