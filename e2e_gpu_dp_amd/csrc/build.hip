@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
@@ -788,10 +789,12 @@ constexpr int kCntModes = 8;                   // DO modes a counting wave holds
 // then carries the LDS address of its position itself (the base OR the offset).
 struct alignas(256) CountWave {
     uint32_t u[2][64];                  // the group's boundary keys per axis, ascending; padding 0xffffffff
+    uint32_t d[kCntMax + 1];            // boundary j of the axis being ranked: action << 16 | ord, then its
+                                        // key (padding 0xffffffff), read four at a time
     uint32_t grid[(kCntArea + 1) / 2];  // 16-bit counts, two per word (a count is at most nrzns < 2^16)
     uint8_t c[2][kCntMax + 1];          // boundary j of the group: #{u <= its key}
-    uint32_t d[kCntMax + 1];            // boundary j of the axis being ranked: action << 16 | ord
 };
+static_assert(offsetof(CountWave, d) % 16 == 0, "the keys are read as uint4");
 static_assert(sizeof(CountWave) % 256 == 0, "counting-wave tables keep u on 256-byte boundaries");
 
 // 32-bit LDS address of a pointer into shared memory, and a load from such an address
@@ -1094,16 +1097,24 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
                 const int a = (int)(dsc >> 16), o = (int)(dsc & 0xffffu);
                 key = ax ? boundary_key<1>(G, T.fsin[a], o, ky0, ky1) : boundary_key<0>(G, T.fcos[a], o, kx0, kx1);
             }
-            uint32_t cnt = 0u, pos = 0u;  // #{keys <= mine}, and a stable sort
-            for (int i = 0; i < nb; i++) {
-                const uint32_t ki = __builtin_amdgcn_readlane(key, i);
-                cnt += ki <= key ? 1u : 0u;
-                pos += (ki < key || (ki == key && i < lane)) ? 1u : 0u;
+            // the descriptors are read: the keys take their place (0xffffffff past nb, above
+            // every finite sample's key), and every lane counts #{keys <= mine} and
+            // #{keys < mine} over them, four keys per LDS read
+            __builtin_amdgcn_wave_barrier();
+            Wv.d[lane] = key;
+            __builtin_amdgcn_wave_barrier();
+            uint32_t cnt = 0u, lt = 0u;
+            const uint4* d4 = reinterpret_cast<const uint4*>(Wv.d);
+            for (int i = 0; i < nb; i += 4) {
+                const uint4 q = d4[i >> 2];
+                cnt += (q.x <= key ? 1u : 0u) + (q.y <= key ? 1u : 0u) + (q.z <= key ? 1u : 0u) + (q.w <= key ? 1u : 0u);
+                lt += (q.x < key ? 1u : 0u) + (q.y < key ? 1u : 0u) + (q.z < key ? 1u : 0u) + (q.w < key ? 1u : 0u);
             }
             __builtin_amdgcn_wave_barrier();
+            // sorted: equal keys share the run [lt, cnt), each lane of the run writes all of it
             if (lane < nb) {
                 Wv.c[ax][lane] = (uint8_t)cnt;
-                Wv.u[ax][pos] = key;
+                for (uint32_t j = lt; j < cnt; j++) Wv.u[ax][j] = key;
             } else {
                 Wv.u[ax][lane] = 0xffffffffu;
             }
