@@ -1139,7 +1139,7 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
         }
         __builtin_amdgcn_wave_barrier();
         // inclusive prefix: down the columns (lane x - 1 walks column x), then along the
-        // rows (a wave scan per row)
+        // rows (lane y - 1 walks row y)
         uint16_t* g16 = reinterpret_cast<uint16_t*>(Wv.grid);
         if (lane <= Tx) {
             int acc = 0;
@@ -1149,9 +1149,13 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
             }
         }
         __builtin_amdgcn_wave_barrier();
-        for (int y = 1; y <= Ty + 1; y++) {
-            const int sc = wave_incl_scan(lane <= Tx ? (int)g16[y * P + lane + 1] : 0);
-            if (lane <= Tx) g16[y * P + lane + 1] = (uint16_t)sc;
+        if (lane <= Ty) {
+            uint16_t* row = g16 + (lane + 1) * P + 1;
+            int acc = 0;
+            for (int x = 0; x <= Tx; x++) {
+                acc += row[x];
+                row[x] = (uint16_t)acc;
+            }
         }
         __builtin_amdgcn_wave_barrier();
 
