@@ -1013,10 +1013,10 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
         return make_float2(u0 + sx, v0 + sy);
     };
     // the cell's sample range (-0 and +0 have the same ords, so the float order serves).
-    // With Yi in LDS two rounds at a time: a sample with a non-finite component has a
-    // non-finite x + y; so has a finite one whose sum overflows, which only hands the
-    // cell to the per-lane pass (the same rows). With Yi read from L2, one round at a
-    // time (two were slower on the 200x200x200 fields).
+    // A sample with a non-finite component has a non-finite x + y; so has a finite one
+    // whose sum overflows, which only hands the cell to the per-lane pass (the same rows).
+    // With Yi in LDS two rounds at a time; with Yi read from L2, one round at a time (two
+    // were slower on the 200x200x200 fields).
     float x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
     bool nf = false;
     if constexpr (YLDS) {
@@ -1031,7 +1031,7 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
     } else {
         for (int w = lane; w < nr; w += 64) {
             const float2 v = sample(w);
-            nf |= !(isfinite(v.x) && isfinite(v.y));
+            nf |= !isfinite(v.x + v.y);
             x0 = fminf(x0, v.x);
             x1 = fmaxf(x1, v.x);
             y0 = fminf(y0, v.y);
