@@ -780,7 +780,10 @@ __device__ __forceinline__ RowRewards row_rewards(const BuildArgs& A, float rste
 // group (on the 100x100x120 fields 1.28 groups per cell, against 1.54 with at most 47
 // boundaries per axis in a square grid of the same area).
 constexpr int kCntMax = 63;                    // boundaries per axis per group (ranks fit 64 keys)
-constexpr int kCntArea = 49 * 49;              // grid [I_y + 1][I_x + 1] bins, row pitch I_x + 2
+constexpr int kCntArea = 49 * 49;              // grid [I_y + 1][I_x + 1] bins, row pitch cnt_pitch(I_x)
+// Row pitch of a group's grid for tx boundaries on x: tx + 2 bins, one more when that is
+// a multiple of 4, so that the lanes walking the rows (one row each) spread over the LDS banks
+__device__ __forceinline__ int cnt_pitch(int tx) { return tx + 2 + (((tx + 2) & 3) == 0 ? 1 : 0); }
 constexpr int kCntWaves = 16;                  // waves of a counting block
 constexpr int kCntModes = 8;                   // DO modes a counting wave holds in registers
 
@@ -1055,7 +1058,7 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
     const int bxlo = col_of(G, olx), bylo = row_of(G, ohy);
     const int bw = col_of(G, ohx) - bxlo + 1, ncell = bw * (row_of(G, oly) - bylo + 1);
     const RowRewards rr = row_rewards(A, step_reward(A, T.speed[aa], 0.0f, 0.0f), late_slice, false);
-    bool ok = rr.seg_lin && nx >= 0 && ny >= 0 && nx <= kCntMax && ny <= kCntMax && (nx + 2) * (ny + 2) <= kCntArea;
+    bool ok = rr.seg_lin && nx >= 0 && ny >= 0 && nx <= kCntMax && ny <= kCntMax && cnt_pitch(nx) * (ny + 2) <= kCntArea;
     if (OBS && obs_slice)  // no obstacle within the fixed window: boxes must stay inside it
         ok = ok && bxlo >= j1 - kFixHalf && col_of(G, ohx) <= j1 + kFixHalf && bylo >= i1 - kFixHalf &&
              row_of(G, oly) <= i1 + kFixHalf;
@@ -1077,11 +1080,11 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
         const int bx = start ? __builtin_amdgcn_readlane(ix, start - 1) : 0;
         const int by = start ? __builtin_amdgcn_readlane(iy, start - 1) : 0;
         const bool fits = act && lane >= start && ix - bx <= kCntMax && iy - by <= kCntMax &&
-                          (ix - bx + 2) * (iy - by + 2) <= kCntArea;
+                          cnt_pitch(ix - bx) * (iy - by + 2) <= kCntArea;
         const int end = start + max(1, (int)__popcll(__ballot(fits)));
         const int Tx = min(kCntMax, __builtin_amdgcn_readlane(ix, end - 1) - bx);
         const int Ty = min(kCntMax, __builtin_amdgcn_readlane(iy, end - 1) - by);
-        const int P = Tx + 2;  // the grid's row pitch
+        const int P = cnt_pitch(Tx);  // the grid's row pitch
 
         // the boundaries, x then y: one lane each, bisection, rank
 #pragma unroll
