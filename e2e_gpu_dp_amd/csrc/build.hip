@@ -1174,6 +1174,9 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
             const int offx = __shfl(fx, src, 64) - bx, offy = __shfl(fy, src, 64) - by;
             const int xlo = __shfl(bxlo, src, 64), ylo = __shfl(bylo, src, 64), W = __shfl(bw, src, 64);
             const int nce = arow ? __shfl(ncell, src, 64) : 0;
+            // the row's integer rewards, formed once per cell by its action's lane
+            const int r_tn0 = __shfl(rr.tn0, src, 64), r_tn1 = __shfl(rr.tn1, src, 64);
+            const int r_tt0 = __shfl(rr.tt0, src, 64), r_tt1 = __shfl(rr.tt1, src, 64);
             const float rW = __builtin_amdgcn_rcpf((float)W);
             // first grid column (row) of the realizations with ord >= o
             auto gx = [&](int o) __attribute__((always_inline)) {
@@ -1259,9 +1262,8 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
             }
             if (lead) {
                 // the fast pass's sum: per landing cell, plus r_outbound per clamp
-                const RowRewards rw = row_rewards(A, step_reward(A, T.speed[ar], 0.0f, 0.0f), late_slice, false);
-                const int64_t tot = (int64_t)n0 * rw.tn0 + (int64_t)n1c * rw.tn1 + (int64_t)n2 * rw.tt0 +
-                                    (int64_t)n3 * rw.tt1 + (int64_t)A.r_out * (int64_t)kcl;
+                const int64_t tot = (int64_t)n0 * r_tn0 + (int64_t)n1c * r_tn1 + (int64_t)n2 * r_tt0 +
+                                    (int64_t)n3 * r_tt1 + (int64_t)A.r_out * (int64_t)kcl;
                 const float fs_ = (float)(double)tot;  // compute_mean :563-570
                 A.row_nnz[row] = (uint32_t)nuq;
                 A.Rout[row] = fs_ / nr;
