@@ -814,7 +814,8 @@ __device__ __forceinline__ void* lds_ptr(uint32_t a) {
 // LDS of a counting block: this table, the waves' tables, then Yi [nm][nr] when it fits
 struct CountShared {
     float fcos[kMaxActions], fsin[kMaxActions], speed[kMaxActions];  // the action table, for per-lane reads
-    unsigned next_cell;                                              // the block's next cell for a free wave
+    RowRewards rew[kMaxActions];  // each action's row rewards on the block's slice (row_rewards)
+    unsigned next_cell;           // the block's next cell for a free wave
 };
 
 __device__ __forceinline__ uint32_t fkey(float f) {
@@ -1057,7 +1058,7 @@ __device__ __forceinline__ bool count_cell(const BuildArgs& A, const float* __re
     const int nx = act ? ohx - olx : 0, ny = act ? ohy - oly : 0;
     const int bxlo = col_of(G, olx), bylo = row_of(G, ohy);
     const int bw = col_of(G, ohx) - bxlo + 1, ncell = bw * (row_of(G, oly) - bylo + 1);
-    const RowRewards rr = row_rewards(A, step_reward(A, T.speed[aa], 0.0f, 0.0f), late_slice, false);
+    const RowRewards rr = T.rew[aa];
     bool ok = rr.seg_lin && nx >= 0 && ny >= 0 && nx <= kCntMax && ny <= kCntMax && cnt_pitch(nx) * (ny + 2) <= kCntArea;
     if (OBS && obs_slice)  // no obstacle within the fixed window: boxes must stay inside it
         ok = ok && bxlo >= j1 - kFixHalf && col_of(G, ohx) <= j1 + kFixHalf && bylo >= i1 - kFixHalf &&
@@ -1761,6 +1762,8 @@ __global__ __launch_bounds__(kCntWaves * 64) void mc_count(const BuildArgs A) {
             T.fcos[a] = A.act_fcos[a];
             T.fsin[a] = A.act_fsin[a];
             T.speed[a] = A.act_speed[a];
+            // the rewards of a row depend on its action and the slice only: once per block
+            T.rew[a] = row_rewards(A, step_reward(A, A.act_speed[a], 0.0f, 0.0f), A.t0 + (int)blockIdx.y > A.nt - 20, false);
         }
     const int c_begin = blockIdx.x * A.tc_count, c_end = min(c_begin + A.tc_count, nc);
     if (tid == 0) T.next_cell = (unsigned)c_begin;
